@@ -1,0 +1,115 @@
+#!/usr/bin/env python3
+"""Benchmark: drop-impact time steps per second at 1024x512, every projection
+(MAC, FEM, implicit viscosity u and v) converged to relative residual 1e-10.
+
+One step = fluid_2d::step_forward(dt_reg) on GPU-resident state (gas-layer
+update, Godunov passes, MAC projection, boundary conditions, two implicit
+viscous solves, FEM projection).  Workload: 32 cSt drop, R=1 mm, V=0.5 m/s,
+h_min=5 um, x_sym, implicit viscosity, synthetic (no input data needed).
+
+Prints one JSON line.  The CPU baseline is the FP64 oracle (a C restatement of
+the reference step) timed on the host's cores on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = "steps/s at 1024x512 to 1e-10 residual: plain FP64 vs mixed+learned; HBM %"
+
+
+def cpu_baseline(m, n, seconds, warmup):
+    """Times the oracle on the host (test infrastructure used as the checker
+    and as the reported CPU baseline)."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import oracle_lib
+    from vdropimpact_amd.params import drop_impact_params
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    p = drop_impact_params(m, n)
+    s = oracle_lib.OracleSim(p, rel_tol=1e-10)
+    s.init_fields()
+    for _ in range(warmup):
+        s.step_forward(p.dt_reg)
+    k = 0
+    t0 = time.perf_counter()
+    while True:
+        s.step_forward(p.dt_reg)
+        k += 1
+        el = time.perf_counter() - t0
+        if el >= seconds and k >= 2:
+            break
+    s.close()
+    return {"value": k / el, "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": f"{k} steps at {m}x{n} after {warmup} warm-up steps, {el:.1f} s, "
+                      f"OMP_NUM_THREADS={threads}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--m", type=int, default=1024)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this solver runs one simulation on one GPU; "
+              "multi-GPU runs are not part of this build (use --gpus 1).", file=sys.stderr)
+        sys.exit(2)
+
+    import vdropimpact_amd as vdi
+    from vdropimpact_amd.params import drop_impact_params
+
+    p = drop_impact_params(args.m, args.n)
+    sim = vdi.Fluid2D(p, rel_tol=1e-10)
+    sim.init_fields()
+    for _ in range(args.warmup):
+        sim.step_forward(p.dt_reg)
+    sim.synchronize()
+    sim.reset_stats()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sim.step_forward(p.dt_reg)
+    sim.synchronize()
+    el = time.perf_counter() - t0
+    st = sim.stats()
+    cyc = {name: st.total_cycles[k] / max(1, st.solves[k]) for k, name in enumerate(["mac", "fem", "vx", "vy"])}
+    value = args.steps / el
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * el / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"single drop {args.m}x{args.n}, 32 cSt, V=0.5 m/s, R=1 mm, h_min=5 um, "
+                               "x_sym, implicit viscosity, all solves to rel. residual 1e-10",
+                   "grid": [args.m, args.n], "form": "fp64", "rel_tol": 1e-10,
+                   "avg_vcycles": cyc},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    sim.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.m, args.n, args.cpu_seconds, 3)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

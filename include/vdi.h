@@ -1,0 +1,156 @@
+/* vdi.h -- C ABI of the MI355X (gfx950) per-timestep fluid solve.
+ *
+ * The reference (chr1shr/vdropimpact) has no FFI layer: its boundary is the
+ * C++ class API of fluid_2d, mgs_mac, mgs_fem, visco_impl and gas_layer.
+ * Every entry point below replaces one of those calls; the comment on each
+ * cites the reference interface it stands in for.  Plain C types only: no
+ * HIP or torch types cross this boundary.
+ *
+ * Conventions
+ *   - Every function returns an int status (VDI_OK on success).  The
+ *     reference is fail-stop (fatal_error -> exit(1), common.cc:44-47);
+ *     here the caller decides, and vdi_status_string() gives the reference's
+ *     message for each status.
+ *   - A context owns all of its device memory and one HIP stream.  Host
+ *     buffers passed in are copied, never retained.
+ *   - A context is used by one host thread at a time (like the reference's
+ *     objects, which share the src array between solvers).
+ *   - Padded cell arrays have the reference layout: (m+4) x (n+4) doubles,
+ *     cell (i,j), i in [-2,m+1], j in [-2,n+1], at (i+2) + (m+4)(j+2)
+ *     (fluid_2d.cc:18, fbase/fm).
+ */
+#ifndef VDI_ABI_H
+#define VDI_ABI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define VDI_OK 0
+#define VDI_ERR_MG_NOCONV 1     /* "Multigrid routine for ... failed" (mgs_mac.hh:68-70, mgs_fem.hh:75-77, visco_impl.cc:34-37) */
+#define VDI_ERR_NEWTON_NOCONV 2 /* "Too many Newton steps" (gas_layer_solve.cc:204) */
+#define VDI_ERR_TRIDIAG 3       /* "LAPACK routine failed" (tri_solve.cc:29-32) */
+#define VDI_ERR_HIP 4           /* HIP runtime error (no reference counterpart) */
+#define VDI_ERR_ARG 5           /* invalid argument (no reference counterpart) */
+
+/* ---- linear-system kinds (the four solvers of fluid_2d, fluid_2d.hh:135-144) ---- */
+#define VDI_MAC 0
+#define VDI_FEM 1
+#define VDI_VX 2
+#define VDI_VY 3
+
+/* ---- arithmetic forms ---- */
+#define VDI_FORM_FP64 0   /* plain FP64 V-cycles */
+#define VDI_FORM_MIXED 1  /* FP32 smoothing and transfers inside FP64 defect correction */
+
+/* ---- array identifiers for vdi_get_array / vdi_set_array ---- */
+enum {
+    VDI_U = 0, VDI_V, VDI_P,                        /* padded (m+4)(n+4): field::u,v,p */
+    VDI_UL, VDI_VL, VDI_UD, VDI_VD, VDI_C0, VDI_C1, /* padded: edge velocities, c0/c1 */
+    VDI_SRC,                                        /* (m+1)(n+1): shared RHS, s_bc = src+mn */
+    VDI_GP, VDI_GTAU, VDI_GP_OLD, VDI_GTAU_OLD,     /* gas_layer p[me], tau[ml], p_old, tau_old */
+    VDI_H, VDI_HX, VDI_HT, VDI_FBD, VDI_PG, VDI_PG_OLD, /* gas_layer_solve h[m],hx[me],ht[m],fbd[m],pg[me],pg_old[me] */
+    VDI_Z_MAC, VDI_Z_FEM, VDI_Z_VX, VDI_Z_VY,       /* solver solutions mgs_common::z */
+    VDI_NARRAYS
+};
+
+/* Constants the step needs, derived from a reference .cfg exactly as
+ * fileinfo.cc:141-152 and fluid_2d.cc:15-17 derive them.  Field order is
+ * part of the ABI. */
+typedef struct vdi_params {
+    int m, n;              /* grid cells (fileinfo::m, n) */
+    int x_sym;             /* fileinfo::x_sym */
+    int implicit_visc;     /* fileinfo::implicit_visc */
+    double ax, bx, ay, by; /* domain (fileinfo.cc:148-152) */
+    double rho, mu;        /* liquid density and dynamic viscosity mu = nul*rho */
+    double dt_reg;         /* fluid_2d::dt_reg (fluid_2d.cc:17) */
+    double alpha, mug, Pamb, h0, R, V, sigma; /* gl_params (gas_layer_solve.hh:20-41) */
+} vdi_params;
+
+/* Per-solver and per-step counters, mirroring the reference's machine-
+ * readable timing line (fluid_2d.cc:66-77). */
+typedef struct vdi_stats {
+    int last_cycles[4];    /* V-cycles of the most recent solve, by VDI_MAC.. */
+    long total_cycles[4];  /* since creation or the last vdi_reset_stats */
+    long solves[4];
+    int last_newton;       /* Newton steps of the most recent gas-layer update */
+    long steps;
+    double time;           /* fluid_2d::time */
+} vdi_stats;
+
+typedef struct vdi_ctx vdi_ctx;
+typedef struct vdi_mg vdi_mg;
+
+const char *vdi_status_string(int status);
+/* Text of the last HIP error seen by this library (or ""). */
+const char *vdi_last_error(void);
+
+/* fluid_2d::fluid_2d(const char*) (fluid_2d.cc:14-24): allocate the state,
+ * the four solvers and the gas layer on the current HIP device. */
+int vdi_create(const vdi_params *p, vdi_ctx **out);
+/* fluid_2d::~fluid_2d (fluid_2d.cc:27-36) */
+int vdi_destroy(vdi_ctx *ctx);
+/* Solver schedule and stopping rule: nu1/nu2 smoothing sweeps, relative
+ * residual tolerance ||b-Az||/||b||, V-cycle cap (replaces the TGMG
+ * constants and tgmg_accuracy, mgs_mac.cc:11, mgs_fem.cc:11, visco_impl.cc:13). */
+int vdi_set_solver(vdi_ctx *ctx, int nu1, int nu2, double rel_tol, int max_cycles);
+/* Arithmetic form of the V-cycles (VDI_FORM_*). */
+int vdi_set_form(vdi_ctx *ctx, int form);
+/* fluid_2d::init_fields (fluid_2d_sio.cc:14-35) */
+int vdi_init_fields(vdi_ctx *ctx);
+/* fluid_2d::step_forward(double dt) (fluid_2d.cc:93-405) */
+int vdi_step_forward(vdi_ctx *ctx, double dt);
+/* gas_layer::update(double t, double dt) (gas_layer.cc:36-46) */
+int vdi_gas_layer_update(vdi_ctx *ctx, double t, double dt);
+/* mgs_mac::solve_v_cycle(double t) (mgs_mac.hh:54-59) and
+ * mgs_fem::solve_v_cycle(double t) (mgs_fem.hh:61-66): time-extrapolated
+ * initial guess, then V-cycles on the shared src.  which = VDI_MAC/VDI_FEM. */
+int vdi_mg_solve_v_cycle(vdi_ctx *ctx, int which, double t, int *vcycles);
+/* visco_impl::setup_and_solve(double vfx, double vfy, double t)
+ * (visco_impl.cc:21-39).  which = VDI_VX or VDI_VY. */
+int vdi_visco_setup_and_solve(vdi_ctx *ctx, int which, double vfx, double vfy, double t, int *vcycles);
+/* Host copies of state arrays (ids above); len must equal vdi_array_len. */
+long vdi_array_len(vdi_ctx *ctx, int id);
+int vdi_get_array(vdi_ctx *ctx, int id, double *host, long len);
+int vdi_set_array(vdi_ctx *ctx, int id, const double *host, long len);
+int vdi_get_time(vdi_ctx *ctx, double *t);
+int vdi_set_time(vdi_ctx *ctx, double t);
+int vdi_get_stats(vdi_ctx *ctx, vdi_stats *st);
+int vdi_reset_stats(vdi_ctx *ctx);
+/* Blocks until all work queued on the context's stream has finished. */
+int vdi_synchronize(vdi_ctx *ctx);
+/* Restart-file body (fluid_2d_sio.cc:97-119 + gas_layer_solve.cc:326-332):
+ * time, (n+4) rows x (m+4) x {u,v,p}, Vframe, p[me], tau[2..m+1], h[m],
+ * ht[m], pg[me].  Length: vdi_state_len. */
+long vdi_state_len(vdi_ctx *ctx);
+int vdi_download_state(vdi_ctx *ctx, double *host, long len);
+int vdi_upload_state(vdi_ctx *ctx, const double *host, long len);
+
+/* ---- standalone linear systems (mg_test.cc:19-45, operator-level use) ----
+ * kind VDI_MAC: (cx,cy) = (1/dx^2, 1/dy^2), m x n cells (mgs_mac.cc:23-27).
+ * kind VDI_FEM: (cx,cy) = (dx,dy), m x n nodes (mgs_fem.cc:23-27).
+ * kind VDI_VX/VDI_VY: (cx,cy) = (vfx,vfy), m x n cells, x_sym as in fluid_2d. */
+int vdi_mg_create(int kind, int m, int n, double cx, double cy, int x_sym, vdi_mg **out);
+int vdi_mg_destroy(vdi_mg *g);
+int vdi_mg_set_schedule(vdi_mg *g, int nu1, int nu2);
+int vdi_mg_set_form(vdi_mg *g, int form);
+int vdi_mg_levels(vdi_mg *g);
+/* Host <-> device copies of the right-hand side b and the solution z (m*n). */
+int vdi_mg_set_b(vdi_mg *g, const double *host, long len);
+int vdi_mg_set_z(vdi_mg *g, const double *host, long len);
+int vdi_mg_get_z(vdi_mg *g, double *host, long len);
+/* out = A z (host arrays, for operator checks). */
+int vdi_mg_apply(vdi_mg *g, const double *z, double *out, long len);
+/* tgmg::solve_v_cycle: V-cycles until ||b-Az|| <= rel_tol ||b||. */
+int vdi_mg_solve(vdi_mg *g, double rel_tol, int max_cycles, int *vcycles);
+/* tgmg::v_cycle (mg_test.cc:38): exactly k V-cycles, no convergence test. */
+int vdi_mg_vcycles(vdi_mg *g, int k);
+/* ||b - A z||_2 and ||b||_2 of the device arrays. */
+int vdi_mg_residual(vdi_mg *g, double *rnorm, double *bnorm);
+int vdi_mg_synchronize(vdi_mg *g);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

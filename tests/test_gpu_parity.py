@@ -1,0 +1,239 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU FP64 oracle
+on the same inputs, and against closed forms.
+
+Tolerance: the north-star bar is <= 1e-9 relative L2 per field once every
+solve has converged to the same residual; parity runs drive both sides'
+multigrid to a relative residual of 1e-13 so that the comparison measures
+the step, not the stopping point of two different iterations.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import closed_forms as cf
+from vdropimpact_amd.params import PhysicalConfig
+
+pytestmark = pytest.mark.gpu
+
+PARITY = 1e-9
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+# ---------------------------------------------------------------- operators
+
+@pytest.mark.parametrize("kind", [cf.MAC, cf.FEM, cf.VX, cf.VY])
+def test_gpu_operator_closed_form(vdi, kind):
+    say("operator closed form, kind", kind)
+    if kind == cf.FEM:
+        M, N, dx, dy = 33, 17, 0.1, 0.07
+        g = vdi.MGSystem(kind, M, N, dx, dy)
+        modes = [cf.fem_mode(M, N, k, l, dx, dy) for k, l in [(0, 0), (3, 2), (M - 1, N - 2)]]
+    elif kind == cf.MAC:
+        m, n, cx, cy = 32, 16, 100.0, 180.0
+        g = vdi.MGSystem(kind, m, n, cx, cy)
+        modes = [cf.mac_mode(m, n, k, l, cx, cy) for k, l in [(0, 0), (3, 2), (m - 1, n - 1)]]
+    else:
+        m, n, cx, cy = 32, 16, 0.3, 0.17
+        g = vdi.MGSystem(kind, m, n, cx, cy, 1)
+        modes = [cf.visco_mode(m, n, k, l, cx, cy, 1, kind == cf.VX) for k, l in [(1, 0), (3, 2), (m - 1, n - 1)]]
+    for z, az in modes:
+        np.testing.assert_allclose(g.apply(z), az, rtol=0, atol=1e-11 * max(1.0, np.abs(az).max()))
+    g.close()
+
+
+# ---------------------------------------------------------------- multigrid
+
+MG_CASES = [
+    (cf.MAC, 256, 128, 1 / 0.01 ** 2, 1 / 0.013 ** 2),
+    (cf.FEM, 257, 129, 0.01, 0.013),
+    (cf.VX, 256, 128, 0.13, 0.09),
+    (cf.VY, 256, 128, 0.13, 0.09),
+    (cf.MAC, 1024, 512, 1.0, 1.0),
+    (cf.FEM, 1025, 513, 1.0, 1.0),
+]
+
+
+@pytest.mark.parametrize("kind,m,n,cx,cy", MG_CASES)
+def test_gpu_vcycles_match_oracle(vdi, oracle, kind, m, n, cx, cy):
+    """The GPU runs the same V-cycle as the oracle: after each of a few
+    cycles the iterates agree to rounding."""
+    say("vcycles", kind, m, n)
+    rng = np.random.default_rng(kind * 100 + m)
+    b = rng.standard_normal(m * n)
+    if kind == cf.FEM:
+        b[:m] = rng.standard_normal(m)
+    g = vdi.MGSystem(kind, m, n, cx, cy, 1)
+    o = oracle.OracleMG(kind, m, n, cx, cy, 1)
+    assert g.levels == o.levels
+    g.set_b(b)
+    z0 = np.zeros(m * n)
+    g.set_z(z0)
+    zo = z0
+    for k in range(3):
+        g.vcycles(1)
+        zo = o.vcycles(b, zo, 1)
+        zg = g.get_z()
+        assert rel(zg, zo) < 1e-11, (k, rel(zg, zo))
+    g.close()
+
+
+@pytest.mark.parametrize("kind,m,n,cx,cy", MG_CASES)
+def test_gpu_solve_reaches_tolerance_and_oracle(vdi, oracle, kind, m, n, cx, cy):
+    say("solve", kind, m, n)
+    rng = np.random.default_rng(kind * 7 + n)
+    b = rng.standard_normal(m * n)
+    g = vdi.MGSystem(kind, m, n, cx, cy, 1)
+    g.set_b(b)
+    g.set_z(np.zeros(m * n))
+    cyc = g.solve(1e-10, 100)
+    r, bn = g.residual()
+    assert r <= 1e-10 * bn and cyc > 0
+    o = oracle.OracleMG(kind, m, n, cx, cy, 1)
+    zo, co = o.solve(b, tol=1e-10)
+    assert abs(cyc - co) <= 1
+    # tight solve on both sides: solutions agree
+    g.solve(1e-13, 100)
+    zt, _ = o.solve(b, zo, tol=1e-13)
+    assert rel(g.get_z(), zt) < PARITY
+    g.close()
+
+
+def test_gpu_mg_closed_form_solution(vdi):
+    say("closed-form solve MAC 512x256")
+    m, n, cx, cy = 512, 256, 1 / 0.002 ** 2, 1 / 0.0025 ** 2
+    z, b = cf.mac_mode(m, n, 5, 3, cx, cy)
+    g = vdi.MGSystem(cf.MAC, m, n, cx, cy)
+    g.set_b(b)
+    g.set_z(np.zeros_like(b))
+    g.solve(1e-13, 100)
+    assert rel(g.get_z(), z) < 1e-10
+
+
+def test_gpu_mg_nonconvergence_is_reported(vdi):
+    say("mg non-convergence status")
+    g = vdi.MGSystem(cf.MAC, 64, 32, 1.0, 1.0)
+    g.set_b(np.random.default_rng(0).standard_normal(64 * 32))
+    g.set_z(np.zeros(64 * 32))
+    with pytest.raises(vdi.VdiError) as e:
+        g.solve(1e-15, 1)
+    assert e.value.status == vdi.ERR_MG_NOCONV
+
+
+# ---------------------------------------------------------------- whole step
+
+ARRS = {"u": 0, "v": 1, "p": 2, "gp": 10, "gtau": 11, "h": 14, "ht": 16, "fbd": 17, "pg": 18}
+
+
+def make_pair(vdi, oracle, cfg, tol=1e-13):
+    p = cfg.params()
+    gpu = vdi.Fluid2D(p, rel_tol=tol)
+    orc = oracle.OracleSim(p, rel_tol=tol)
+    gpu.init_fields()
+    orc.init_fields()
+    return p, gpu, orc
+
+
+def compare(gpu, orc, where, bar=PARITY):
+    worst = 0.0
+    for name, aid in ARRS.items():
+        e = rel(gpu.get(aid), orc.get(aid))
+        worst = max(worst, e)
+        assert e < bar, f"{where}: {name} rel L2 {e:.3e}"
+    assert gpu.time == orc.time
+    return worst
+
+
+@pytest.mark.parametrize("m,n,steps,x_sym,implicit", [
+    (64, 32, 20, True, True),
+    (256, 128, 10, True, True),
+    (64, 32, 10, False, True),
+    (64, 32, 10, True, False),
+])
+def test_gpu_step_parity(vdi, oracle, m, n, steps, x_sym, implicit):
+    say("step parity", m, n, steps, x_sym, implicit)
+    cfg = PhysicalConfig(m=m, n=n, x_sym=x_sym, implicit_visc=implicit)
+    p, gpu, orc = make_pair(vdi, oracle, cfg)
+    compare(gpu, orc, "init", bar=1e-13)
+    worst = 0.0
+    for k in range(steps):
+        gpu.step_forward(p.dt_reg)
+        orc.step_forward(p.dt_reg)
+        worst = max(worst, compare(gpu, orc, f"step {k}"))
+    say("  worst rel L2", worst)
+    gpu.close()
+
+
+def test_gpu_step_parity_metric_grid(vdi, oracle):
+    say("step parity 1024x512")
+    cfg = PhysicalConfig(m=1024, n=512)
+    p, gpu, orc = make_pair(vdi, oracle, cfg, tol=1e-12)
+    for k in range(3):
+        gpu.step_forward(p.dt_reg)
+        orc.step_forward(p.dt_reg)
+        compare(gpu, orc, f"step {k}")
+    gpu.close()
+
+
+def test_gpu_step_at_bench_tolerance(vdi, oracle):
+    """At the benchmark's stopping rule (relative residual 1e-10 on both
+    sides) the fields still agree to the parity bar after 30 steps."""
+    say("step parity at 1e-10, 256x128")
+    cfg = PhysicalConfig(m=256, n=128)
+    p, gpu, orc = make_pair(vdi, oracle, cfg, tol=1e-10)
+    for k in range(30):
+        gpu.step_forward(p.dt_reg)
+        orc.step_forward(p.dt_reg)
+    compare(gpu, orc, "step 30")
+    gpu.close()
+
+
+def test_gpu_gas_update_matches_oracle(vdi, oracle):
+    say("gas update parity 512x256")
+    cfg = PhysicalConfig(m=512, n=256)
+    p, gpu, orc = make_pair(vdi, oracle, cfg)
+    for _ in range(4):
+        orc.step_forward(p.dt_reg)
+    for aid in [0, 1, 2, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19]:
+        gpu.set(aid, orc.get(aid))
+    gpu.gas_layer_update(orc.time, p.dt_reg)
+    orc.gas_update(orc.time, p.dt_reg)
+    for name, aid in [("h", 14), ("ht", 16), ("hx", 15), ("pg", 18), ("gp", 10), ("gtau", 11), ("fbd", 17)]:
+        assert rel(gpu.get(aid), orc.get(aid)) < 1e-11, name
+    assert gpu.stats().last_newton == orc.last_newton()
+    gpu.close()
+
+
+def test_gpu_restart_round_trip(vdi):
+    say("restart round trip")
+    p = PhysicalConfig(m=64, n=32).params()
+    a = vdi.Fluid2D(p)
+    a.init_fields()
+    for _ in range(3):
+        a.step_forward(p.dt_reg)
+    st = a.download_state()
+    b = vdi.Fluid2D(p)
+    b.upload_state(st)
+    np.testing.assert_array_equal(b.download_state(), st)
+    a.close()
+    b.close()
+
+
+def test_gpu_step_nonconvergence_status(vdi):
+    say("step non-convergence status")
+    p = PhysicalConfig(m=64, n=32).params()
+    g = vdi.Fluid2D(p, rel_tol=1e-16, max_cycles=1)
+    g.init_fields()
+    with pytest.raises(vdi.VdiError) as e:
+        g.step_forward(p.dt_reg)
+    assert e.value.status == vdi.ERR_MG_NOCONV
+    g.close()

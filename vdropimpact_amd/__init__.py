@@ -1,0 +1,304 @@
+"""MI355X (gfx950) per-timestep liquid solve of the vdropimpact drop-impact code.
+
+The compute path is the HIP library ``_lib/libvdi_hip.so`` behind the C ABI
+of include/vdi.h.  This module is the Python mirror of the reference's C++
+entry points for that path:
+
+  Fluid2D.step_forward(dt)          fluid_2d::step_forward (fluid_2d.cc:93)
+  Fluid2D.init_fields()             fluid_2d::init_fields (fluid_2d_sio.cc:14)
+  Fluid2D.gas_layer_update(t, dt)   gas_layer::update (gas_layer.cc:36)
+  Fluid2D.mg_solve_v_cycle(k, t)    mgs_mac/mgs_fem::solve_v_cycle (mgs_mac.hh:54, mgs_fem.hh:61)
+  Fluid2D.visco_setup_and_solve()   visco_impl::setup_and_solve (visco_impl.cc:21)
+  MGSystem                          the standalone systems of mg_test.cc
+
+There is no CPU fallback: if the library is missing, loading fails loudly.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .params import PhysicalConfig, VdiParams, drop_impact_params, h0_nd_for_hmin
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "_lib", "libvdi_hip.so")
+
+# status codes (include/vdi.h)
+OK, ERR_MG_NOCONV, ERR_NEWTON_NOCONV, ERR_TRIDIAG, ERR_HIP, ERR_ARG = range(6)
+# solver kinds
+MAC, FEM, VX, VY = 0, 1, 2, 3
+# arithmetic forms
+FORM_FP64, FORM_MIXED = 0, 1
+# array ids
+(U, V, P, UL, VL, UD, VD, C0, C1, SRC, GP, GTAU, GP_OLD, GTAU_OLD, H, HX, HT, FBD, PG, PG_OLD,
+ Z_MAC, Z_FEM, Z_VX, Z_VY) = range(24)
+
+# Exported C ABI: name -> (restype, argtypes).  Kept in step with include/vdi.h.
+_vp = ctypes.c_void_p
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_i, _d, _l = ctypes.c_int, ctypes.c_double, ctypes.c_long
+
+
+class VdiStats(ctypes.Structure):
+    _fields_ = [
+        ("last_cycles", ctypes.c_int * 4),
+        ("total_cycles", ctypes.c_long * 4),
+        ("solves", ctypes.c_long * 4),
+        ("last_newton", ctypes.c_int),
+        ("steps", ctypes.c_long),
+        ("time", ctypes.c_double),
+    ]
+
+
+ABI = {
+    "vdi_status_string": (ctypes.c_char_p, [_i]),
+    "vdi_last_error": (ctypes.c_char_p, []),
+    "vdi_create": (_i, [ctypes.POINTER(VdiParams), ctypes.POINTER(_vp)]),
+    "vdi_destroy": (_i, [_vp]),
+    "vdi_set_solver": (_i, [_vp, _i, _i, _d, _i]),
+    "vdi_set_form": (_i, [_vp, _i]),
+    "vdi_init_fields": (_i, [_vp]),
+    "vdi_step_forward": (_i, [_vp, _d]),
+    "vdi_gas_layer_update": (_i, [_vp, _d, _d]),
+    "vdi_mg_solve_v_cycle": (_i, [_vp, _i, _d, _ip]),
+    "vdi_visco_setup_and_solve": (_i, [_vp, _i, _d, _d, _d, _ip]),
+    "vdi_array_len": (_l, [_vp, _i]),
+    "vdi_get_array": (_i, [_vp, _i, _dp, _l]),
+    "vdi_set_array": (_i, [_vp, _i, _dp, _l]),
+    "vdi_get_time": (_i, [_vp, _dp]),
+    "vdi_set_time": (_i, [_vp, _d]),
+    "vdi_get_stats": (_i, [_vp, ctypes.POINTER(VdiStats)]),
+    "vdi_reset_stats": (_i, [_vp]),
+    "vdi_synchronize": (_i, [_vp]),
+    "vdi_state_len": (_l, [_vp]),
+    "vdi_download_state": (_i, [_vp, _dp, _l]),
+    "vdi_upload_state": (_i, [_vp, _dp, _l]),
+    "vdi_mg_create": (_i, [_i, _i, _i, _d, _d, _i, ctypes.POINTER(_vp)]),
+    "vdi_mg_destroy": (_i, [_vp]),
+    "vdi_mg_set_schedule": (_i, [_vp, _i, _i]),
+    "vdi_mg_set_form": (_i, [_vp, _i]),
+    "vdi_mg_levels": (_i, [_vp]),
+    "vdi_mg_set_b": (_i, [_vp, _dp, _l]),
+    "vdi_mg_set_z": (_i, [_vp, _dp, _l]),
+    "vdi_mg_get_z": (_i, [_vp, _dp, _l]),
+    "vdi_mg_apply": (_i, [_vp, _dp, _dp, _l]),
+    "vdi_mg_solve": (_i, [_vp, _d, _i, _ip]),
+    "vdi_mg_vcycles": (_i, [_vp, _i]),
+    "vdi_mg_residual": (_i, [_vp, _dp, _dp]),
+    "vdi_mg_synchronize": (_i, [_vp]),
+}
+
+_lib = None
+
+
+class VdiError(RuntimeError):
+    def __init__(self, status: int, where: str):
+        lib = _lib
+        msg = lib.vdi_status_string(status).decode() if lib else str(status)
+        detail = lib.vdi_last_error().decode() if (lib and status == ERR_HIP) else ""
+        super().__init__(f"{where}: {msg}" + (f" [{detail}]" if detail else ""))
+        self.status = status
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Loads libvdi_hip.so and binds the C ABI.  Raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"libvdi_hip.so not built ({path}); run python -m vdropimpact_amd.build")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in ABI.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(status: int, where: str):
+    if status != OK:
+        raise VdiError(status, where)
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+class Fluid2D:
+    """GPU-resident drop-impact liquid solve (the reference's fluid_2d)."""
+
+    def __init__(self, params: VdiParams, nu1: int = 2, nu2: int = 2, rel_tol: float = 1e-10,
+                 max_cycles: int = 200):
+        self.lib = load_library()
+        self.params = params
+        self.m, self.n = params.m, params.n
+        h = _vp()
+        _check(self.lib.vdi_create(ctypes.byref(params), ctypes.byref(h)), "vdi_create")
+        self.h = h
+        self.set_solver(nu1, nu2, rel_tol, max_cycles)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.vdi_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_solver(self, nu1=2, nu2=2, rel_tol=1e-10, max_cycles=200):
+        _check(self.lib.vdi_set_solver(self.h, nu1, nu2, rel_tol, max_cycles), "vdi_set_solver")
+
+    def set_form(self, form: int):
+        _check(self.lib.vdi_set_form(self.h, form), "vdi_set_form")
+
+    def init_fields(self):
+        _check(self.lib.vdi_init_fields(self.h), "init_fields")
+
+    def step_forward(self, dt: float):
+        _check(self.lib.vdi_step_forward(self.h, dt), "step_forward")
+
+    def gas_layer_update(self, t: float, dt: float):
+        _check(self.lib.vdi_gas_layer_update(self.h, t, dt), "gas_layer::update")
+
+    def mg_solve_v_cycle(self, which: int, t: float) -> int:
+        c = ctypes.c_int(0)
+        _check(self.lib.vdi_mg_solve_v_cycle(self.h, which, t, ctypes.byref(c)), "solve_v_cycle")
+        return c.value
+
+    def visco_setup_and_solve(self, which: int, vfx: float, vfy: float, t: float) -> int:
+        c = ctypes.c_int(0)
+        _check(self.lib.vdi_visco_setup_and_solve(self.h, which, vfx, vfy, t, ctypes.byref(c)),
+               "visco_impl::setup_and_solve")
+        return c.value
+
+    def synchronize(self):
+        _check(self.lib.vdi_synchronize(self.h), "synchronize")
+
+    @property
+    def time(self) -> float:
+        t = ctypes.c_double()
+        _check(self.lib.vdi_get_time(self.h, ctypes.byref(t)), "get_time")
+        return t.value
+
+    @time.setter
+    def time(self, t: float):
+        _check(self.lib.vdi_set_time(self.h, t), "set_time")
+
+    def get(self, aid: int) -> np.ndarray:
+        n = self.lib.vdi_array_len(self.h, aid)
+        if n <= 0:
+            raise ValueError(f"array {aid} not available")
+        out = np.empty(n, np.float64)
+        _check(self.lib.vdi_get_array(self.h, aid, _ptr(out), n), "get_array")
+        return out
+
+    def set(self, aid: int, arr):
+        a = np.ascontiguousarray(arr, np.float64).ravel()
+        _check(self.lib.vdi_set_array(self.h, aid, _ptr(a), a.size), "set_array")
+
+    def padded(self, aid: int) -> np.ndarray:
+        return self.get(aid).reshape(self.n + 4, self.m + 4)
+
+    def stats(self) -> VdiStats:
+        s = VdiStats()
+        _check(self.lib.vdi_get_stats(self.h, ctypes.byref(s)), "get_stats")
+        return s
+
+    def reset_stats(self):
+        _check(self.lib.vdi_reset_stats(self.h), "reset_stats")
+
+    def download_state(self) -> np.ndarray:
+        n = self.lib.vdi_state_len(self.h)
+        out = np.empty(n, np.float64)
+        _check(self.lib.vdi_download_state(self.h, _ptr(out), n), "download_state")
+        return out
+
+    def upload_state(self, state):
+        a = np.ascontiguousarray(state, np.float64).ravel()
+        _check(self.lib.vdi_upload_state(self.h, _ptr(a), a.size), "upload_state")
+
+
+class MGSystem:
+    """A standalone linear system (mgs_mac / mgs_fem / visco_impl test
+    constructors, mg_test.cc) solved on the GPU."""
+
+    def __init__(self, kind: int, m: int, n: int, cx: float, cy: float, x_sym: int = 1,
+                 nu1: int = 2, nu2: int = 2):
+        self.lib = load_library()
+        self.kind, self.m, self.n = kind, m, n
+        h = _vp()
+        _check(self.lib.vdi_mg_create(kind, m, n, cx, cy, x_sym, ctypes.byref(h)), "vdi_mg_create")
+        self.h = h
+        _check(self.lib.vdi_mg_set_schedule(self.h, nu1, nu2), "vdi_mg_set_schedule")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.vdi_mg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def levels(self) -> int:
+        return self.lib.vdi_mg_levels(self.h)
+
+    def set_form(self, form: int):
+        _check(self.lib.vdi_mg_set_form(self.h, form), "vdi_mg_set_form")
+
+    def set_b(self, b):
+        a = np.ascontiguousarray(b, np.float64).ravel()
+        _check(self.lib.vdi_mg_set_b(self.h, _ptr(a), a.size), "vdi_mg_set_b")
+
+    def set_z(self, z):
+        a = np.ascontiguousarray(z, np.float64).ravel()
+        _check(self.lib.vdi_mg_set_z(self.h, _ptr(a), a.size), "vdi_mg_set_z")
+
+    def get_z(self) -> np.ndarray:
+        out = np.empty(self.m * self.n, np.float64)
+        _check(self.lib.vdi_mg_get_z(self.h, _ptr(out), out.size), "vdi_mg_get_z")
+        return out
+
+    def apply(self, z) -> np.ndarray:
+        a = np.ascontiguousarray(z, np.float64).ravel()
+        out = np.empty_like(a)
+        _check(self.lib.vdi_mg_apply(self.h, _ptr(a), _ptr(out), a.size), "vdi_mg_apply")
+        return out
+
+    def solve(self, rel_tol: float = 1e-10, max_cycles: int = 200) -> int:
+        c = ctypes.c_int(0)
+        _check(self.lib.vdi_mg_solve(self.h, rel_tol, max_cycles, ctypes.byref(c)), "vdi_mg_solve")
+        return c.value
+
+    def vcycles(self, k: int):
+        _check(self.lib.vdi_mg_vcycles(self.h, k), "vdi_mg_vcycles")
+
+    def residual(self):
+        r, b = ctypes.c_double(), ctypes.c_double()
+        _check(self.lib.vdi_mg_residual(self.h, ctypes.byref(r), ctypes.byref(b)), "vdi_mg_residual")
+        return r.value, b.value
+
+    def synchronize(self):
+        _check(self.lib.vdi_mg_synchronize(self.h), "vdi_mg_synchronize")
+
+
+__all__ = [
+    "Fluid2D", "MGSystem", "VdiParams", "VdiStats", "VdiError", "PhysicalConfig",
+    "drop_impact_params", "h0_nd_for_hmin", "load_library", "LIB_PATH", "ABI",
+]

@@ -1,0 +1,1139 @@
+// step.hip -- one time step of the drop-impact liquid solve on gfx950, and
+// the C ABI (include/vdi.h).
+//
+// Data layout in HBM: every cell field is its own FP64 array (structure of
+// arrays) with the reference's ghost padding, (m+4) x (n+4), cell (i,j) at
+// (i+2) + (m+4)(j+2) -- the reference keeps the same fields in a 152-byte
+// record per cell (fields.hh:7-19).  Node arrays (the FEM pressure) are
+// (m+1) x (n+1).  The gas layer is a handful of 1D arrays.
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <algorithm>
+
+#include "mg.hpp"
+
+namespace vdi {
+
+static std::string g_last_error;
+
+void set_error(const char *what, hipError_t e, const char *file, int line) {
+    char buf[512];
+    snprintf(buf, sizeof buf, "%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
+    g_last_error = buf;
+}
+
+// Grid constants (fluid_2d.hh:59-80).
+struct Geo {
+    int m, n, ml, me, ne, x_sym;
+    double dx, dy, xsp, ysp, xxsp, yysp;
+    double ax, bx, ay, by;
+    double rho, rhoinv, mu, muinv;
+};
+
+// Device pointers of the cell fields (fields.hh:7-19, one array each).
+struct Fields {
+    double *u, *v, *p;
+    double *ux, *uy, *vx, *vy;
+    double *ud, *vd, *ul, *vl, *ur, *vr, *uu, *vu;
+    double *c0, *c1, *c2, *c3;
+    double *src;
+};
+
+// Gas-layer arrays (gas_layer.hh:26-34, gas_layer_solve.hh:163-173).
+struct Gas {
+    double *p, *tau, *p_old, *tau_old, *h, *hx, *ht, *fbd, *pg, *pg_old;
+    int *status;  // [0] = status code, [1] = Newton steps
+};
+
+struct GasConst {
+    double alpha, mug, Pamb, h0, R, V, sigma, gxsp, gxxsp;
+};
+
+#define IX(i, j) (((size_t)(j) + 2) * (size_t)g.ml + (size_t)((i) + 2))
+
+// ------------------------------------------------------------------------
+// Limiter and Godunov selection (fluid_2d.cc:419-476)
+// ------------------------------------------------------------------------
+
+__device__ __forceinline__ double delta_lim(double f0, double f1, double f2) {
+    double drm = f1 - f0, drp = f2 - f1;
+    return drm > 0 ? (drp > 0 ? 2 * fmin(drm, drp) : 0.0) : (drp > 0 ? 0.0 : -2 * fmax(drm, drp));
+}
+__device__ __forceinline__ double delta_f(double f0, double f1, double f2) {
+    double drc = 0.5 * (f2 - f0);
+    return drc > 0 ? fmin(drc, delta_lim(f0, f1, f2)) : fmax(drc, -delta_lim(f0, f1, f2));
+}
+__device__ __forceinline__ double mono_diff(double f0, double f1, double f2, double f3, double f4) {
+    double tdrc = f3 - f1, s = (1 / 6.) * (4 * tdrc - delta_f(f0, f1, f2) - delta_f(f2, f3, f4)),
+           t = delta_lim(f1, f2, f3);
+    return tdrc > 0 ? fmin(s, t) : fmax(s, -t);
+}
+// fluid_2d.cc:455-461
+__device__ __forceinline__ void godunov_set(double &u0, double &u1, double &v0, double &v1) {
+    if (u0 > -u1) {
+        if (u0 > 0) { u1 = u0; v1 = v0; return; }
+    } else if (u1 < 0) return;
+    u1 = 0;
+    v1 = 0.5 * (v0 + v1);
+}
+// fluid_2d.cc:469-476
+__device__ __forceinline__ void godunov_set_tang(double &u0, double &u1, double &v0, double &v1) {
+    if (u0 > -u1) {
+        if (u0 > 0) { u1 = u0; v1 = v0; return; }
+    } else if (u1 < 0) { u0 = u1; return; }
+    v1 = 0.5 * (v0 + v1);
+    u0 = 0.5 * (u0 + u1);
+    u1 = 0;
+}
+
+// ------------------------------------------------------------------------
+// Godunov passes K1..K8 (fluid_2d.cc:105-261)
+// ------------------------------------------------------------------------
+
+#define CELL_IDX                                              \
+    int i = blockIdx.x * blockDim.x + threadIdx.x;            \
+    int j = blockIdx.y * blockDim.y + threadIdx.y;            \
+    if (i >= g.m || j >= g.n) return;
+
+// K1 (fluid_2d.cc:105-132) + zero_edge_velocities (:480-482).
+__global__ void k_slopes(Geo g, Fields F, double dt) {
+    CELL_IDX
+    const double *u = F.u, *v = F.v;
+    double fu = u[IX(i, j)], fv = v[IX(i, j)];
+    double uy = g.ysp * mono_diff(u[IX(i, j - 2)], u[IX(i, j - 1)], fu, u[IX(i, j + 1)], u[IX(i, j + 2)]);
+    double vy = g.ysp * mono_diff(v[IX(i, j - 2)], v[IX(i, j - 1)], fv, v[IX(i, j + 1)], v[IX(i, j + 2)]);
+    double ux = g.xsp * mono_diff(u[IX(i - 2, j)], u[IX(i - 1, j)], fu, u[IX(i + 1, j)], u[IX(i + 2, j)]);
+    double vx = g.xsp * mono_diff(v[IX(i - 2, j)], v[IX(i - 1, j)], fv, v[IX(i + 1, j)], v[IX(i + 2, j)]);
+    size_t c = IX(i, j);
+    F.ux[c] = ux; F.uy[c] = uy; F.vx[c] = vx; F.vy[c] = vy;
+    F.ud[c] = fu - 0.5 * (g.dy + dt * fv) * uy;
+    F.vd[c] = fv - 0.5 * (g.dy + dt * fv) * vy;
+    F.ul[c] = (g.x_sym && i == 0) ? 0.0 : fu - 0.5 * (g.dx + dt * fu) * ux;
+    F.vl[c] = fv - 0.5 * (g.dx + dt * fu) * vx;
+    F.ur[c] = fu + 0.5 * (g.dx - dt * fu) * ux;
+    F.vr[c] = fv + 0.5 * (g.dx - dt * fu) * vx;
+    F.uu[c] = fu + 0.5 * (g.dy - dt * fv) * uy;
+    F.vu[c] = fv + 0.5 * (g.dy - dt * fv) * vy;
+}
+
+// K2 (fluid_2d.cc:136-150): tangential Godunov selection on each cell's
+// right and top faces; each face is owned by its left/lower cell.
+__global__ void k_godunov_tang(Geo g, Fields F) {
+    CELL_IDX
+    size_t c = IX(i, j);
+    if (i < g.m - 1) {
+        size_t r = IX(i + 1, j);
+        godunov_set_tang(F.ur[c], F.ul[r], F.vr[c], F.vl[r]);
+        if (i == 0) F.ur[IX(-1, j)] = F.ul[c];
+    } else {
+        F.ul[IX(g.m, j)] = F.ur[c];
+        F.vl[IX(g.m, j)] = F.vr[c];
+    }
+    if (j < g.n - 1) {
+        size_t t = IX(i, j + 1);
+        godunov_set_tang(F.vu[c], F.vd[t], F.uu[c], F.ud[t]);
+        if (j == 0) F.vu[IX(i, -1)] = F.vd[c];
+    } else {
+        F.vd[IX(i, g.n)] = F.vu[c];
+        F.ud[IX(i, g.n)] = F.uu[c];
+    }
+}
+
+// K3 (fluid_2d.cc:153-160)
+__global__ void k_tang_terms(Geo g, Fields F) {
+    CELL_IDX
+    size_t c = IX(i, j), r = IX(i + 1, j), l = IX(i - 1, j), t = IX(i, j + 1), d = IX(i, j - 1);
+    double sx = 0.5 * g.xsp * (F.ul[r] + F.ul[c]), sy = 0.5 * g.ysp * (F.vd[t] + F.vd[c]);
+    F.c0[c] = sx * (F.ur[c] - F.ur[l]);
+    F.c1[c] = sx * (F.vl[r] - F.vl[c]);
+    F.c2[c] = sy * (F.ud[t] - F.ud[c]);
+    F.c3[c] = sy * (F.vu[c] - F.vu[d]);
+}
+
+// K4 (fluid_2d.cc:164-201, centered_diff :412-417) + zero_edge_velocities.
+__global__ void k_predictor(Geo g, Fields F, double dt, double viscfac) {
+    CELL_IDX
+    const double *u = F.u, *v = F.v, *p = F.p;
+    size_t c = IX(i, j);
+    double px = 0.5 * g.xsp * (p[IX(i + 1, j + 1)] + p[IX(i + 1, j)] - p[IX(i, j + 1)] - p[c]);
+    double py = 0.5 * g.ysp * (p[IX(i + 1, j + 1)] - p[IX(i + 1, j)] + p[IX(i, j + 1)] - p[c]);
+    double fu = u[c], fv = v[c];
+    double uyy = g.yysp * (u[IX(i, j - 1)] - 2 * fu + u[IX(i, j + 1)]);
+    double vyy = g.yysp * (v[IX(i, j - 1)] - 2 * fv + v[IX(i, j + 1)]);
+    double uxx = g.xxsp * (u[IX(i - 1, j)] - 2 * fu + u[IX(i + 1, j)]);
+    double vxx = g.xxsp * (v[IX(i - 1, j)] - 2 * fv + v[IX(i + 1, j)]);
+    double f_x = -px * g.rhoinv + viscfac * (uxx + uyy);
+    double f_y = -py * g.rhoinv + viscfac * (vxx + vyy);
+    double fux = F.ux[c], fuy = F.uy[c], fvx = F.vx[c], fvy = F.vy[c];
+    double fc0 = F.c0[c], fc1 = F.c1[c], fc2 = F.c2[c], fc3 = F.c3[c];
+    double dt_, dx = g.dx, dy = g.dy;
+    dt_ = dt;
+    F.ud[c] = fu + 0.5 * (-dy * fuy + dt_ * (-fc0 - fv * fuy + f_x));
+    F.vd[c] = fv + 0.5 * (-dy * fvy + dt_ * (-fc1 - fv * fvy + f_y));
+    F.ul[c] = (g.x_sym && i == 0) ? 0.0 : fu + 0.5 * (-dx * fux + dt_ * (-fu * fux - fc2 + f_x));
+    F.vl[c] = fv + 0.5 * (-dx * fvx + dt_ * (-fu * fvx - fc3 + f_y));
+    F.ur[c] = fu + 0.5 * (dx * fux + dt_ * (-fu * fux - fc2 + f_x));
+    F.vr[c] = fv + 0.5 * (dx * fvx + dt_ * (-fu * fvx - fc3 + f_y));
+    F.uu[c] = fu + 0.5 * (dy * fuy + dt_ * (-fc0 - fv * fuy + f_x));
+    F.vu[c] = fv + 0.5 * (dy * fvy + dt_ * (-fc1 - fv * fvy + f_y));
+    F.c0[c] = viscfac * (uxx + uyy);
+    F.c1[c] = viscfac * (vxx + vyy);
+}
+
+// K5 (fluid_2d.cc:205-215)
+__global__ void k_godunov_normal(Geo g, Fields F) {
+    CELL_IDX
+    size_t c = IX(i, j);
+    if (i < g.m - 1) {
+        size_t r = IX(i + 1, j);
+        godunov_set(F.ur[c], F.ul[r], F.vr[c], F.vl[r]);
+    } else {
+        F.ul[IX(g.m, j)] = F.ur[c];
+        F.vl[IX(g.m, j)] = F.vr[c];
+    }
+    if (j < g.n - 1) {
+        size_t t = IX(i, j + 1);
+        godunov_set(F.vu[c], F.vd[t], F.uu[c], F.ud[t]);
+    } else {
+        F.vd[IX(i, g.n)] = F.vu[c];
+        F.ud[IX(i, g.n)] = F.uu[c];
+    }
+}
+
+// K6 (fluid_2d.cc:218-232): face divergence, plus the bottom Dirichlet row
+// s_bc = dt/(8 rho) (p_old[i]+p_old[i+1]+p[i]+p[i+1]), src -= 2 yysp s_bc.
+__global__ void k_mac_rhs(Geo g, Fields F, Gas G, double macfac) {
+    CELL_IDX
+    double s = g.xsp * (F.ul[IX(i + 1, j)] - F.ul[IX(i, j)]) + g.ysp * (F.vd[IX(i, j + 1)] - F.vd[IX(i, j)]);
+    if (j == 0) {
+        double sbc = macfac * (G.p_old[i] + G.p_old[i + 1] + G.p[i] + G.p[i + 1]);
+        F.src[(size_t)g.m * g.n + i] = sbc;
+        s -= 2 * g.yysp * sbc;
+    }
+    F.src[i + (size_t)g.m * j] = s;
+}
+
+// K7 (fluid_2d.cc:236-248)
+__global__ void k_mac_correct(Geo g, Fields F, const double *z) {
+    CELL_IDX
+    size_t q = i + (size_t)g.m * j;
+    if (i > 0) F.ul[IX(i, j)] -= (z[q] - z[q - 1]) * g.xsp;
+    if (j != 0) F.vd[IX(i, j)] -= (z[q] - z[q - g.m]) * g.ysp;
+    else F.vd[IX(i, 0)] -= 2 * g.ysp * (z[q] - F.src[(size_t)g.m * g.n + i]);
+}
+
+// K8 (fluid_2d.cc:253-261)
+__global__ void k_advect(Geo g, Fields F, double dt) {
+    CELL_IDX
+    size_t c = IX(i, j), r = IX(i + 1, j), t = IX(i, j + 1);
+    double sx = g.xsp * (F.ul[r] + F.ul[c]), sy = g.ysp * (F.vd[t] + F.vd[c]);
+    F.c0[c] = F.u[c] + dt * (-0.5 * (sx * (F.ul[r] - F.ul[c]) + sy * (F.ud[t] - F.ud[c])) + F.c0[c]);
+    F.c1[c] = F.v[c] + dt * (-0.5 * (sx * (F.vl[r] - F.vl[c]) + sy * (F.vd[t] - F.vd[c])) + F.c1[c]);
+}
+
+// ------------------------------------------------------------------------
+// Boundary conditions (fluid_2d_sio.cc:256-405)
+// ------------------------------------------------------------------------
+
+__device__ __forceinline__ void simpson_term(int x_sym, double xms, double xps, double ysq, double p,
+                                             double &fsu, double &fsv) {
+    double rm = xms * xms + ysq;
+    rm = 1 / (rm * rm);
+    if (x_sym) {
+        double rp = xps * xps + ysq;
+        rp = 1 / (rp * rp);
+        fsu = p * (xms * rm + xps * rp);
+        fsv = -p * ((xms * xms - ysq) * rm + (xps * xps - ysq) * rp);
+    } else {
+        fsu = p * xms * rm;
+        fsv = -p * (xms * xms - ysq) * rm;
+    }
+}
+
+// set_inviscid_boundaries: one wavefront per ghost line.  Waves [0,n) do the
+// two right ghost cells of row j; waves [n, n+ml) the two top ghost cells of
+// column c = w-n-2.  Composite Simpson over the gas pressure nodes, then the
+// forward-Euler accumulation (fluid_2d_sio.cc:383-384, quirk Q4).
+__global__ void __launch_bounds__(256) k_simpson(Geo g, Fields F, const double *gp, double prefac) {
+    int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+    if (w >= g.n + g.ml) return;
+    double x1, x2, y1, y2;
+    bool right = w < g.n;
+    int j = w, c = w - g.n - 2;
+    if (right) {
+        y1 = y2 = g.ay + (j + 0.5) * g.dy;
+        x1 = g.bx + g.dx * 0.5;
+        x2 = g.bx + g.dx * 1.5;
+    } else {
+        x1 = x2 = g.ax + (c + 0.5) * g.dx;
+        y1 = g.by + g.dy * 0.5;
+        y2 = g.by + g.dy * 1.5;
+    }
+    double su1 = 0, sv1 = 0, su2 = 0, sv2 = 0, ysq1 = y1 * y1, ysq2 = y2 * y2;
+    for (int k = lane; k < g.me; k += 64) {
+        double wgt = (k == 0 || k == g.me - 1) ? 1.0 : ((k & 1) ? 4.0 : 2.0);
+        double s = k * g.dx, pk = gp[k], fsu, fsv;
+        simpson_term(g.x_sym, x1 - g.ax - s, x1 + g.ax + s, ysq1, pk, fsu, fsv);
+        su1 += wgt * fsu; sv1 += wgt * fsv;
+        simpson_term(g.x_sym, x2 - g.ax - s, x2 + g.ax + s, ysq2, pk, fsu, fsv);
+        su2 += wgt * fsu; sv2 += wgt * fsv;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        su1 += __shfl_xor(su1, o); sv1 += __shfl_xor(sv1, o);
+        su2 += __shfl_xor(su2, o); sv2 += __shfl_xor(sv2, o);
+    }
+    if (lane) return;
+    if (right) {
+        size_t a = IX(g.m, j), b = IX(g.m + 1, j);
+        F.u[a] += 2 * y1 * prefac * su1; F.v[a] += prefac * sv1;
+        F.u[b] += 2 * y2 * prefac * su2; F.v[b] += prefac * sv2;
+        if (!g.x_sym) {
+            F.u[IX(-1, j)] = -F.u[a]; F.v[IX(-1, j)] = F.v[a];
+            F.u[IX(-2, j)] = -F.u[b]; F.v[IX(-2, j)] = F.v[b];
+        }
+    } else {
+        size_t a = IX(c, g.n), b = IX(c, g.n + 1);
+        F.u[a] += 2 * y1 * prefac * su1; F.v[a] += prefac * sv1;
+        F.u[b] += 2 * y2 * prefac * su2; F.v[b] += prefac * sv2;
+    }
+}
+
+// set_viscous_boundaries (fluid_2d_sio.cc:294-343), phases in the
+// reference's order (quirk Q5) with workgroup barriers between them.  The
+// reference's ghost-row ux scratch stores are dead and are not made.
+__global__ void __launch_bounds__(1024) k_viscous_bc(Geo g, Fields F, const double *tau) {
+    double *u = F.u, *v = F.v;
+    const int T = blockDim.x;
+    if (g.x_sym)
+        for (int j = threadIdx.x; j < g.n; j += T) {
+            u[IX(-1, j)] = -u[IX(0, j)]; v[IX(-1, j)] = v[IX(0, j)];
+            u[IX(-2, j)] = -u[IX(1, j)]; v[IX(-2, j)] = v[IX(1, j)];
+        }
+    __syncthreads();
+    for (int k = threadIdx.x; k < g.ml; k += T) {
+        int c = k - 2;
+        double un = g.muinv * tau[k];
+        u[IX(c, -1)] = u[IX(c, 0)] - g.dy * un;
+        u[IX(c, -2)] = u[IX(c, 1)] - 3 * g.dy * un;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < g.m + 1; k += T) {
+        int c = k - 1;
+        double ux1 = 0.5 * g.xsp * (u[IX(c + 1, -1)] - u[IX(c - 1, -1)]);
+        double ux2 = 0.5 * g.xsp * (u[IX(c + 1, -2)] - u[IX(c - 1, -2)]);
+        double v1 = v[IX(c, 0)] + g.dy * ux1;
+        v[IX(c, -1)] = v1;
+        v[IX(c, -2)] = v1 + g.dy * ux2;
+    }
+}
+
+// ------------------------------------------------------------------------
+// Implicit viscosity RHS (fluid_2d.cc:272-334) and FEM projection
+// ------------------------------------------------------------------------
+
+__global__ void k_visco_rhs(Geo g, Fields F, Gas G, double vfx, double vfy, int comp) {
+    CELL_IDX
+    const double *w = comp ? F.v : F.u;
+    double r = comp ? F.c1[IX(i, j)] : F.c0[IX(i, j)];
+    if (!g.x_sym && i == 0) r += vfx * w[IX(-1, j)];
+    if (i == g.m - 1) r += vfx * w[IX(g.m, j)];
+    if (j == g.n - 1) r += vfy * w[IX(i, g.n)];
+    if (j == 0) {
+        if (!comp) {
+            double dudy = g.muinv * 0.5 * (G.tau_old[i + 2] + G.tau[i + 2]);
+            r -= vfy * g.dy * dudy;
+        } else {
+            double dvdy = -0.5 * g.xsp * (F.u[IX(i + 1, 0)] - F.u[IX(i - 1, 0)]);
+            r -= vfy * g.dy * dvdy;
+        }
+    }
+    F.src[i + (size_t)g.m * j] = r;
+}
+
+__global__ void k_visco_copy(Geo g, Fields F, const double *zx, const double *zy) {
+    CELL_IDX
+    size_t q = i + (size_t)g.m * j;
+    F.c0[IX(i, j)] = zx[q];
+    F.c1[IX(i, j)] = zy[q];
+}
+
+// K14 (fluid_2d.cc:338-377): nodal weak divergence, boundary fluxes, and the
+// Dirichlet bottom row src = a_cc p_gas.
+__global__ void k_fem_rhs(Geo g, Fields F, const double *gp, double rdt, double fm) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= g.me || j >= g.ne) return;
+    const int m = g.m, n = g.n;
+    if (j == 0) {
+        F.src[i] = (i > 0 && i < g.me - 1 ? 1.0 : 0.5) * 0.5 * fm * gp[i];
+        return;
+    }
+    double spx = 0, spy = 0;
+    if (i > 0) { spx += F.c0[IX(i - 1, j - 1)]; spy += F.c1[IX(i - 1, j - 1)]; }
+    if (i > 0 && j < n) { spx += F.c0[IX(i - 1, j)]; spy -= F.c1[IX(i - 1, j)]; }
+    if (i < m) { spx -= F.c0[IX(i, j - 1)]; spy += F.c1[IX(i, j - 1)]; }
+    if (i < m && j < n) { spx -= F.c0[IX(i, j)]; spy -= F.c1[IX(i, j)]; }
+    double s = 0.5 * (g.dy * spx + g.dx * spy) * rdt;
+    if (!g.x_sym && i == 0) {
+        if (j < n) s += 0.25 * g.dy * rdt * (F.c0[IX(0, j)] + F.u[IX(-1, j)]);
+        s += 0.25 * g.dy * rdt * (F.c0[IX(0, j - 1)] + F.u[IX(-1, j - 1)]);
+    }
+    if (i == m) {
+        if (j < n) s -= 0.25 * g.dy * rdt * (F.c0[IX(m - 1, j)] + F.u[IX(m, j)]);
+        s -= 0.25 * g.dy * rdt * (F.c0[IX(m - 1, j - 1)] + F.u[IX(m, j - 1)]);
+    }
+    if (j == n) {
+        if (i > 0) s -= 0.25 * g.dx * rdt * (F.c1[IX(i - 1, n - 1)] + F.v[IX(i - 1, n)]);
+        if (i < m) s -= 0.25 * g.dx * rdt * (F.c1[IX(i, n - 1)] + F.v[IX(i, n)]);
+    }
+    F.src[i + (size_t)g.me * j] = s;
+}
+
+// K15 (fluid_2d.cc:383-396)
+__global__ void k_update(Geo g, Fields F, const double *zf, double dt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= g.me || j >= g.ne) return;
+    const int me = g.me;
+    const double *q = zf + i + (size_t)me * j;
+    if (i < g.m && j < g.n) {
+        size_t c = IX(i, j);
+        F.u[c] = F.c0[c] - 0.5 * dt * g.rhoinv * g.xsp * (q[me + 1] + q[1] - q[me] - q[0]);
+        F.v[c] = F.c1[c] - 0.5 * dt * g.rhoinv * g.ysp * (q[me + 1] - q[1] + q[me] - q[0]);
+    }
+    F.p[IX(i, j)] = q[0];
+}
+
+// ------------------------------------------------------------------------
+// Gas layer: one workgroup (gas_layer.cc:36-46, gas_layer_solve.cc:133-249)
+// ------------------------------------------------------------------------
+
+__device__ __forceinline__ double eno2(double p0, double p1, double p2, double p3) {
+    return fabs(p0 - 2 * p1 + p2) > fabs(p1 - 2 * p2 + p3) ? 3 * p1 - 4 * p2 + p3 : p0 - p2;
+}
+
+// gas_layer_solve.cc:133-162, run by one lane.  The reference updates h in
+// place while i sweeps upward, so the stencil at i sees new h[i-1], h[i-2]
+// and old h[i..i+2]; the lane keeps exactly those five values in registers.
+__device__ void gas_height_serial(int m, int x_sym, double dt, double Vframe, const double *__restrict__ HO,
+                                  const double *__restrict__ UC, const double *__restrict__ VC,
+                                  double *__restrict__ h, double *__restrict__ ht, double *__restrict__ fbd) {
+    double hn1 = 0, hn2 = 0;
+    double o0 = HO[0], o1 = HO[1], o2 = HO[2];
+    for (int i = 0; i < m; i++) {
+        double o3 = (i + 3 < m) ? HO[i + 3] : 0.0;
+        double uc = UC[i], vc = VC[i];
+        // current value of h[k] for k in [i-2, i+2]
+        auto cur = [&](int k) -> double {
+            return k == i - 1 ? hn1 : (k == i - 2 ? hn2 : (k == i ? o0 : (k == i + 1 ? o1 : o2)));
+        };
+        // h_ref (gas_layer_solve.cc:176-180)
+        auto href = [&](int k) -> double {
+            if (k >= 0 && k < m) return cur(k);
+            if (k >= m) return k == m ? 2 * cur(m - 1) - cur(m - 2) : 3 * cur(m - 1) - 2 * cur(m - 2);
+            return x_sym ? (k == -1 ? cur(0) : cur(1)) : (k == -1 ? 2 * cur(0) - cur(1) : 3 * cur(0) - 2 * cur(1));
+        };
+        double dhdx = uc > 0 ? eno2(href(i + 1), o0, href(i - 1), href(i - 2))
+                             : -eno2(href(i - 1), o0, href(i + 1), href(i + 2));
+        double htv = -uc * dhdx + vc - Vframe;
+        double hnew = o0 + dt * htv;
+        ht[i] = htv;
+        h[i] = hnew;
+        fbd[i] += dt * vc;
+        hn2 = hn1; hn1 = hnew;
+        o0 = o1; o1 = o2; o2 = o3;
+    }
+}
+
+// Deterministic workgroup sum; result broadcast to every thread.
+__device__ double wg_sum(double x, double *red) {
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    int w = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = (blockDim.x + 63) >> 6;
+    __syncthreads();
+    if (lane == 0) red[w] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0;
+        for (int k = 0; k < nw; k++) s += red[k];
+        red[32] = s;
+    }
+    __syncthreads();
+    return red[32];
+}
+
+constexpr int kGasThreads = 1024;
+constexpr int kGasRows = 5;  // rows per thread: me <= 5120
+
+__global__ void __launch_bounds__(kGasThreads) k_gas(Geo g, Fields F, Gas G, GasConst K, double dt, double Vframe) {
+    extern __shared__ double lds[];
+    __shared__ double red[33];
+    const int m = g.m, me = g.me, ml = g.ml, T = blockDim.x, tid = threadIdx.x;
+    const int x_sym = g.x_sym;
+
+    // gas_layer.cc:39-40
+    for (int k = tid; k < me; k += T) G.p_old[k] = G.p[k];
+    for (int k = tid; k < ml; k += T) G.tau_old[k] = G.tau[k];
+
+    // update_height: stage old h and the interface velocities in LDS
+    double *HO = lds, *UC = lds + m, *VC = lds + 2 * m;
+    for (int i = tid; i < m; i += T) {
+        HO[i] = G.h[i];
+        UC[i] = 0.25 * K.gxsp * (F.u[IX(i, 0)] + F.u[IX(i, -1)]);
+        VC[i] = 0.5 * (F.v[IX(i, 0)] + F.v[IX(i, -1)]);
+    }
+    __syncthreads();
+    if (tid == 0) gas_height_serial(m, x_sym, dt, Vframe, HO, UC, VC, G.h, G.ht, G.fbd);
+    __syncthreads();
+    // calculate_hx (gas_layer_solve.cc:58-62)
+    for (int i = tid; i <= m; i += T) {
+        double v;
+        if (i >= 1 && i < m) v = K.gxsp * (G.h[i] - G.h[i - 1]);
+        else if (i == 0) v = x_sym ? 0.0 : K.gxsp * (G.h[1] - G.h[0]);
+        else v = K.gxsp * (G.h[m - 1] - G.h[m - 2]);
+        G.hx[i] = v;
+    }
+    // update_gas_pressure: pg_old = pg
+    for (int i = tid; i < me; i += T) G.pg_old[i] = G.pg[i];
+    __syncthreads();
+
+    // implicit_step / newton_step (gas_layer_solve.cc:67-129, :197-212)
+    double *A = lds, *B = lds + me, *C = lds + 2 * me, *D = lds + 3 * me;
+    const double mu = K.mug, alpha = K.alpha, Pamb = K.Pamb, xsp = K.gxsp, xxsp = K.gxxsp, dtinv = 1. / dt;
+    const double pre = std::numeric_limits<double>::epsilon() * 1e8, thresh = pre * pre;
+    int nstep = 0, naccept = 0, status = VDI_OK;
+    while (true) {
+        if (++nstep > 256) { status = VDI_ERR_NEWTON_NOCONV; nstep--; break; }
+        double part = 0;
+        for (int i = tid; i < me; i += T) {
+            double a = 0, b = 1, c = 0, d;
+            if (i == me - 1) d = -G.pg[me - 1];
+            else if (i == 0 && !x_sym) d = -G.pg[0];
+            else {
+                const double *h = G.h, *ht = G.ht, *pg = G.pg;
+                double h_ns = i > 0 ? 0.5 * (h[i - 1] + h[i]) : h[i];
+                double ht_ns = i > 0 ? 0.5 * (ht[i - 1] + ht[i]) : ht[i];
+                double pgi = pg[i], pgp = pg[i + 1], pgm = i > 0 ? pg[i - 1] : pg[i + 1];
+                double px = i > 0 ? 0.5 * xsp * (pgp - pgm) : 0.0;
+                double pxx = xxsp * (pgp - 2 * pgi + pgm);
+                double hsq = h_ns * h_ns, hcu = hsq * h_ns, hxi = G.hx[i];
+                double br = 3 * hsq * hxi * px + hcu * pxx - 12 * mu * alpha * h_ns * dtinv - 12 * mu * ht_ns;
+                a = -xsp * alpha * hcu * px + pgi * (-1.5 * xsp * hsq * hxi + hcu * xxsp) +
+                    Pamb * (-1.5 * xsp * hsq * hxi + hcu * xxsp);
+                b = br - 2 * pgi * hcu * xxsp - 2 * Pamb * hcu * xxsp;
+                c = xsp * alpha * hcu * px + pgi * (1.5 * xsp * hsq * hxi + hcu * xxsp) +
+                    Pamb * (1.5 * xsp * hsq * hxi + hcu * xxsp);
+                if (i == 0) { c += a; a = 0; }
+                d = -(12 * mu * alpha * h_ns * dtinv * G.pg_old[i] + alpha * hcu * px * px + pgi * br +
+                      Pamb * (-12 * mu * ht_ns + 3 * hsq * hxi * px + hcu * pxx));
+                part += d * d;
+            }
+            A[i] = a; B[i] = b; C[i] = c; D[i] = d;
+        }
+        double ressq = wg_sum(part, red) / me;  // quirk Q7: divided by me
+        // parallel cyclic reduction (no pivoting; the Newton matrix is
+        // diagonally dominant)
+        for (int s = 1; s < me; s <<= 1) {
+            double na[kGasRows], nb[kGasRows], nc[kGasRows], nd[kGasRows];
+#pragma unroll
+            for (int r = 0; r < kGasRows; r++) {
+                int i = tid + r * T;
+                if (i >= me) continue;
+                double ai = A[i], bi = B[i], ci = C[i], di = D[i];
+                double k1 = 0, k2 = 0, am = 0, cm = 0, dm = 0, ap = 0, cp = 0, dp = 0;
+                if (i - s >= 0) { k1 = ai / B[i - s]; am = A[i - s]; cm = C[i - s]; dm = D[i - s]; }
+                if (i + s < me) { k2 = ci / B[i + s]; ap = A[i + s]; cp = C[i + s]; dp = D[i + s]; }
+                na[r] = -am * k1; nc[r] = -cp * k2;
+                nb[r] = bi - cm * k1 - ap * k2;
+                nd[r] = di - dm * k1 - dp * k2;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < kGasRows; r++) {
+                int i = tid + r * T;
+                if (i >= me) continue;
+                A[i] = na[r]; B[i] = nb[r]; C[i] = nc[r]; D[i] = nd[r];
+            }
+            __syncthreads();
+        }
+        double bad = 0;
+        for (int i = tid; i < me; i += T) {
+            double x = D[i] / B[i];
+            if (!(x == x) || x - x != 0) bad = 1;
+            G.pg[i] += x;
+        }
+        if (wg_sum(bad, red) != 0) { status = VDI_ERR_TRIDIAG; break; }
+        if (ressq < thresh) naccept++;
+        else naccept = 0;
+        if (naccept >= 2) break;
+    }
+    __syncthreads();
+
+    // calculate_liquid_fields (gas_layer_solve.cc:229-249)
+    const double fac = 0.5 * K.sigma * xxsp, fac2 = K.sigma / K.R;
+    const double *h = G.h, *pg = G.pg;
+    for (int i = tid; i < me; i += T) {
+        double v;
+        if (i == 0) v = x_sym ? pg[0] + 2 * fac * (h[1] - h[0]) - fac2 : pg[0];
+        else if (i == 1) v = x_sym ? pg[1] + fac * (h[2] - h[1]) - fac2 : pg[1];
+        else if (i < m - 1) v = pg[i] + fac * (h[i + 1] - h[i] - h[i - 1] + h[i - 2]) - fac2;
+        else v = pg[i];
+        G.p[i] = v;
+    }
+    __syncthreads();
+    for (int i = tid; i < m; i += T) G.tau[i] = -0.5 * xsp * mu * (G.p[i + 1] - G.p[i]) * h[i];
+    __syncthreads();
+    if (tid == 0) {
+        // gas_layer.cc:25-30 (quirk Q1: tau[m], tau[m+1] are never written by the model)
+        double *tau = G.tau;
+        tau[0] = -tau[3]; tau[1] = -tau[2];
+        tau[ml - 2] = -tau[ml - 3]; tau[ml - 1] = -tau[ml - 4];
+        G.status[0] = status;
+        G.status[1] = nstep;
+    }
+}
+
+// gas_layer_solve.cc:32-53 + :229-249 (initial liquid fields, no tau BC: quirk Q6)
+__global__ void __launch_bounds__(1024) k_gas_init(Geo g, Gas G, GasConst K) {
+    const int m = g.m, me = g.me, T = blockDim.x, tid = threadIdx.x;
+    const double dx = (g.bx - g.ax) / m;
+    for (int i = tid; i < m; i += T) {
+        G.fbd[i] = 0;
+        double x = g.ax + (i + 0.5) * dx;
+        G.h[i] = K.h0 + 0.5 / K.R * x * x;
+        G.ht[i] = -K.V;
+    }
+    for (int i = tid; i < me; i += T) G.pg[i] = 0;
+    __syncthreads();
+    for (int i = tid; i <= m; i += T) {
+        double v;
+        if (i >= 1 && i < m) v = K.gxsp * (G.h[i] - G.h[i - 1]);
+        else if (i == 0) v = g.x_sym ? 0.0 : K.gxsp * (G.h[1] - G.h[0]);
+        else v = K.gxsp * (G.h[m - 1] - G.h[m - 2]);
+        G.hx[i] = v;
+    }
+    const double fac = 0.5 * K.sigma * K.gxxsp, fac2 = K.sigma / K.R;
+    const double *h = G.h, *pg = G.pg;
+    for (int i = tid; i < me; i += T) {
+        double v;
+        if (i == 0) v = g.x_sym ? pg[0] + 2 * fac * (h[1] - h[0]) - fac2 : pg[0];
+        else if (i == 1) v = g.x_sym ? pg[1] + fac * (h[2] - h[1]) - fac2 : pg[1];
+        else if (i < m - 1) v = pg[i] + fac * (h[i + 1] - h[i] - h[i - 1] + h[i - 2]) - fac2;
+        else v = pg[i];
+        G.p[i] = v;
+    }
+    __syncthreads();
+    for (int i = tid; i < m; i += T) G.tau[i] = -0.5 * K.gxsp * K.mug * (G.p[i + 1] - G.p[i]) * h[i];
+}
+
+#undef IX
+
+}  // namespace vdi
+
+// ------------------------------------------------------------------------
+// Context and C ABI
+// ------------------------------------------------------------------------
+
+using namespace vdi;
+
+struct vdi_ctx {
+    vdi_params P{};
+    Geo g{};
+    GasConst K{};
+    hipStream_t st = nullptr;
+    double time = 0, Vframe = 0;
+    double rel_tol = 1e-10;
+    int max_cycles = 200;
+    int form = VDI_FORM_FP64;
+    long steps = 0;
+    int last_newton = 0;
+    DevBuf fields;  // 19 padded arrays
+    DevBuf srcbuf, gasbuf, zbuf, statusbuf;
+    Fields F{};
+    Gas G{};
+    double *zmac = nullptr, *zfem = nullptr, *zvx = nullptr, *zvy = nullptr;
+    MG mac, fem, vx, vy;
+    int *host_status = nullptr;
+    size_t gas_lds = 0;
+    ~vdi_ctx() {
+        if (host_status) (void)hipHostFree(host_status);
+        if (st) (void)hipStreamDestroy(st);
+    }
+};
+
+static const dim3 kB2(64, 4);
+
+static dim3 cells_grid(const Geo &g) { return dim3(cdiv(g.m, kB2.x), cdiv(g.n, kB2.y)); }
+static dim3 nodes_grid(const Geo &g) { return dim3(cdiv(g.me, kB2.x), cdiv(g.ne, kB2.y)); }
+
+extern "C" {
+
+const char *vdi_status_string(int s) {
+    switch (s) {
+    case VDI_OK: return "OK";
+    case VDI_ERR_MG_NOCONV: return "Multigrid routine failed";
+    case VDI_ERR_NEWTON_NOCONV: return "Too many Newton steps";
+    case VDI_ERR_TRIDIAG: return "Tridiagonal solve failed";
+    case VDI_ERR_HIP: return "HIP runtime error";
+    case VDI_ERR_ARG: return "Invalid argument";
+    default: return "Unknown status";
+    }
+}
+
+const char *vdi_last_error(void) { return g_last_error.c_str(); }
+
+int vdi_create(const vdi_params *p, vdi_ctx **out) {
+    if (!p || !out) return VDI_ERR_ARG;
+    *out = nullptr;
+    if (p->m < 4 || p->n < 4 || p->m + 1 > kGasThreads * kGasRows) return VDI_ERR_ARG;
+    if (!(p->bx > p->ax) || !(p->by > p->ay) || !(p->rho > 0) || !(p->mu > 0)) return VDI_ERR_ARG;
+    std::unique_ptr<vdi_ctx> c(new vdi_ctx());
+    c->P = *p;
+    Geo &g = c->g;
+    g.m = p->m; g.n = p->n; g.ml = p->m + 4; g.me = p->m + 1; g.ne = p->n + 1; g.x_sym = p->x_sym;
+    g.ax = p->ax; g.bx = p->bx; g.ay = p->ay; g.by = p->by;
+    g.dx = (p->bx - p->ax) / p->m; g.dy = (p->by - p->ay) / p->n;
+    g.xsp = 1. / g.dx; g.ysp = 1. / g.dy; g.xxsp = g.xsp * g.xsp; g.yysp = g.ysp * g.ysp;
+    g.rho = p->rho; g.rhoinv = 1. / p->rho; g.mu = p->mu; g.muinv = 1. / p->mu;
+    GasConst &K = c->K;
+    K.alpha = p->alpha; K.mug = p->mug; K.Pamb = p->Pamb; K.h0 = p->h0; K.R = p->R; K.V = p->V;
+    K.sigma = p->sigma;
+    K.gxsp = 1. / ((p->bx - p->ax) / p->m);
+    K.gxxsp = K.gxsp * K.gxsp;
+    c->Vframe = p->V;
+    VDI_HIP(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+    size_t NP = (size_t)g.ml * (g.n + 4);
+    VDI_TRY(c->fields.alloc(19 * NP * sizeof(double)));
+    double *f = c->fields.as<double>();
+    double **dst[19] = {&c->F.u, &c->F.v, &c->F.p, &c->F.ux, &c->F.uy, &c->F.vx, &c->F.vy, &c->F.ud, &c->F.vd,
+                        &c->F.ul, &c->F.vl, &c->F.ur, &c->F.vr, &c->F.uu, &c->F.vu, &c->F.c0, &c->F.c1,
+                        &c->F.c2, &c->F.c3};
+    for (int k = 0; k < 19; k++) *dst[k] = f + k * NP;
+    VDI_TRY(c->srcbuf.alloc((size_t)g.me * g.ne * sizeof(double)));
+    c->F.src = c->srcbuf.as<double>();
+    // gas arrays: p[me], tau[ml], p_old[me], tau_old[ml], h[m], hx[me], ht[m], fbd[m], pg[me], pg_old[me]
+    size_t ng = 5 * (size_t)g.me + 2 * (size_t)g.ml + 3 * (size_t)g.m;
+    VDI_TRY(c->gasbuf.alloc(ng * sizeof(double)));
+    double *q = c->gasbuf.as<double>();
+    c->G.p = q; q += g.me;
+    c->G.tau = q; q += g.ml;
+    c->G.p_old = q; q += g.me;
+    c->G.tau_old = q; q += g.ml;
+    c->G.h = q; q += g.m;
+    c->G.hx = q; q += g.me;
+    c->G.ht = q; q += g.m;
+    c->G.fbd = q; q += g.m;
+    c->G.pg = q; q += g.me;
+    c->G.pg_old = q; q += g.me;
+    VDI_TRY(c->statusbuf.alloc(4 * sizeof(int)));
+    c->G.status = c->statusbuf.as<int>();
+    VDI_HIP(hipHostMalloc((void **)&c->host_status, 4 * sizeof(int), hipHostMallocDefault));
+    size_t mn = (size_t)g.m * g.n;
+    VDI_TRY(c->zbuf.alloc((3 * mn + (size_t)g.me * g.ne) * sizeof(double)));
+    double *z = c->zbuf.as<double>();
+    c->zmac = z; c->zvx = z + mn; c->zvy = z + 2 * mn; c->zfem = z + 3 * mn;
+    VDI_TRY(c->mac.init(VDI_MAC, g.m, g.n, g.xxsp, g.yysp, g.x_sym, c->zmac, c->F.src, c->st));
+    VDI_TRY(c->fem.init(VDI_FEM, g.me, g.ne, g.dx, g.dy, g.x_sym, c->zfem, c->F.src, c->st));
+    if (p->implicit_visc) {
+        VDI_TRY(c->vx.init(VDI_VX, g.m, g.n, 0, 0, g.x_sym, c->zvx, c->F.src, c->st));
+        VDI_TRY(c->vy.init(VDI_VY, g.m, g.n, 0, 0, g.x_sym, c->zvy, c->F.src, c->st));
+    }
+    c->gas_lds = std::max<size_t>(3 * (size_t)g.m, 4 * (size_t)g.me) * sizeof(double);
+    if (c->gas_lds > 64 * 1024)
+        VDI_HIP(hipFuncSetAttribute((const void *)k_gas, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->gas_lds));
+    *out = c.release();
+    return VDI_OK;
+}
+
+int vdi_destroy(vdi_ctx *ctx) {
+    if (!ctx) return VDI_ERR_ARG;
+    (void)hipStreamSynchronize(ctx->st);
+    delete ctx;
+    return VDI_OK;
+}
+
+int vdi_set_solver(vdi_ctx *c, int nu1, int nu2, double rel_tol, int max_cycles) {
+    if (!c || nu1 < 0 || nu2 < 0 || nu1 + nu2 == 0 || !(rel_tol > 0) || max_cycles < 1) return VDI_ERR_ARG;
+    MG *gs[4] = {&c->mac, &c->fem, &c->vx, &c->vy};
+    for (MG *g : gs) { g->nu1 = nu1; g->nu2 = nu2; }
+    c->rel_tol = rel_tol;
+    c->max_cycles = max_cycles;
+    return VDI_OK;
+}
+
+int vdi_set_form(vdi_ctx *c, int form) {
+    if (!c || (form != VDI_FORM_FP64)) return VDI_ERR_ARG;
+    c->form = form;
+    return VDI_OK;
+}
+
+static int enqueue_viscous_bc(vdi_ctx *c) {
+    hipLaunchKernelGGL(k_viscous_bc, dim3(1), dim3(1024), 0, c->st, c->g, c->F, (const double *)c->G.tau);
+    VDI_LAUNCHED();
+    return VDI_OK;
+}
+
+static int enqueue_inviscid_bc(vdi_ctx *c, double dt) {
+    double prefac = c->g.dx * c->g.rhoinv * dt / (3 * M_PI);
+    int waves = c->g.n + c->g.ml;
+    hipLaunchKernelGGL(k_simpson, dim3(cdiv(waves, 4)), dim3(256), 0, c->st, c->g, c->F, (const double *)c->G.p, prefac);
+    VDI_LAUNCHED();
+    return VDI_OK;
+}
+
+int vdi_init_fields(vdi_ctx *c) {
+    if (!c) return VDI_ERR_ARG;
+    c->time = 0;
+    size_t NP = (size_t)c->g.ml * (c->g.n + 4);
+    VDI_HIP(hipMemsetAsync(c->F.u, 0, NP * sizeof(double), c->st));
+    VDI_HIP(hipMemsetAsync(c->F.v, 0, NP * sizeof(double), c->st));
+    VDI_HIP(hipMemsetAsync(c->F.p, 0, NP * sizeof(double), c->st));
+    hipLaunchKernelGGL(k_gas_init, dim3(1), dim3(1024), 0, c->st, c->g, c->G, c->K);
+    VDI_LAUNCHED();
+    VDI_TRY(enqueue_inviscid_bc(c, 0));
+    VDI_TRY(enqueue_viscous_bc(c));
+    VDI_HIP(hipStreamSynchronize(c->st));
+    return VDI_OK;
+}
+
+static int gas_update(vdi_ctx *c, double dt) {
+    hipLaunchKernelGGL(k_gas, dim3(1), dim3(kGasThreads), c->gas_lds, c->st, c->g, c->F, c->G, c->K, dt, c->Vframe);
+    VDI_LAUNCHED();
+    VDI_HIP(hipMemcpyAsync(c->host_status, c->G.status, 2 * sizeof(int), hipMemcpyDeviceToHost, c->st));
+    VDI_HIP(hipStreamSynchronize(c->st));
+    c->last_newton = c->host_status[1];
+    return c->host_status[0];
+}
+
+int vdi_gas_layer_update(vdi_ctx *c, double t, double dt) {
+    (void)t;
+    if (!c || !(dt > 0)) return VDI_ERR_ARG;
+    return gas_update(c, dt);
+}
+
+static int solve_mg(vdi_ctx *c, MG &g, double t, int *cyc) {
+    VDI_TRY(g.initial_guess(t));
+    return g.solve(c->rel_tol, c->max_cycles, cyc);
+}
+
+int vdi_mg_solve_v_cycle(vdi_ctx *c, int which, double t, int *vcycles) {
+    if (!c || (which != VDI_MAC && which != VDI_FEM)) return VDI_ERR_ARG;
+    return solve_mg(c, which == VDI_MAC ? c->mac : c->fem, t, vcycles);
+}
+
+static int visco_solve(vdi_ctx *c, MG &g, double vfx, double vfy, double t, int *cyc) {
+    // visco_impl.cc:21-39
+    if (std::fabs(vfx - g.vfx) > 1e2 * std::numeric_limits<double>::epsilon() * g.vfx) {
+        g.vfx = vfx;
+        g.vfy = vfy;
+        g.set_coeffs(vfx, vfy);
+    }
+    return solve_mg(c, g, t, cyc);
+}
+
+int vdi_visco_setup_and_solve(vdi_ctx *c, int which, double vfx, double vfy, double t, int *vcycles) {
+    if (!c || !c->P.implicit_visc || (which != VDI_VX && which != VDI_VY)) return VDI_ERR_ARG;
+    return visco_solve(c, which == VDI_VX ? c->vx : c->vy, vfx, vfy, t, vcycles);
+}
+
+int vdi_step_forward(vdi_ctx *c, double dt) {
+    if (!c || !(dt > 0)) return VDI_ERR_ARG;
+    const Geo &g = c->g;
+    const dim3 gc = cells_grid(g), gn = nodes_grid(g);
+    // fluid_2d.cc:99-102
+    VDI_TRY(gas_update(c, dt));
+    c->time += 0.5 * dt;
+    const double viscfac = c->P.implicit_visc ? g.rhoinv * c->P.mu * 0.5 : g.rhoinv * c->P.mu;
+    hipLaunchKernelGGL(k_slopes, gc, kB2, 0, c->st, g, c->F, dt);
+    hipLaunchKernelGGL(k_godunov_tang, gc, kB2, 0, c->st, g, c->F);
+    hipLaunchKernelGGL(k_tang_terms, gc, kB2, 0, c->st, g, c->F);
+    hipLaunchKernelGGL(k_predictor, gc, kB2, 0, c->st, g, c->F, dt, viscfac);
+    hipLaunchKernelGGL(k_godunov_normal, gc, kB2, 0, c->st, g, c->F);
+    hipLaunchKernelGGL(k_mac_rhs, gc, kB2, 0, c->st, g, c->F, c->G, 0.125 * g.rhoinv * dt);
+    VDI_LAUNCHED();
+    int cyc;
+    VDI_TRY(solve_mg(c, c->mac, c->time, &cyc));
+    hipLaunchKernelGGL(k_mac_correct, gc, kB2, 0, c->st, g, c->F, (const double *)c->zmac);
+    c->time += 0.5 * dt;
+    hipLaunchKernelGGL(k_advect, gc, kB2, 0, c->st, g, c->F, dt);
+    VDI_LAUNCHED();
+    VDI_TRY(enqueue_inviscid_bc(c, dt));
+    VDI_TRY(enqueue_viscous_bc(c));
+    if (c->P.implicit_visc) {
+        const double vfx = viscfac * g.xxsp * dt, vfy = viscfac * g.yysp * dt;
+        hipLaunchKernelGGL(k_visco_rhs, gc, kB2, 0, c->st, g, c->F, c->G, vfx, vfy, 0);
+        VDI_LAUNCHED();
+        VDI_TRY(visco_solve(c, c->vx, vfx, vfy, c->time, &cyc));
+        hipLaunchKernelGGL(k_visco_rhs, gc, kB2, 0, c->st, g, c->F, c->G, vfx, vfy, 1);
+        VDI_LAUNCHED();
+        VDI_TRY(visco_solve(c, c->vy, vfx, vfy, c->time, &cyc));
+        hipLaunchKernelGGL(k_visco_copy, gc, kB2, 0, c->st, g, c->F, (const double *)c->zvx, (const double *)c->zvy);
+        VDI_LAUNCHED();
+    }
+    hipLaunchKernelGGL(k_fem_rhs, gn, kB2, 0, c->st, g, c->F, (const double *)c->G.p, g.rho / dt, c->fem.fe.fm);
+    VDI_LAUNCHED();
+    VDI_TRY(solve_mg(c, c->fem, c->time, &cyc));
+    hipLaunchKernelGGL(k_update, gn, kB2, 0, c->st, g, c->F, (const double *)c->zfem, dt);
+    VDI_LAUNCHED();
+    VDI_TRY(enqueue_viscous_bc(c));
+    c->steps++;
+    return VDI_OK;
+}
+
+static double *array_ptr(vdi_ctx *c, int id, long *len) {
+    long NP = (long)c->g.ml * (c->g.n + 4), mn = (long)c->g.m * c->g.n, nn = (long)c->g.me * c->g.ne;
+    *len = 0;
+    switch (id) {
+    case VDI_U: *len = NP; return c->F.u;
+    case VDI_V: *len = NP; return c->F.v;
+    case VDI_P: *len = NP; return c->F.p;
+    case VDI_UL: *len = NP; return c->F.ul;
+    case VDI_VL: *len = NP; return c->F.vl;
+    case VDI_UD: *len = NP; return c->F.ud;
+    case VDI_VD: *len = NP; return c->F.vd;
+    case VDI_C0: *len = NP; return c->F.c0;
+    case VDI_C1: *len = NP; return c->F.c1;
+    case VDI_SRC: *len = nn; return c->F.src;
+    case VDI_GP: *len = c->g.me; return c->G.p;
+    case VDI_GTAU: *len = c->g.ml; return c->G.tau;
+    case VDI_GP_OLD: *len = c->g.me; return c->G.p_old;
+    case VDI_GTAU_OLD: *len = c->g.ml; return c->G.tau_old;
+    case VDI_H: *len = c->g.m; return c->G.h;
+    case VDI_HX: *len = c->g.me; return c->G.hx;
+    case VDI_HT: *len = c->g.m; return c->G.ht;
+    case VDI_FBD: *len = c->g.m; return c->G.fbd;
+    case VDI_PG: *len = c->g.me; return c->G.pg;
+    case VDI_PG_OLD: *len = c->g.me; return c->G.pg_old;
+    case VDI_Z_MAC: *len = mn; return c->zmac;
+    case VDI_Z_FEM: *len = nn; return c->zfem;
+    case VDI_Z_VX: *len = c->P.implicit_visc ? mn : 0; return c->P.implicit_visc ? c->zvx : nullptr;
+    case VDI_Z_VY: *len = c->P.implicit_visc ? mn : 0; return c->P.implicit_visc ? c->zvy : nullptr;
+    default: return nullptr;
+    }
+}
+
+long vdi_array_len(vdi_ctx *c, int id) {
+    if (!c) return -1;
+    long len;
+    array_ptr(c, id, &len);
+    return len;
+}
+
+int vdi_get_array(vdi_ctx *c, int id, double *host, long len) {
+    if (!c || !host) return VDI_ERR_ARG;
+    long n;
+    double *p = array_ptr(c, id, &n);
+    if (!p || n != len) return VDI_ERR_ARG;
+    VDI_HIP(hipStreamSynchronize(c->st));
+    VDI_HIP(hipMemcpy(host, p, n * sizeof(double), hipMemcpyDeviceToHost));
+    return VDI_OK;
+}
+
+int vdi_set_array(vdi_ctx *c, int id, const double *host, long len) {
+    if (!c || !host) return VDI_ERR_ARG;
+    long n;
+    double *p = array_ptr(c, id, &n);
+    if (!p || n != len) return VDI_ERR_ARG;
+    VDI_HIP(hipStreamSynchronize(c->st));
+    VDI_HIP(hipMemcpy(p, host, n * sizeof(double), hipMemcpyHostToDevice));
+    return VDI_OK;
+}
+
+int vdi_get_time(vdi_ctx *c, double *t) {
+    if (!c || !t) return VDI_ERR_ARG;
+    *t = c->time;
+    return VDI_OK;
+}
+
+int vdi_set_time(vdi_ctx *c, double t) {
+    if (!c) return VDI_ERR_ARG;
+    c->time = t;
+    return VDI_OK;
+}
+
+int vdi_get_stats(vdi_ctx *c, vdi_stats *s) {
+    if (!c || !s) return VDI_ERR_ARG;
+    MG *gs[4] = {&c->mac, &c->fem, &c->vx, &c->vy};
+    for (int k = 0; k < 4; k++) {
+        s->last_cycles[k] = gs[k]->last_cycles;
+        s->total_cycles[k] = gs[k]->total_cycles;
+        s->solves[k] = gs[k]->solves;
+    }
+    s->last_newton = c->last_newton;
+    s->steps = c->steps;
+    s->time = c->time;
+    return VDI_OK;
+}
+
+int vdi_reset_stats(vdi_ctx *c) {
+    if (!c) return VDI_ERR_ARG;
+    MG *gs[4] = {&c->mac, &c->fem, &c->vx, &c->vy};
+    for (MG *g : gs) { g->total_cycles = 0; g->solves = 0; }
+    c->steps = 0;
+    return VDI_OK;
+}
+
+int vdi_synchronize(vdi_ctx *c) {
+    if (!c) return VDI_ERR_ARG;
+    VDI_HIP(hipStreamSynchronize(c->st));
+    return VDI_OK;
+}
+
+long vdi_state_len(vdi_ctx *c) {
+    if (!c) return -1;
+    const Geo &g = c->g;
+    return 1 + 3L * g.ml * (g.n + 4) + 1 + g.me + g.m + g.m + g.m + g.me;
+}
+
+int vdi_download_state(vdi_ctx *c, double *host, long len) {
+    if (!c || !host || len != vdi_state_len(c)) return VDI_ERR_ARG;
+    const Geo &g = c->g;
+    long NP = (long)g.ml * (g.n + 4);
+    std::vector<double> u(NP), v(NP), p(NP);
+    VDI_TRY(vdi_get_array(c, VDI_U, u.data(), NP));
+    VDI_TRY(vdi_get_array(c, VDI_V, v.data(), NP));
+    VDI_TRY(vdi_get_array(c, VDI_P, p.data(), NP));
+    double *o = host;
+    *o++ = c->time;
+    for (long k = 0; k < NP; k++) { *o++ = u[k]; *o++ = v[k]; *o++ = p[k]; }
+    *o++ = c->Vframe;
+    std::vector<double> tau(g.ml);
+    VDI_HIP(hipMemcpy(o, c->G.p, g.me * sizeof(double), hipMemcpyDeviceToHost)); o += g.me;
+    VDI_HIP(hipMemcpy(tau.data(), c->G.tau, g.ml * sizeof(double), hipMemcpyDeviceToHost));
+    memcpy(o, tau.data() + 2, g.m * sizeof(double)); o += g.m;
+    VDI_HIP(hipMemcpy(o, c->G.h, g.m * sizeof(double), hipMemcpyDeviceToHost)); o += g.m;
+    VDI_HIP(hipMemcpy(o, c->G.ht, g.m * sizeof(double), hipMemcpyDeviceToHost)); o += g.m;
+    VDI_HIP(hipMemcpy(o, c->G.pg, g.me * sizeof(double), hipMemcpyDeviceToHost)); o += g.me;
+    return VDI_OK;
+}
+
+int vdi_upload_state(vdi_ctx *c, const double *host, long len) {
+    if (!c || !host || len != vdi_state_len(c)) return VDI_ERR_ARG;
+    const Geo &g = c->g;
+    long NP = (long)g.ml * (g.n + 4);
+    std::vector<double> u(NP), v(NP), p(NP);
+    const double *o = host;
+    c->time = *o++;
+    for (long k = 0; k < NP; k++) { u[k] = *o++; v[k] = *o++; p[k] = *o++; }
+    c->Vframe = *o++;
+    VDI_TRY(vdi_set_array(c, VDI_U, u.data(), NP));
+    VDI_TRY(vdi_set_array(c, VDI_V, v.data(), NP));
+    VDI_TRY(vdi_set_array(c, VDI_P, p.data(), NP));
+    // gas_layer_solve.cc:313-324: p, tau[2..m+1] + tau BC, h + hx, ht, pg
+    std::vector<double> tau(g.ml, 0.0), h(o + g.me, o + g.me + g.m), hx(g.me);
+    VDI_HIP(hipMemcpy(c->G.p, o, g.me * sizeof(double), hipMemcpyHostToDevice)); o += g.me;
+    memcpy(tau.data() + 2, o, g.m * sizeof(double)); o += g.m;
+    tau[0] = -tau[3]; tau[1] = -tau[2]; tau[g.ml - 2] = -tau[g.ml - 3]; tau[g.ml - 1] = -tau[g.ml - 4];
+    VDI_HIP(hipMemcpy(c->G.tau, tau.data(), g.ml * sizeof(double), hipMemcpyHostToDevice));
+    VDI_HIP(hipMemcpy(c->G.h, o, g.m * sizeof(double), hipMemcpyHostToDevice)); o += g.m;
+    for (int i = 1; i < g.m; i++) hx[i] = c->K.gxsp * (h[i] - h[i - 1]);
+    hx[0] = g.x_sym ? 0 : hx[1];
+    hx[g.m] = hx[g.m - 1];
+    VDI_HIP(hipMemcpy(c->G.hx, hx.data(), g.me * sizeof(double), hipMemcpyHostToDevice));
+    VDI_HIP(hipMemcpy(c->G.ht, o, g.m * sizeof(double), hipMemcpyHostToDevice)); o += g.m;
+    VDI_HIP(hipMemcpy(c->G.pg, o, g.me * sizeof(double), hipMemcpyHostToDevice)); o += g.me;
+    return VDI_OK;
+}
+
+// ---- standalone linear systems ----
+
+}  // extern "C"
+
+struct vdi_mg {
+    MG g;
+    hipStream_t st = nullptr;
+    ~vdi_mg() {
+        if (st) (void)hipStreamDestroy(st);
+    }
+};
+
+extern "C" {
+
+int vdi_mg_create(int kind, int m, int n, double cx, double cy, int x_sym, vdi_mg **out) {
+    if (!out || kind < VDI_MAC || kind > VDI_VY || m < 2 || n < 2) return VDI_ERR_ARG;
+    *out = nullptr;
+    std::unique_ptr<vdi_mg> h(new vdi_mg());
+    VDI_HIP(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
+    VDI_TRY(h->g.init(kind, m, n, cx, cy, x_sym, nullptr, nullptr, h->st));
+    *out = h.release();
+    return VDI_OK;
+}
+
+int vdi_mg_destroy(vdi_mg *h) {
+    if (!h) return VDI_ERR_ARG;
+    (void)hipStreamSynchronize(h->st);
+    delete h;
+    return VDI_OK;
+}
+
+int vdi_mg_set_schedule(vdi_mg *h, int nu1, int nu2) {
+    if (!h || nu1 < 0 || nu2 < 0 || nu1 + nu2 == 0) return VDI_ERR_ARG;
+    h->g.nu1 = nu1;
+    h->g.nu2 = nu2;
+    return VDI_OK;
+}
+
+int vdi_mg_set_form(vdi_mg *h, int form) {
+    if (!h || form != VDI_FORM_FP64) return VDI_ERR_ARG;
+    h->g.form = form;
+    return VDI_OK;
+}
+
+int vdi_mg_levels(vdi_mg *h) { return h ? h->g.nl : -1; }
+
+int vdi_mg_set_b(vdi_mg *h, const double *host, long len) {
+    if (!h || !host || len != h->g.unknowns()) return VDI_ERR_ARG;
+    VDI_HIP(hipMemcpyAsync(h->g.b(), host, len * sizeof(double), hipMemcpyHostToDevice, h->st));
+    VDI_HIP(hipStreamSynchronize(h->st));
+    return VDI_OK;
+}
+
+int vdi_mg_set_z(vdi_mg *h, const double *host, long len) {
+    if (!h || !host || len != h->g.unknowns()) return VDI_ERR_ARG;
+    VDI_HIP(hipMemcpyAsync(h->g.z(), host, len * sizeof(double), hipMemcpyHostToDevice, h->st));
+    VDI_HIP(hipStreamSynchronize(h->st));
+    return VDI_OK;
+}
+
+int vdi_mg_get_z(vdi_mg *h, double *host, long len) {
+    if (!h || !host || len != h->g.unknowns()) return VDI_ERR_ARG;
+    VDI_HIP(hipStreamSynchronize(h->st));
+    VDI_HIP(hipMemcpy(host, h->g.z(), len * sizeof(double), hipMemcpyDeviceToHost));
+    return VDI_OK;
+}
+
+int vdi_mg_apply(vdi_mg *h, const double *z, double *out, long len) {
+    if (!h || !z || !out || len != h->g.unknowns()) return VDI_ERR_ARG;
+    DevBuf dz, dout;
+    VDI_TRY(dz.alloc(len * sizeof(double)));
+    VDI_TRY(dout.alloc(len * sizeof(double)));
+    VDI_HIP(hipMemcpy(dz.p, z, len * sizeof(double), hipMemcpyHostToDevice));
+    VDI_TRY(h->g.apply(dz.as<double>(), dout.as<double>()));
+    VDI_HIP(hipStreamSynchronize(h->st));
+    VDI_HIP(hipMemcpy(out, dout.p, len * sizeof(double), hipMemcpyDeviceToHost));
+    return VDI_OK;
+}
+
+int vdi_mg_solve(vdi_mg *h, double rel_tol, int max_cycles, int *vcycles) {
+    if (!h || !(rel_tol > 0) || max_cycles < 0) return VDI_ERR_ARG;
+    return h->g.solve(rel_tol, max_cycles, vcycles);
+}
+
+int vdi_mg_vcycles(vdi_mg *h, int k) {
+    if (!h || k < 0) return VDI_ERR_ARG;
+    return h->g.vcycles(k);
+}
+
+int vdi_mg_residual(vdi_mg *h, double *rnorm, double *bnorm) {
+    if (!h || !rnorm || !bnorm) return VDI_ERR_ARG;
+    return h->g.residual(rnorm, bnorm);
+}
+
+int vdi_mg_synchronize(vdi_mg *h) {
+    if (!h) return VDI_ERR_ARG;
+    VDI_HIP(hipStreamSynchronize(h->st));
+    return VDI_OK;
+}
+
+}  // extern "C"
