@@ -24,6 +24,10 @@ sys.path.insert(0, REPO)
 METRIC = "steps/s at 1024x512 to 1e-10 residual: plain FP64 vs mixed+learned; HBM %"
 
 
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
 def cpu_baseline(m, n, seconds, warmup):
     """Times the oracle on the host (test infrastructure used as the checker
     and as the reported CPU baseline)."""
@@ -75,16 +79,47 @@ def main():
     sim.init_fields()
     for _ in range(args.warmup):
         sim.step_forward(p.dt_reg)
+    log(f"{args.warmup} warm-up steps done")
+
+    # Which kernel class dominates the step: time each class over a few steps.
+    class_ms = {}
+    for kc in vdi.KCLASS_NAMES:
+        sim.time_kernels(kc, 4096)
+        for _ in range(3):
+            sim.step_forward(p.dt_reg)
+        n, ms, _b = sim.kernel_times()
+        class_ms[kc] = ms / 3.0
+    sim.time_kernels(-1)
+    dominant = max(class_ms, key=class_ms.get)
+    log("kernel classes, ms/step:", {vdi.KCLASS_NAMES[k]: round(v, 4) for k, v in class_ms.items()})
+
+    # Timed region: the dominant class's launches are bracketed by HIP events
+    # on the stream that runs them.
     sim.synchronize()
     sim.reset_stats()
+    sim.time_kernels(dominant, 8192)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         sim.step_forward(p.dt_reg)
     sim.synchronize()
     el = time.perf_counter() - t0
+    launches, kms, kbytes = sim.kernel_times()
+    sim.time_kernels(-1)
+    log(f"timed region: {args.steps} steps in {el:.3f} s")
     st = sim.stats()
     cyc = {name: st.total_cycles[k] / max(1, st.solves[k]) for k, name in enumerate(["mac", "fem", "vx", "vy"])}
+
+    # The same number of steps without events, for the event overhead.
+    sim.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        sim.step_forward(p.dt_reg)
+    sim.synchronize()
+    el_clean = time.perf_counter() - t1
+
     value = args.steps / el
+    achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
+    peak = 8000.0
     out = {
         "metric": METRIC,
         "value": value,
@@ -101,12 +136,18 @@ def main():
         "config": {"workload": f"single drop {args.m}x{args.n}, 32 cSt, V=0.5 m/s, R=1 mm, h_min=5 um, "
                                "x_sym, implicit viscosity, all solves to rel. residual 1e-10",
                    "grid": [args.m, args.n], "form": "fp64", "rel_tol": 1e-10,
-                   "avg_vcycles": cyc},
-        "roofline": None,
+                   "avg_vcycles": cyc, "steps_per_s_without_events": args.steps / el_clean},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": peak, "unit": "GB/s",
+                     "frac": achieved / peak, "traffic": None,
+                     "kernel": vdi.KCLASS_NAMES[dominant], "launches": launches,
+                     "avg_launch_us": 1e3 * kms / max(1, launches),
+                     "alg_bytes_per_launch": kbytes / max(1, launches),
+                     "class_ms_per_step": {vdi.KCLASS_NAMES[k]: v for k, v in class_ms.items()}},
         "cpu_baseline": None,
     }
     sim.close()
     if not args.no_cpu:
+        log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(args.m, args.n, args.cpu_seconds, 3)
     print(json.dumps(out), flush=True)
 

@@ -79,6 +79,14 @@ typedef struct vdi_stats {
     double time;           /* fluid_2d::time */
 } vdi_stats;
 
+/* Kernel classes for vdi_time_kernels (measurement only). */
+#define VDI_KCLASS_MG_DOWN0 0   /* level-0 smooth+restrict of every solve */
+#define VDI_KCLASS_MG_UP0 1     /* level-0 prolong+smooth+residual of every solve */
+#define VDI_KCLASS_MG_MID 2     /* down/up kernels of levels 1..lc-1 */
+#define VDI_KCLASS_MG_COARSE 3  /* single-workgroup coarse-level V-cycle */
+#define VDI_KCLASS_GAS 4        /* gas-layer update */
+#define VDI_KCLASS_SIMPSON 5    /* far-field boundary integrals */
+
 typedef struct vdi_ctx vdi_ctx;
 typedef struct vdi_mg vdi_mg;
 
@@ -120,6 +128,19 @@ int vdi_get_stats(vdi_ctx *ctx, vdi_stats *st);
 int vdi_reset_stats(vdi_ctx *ctx);
 /* Blocks until all work queued on the context's stream has finished. */
 int vdi_synchronize(vdi_ctx *ctx);
+/* Measurement (no reference counterpart): time every launch of one kernel
+ * class (VDI_KCLASS_*, -1 to stop) with HIP events on the context's stream,
+ * up to max_launches launches.  vdi_kernel_times returns the number of timed
+ * launches, their summed duration in ms and their summed algorithmic bytes. */
+int vdi_time_kernels(vdi_ctx *ctx, int kclass, int max_launches);
+int vdi_kernel_times(vdi_ctx *ctx, long *launches, double *total_ms, double *alg_bytes);
+/* Plain device copy of n doubles, repeated reps times, for calibrating
+ * hardware byte counters against a kernel of known traffic. */
+int vdi_copy_probe(long n, int reps, double *ms);
+/* Device facts used to size launches: info[0..8] = LDS bytes per workgroup,
+ * compute units, threads per workgroup, wavefront size, clock kHz, memory
+ * clock kHz, memory bus bits, LDS bytes per compute unit, L2 bytes. */
+int vdi_device_info(int *info, int n);
 /* Restart-file body (fluid_2d_sio.cc:97-119 + gas_layer_solve.cc:326-332):
  * time, (n+4) rows x (m+4) x {u,v,p}, Vframe, p[me], tau[2..m+1], h[m],
  * ht[m], pg[me].  Length: vdi_state_len. */

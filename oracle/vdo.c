@@ -83,7 +83,7 @@ vdo_mg *vdo_mg_create(int kind, int m, int n, double cx, double cy, int x_sym) {
     vdo_mg *g = (vdo_mg *)xcalloc(1, sizeof(vdo_mg));
     g->kind = kind;
     g->fem = kind == VDO_KIND_FEM;
-    g->nu1 = 2; g->nu2 = 2; g->ncs = 64;
+    g->nu1 = 2; g->nu2 = 2; g->ncs = 8;
     if (kind == VDO_KIND_MAC) {
         /* mgs_mac.hh:37-50: Neumann left/right/top, Dirichlet face at bottom */
         g->sigma = 0; g->s = 1;

@@ -151,7 +151,7 @@ class OracleSim:
 class OracleMG:
     """A standalone oracle linear system (vdo_mg)."""
 
-    def __init__(self, kind, m, n, cx, cy, x_sym=1, nu1=2, nu2=2, ncs=64):
+    def __init__(self, kind, m, n, cx, cy, x_sym=1, nu1=2, nu2=2, ncs=8):
         self.kind, self.m, self.n = kind, m, n
         self.h = lib().vdo_mg_create(kind, m, n, cx, cy, x_sym)
         lib().vdo_mg_set(self.h, nu1, nu2, ncs)

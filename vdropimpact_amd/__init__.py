@@ -31,6 +31,10 @@ OK, ERR_MG_NOCONV, ERR_NEWTON_NOCONV, ERR_TRIDIAG, ERR_HIP, ERR_ARG = range(6)
 MAC, FEM, VX, VY = 0, 1, 2, 3
 # arithmetic forms
 FORM_FP64, FORM_MIXED = 0, 1
+# kernel classes for Fluid2D.time_kernels
+KCLASS_MG_DOWN0, KCLASS_MG_UP0, KCLASS_MG_MID, KCLASS_MG_COARSE, KCLASS_GAS, KCLASS_SIMPSON = range(6)
+KCLASS_NAMES = {0: "k_cc_down/k_fem_down (level 0)", 1: "k_cc_up/k_fem_up (level 0)", 2: "level kernels 1..lc-1",
+                3: "k_coarse", 4: "k_gas", 5: "k_simpson"}
 # array ids
 (U, V, P, UL, VL, UD, VD, C0, C1, SRC, GP, GTAU, GP_OLD, GTAU_OLD, H, HX, HT, FBD, PG, PG_OLD,
  Z_MAC, Z_FEM, Z_VX, Z_VY) = range(24)
@@ -73,6 +77,10 @@ ABI = {
     "vdi_get_stats": (_i, [_vp, ctypes.POINTER(VdiStats)]),
     "vdi_reset_stats": (_i, [_vp]),
     "vdi_synchronize": (_i, [_vp]),
+    "vdi_time_kernels": (_i, [_vp, _i, _i]),
+    "vdi_kernel_times": (_i, [_vp, ctypes.POINTER(_l), _dp, _dp]),
+    "vdi_copy_probe": (_i, [_l, _i, _dp]),
+    "vdi_device_info": (_i, [_ip, _i]),
     "vdi_state_len": (_l, [_vp]),
     "vdi_download_state": (_i, [_vp, _dp, _l]),
     "vdi_upload_state": (_i, [_vp, _dp, _l]),
@@ -220,6 +228,16 @@ class Fluid2D:
     def reset_stats(self):
         _check(self.lib.vdi_reset_stats(self.h), "reset_stats")
 
+    def time_kernels(self, kclass: int, max_launches: int = 8192):
+        """Times every launch of one kernel class (KCLASS_*) with HIP events."""
+        _check(self.lib.vdi_time_kernels(self.h, kclass, max_launches), "time_kernels")
+
+    def kernel_times(self):
+        """(launches, total ms, algorithmic bytes) of the timed launches."""
+        n, t, b = ctypes.c_long(), ctypes.c_double(), ctypes.c_double()
+        _check(self.lib.vdi_kernel_times(self.h, ctypes.byref(n), ctypes.byref(t), ctypes.byref(b)), "kernel_times")
+        return n.value, t.value, b.value
+
     def download_state(self) -> np.ndarray:
         n = self.lib.vdi_state_len(self.h)
         out = np.empty(n, np.float64)
@@ -229,6 +247,24 @@ class Fluid2D:
     def upload_state(self, state):
         a = np.ascontiguousarray(state, np.float64).ravel()
         _check(self.lib.vdi_upload_state(self.h, _ptr(a), a.size), "upload_state")
+
+
+def copy_probe(n: int, reps: int = 10) -> float:
+    """Average ms of a plain device copy of n doubles (counter calibration)."""
+    lib = load_library()
+    ms = ctypes.c_double()
+    _check(lib.vdi_copy_probe(n, reps, ctypes.byref(ms)), "copy_probe")
+    return ms.value
+
+
+def device_info() -> dict:
+    """LDS per workgroup, CU count and other facts of device 0."""
+    lib = load_library()
+    arr = (ctypes.c_int * 9)()
+    _check(lib.vdi_device_info(arr, 9), "device_info")
+    keys = ["lds_per_block", "cus", "threads_per_block", "wavefront", "clock_khz", "mem_clock_khz",
+            "mem_bus_bits", "lds_per_cu", "l2_bytes"]
+    return dict(zip(keys, list(arr)))
 
 
 class MGSystem:

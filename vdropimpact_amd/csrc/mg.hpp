@@ -10,34 +10,65 @@
 //     rediscretised coarse operators (couplings / 4 per level);
 //   nodal FEM: four-colour Gauss-Seidel, full-weighting restriction (the
 //     transpose of bilinear prolongation) so that the coarse Q1 stiffness is
-//     the Galerkin one, Dirichlet row kept at zero correction.
-//   coarsest level: a fixed number of smoothing sweeps in one workgroup.
+//     the Galerkin one, Dirichlet row kept at zero correction;
+//   coarsest level: a fixed number of smoothing sweeps.
 #pragma once
 
 #include "vdi_internal.hpp"
 
 namespace vdi {
 
+constexpr int kMaxCoarse = 12;  // levels handled by the single-workgroup kernel
+constexpr int kMaxNu = 3;       // largest nu1/nu2 the tiled kernels accept
+
+// Device-side solve state: convergence flag (0 running, 1 converged,
+// 2 NaN, 3 zero right-hand side), V-cycles run, ticket of the last-
+// workgroup reduction, ||b-Az||^2 and ||b||^2.
+struct MGState {
+    int done;
+    int cycles;
+    unsigned ticket;
+    int pad;
+    double rr, bb;
+};
+
+struct MGCtl {
+    MGState *state;
+    double *partial;  // one (or two) doubles per workgroup
+};
+
+// Levels lc..L-1 for the single-workgroup kernel (LDS offsets in doubles;
+// z at off, b at off + m*n).
+struct CoarseDesc {
+    int nl, top;
+    int m[kMaxCoarse], n[kMaxCoarse], off[kMaxCoarse];
+    double cx[kMaxCoarse], cy[kMaxCoarse];
+    double *gz, *gb;
+};
+
 class MG {
   public:
     int kind = VDI_MAC;
     bool fem = false;
-    int nl = 0;
-    std::vector<Level> L;      // FP64 hierarchy
-    std::vector<int> lm, ln;   // level sizes
+    int nl = 0;   // levels
+    int lc = 0;   // first level of the single-workgroup kernel
+    std::vector<Level> L;    // FP64 hierarchy
+    std::vector<LevelF> LF;  // FP32 hierarchy of the mixed form (level 0: e32, r32)
+    std::vector<int> lm, ln;
     CCOp cc{};
     FemOp fe{};
-    int nu1 = 2, nu2 = 2, ncs = 64;
+    int nu1 = 2, nu2 = 2, ncs = 8;
     int form = VDI_FORM_FP64;
     hipStream_t st = nullptr;
 
     // time-extrapolation history (mgs_common.hh:21-38, mgs_common.cc:17-50)
     int scount = 0;
     double t0 = 0, t_prev = 0;
-    double vfx = 0, vfy = 0;   // visco couplings set up (visco_impl.cc:26-30)
+    double vfx = 0, vfy = 0;  // visco couplings set up (visco_impl.cc:26-30)
 
     int last_cycles = 0;
     long total_cycles = 0, solves = 0;
+    KernelTimer *timer = nullptr;  // optional launch timing
 
     // Sets up the hierarchy.  z0/b0 are the level-0 solution and RHS buffers
     // (owned by the caller); when null the MG allocates its own.
@@ -48,16 +79,24 @@ class MG {
     double *z() const { return L[0].z; }
     double *b() const { return L[0].b; }
     double *z_prev() const { return zprev_.as<double>(); }
+    MGCtl ctl() const;
 
-    int vcycle();
-    // Enqueues ||b-Az||^2 and ||b||^2 into the device pair norms_dev();
-    int enqueue_norms();
-    // Synchronous residual norms.
-    int residual(double *rnorm, double *bnorm);
     // FEM Dirichlet row z = b / a_cc on level 0 (no-op for other kinds).
     int enforce_dirichlet();
+    // Enqueues the start of a solve: Dirichlet row, ||b||, ||b - A z||.
+    int begin(double tol);
+    // Enqueues one V-cycle; kernels return at once when the device flag says
+    // the solve has converged (tol2 < 0: never converge).
+    int launch_cycle(double tol2);
+    // Mixed form in use (needs at least one tiled level).
+    bool mixed_active() const;
+    int mixed_finish(int executed);
+    int read_state(MGState *out);
+    // Synchronous residual norms.
+    int residual(double *rnorm, double *bnorm);
     // V-cycles until ||b-Az|| <= tol ||b||; *cycles = V-cycles run.
     int solve(double tol, int max_cycles, int *cycles);
+    // Exactly k V-cycles.
     int vcycles(int k);
     // mgs_common::initial_guess(t)
     int initial_guess(double t);
@@ -65,15 +104,20 @@ class MG {
     int apply(const double *z, double *out);
 
   private:
-    DevBuf store_;   // coarse z/b
+    template <class T> int launch_down(const std::vector<LevelT<T>> &H, int l, int zero_z);
+    template <class T> int launch_up(const std::vector<LevelT<T>> &H, int l, int norm, double tol2);
+    template <class T> int launch_coarse(const std::vector<LevelT<T>> &H, double tol2);
+    DevBuf store_;  // coarse z/b
+    DevBuf store_f_;
+    float *r32_[2] = {nullptr, nullptr};  // FP32 residual, ping-pong
+    int mx_enqueued_ = 0;
     DevBuf zprev_;
     DevBuf partial_;
-    DevBuf norms_;   // 2 doubles
+    DevBuf state_;
     DevBuf own_z_, own_b_;
-    double *host_norms_ = nullptr;  // pinned
-    int nblk_norm_ = 0;
-    int vcycle_level(int l);
-    int sweep(int l, int count);
+    MGState *host_state_ = nullptr;  // pinned
+    size_t coarse_lds_ = 0;
+
   public:
     ~MG();
 };

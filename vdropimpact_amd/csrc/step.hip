@@ -88,149 +88,204 @@ __device__ __forceinline__ void godunov_set_tang(double &u0, double &u1, double 
     u1 = 0;
 }
 
-// ------------------------------------------------------------------------
-// Godunov passes K1..K8 (fluid_2d.cc:105-261)
-// ------------------------------------------------------------------------
-
 #define CELL_IDX                                              \
     int i = blockIdx.x * blockDim.x + threadIdx.x;            \
     int j = blockIdx.y * blockDim.y + threadIdx.y;            \
     if (i >= g.m || j >= g.n) return;
 
-// K1 (fluid_2d.cc:105-132) + zero_edge_velocities (:480-482).
-__global__ void k_slopes(Geo g, Fields F, double dt) {
-    CELL_IDX
-    const double *u = F.u, *v = F.v;
-    double fu = u[IX(i, j)], fv = v[IX(i, j)];
-    double uy = g.ysp * mono_diff(u[IX(i, j - 2)], u[IX(i, j - 1)], fu, u[IX(i, j + 1)], u[IX(i, j + 2)]);
-    double vy = g.ysp * mono_diff(v[IX(i, j - 2)], v[IX(i, j - 1)], fv, v[IX(i, j + 1)], v[IX(i, j + 2)]);
-    double ux = g.xsp * mono_diff(u[IX(i - 2, j)], u[IX(i - 1, j)], fu, u[IX(i + 1, j)], u[IX(i + 2, j)]);
-    double vx = g.xsp * mono_diff(v[IX(i - 2, j)], v[IX(i - 1, j)], fv, v[IX(i + 1, j)], v[IX(i + 2, j)]);
-    size_t c = IX(i, j);
-    F.ux[c] = ux; F.uy[c] = uy; F.vx[c] = vx; F.vy[c] = vy;
-    F.ud[c] = fu - 0.5 * (g.dy + dt * fv) * uy;
-    F.vd[c] = fv - 0.5 * (g.dy + dt * fv) * vy;
-    F.ul[c] = (g.x_sym && i == 0) ? 0.0 : fu - 0.5 * (g.dx + dt * fu) * ux;
-    F.vl[c] = fv - 0.5 * (g.dx + dt * fu) * vx;
-    F.ur[c] = fu + 0.5 * (g.dx - dt * fu) * ux;
-    F.vr[c] = fv + 0.5 * (g.dx - dt * fu) * vx;
-    F.uu[c] = fu + 0.5 * (g.dy - dt * fv) * uy;
-    F.vu[c] = fv + 0.5 * (g.dy - dt * fv) * vy;
+// ------------------------------------------------------------------------
+// Fused Godunov passes K1..K6 on LDS tiles
+// ------------------------------------------------------------------------
+//
+// One workgroup owns kGX x kGY cells.  With T the tile and T+k its ring of
+// width k (clipped to the domain):
+//   K1 (slopes, first extrapolation) on T+2 from u,v on T+4;
+//   post-K2 face states (tangential Godunov select, recomputed for every
+//   face a cell needs) and K3 on T+1;
+//   K4 (full predictor) on T+1;
+//   K5 (normal Godunov select) and K6 (MAC RHS) on T.
+// Only the final face states, the viscous force c0/c1 and src are written:
+// about 10 words of traffic per cell instead of the ~75 of one pass per
+// reference loop.  Boundary faces follow fluid_2d.cc:141-148 and :210-213.
+constexpr int kGX = 32, kGY = 8;
+constexpr int kGW = kGX + 8, kGH = kGY + 8;  // LDS frame: tile + 4 on each side
+
+struct GTile {
+    int x0, y0;
+    __device__ int L(int i, int j) const { return (j - y0 + 4) * kGW + (i - x0 + 4); }
+};
+
+// Tangential Godunov select on face between left state a and right state b
+// (fluid_2d.cc:469-476); returns (u0', u1', v1').
+__device__ __forceinline__ void tang3(double u0, double u1, double v0, double v1, double &ou0, double &ou1,
+                                      double &ov1) {
+    godunov_set_tang(u0, u1, v0, v1);
+    ou0 = u0; ou1 = u1; ov1 = v1;
 }
 
-// K2 (fluid_2d.cc:136-150): tangential Godunov selection on each cell's
-// right and top faces; each face is owned by its left/lower cell.
-__global__ void k_godunov_tang(Geo g, Fields F) {
-    CELL_IDX
-    size_t c = IX(i, j);
-    if (i < g.m - 1) {
-        size_t r = IX(i + 1, j);
-        godunov_set_tang(F.ur[c], F.ul[r], F.vr[c], F.vl[r]);
-        if (i == 0) F.ur[IX(-1, j)] = F.ul[c];
-    } else {
-        F.ul[IX(g.m, j)] = F.ur[c];
-        F.vl[IX(g.m, j)] = F.vr[c];
+__global__ void __launch_bounds__(256) k_godunov(Geo g, Fields F, Gas G, double dt, double viscfac, double macfac) {
+    __shared__ double su[kGW * kGH], sv[kGW * kGH];
+    // K1 edges, later K4 edges: ul, vl, ur, vr, ud, vd, uu, vu
+    __shared__ double se[8][kGW * kGH];
+    __shared__ double sc[4][kGW * kGH];         // c0..c3 of K3
+    __shared__ double sp[(kGX + 3) * (kGY + 3)]; // nodal p on T+1 (+1 node)
+    enum { UL, VL, UR, VR, UD, VD, UU, VU };
+    const int tid = threadIdx.x;
+    GTile t;
+    t.x0 = blockIdx.x * kGX;
+    t.y0 = blockIdx.y * kGY;
+    const int m = g.m, n = g.n;
+    // u, v on T+4 within the padded array
+    for (int k = tid; k < kGW * kGH; k += 256) {
+        const int i = t.x0 - 4 + k % kGW, j = t.y0 - 4 + k / kGW;
+        const bool in = i >= -2 && i < m + 2 && j >= -2 && j < n + 2;
+        su[k] = in ? F.u[IX(i, j)] : 0.0;
+        sv[k] = in ? F.v[IX(i, j)] : 0.0;
     }
-    if (j < g.n - 1) {
-        size_t t = IX(i, j + 1);
-        godunov_set_tang(F.vu[c], F.vd[t], F.uu[c], F.ud[t]);
-        if (j == 0) F.vu[IX(i, -1)] = F.vd[c];
-    } else {
-        F.vd[IX(i, g.n)] = F.vu[c];
-        F.ud[IX(i, g.n)] = F.uu[c];
+    // p on nodes [x0-1, x0+kGX+1] x [y0-1, y0+kGY+1]
+    for (int k = tid; k < (kGX + 3) * (kGY + 3); k += 256) {
+        const int i = t.x0 - 1 + k % (kGX + 3), j = t.y0 - 1 + k / (kGX + 3);
+        const bool in = i >= -2 && i < m + 2 && j >= -2 && j < n + 2;
+        sp[k] = in ? F.p[IX(i, j)] : 0.0;
+    }
+    __syncthreads();
+    // K1 on T+2 (fluid_2d.cc:105-132) + zero_edge_velocities
+    const int ax0 = max(0, t.x0 - 2), ax1 = min(m, t.x0 + kGX + 2);
+    const int ay0 = max(0, t.y0 - 2), ay1 = min(n, t.y0 + kGY + 2);
+    for (int k = tid; k < (kGX + 4) * (kGY + 4); k += 256) {
+        const int i = t.x0 - 2 + k % (kGX + 4), j = t.y0 - 2 + k / (kGX + 4);
+        if (i < ax0 || i >= ax1 || j < ay0 || j >= ay1) continue;
+        const int l = t.L(i, j);
+        const double fu = su[l], fv = sv[l];
+        const double uy = g.ysp * mono_diff(su[l - 2 * kGW], su[l - kGW], fu, su[l + kGW], su[l + 2 * kGW]);
+        const double vy = g.ysp * mono_diff(sv[l - 2 * kGW], sv[l - kGW], fv, sv[l + kGW], sv[l + 2 * kGW]);
+        const double ux = g.xsp * mono_diff(su[l - 2], su[l - 1], fu, su[l + 1], su[l + 2]);
+        const double vx = g.xsp * mono_diff(sv[l - 2], sv[l - 1], fv, sv[l + 1], sv[l + 2]);
+        se[UD][l] = fu - 0.5 * (g.dy + dt * fv) * uy;
+        se[VD][l] = fv - 0.5 * (g.dy + dt * fv) * vy;
+        se[UL][l] = (g.x_sym && i == 0) ? 0.0 : fu - 0.5 * (g.dx + dt * fu) * ux;
+        se[VL][l] = fv - 0.5 * (g.dx + dt * fu) * vx;
+        se[UR][l] = fu + 0.5 * (g.dx - dt * fu) * ux;
+        se[VR][l] = fv + 0.5 * (g.dx - dt * fu) * vx;
+        se[UU][l] = fu + 0.5 * (g.dy - dt * fv) * uy;
+        se[VU][l] = fv + 0.5 * (g.dy - dt * fv) * vy;
+    }
+    __syncthreads();
+    // post-K2 face states and K3 on T+1 (fluid_2d.cc:136-160)
+    const int bx0 = max(0, t.x0 - 1), bx1 = min(m, t.x0 + kGX + 1);
+    const int by0 = max(0, t.y0 - 1), by1 = min(n, t.y0 + kGY + 1);
+    for (int k = tid; k < (kGX + 2) * (kGY + 2); k += 256) {
+        const int i = t.x0 - 1 + k % (kGX + 2), j = t.y0 - 1 + k / (kGX + 2);
+        if (i < bx0 || i >= bx1 || j < by0 || j >= by1) continue;
+        const int l = t.L(i, j);
+        double a, b, c;
+        // x faces: left face (i-1/2) gives UR2(i-1), UL2(i), VL2(i); right face gives UR2(i), UL2(i+1), VL2(i+1)
+        double urm, ul0, vl0, ur0, ulp, vlp;
+        if (i == 0) { ul0 = se[UL][l]; vl0 = se[VL][l]; urm = ul0; }
+        else { tang3(se[UR][l - 1], se[UL][l], se[VR][l - 1], se[VL][l], a, b, c); urm = a; ul0 = b; vl0 = c; }
+        if (i == m - 1) { ur0 = se[UR][l]; ulp = se[UR][l]; vlp = se[VR][l]; }
+        else { tang3(se[UR][l], se[UL][l + 1], se[VR][l], se[VL][l + 1], a, b, c); ur0 = a; ulp = b; vlp = c; }
+        // y faces
+        double vum, vd0, ud0, vu0, vdp, udp;
+        if (j == 0) { vd0 = se[VD][l]; ud0 = se[UD][l]; vum = vd0; }
+        else { tang3(se[VU][l - kGW], se[VD][l], se[UU][l - kGW], se[UD][l], a, b, c); vum = a; vd0 = b; ud0 = c; }
+        if (j == n - 1) { vu0 = se[VU][l]; vdp = se[VU][l]; udp = se[UU][l]; }
+        else { tang3(se[VU][l], se[VD][l + kGW], se[UU][l], se[UD][l + kGW], a, b, c); vu0 = a; vdp = b; udp = c; }
+        const double sx = 0.5 * g.xsp * (ulp + ul0), sy = 0.5 * g.ysp * (vdp + vd0);
+        sc[0][l] = sx * (ur0 - urm);
+        sc[1][l] = sx * (vlp - vl0);
+        sc[2][l] = sy * (udp - ud0);
+        sc[3][l] = sy * (vu0 - vum);
+    }
+    __syncthreads();
+    // K4 on T+1 (fluid_2d.cc:164-201); tile cells also store c0,c1
+    for (int k = tid; k < (kGX + 2) * (kGY + 2); k += 256) {
+        const int i = t.x0 - 1 + k % (kGX + 2), j = t.y0 - 1 + k / (kGX + 2);
+        if (i < bx0 || i >= bx1 || j < by0 || j >= by1) continue;
+        const int l = t.L(i, j);
+        const int q = (j - t.y0 + 1) * (kGX + 3) + (i - t.x0 + 1);
+        const double px = 0.5 * g.xsp * (sp[q + kGX + 4] + sp[q + 1] - sp[q + kGX + 3] - sp[q]);
+        const double py = 0.5 * g.ysp * (sp[q + kGX + 4] - sp[q + 1] + sp[q + kGX + 3] - sp[q]);
+        const double fu = su[l], fv = sv[l];
+        const double uyy = g.yysp * (su[l - kGW] - 2 * fu + su[l + kGW]);
+        const double vyy = g.yysp * (sv[l - kGW] - 2 * fv + sv[l + kGW]);
+        const double uxx = g.xxsp * (su[l - 1] - 2 * fu + su[l + 1]);
+        const double vxx = g.xxsp * (sv[l - 1] - 2 * fv + sv[l + 1]);
+        const double f_x = -px * g.rhoinv + viscfac * (uxx + uyy);
+        const double f_y = -py * g.rhoinv + viscfac * (vxx + vyy);
+        const double fuy = g.ysp * mono_diff(su[l - 2 * kGW], su[l - kGW], fu, su[l + kGW], su[l + 2 * kGW]);
+        const double fvy = g.ysp * mono_diff(sv[l - 2 * kGW], sv[l - kGW], fv, sv[l + kGW], sv[l + 2 * kGW]);
+        const double fux = g.xsp * mono_diff(su[l - 2], su[l - 1], fu, su[l + 1], su[l + 2]);
+        const double fvx = g.xsp * mono_diff(sv[l - 2], sv[l - 1], fv, sv[l + 1], sv[l + 2]);
+        const double fc0 = sc[0][l], fc1 = sc[1][l], fc2 = sc[2][l], fc3 = sc[3][l];
+        const double dx = g.dx, dy = g.dy;
+        se[UD][l] = fu + 0.5 * (-dy * fuy + dt * (-fc0 - fv * fuy + f_x));
+        se[VD][l] = fv + 0.5 * (-dy * fvy + dt * (-fc1 - fv * fvy + f_y));
+        se[UL][l] = (g.x_sym && i == 0) ? 0.0 : fu + 0.5 * (-dx * fux + dt * (-fu * fux - fc2 + f_x));
+        se[VL][l] = fv + 0.5 * (-dx * fvx + dt * (-fu * fvx - fc3 + f_y));
+        se[UR][l] = fu + 0.5 * (dx * fux + dt * (-fu * fux - fc2 + f_x));
+        se[VR][l] = fv + 0.5 * (dx * fvx + dt * (-fu * fvx - fc3 + f_y));
+        se[UU][l] = fu + 0.5 * (dy * fuy + dt * (-fc0 - fv * fuy + f_x));
+        se[VU][l] = fv + 0.5 * (dy * fvy + dt * (-fc1 - fv * fvy + f_y));
+        if (i >= t.x0 && i < t.x0 + kGX && j >= t.y0 && j < t.y0 + kGY) {
+            F.c0[IX(i, j)] = viscfac * (uxx + uyy);
+            F.c1[IX(i, j)] = viscfac * (vxx + vyy);
+        }
+    }
+    __syncthreads();
+    // K5 + K6 on T (fluid_2d.cc:205-232)
+    {
+        const int i = t.x0 + tid % kGX, j = t.y0 + tid / kGX;
+        if (i >= m || j >= n) return;
+        const int l = t.L(i, j);
+        double a0, a1, b0, b1;
+        // face i-1/2
+        double ulL, vlL;
+        if (i == 0) { ulL = se[UL][l]; vlL = se[VL][l]; }
+        else { a0 = se[UR][l - 1]; a1 = se[UL][l]; b0 = se[VR][l - 1]; b1 = se[VL][l]; godunov_set(a0, a1, b0, b1); ulL = a1; vlL = b1; }
+        // face i+1/2
+        double ulR;
+        if (i == m - 1) ulR = se[UR][l];
+        else { a0 = se[UR][l]; a1 = se[UL][l + 1]; b0 = se[VR][l]; b1 = se[VL][l + 1]; godunov_set(a0, a1, b0, b1); ulR = a1; }
+        // face j-1/2
+        double vdB, udB;
+        if (j == 0) { vdB = se[VD][l]; udB = se[UD][l]; }
+        else { a0 = se[VU][l - kGW]; a1 = se[VD][l]; b0 = se[UU][l - kGW]; b1 = se[UD][l]; godunov_set(a0, a1, b0, b1); vdB = a1; udB = b1; }
+        // face j+1/2
+        double vdT;
+        if (j == n - 1) vdT = se[VU][l];
+        else { a0 = se[VU][l]; a1 = se[VD][l + kGW]; b0 = se[UU][l]; b1 = se[UD][l + kGW]; godunov_set(a0, a1, b0, b1); vdT = a1; }
+        const size_t c = IX(i, j);
+        F.ul[c] = ulL; F.vl[c] = vlL; F.vd[c] = vdB; F.ud[c] = udB;
+        if (i == m - 1) { F.ul[IX(m, j)] = se[UR][l]; F.vl[IX(m, j)] = se[VR][l]; }
+        if (j == n - 1) { F.vd[IX(i, n)] = se[VU][l]; F.ud[IX(i, n)] = se[UU][l]; }
+        double s = g.xsp * (ulR - ulL) + g.ysp * (vdT - vdB);
+        if (j == 0) {
+            const double sbc = macfac * (G.p_old[i] + G.p_old[i + 1] + G.p[i] + G.p[i + 1]);
+            F.src[(size_t)m * n + i] = sbc;
+            s -= 2 * g.yysp * sbc;
+        }
+        F.src[i + (size_t)m * j] = s;
     }
 }
 
-// K3 (fluid_2d.cc:153-160)
-__global__ void k_tang_terms(Geo g, Fields F) {
+// K7 + K8 (fluid_2d.cc:236-261): the MAC gradient correction is applied to
+// the four faces of the cell on the fly and the advective update is formed;
+// the corrected faces are not needed afterwards, so they are not stored.
+__global__ void k_correct_advect(Geo g, Fields F, const double *z, double dt) {
     CELL_IDX
-    size_t c = IX(i, j), r = IX(i + 1, j), l = IX(i - 1, j), t = IX(i, j + 1), d = IX(i, j - 1);
-    double sx = 0.5 * g.xsp * (F.ul[r] + F.ul[c]), sy = 0.5 * g.ysp * (F.vd[t] + F.vd[c]);
-    F.c0[c] = sx * (F.ur[c] - F.ur[l]);
-    F.c1[c] = sx * (F.vl[r] - F.vl[c]);
-    F.c2[c] = sy * (F.ud[t] - F.ud[c]);
-    F.c3[c] = sy * (F.vu[c] - F.vu[d]);
-}
-
-// K4 (fluid_2d.cc:164-201, centered_diff :412-417) + zero_edge_velocities.
-__global__ void k_predictor(Geo g, Fields F, double dt, double viscfac) {
-    CELL_IDX
-    const double *u = F.u, *v = F.v, *p = F.p;
-    size_t c = IX(i, j);
-    double px = 0.5 * g.xsp * (p[IX(i + 1, j + 1)] + p[IX(i + 1, j)] - p[IX(i, j + 1)] - p[c]);
-    double py = 0.5 * g.ysp * (p[IX(i + 1, j + 1)] - p[IX(i + 1, j)] + p[IX(i, j + 1)] - p[c]);
-    double fu = u[c], fv = v[c];
-    double uyy = g.yysp * (u[IX(i, j - 1)] - 2 * fu + u[IX(i, j + 1)]);
-    double vyy = g.yysp * (v[IX(i, j - 1)] - 2 * fv + v[IX(i, j + 1)]);
-    double uxx = g.xxsp * (u[IX(i - 1, j)] - 2 * fu + u[IX(i + 1, j)]);
-    double vxx = g.xxsp * (v[IX(i - 1, j)] - 2 * fv + v[IX(i + 1, j)]);
-    double f_x = -px * g.rhoinv + viscfac * (uxx + uyy);
-    double f_y = -py * g.rhoinv + viscfac * (vxx + vyy);
-    double fux = F.ux[c], fuy = F.uy[c], fvx = F.vx[c], fvy = F.vy[c];
-    double fc0 = F.c0[c], fc1 = F.c1[c], fc2 = F.c2[c], fc3 = F.c3[c];
-    double dt_, dx = g.dx, dy = g.dy;
-    dt_ = dt;
-    F.ud[c] = fu + 0.5 * (-dy * fuy + dt_ * (-fc0 - fv * fuy + f_x));
-    F.vd[c] = fv + 0.5 * (-dy * fvy + dt_ * (-fc1 - fv * fvy + f_y));
-    F.ul[c] = (g.x_sym && i == 0) ? 0.0 : fu + 0.5 * (-dx * fux + dt_ * (-fu * fux - fc2 + f_x));
-    F.vl[c] = fv + 0.5 * (-dx * fvx + dt_ * (-fu * fvx - fc3 + f_y));
-    F.ur[c] = fu + 0.5 * (dx * fux + dt_ * (-fu * fux - fc2 + f_x));
-    F.vr[c] = fv + 0.5 * (dx * fvx + dt_ * (-fu * fvx - fc3 + f_y));
-    F.uu[c] = fu + 0.5 * (dy * fuy + dt_ * (-fc0 - fv * fuy + f_x));
-    F.vu[c] = fv + 0.5 * (dy * fvy + dt_ * (-fc1 - fv * fvy + f_y));
-    F.c0[c] = viscfac * (uxx + uyy);
-    F.c1[c] = viscfac * (vxx + vyy);
-}
-
-// K5 (fluid_2d.cc:205-215)
-__global__ void k_godunov_normal(Geo g, Fields F) {
-    CELL_IDX
-    size_t c = IX(i, j);
-    if (i < g.m - 1) {
-        size_t r = IX(i + 1, j);
-        godunov_set(F.ur[c], F.ul[r], F.vr[c], F.vl[r]);
-    } else {
-        F.ul[IX(g.m, j)] = F.ur[c];
-        F.vl[IX(g.m, j)] = F.vr[c];
-    }
-    if (j < g.n - 1) {
-        size_t t = IX(i, j + 1);
-        godunov_set(F.vu[c], F.vd[t], F.uu[c], F.ud[t]);
-    } else {
-        F.vd[IX(i, g.n)] = F.vu[c];
-        F.ud[IX(i, g.n)] = F.uu[c];
-    }
-}
-
-// K6 (fluid_2d.cc:218-232): face divergence, plus the bottom Dirichlet row
-// s_bc = dt/(8 rho) (p_old[i]+p_old[i+1]+p[i]+p[i+1]), src -= 2 yysp s_bc.
-__global__ void k_mac_rhs(Geo g, Fields F, Gas G, double macfac) {
-    CELL_IDX
-    double s = g.xsp * (F.ul[IX(i + 1, j)] - F.ul[IX(i, j)]) + g.ysp * (F.vd[IX(i, j + 1)] - F.vd[IX(i, j)]);
-    if (j == 0) {
-        double sbc = macfac * (G.p_old[i] + G.p_old[i + 1] + G.p[i] + G.p[i + 1]);
-        F.src[(size_t)g.m * g.n + i] = sbc;
-        s -= 2 * g.yysp * sbc;
-    }
-    F.src[i + (size_t)g.m * j] = s;
-}
-
-// K7 (fluid_2d.cc:236-248)
-__global__ void k_mac_correct(Geo g, Fields F, const double *z) {
-    CELL_IDX
-    size_t q = i + (size_t)g.m * j;
-    if (i > 0) F.ul[IX(i, j)] -= (z[q] - z[q - 1]) * g.xsp;
-    if (j != 0) F.vd[IX(i, j)] -= (z[q] - z[q - g.m]) * g.ysp;
-    else F.vd[IX(i, 0)] -= 2 * g.ysp * (z[q] - F.src[(size_t)g.m * g.n + i]);
-}
-
-// K8 (fluid_2d.cc:253-261)
-__global__ void k_advect(Geo g, Fields F, double dt) {
-    CELL_IDX
-    size_t c = IX(i, j), r = IX(i + 1, j), t = IX(i, j + 1);
-    double sx = g.xsp * (F.ul[r] + F.ul[c]), sy = g.ysp * (F.vd[t] + F.vd[c]);
-    F.c0[c] = F.u[c] + dt * (-0.5 * (sx * (F.ul[r] - F.ul[c]) + sy * (F.ud[t] - F.ud[c])) + F.c0[c]);
-    F.c1[c] = F.v[c] + dt * (-0.5 * (sx * (F.vl[r] - F.vl[c]) + sy * (F.vd[t] - F.vd[c])) + F.c1[c]);
+    const int m = g.m, n = g.n;
+    const size_t q = i + (size_t)m * j, c = IX(i, j), r = IX(i + 1, j), t = IX(i, j + 1);
+    const double *sbc = F.src + (size_t)m * n;
+    double ulL = F.ul[c], ulR = F.ul[r], vdB = F.vd[c], vdT = F.vd[t];
+    if (i > 0) ulL -= (z[q] - z[q - 1]) * g.xsp;
+    if (i + 1 < m) ulR -= (z[q + 1] - z[q]) * g.xsp;
+    if (j != 0) vdB -= (z[q] - z[q - m]) * g.ysp;
+    else vdB -= 2 * g.ysp * (z[q] - sbc[i]);
+    if (j + 1 < n) vdT -= (z[q + m] - z[q]) * g.ysp;
+    const double sx = g.xsp * (ulR + ulL), sy = g.ysp * (vdT + vdB);
+    F.c0[c] = F.u[c] + dt * (-0.5 * (sx * (ulR - ulL) + sy * (F.ud[t] - F.ud[c])) + F.c0[c]);
+    F.c1[c] = F.v[c] + dt * (-0.5 * (sx * (F.vl[r] - F.vl[c]) + sy * (vdT - vdB)) + F.c1[c]);
 }
 
 // ------------------------------------------------------------------------
@@ -629,6 +684,10 @@ __global__ void __launch_bounds__(1024) k_gas_init(Geo g, Gas G, GasConst K) {
 
 #undef IX
 
+__global__ void k_copy(const double *__restrict__ a, double *__restrict__ b, long n) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) b[i] = a[i];
+}
+
 }  // namespace vdi
 
 // ------------------------------------------------------------------------
@@ -656,6 +715,7 @@ struct vdi_ctx {
     MG mac, fem, vx, vy;
     int *host_status = nullptr;
     size_t gas_lds = 0;
+    KernelTimer timer;
     ~vdi_ctx() {
         if (host_status) (void)hipHostFree(host_status);
         if (st) (void)hipStreamDestroy(st);
@@ -754,7 +814,9 @@ int vdi_destroy(vdi_ctx *ctx) {
 }
 
 int vdi_set_solver(vdi_ctx *c, int nu1, int nu2, double rel_tol, int max_cycles) {
-    if (!c || nu1 < 0 || nu2 < 0 || nu1 + nu2 == 0 || !(rel_tol > 0) || max_cycles < 1) return VDI_ERR_ARG;
+    if (!c || nu1 < 0 || nu2 < 0 || nu1 + nu2 == 0 || nu1 > kMaxNu || nu2 > kMaxNu || !(rel_tol > 0) ||
+        max_cycles < 1)
+        return VDI_ERR_ARG;
     MG *gs[4] = {&c->mac, &c->fem, &c->vx, &c->vy};
     for (MG *g : gs) { g->nu1 = nu1; g->nu2 = nu2; }
     c->rel_tol = rel_tol;
@@ -763,8 +825,10 @@ int vdi_set_solver(vdi_ctx *c, int nu1, int nu2, double rel_tol, int max_cycles)
 }
 
 int vdi_set_form(vdi_ctx *c, int form) {
-    if (!c || (form != VDI_FORM_FP64)) return VDI_ERR_ARG;
+    if (!c || (form != VDI_FORM_FP64 && form != VDI_FORM_MIXED)) return VDI_ERR_ARG;
     c->form = form;
+    MG *gs[4] = {&c->mac, &c->fem, &c->vx, &c->vy};
+    for (MG *g : gs) g->form = form;
     return VDI_OK;
 }
 
@@ -777,8 +841,11 @@ static int enqueue_viscous_bc(vdi_ctx *c) {
 static int enqueue_inviscid_bc(vdi_ctx *c, double dt) {
     double prefac = c->g.dx * c->g.rhoinv * dt / (3 * M_PI);
     int waves = c->g.n + c->g.ml;
+    const int slot = c->timer.begin(VDI_KCLASS_SIMPSON, c->st);
     hipLaunchKernelGGL(k_simpson, dim3(cdiv(waves, 4)), dim3(256), 0, c->st, c->g, c->F, (const double *)c->G.p, prefac);
     VDI_LAUNCHED();
+    // reads p once; reads and writes two ghost cells of u and v per line
+    c->timer.end(slot, c->st, 8.0 * (c->g.me + 8.0 * waves));
     return VDI_OK;
 }
 
@@ -798,8 +865,12 @@ int vdi_init_fields(vdi_ctx *c) {
 }
 
 static int gas_update(vdi_ctx *c, double dt) {
+    const int slot = c->timer.begin(VDI_KCLASS_GAS, c->st);
     hipLaunchKernelGGL(k_gas, dim3(1), dim3(kGasThreads), c->gas_lds, c->st, c->g, c->F, c->G, c->K, dt, c->Vframe);
     VDI_LAUNCHED();
+    // reads u,v rows 0,-1, h, ht, fbd, pg, p, tau; writes p_old, tau_old, h, ht, fbd, hx, pg, pg_old, p, tau
+    c->timer.end(slot, c->st, 8.0 * (4.0 * c->g.m + 3.0 * c->g.m + 3.0 * c->g.me + 2.0 * c->g.ml + 3.0 * c->g.m +
+                                      4.0 * c->g.me + 2.0 * c->g.ml));
     VDI_HIP(hipMemcpyAsync(c->host_status, c->G.status, 2 * sizeof(int), hipMemcpyDeviceToHost, c->st));
     VDI_HIP(hipStreamSynchronize(c->st));
     c->last_newton = c->host_status[1];
@@ -845,18 +916,13 @@ int vdi_step_forward(vdi_ctx *c, double dt) {
     VDI_TRY(gas_update(c, dt));
     c->time += 0.5 * dt;
     const double viscfac = c->P.implicit_visc ? g.rhoinv * c->P.mu * 0.5 : g.rhoinv * c->P.mu;
-    hipLaunchKernelGGL(k_slopes, gc, kB2, 0, c->st, g, c->F, dt);
-    hipLaunchKernelGGL(k_godunov_tang, gc, kB2, 0, c->st, g, c->F);
-    hipLaunchKernelGGL(k_tang_terms, gc, kB2, 0, c->st, g, c->F);
-    hipLaunchKernelGGL(k_predictor, gc, kB2, 0, c->st, g, c->F, dt, viscfac);
-    hipLaunchKernelGGL(k_godunov_normal, gc, kB2, 0, c->st, g, c->F);
-    hipLaunchKernelGGL(k_mac_rhs, gc, kB2, 0, c->st, g, c->F, c->G, 0.125 * g.rhoinv * dt);
+    hipLaunchKernelGGL(k_godunov, dim3(cdiv(g.m, kGX), cdiv(g.n, kGY)), dim3(256), 0, c->st, g, c->F, c->G, dt, viscfac,
+                       0.125 * g.rhoinv * dt);
     VDI_LAUNCHED();
     int cyc;
     VDI_TRY(solve_mg(c, c->mac, c->time, &cyc));
-    hipLaunchKernelGGL(k_mac_correct, gc, kB2, 0, c->st, g, c->F, (const double *)c->zmac);
     c->time += 0.5 * dt;
-    hipLaunchKernelGGL(k_advect, gc, kB2, 0, c->st, g, c->F, dt);
+    hipLaunchKernelGGL(k_correct_advect, gc, kB2, 0, c->st, g, c->F, (const double *)c->zmac, dt);
     VDI_LAUNCHED();
     VDI_TRY(enqueue_inviscid_bc(c, dt));
     VDI_TRY(enqueue_viscous_bc(c));
@@ -980,6 +1046,45 @@ int vdi_synchronize(vdi_ctx *c) {
     return VDI_OK;
 }
 
+int vdi_time_kernels(vdi_ctx *c, int kclass, int max_launches) {
+    if (!c || max_launches < 0 || kclass < -1 || kclass > VDI_KCLASS_SIMPSON) return VDI_ERR_ARG;
+    VDI_HIP(hipStreamSynchronize(c->st));
+    c->timer.release();
+    c->timer.cls = kclass;
+    MG *gs[4] = {&c->mac, &c->fem, &c->vx, &c->vy};
+    for (MG *g : gs) g->timer = kclass < 0 ? nullptr : &c->timer;
+    if (kclass < 0) return VDI_OK;
+    c->timer.ev.resize(2 * (size_t)max_launches);
+    c->timer.bytes.assign(max_launches, 0.0);
+    for (auto &e : c->timer.ev) VDI_HIP(hipEventCreate(&e));
+    return VDI_OK;
+}
+
+int vdi_kernel_times(vdi_ctx *c, long *launches, double *total_ms, double *alg_bytes) {
+    if (!c || !launches || !total_ms || !alg_bytes) return VDI_ERR_ARG;
+    VDI_HIP(hipStreamSynchronize(c->st));
+    std::vector<float> ms(c->timer.used);
+    for (size_t k = 0; k < c->timer.used; k++)
+        VDI_HIP(hipEventElapsedTime(&ms[k], c->timer.ev[2 * k], c->timer.ev[2 * k + 1]));
+    // Launches that returned at once (issued after their solve had already
+    // converged) move no data; they are left out.
+    std::vector<float> sorted = ms;
+    std::sort(sorted.begin(), sorted.end());
+    const float med = sorted.empty() ? 0.f : sorted[sorted.size() / 2];
+    long n = 0;
+    double t = 0, b = 0;
+    for (size_t k = 0; k < ms.size(); k++) {
+        if (ms[k] < 0.2f * med) continue;
+        n++;
+        t += ms[k];
+        b += c->timer.bytes[k];
+    }
+    *launches = n;
+    *total_ms = t;
+    *alg_bytes = b;
+    return VDI_OK;
+}
+
 long vdi_state_len(vdi_ctx *c) {
     if (!c) return -1;
     const Geo &g = c->g;
@@ -1050,6 +1155,44 @@ struct vdi_mg {
 
 extern "C" {
 
+int vdi_device_info(int *info, int n) {
+    if (!info || n < 1) return VDI_ERR_ARG;
+    const hipDeviceAttribute_t attrs[] = {
+        hipDeviceAttributeMaxSharedMemoryPerBlock, hipDeviceAttributeMultiprocessorCount,
+        hipDeviceAttributeMaxThreadsPerBlock,      hipDeviceAttributeWarpSize,
+        hipDeviceAttributeClockRate,               hipDeviceAttributeMemoryClockRate,
+        hipDeviceAttributeMemoryBusWidth,          hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+        hipDeviceAttributeL2CacheSize};
+    const int na = (int)(sizeof attrs / sizeof attrs[0]);
+    for (int k = 0; k < n; k++) {
+        info[k] = -1;
+        if (k < na) VDI_HIP(hipDeviceGetAttribute(&info[k], attrs[k], 0));
+    }
+    return VDI_OK;
+}
+
+int vdi_copy_probe(long n, int reps, double *ms) {
+    if (n <= 0 || reps <= 0 || !ms) return VDI_ERR_ARG;
+    DevBuf a, b;
+    VDI_TRY(a.alloc(n * sizeof(double)));
+    VDI_TRY(b.alloc(n * sizeof(double)));
+    hipEvent_t e0, e1;
+    VDI_HIP(hipEventCreate(&e0));
+    VDI_HIP(hipEventCreate(&e1));
+    const unsigned nb = (unsigned)std::min<long>(4096, (n + 255) / 256);
+    VDI_HIP(hipEventRecord(e0, nullptr));
+    for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_copy, dim3(nb), dim3(256), 0, nullptr, a.as<double>(), b.as<double>(), n);
+    VDI_LAUNCHED();
+    VDI_HIP(hipEventRecord(e1, nullptr));
+    VDI_HIP(hipEventSynchronize(e1));
+    float t = 0;
+    VDI_HIP(hipEventElapsedTime(&t, e0, e1));
+    *ms = t / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return VDI_OK;
+}
+
 int vdi_mg_create(int kind, int m, int n, double cx, double cy, int x_sym, vdi_mg **out) {
     if (!out || kind < VDI_MAC || kind > VDI_VY || m < 2 || n < 2) return VDI_ERR_ARG;
     *out = nullptr;
@@ -1068,14 +1211,14 @@ int vdi_mg_destroy(vdi_mg *h) {
 }
 
 int vdi_mg_set_schedule(vdi_mg *h, int nu1, int nu2) {
-    if (!h || nu1 < 0 || nu2 < 0 || nu1 + nu2 == 0) return VDI_ERR_ARG;
+    if (!h || nu1 < 0 || nu2 < 0 || nu1 + nu2 == 0 || nu1 > kMaxNu || nu2 > kMaxNu) return VDI_ERR_ARG;
     h->g.nu1 = nu1;
     h->g.nu2 = nu2;
     return VDI_OK;
 }
 
 int vdi_mg_set_form(vdi_mg *h, int form) {
-    if (!h || form != VDI_FORM_FP64) return VDI_ERR_ARG;
+    if (!h || (form != VDI_FORM_FP64 && form != VDI_FORM_MIXED)) return VDI_ERR_ARG;
     h->g.form = form;
     return VDI_OK;
 }

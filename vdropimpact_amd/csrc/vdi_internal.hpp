@@ -78,11 +78,48 @@ struct FemOp {
     double fm, fex, fey, fc, hex, hey;
 };
 
-// One multigrid level on the device.
-struct Level {
-    int m, n;       // unknowns
-    double cx, cy;  // cell-centred couplings
-    double *z, *b;  // solution / right-hand side (level 0: owned elsewhere)
+// One multigrid level on the device, in FP64 or FP32.
+// A level kernel never reads and writes the same array (its workgroups read
+// halos that neighbouring workgroups would otherwise already have updated):
+// the down kernel reads z and writes zalt, the up kernel reads zalt and
+// writes z.
+template <class T>
+struct LevelT {
+    int m, n;    // unknowns
+    T cx, cy;    // cell-centred couplings
+    T inv_int;   // reciprocal of the interior diagonal
+    T *z, *b;    // solution / right-hand side (level 0: owned elsewhere)
+    T *zalt;     // pre-smoothed iterate between the down and up kernels
+};
+using Level = LevelT<double>;
+using LevelF = LevelT<float>;
+
+// Times every launch of one kernel class with HIP events recorded on the
+// stream that runs it, and accumulates the algorithmic bytes of those
+// launches (arrays read + arrays written, each counted once per launch).
+struct KernelTimer {
+    int cls = -1;  // class being timed (VDI_KCLASS_*), -1 = off
+    std::vector<hipEvent_t> ev;
+    std::vector<double> bytes;
+    size_t used = 0;
+    int begin(int c, hipStream_t st) {
+        if (c != cls || 2 * used + 2 > ev.size()) return -1;
+        (void)hipEventRecord(ev[2 * used], st);
+        return (int)used;
+    }
+    void end(int slot, hipStream_t st, double alg_bytes) {
+        if (slot < 0) return;
+        (void)hipEventRecord(ev[2 * slot + 1], st);
+        bytes[slot] = alg_bytes;
+        used++;
+    }
+    void release() {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        ev.clear();
+        bytes.clear();
+        used = 0;
+    }
+    ~KernelTimer() { release(); }
 };
 
 }  // namespace vdi
