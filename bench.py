@@ -55,6 +55,38 @@ def cpu_baseline(m, n, seconds, warmup):
                       f"OMP_NUM_THREADS={threads}"}
 
 
+def operator_level(vdi, reps=3):
+    """Operator-level solves (projection and Helmholtz systems alone): the
+    64-mode Fourier inputs, every solve from zero to relative residual 1e-10.
+    FP64 at 1024x1024 and 2048x1024, mixed FP32/FP64 at 2048x1024 and
+    4096x2048.  GB/s = V-cycles x algorithmic bytes per V-cycle / time."""
+    import numpy as np
+    from vdropimpact_amd import workloads
+    out = []
+    cases = [(form, m, n) for form, grids in ((vdi.FORM_FP64, [(1024, 1024), (2048, 1024)]),
+                                              (vdi.FORM_MIXED, [(2048, 1024), (4096, 2048)])) for m, n in grids]
+    names = {0: "mac", 1: "fem", 2: "helmholtz"}
+    for form, m, n in cases:
+        for kind in (0, 1, 2):
+            M, N, cx, cy, b = workloads.operator_case(kind, m, n)
+            g = vdi.MGSystem(kind, M, N, cx, cy, 1)
+            g.set_form(form)
+            g.set_b(b)
+            zero = np.zeros(M * N)
+            times, cyc = [], 0
+            for _ in range(reps):
+                g.set_z(zero)
+                t0 = time.perf_counter()
+                cyc = g.solve(1e-10, 100)
+                times.append(time.perf_counter() - t0)
+            t = min(times)
+            gbs = cyc * g.cycle_bytes() / t / 1e9
+            out.append({"system": names[kind], "grid": [m, n], "form": "fp64" if form == 0 else "mixed",
+                        "vcycles": cyc, "ms_per_solve": 1e3 * t, "GB_per_s": gbs, "frac_of_8TBps": gbs / 8000.0})
+            g.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -64,6 +96,7 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-operator", action="store_true")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -146,6 +179,9 @@ def main():
         "cpu_baseline": None,
     }
     sim.close()
+    if not args.no_operator:
+        log("operator-level solves ...")
+        out["operator_level"] = operator_level(vdi)
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(args.m, args.n, args.cpu_seconds, 3)

@@ -105,6 +105,11 @@ int vdi_destroy(vdi_ctx *ctx);
 int vdi_set_solver(vdi_ctx *ctx, int nu1, int nu2, double rel_tol, int max_cycles);
 /* Arithmetic form of the V-cycles (VDI_FORM_*). */
 int vdi_set_form(vdi_ctx *ctx, int form);
+/* Capture steps in HIP graphs (default on): after the first two steps each
+ * step is replayed from a graph with predicted V-cycle budgets, checked once
+ * at its end and redone on the synchronising path if a solve fell short.
+ * No reference counterpart (launch mechanics only; results are the same). */
+int vdi_set_graphs(vdi_ctx *ctx, int enable);
 /* fluid_2d::init_fields (fluid_2d_sio.cc:14-35) */
 int vdi_init_fields(vdi_ctx *ctx);
 /* fluid_2d::step_forward(double dt) (fluid_2d.cc:93-405) */
@@ -170,6 +175,8 @@ int vdi_mg_vcycles(vdi_mg *g, int k);
 /* ||b - A z||_2 and ||b||_2 of the device arrays. */
 int vdi_mg_residual(vdi_mg *g, double *rnorm, double *bnorm);
 int vdi_mg_synchronize(vdi_mg *g);
+/* Algorithmic bytes of one V-cycle in the current form (measurement). */
+int vdi_mg_cycle_bytes(vdi_mg *g, double *bytes);
 
 #ifdef __cplusplus
 }

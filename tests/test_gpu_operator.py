@@ -1,0 +1,65 @@
+"""Operator-level solves on the GPU (projection and Helmholtz systems alone):
+the 64-mode Fourier inputs, FP64 and the mixed FP32/FP64 form, against the
+FP64 oracle."""
+import sys
+
+import numpy as np
+import pytest
+
+from vdropimpact_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2])
+@pytest.mark.parametrize("m,n", [(1024, 1024), (2048, 1024)])
+def test_fp64_operator_parity(vdi, oracle, kind, m, n):
+    say("fp64 operator", kind, m, n)
+    M, N, cx, cy, b = workloads.operator_case(kind, m, n)
+    g = vdi.MGSystem(kind, M, N, cx, cy, 1)
+    g.set_b(b)
+    g.set_z(np.zeros(M * N))
+    cyc = g.solve(1e-10, 100)
+    r, bn = g.residual()
+    assert r <= 1e-10 * bn
+    o = oracle.OracleMG(kind, M, N, cx, cy, 1)
+    zo, co = o.solve(b, tol=1e-10)
+    assert abs(cyc - co) <= 1
+    g.solve(1e-12, 100)
+    zt, _ = o.solve(b, zo, tol=1e-12)
+    e = rel(g.get_z(), zt)
+    say("  cycles", cyc, "rel L2 vs oracle", e)
+    assert e < 1e-9
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2])
+@pytest.mark.parametrize("m,n", [(2048, 1024), (4096, 2048)])
+def test_mixed_operator_error(vdi, oracle, kind, m, n):
+    """FP32 inner cycles, FP64 residual and update: converges to the same FP64
+    relative residual 1e-10; the solution error against the FP64 oracle's
+    tight solve is of the size the stopping rule allows."""
+    say("mixed operator", kind, m, n)
+    M, N, cx, cy, b = workloads.operator_case(kind, m, n)
+    g = vdi.MGSystem(kind, M, N, cx, cy, 1)
+    g.set_form(vdi.FORM_MIXED)
+    g.set_b(b)
+    g.set_z(np.zeros(M * N))
+    cyc = g.solve(1e-10, 100)
+    r, bn = g.residual()
+    assert r <= 1e-10 * bn
+    o = oracle.OracleMG(kind, M, N, cx, cy, 1)
+    zo, co = o.solve(b, tol=1e-12)
+    z = g.get_z()
+    e, emax = rel(z, zo), np.abs(z - zo).max() / np.abs(zo).max()
+    say("  cycles", cyc, "fp64 cycles", co, "rel L2", e, "max", emax)
+    assert cyc <= co + 2
+    assert e < 1e-7

@@ -64,6 +64,7 @@ ABI = {
     "vdi_destroy": (_i, [_vp]),
     "vdi_set_solver": (_i, [_vp, _i, _i, _d, _i]),
     "vdi_set_form": (_i, [_vp, _i]),
+    "vdi_set_graphs": (_i, [_vp, _i]),
     "vdi_init_fields": (_i, [_vp]),
     "vdi_step_forward": (_i, [_vp, _d]),
     "vdi_gas_layer_update": (_i, [_vp, _d, _d]),
@@ -97,6 +98,7 @@ ABI = {
     "vdi_mg_vcycles": (_i, [_vp, _i]),
     "vdi_mg_residual": (_i, [_vp, _dp, _dp]),
     "vdi_mg_synchronize": (_i, [_vp]),
+    "vdi_mg_cycle_bytes": (_i, [_vp, _dp]),
 }
 
 _lib = None
@@ -171,6 +173,9 @@ class Fluid2D:
 
     def set_form(self, form: int):
         _check(self.lib.vdi_set_form(self.h, form), "vdi_set_form")
+
+    def set_graphs(self, enable: bool):
+        _check(self.lib.vdi_set_graphs(self.h, int(enable)), "vdi_set_graphs")
 
     def init_fields(self):
         _check(self.lib.vdi_init_fields(self.h), "init_fields")
@@ -332,6 +337,11 @@ class MGSystem:
 
     def synchronize(self):
         _check(self.lib.vdi_mg_synchronize(self.h), "vdi_mg_synchronize")
+
+    def cycle_bytes(self) -> float:
+        b = ctypes.c_double()
+        _check(self.lib.vdi_mg_cycle_bytes(self.h, ctypes.byref(b)), "vdi_mg_cycle_bytes")
+        return b.value
 
 
 __all__ = [

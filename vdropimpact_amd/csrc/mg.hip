@@ -667,6 +667,27 @@ __global__ void k_extrapolate(double *z, double *zp, long N, double bb, double c
     zp[ij] = zz;
 }
 
+// Same update with the coefficients (b, c) read from device memory, so a
+// captured step graph can be replayed with new times.
+__global__ void k_extrapolate_dev(double *z, double *zp, long N, const double *coef) {
+    long ij = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (ij >= N) return;
+    const double bb = coef[0], c = coef[1];
+    double zz = z[ij];
+    z[ij] = bb * zz - c * zp[ij];
+    zp[ij] = zz;
+}
+
+// End of a solve whose cycle count is only known on the device: the mixed
+// form leaves its iterate in zalt after an odd number of cycles, and a zero
+// right-hand side means z = 0.
+__global__ void k_solve_end(const MGState *st, double *z, const double *zalt, long N, int mixed) {
+    long ij = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (ij >= N) return;
+    if (st->done == 3) z[ij] = 0;
+    else if (mixed && (st->cycles & 1)) z[ij] = zalt[ij];
+}
+
 __global__ void k_zero(double *z, long N) {
     long ij = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (ij < N) z[ij] = 0;
@@ -929,6 +950,22 @@ int MG::mixed_finish(int executed) {
     return VDI_OK;
 }
 
+double MG::cycle_bytes() const {
+    auto N = [&](int l) { return (double)lm[l] * ln[l]; };
+    double b = 0;
+    if (!mixed_active()) {
+        for (int l = 0; l < lc; l++) b += 8.0 * ((l == 0 ? 3.0 : 2.0) * N(l) + N(l + 1));
+        b += 8.0 * (lc == 0 ? 3.0 : 2.0) * N(lc);
+        for (int l = 0; l < lc; l++) b += 8.0 * (3.0 * N(l) + N(l + 1));
+    } else {
+        for (int l = 0; l < lc; l++) b += 4.0 * (2.0 * N(l) + N(l + 1));
+        b += 4.0 * 2.0 * N(lc);
+        for (int l = 1; l < lc; l++) b += 4.0 * (3.0 * N(l) + N(l + 1));
+        b += 4.0 * (3.0 * N(0) + N(1)) + 8.0 * 3.0 * N(0);
+    }
+    return b;
+}
+
 int MG::enforce_dirichlet() {
     if (!fem) return VDI_OK;
     hipLaunchKernelGGL(k_fem_dirichlet, dim3(cdiv(L[0].m, 256)), dim3(256), 0, st, L[0], fe);
@@ -1025,6 +1062,42 @@ int MG::initial_guess(double t) {
     VDI_LAUNCHED();
     t_prev = t0;
     t0 = t;
+    return VDI_OK;
+}
+
+int MG::guess_coeffs(double t, double *bb, double *c) {
+    if (scount < 2) {
+        if (scount == 1) t0 = t;
+        else t_prev = t;
+        scount++;
+        return 0;  // the caller copies z to z_prev when scount was 1
+    }
+    if (std::fabs(t0 - t_prev) < std::numeric_limits<double>::epsilon() * 10) {
+        fputs("Multigrid routine called for duplicate times\n", stderr);
+        return -1;
+    }
+    double a = 1. / (t0 - t_prev);
+    *bb = (t - t_prev) * a;
+    *c = (t - t0) * a;
+    t_prev = t0;
+    t0 = t;
+    return 1;
+}
+
+int MG::enqueue_extrapolate(const double *coef) {
+    const long N = unknowns();
+    hipLaunchKernelGGL(k_extrapolate_dev, dim3(cdiv(N, 256)), dim3(256), 0, st, L[0].z, zprev_.as<double>(), N, coef);
+    VDI_LAUNCHED();
+    return VDI_OK;
+}
+
+int MG::enqueue_fixed(double tol, int budget) {
+    VDI_TRY(begin(tol));
+    for (int c = 0; c < budget; c++) VDI_TRY(launch_cycle(tol * tol));
+    const long N = unknowns();
+    hipLaunchKernelGGL(k_solve_end, dim3(cdiv(N, 256)), dim3(256), 0, st, (const MGState *)state_.as<MGState>(),
+                       L[0].z, (const double *)L[0].zalt, N, (int)mixed_active());
+    VDI_LAUNCHED();
     return VDI_OK;
 }
 

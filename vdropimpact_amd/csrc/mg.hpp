@@ -91,6 +91,8 @@ class MG {
     // Mixed form in use (needs at least one tiled level).
     bool mixed_active() const;
     int mixed_finish(int executed);
+    // Algorithmic bytes of one V-cycle (arrays read + written per kernel).
+    double cycle_bytes() const;
     int read_state(MGState *out);
     // Synchronous residual norms.
     int residual(double *rnorm, double *bnorm);
@@ -100,6 +102,16 @@ class MG {
     int vcycles(int k);
     // mgs_common::initial_guess(t)
     int initial_guess(double t);
+    // Host half of initial_guess for captured steps: updates the time
+    // history and returns 1 with the extrapolation coefficients, 0 while the
+    // history is filling (scount < 2), -1 for duplicate times.
+    int guess_coeffs(double t, double *bb, double *c);
+    // Device half: extrapolation with coefficients read from device memory.
+    int enqueue_extrapolate(const double *coef);
+    // A solve with a fixed cycle budget and no host synchronisation; the
+    // outcome is in state() afterwards.
+    int enqueue_fixed(double tol, int budget);
+    MGState *state() const { return state_.as<MGState>(); }
     // out = A z for level-0 device arrays
     int apply(const double *z, double *out);
 

@@ -684,6 +684,17 @@ __global__ void __launch_bounds__(1024) k_gas_init(Geo g, Gas G, GasConst K) {
 
 #undef IX
 
+// Gathers the four solve states and the gas-layer status for one readback.
+__global__ void k_gather_status(const MGState *s0, const MGState *s1, const MGState *s2, const MGState *s3,
+                                const int *gas, MGState *out_st, int *out_gas) {
+    if (threadIdx.x != 0) return;
+    const MGState *src[4] = {s0, s1, s2, s3};
+    for (int k = 0; k < 4; k++)
+        if (src[k]) out_st[k] = *src[k];
+    out_gas[0] = gas[0];
+    out_gas[1] = gas[1];
+}
+
 __global__ void k_copy(const double *__restrict__ a, double *__restrict__ b, long n) {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) b[i] = a[i];
 }
@@ -716,8 +727,36 @@ struct vdi_ctx {
     int *host_status = nullptr;
     size_t gas_lds = 0;
     KernelTimer timer;
+    // Captured steps (one graph per cycle budget, time step and form).
+    struct GraphEntry {
+        int budget[4];
+        double dt;
+        int form;
+        hipGraphExec_t exec;
+        std::vector<KernelTimer::Slot> slots;
+    };
+    struct Readback {
+        MGState st[4];
+        int gas[2];
+        int pad[2];
+    };
+    int use_graphs = 1;
+    int budget[4] = {0, 0, 0, 0};
+    std::vector<GraphEntry> graphs;
+    DevBuf snap, coef_dev, rb_dev;
+    double *coef_host = nullptr;  // pinned, 2 extrapolation coefficients per solver
+    Readback *rb = nullptr;       // pinned
+    long graph_steps = 0, redo_steps = 0;
+    void drop_graphs() {
+        for (auto &e : graphs) (void)hipGraphExecDestroy(e.exec);
+        graphs.clear();
+    }
     ~vdi_ctx() {
+        if (st) (void)hipStreamSynchronize(st);
+        drop_graphs();
         if (host_status) (void)hipHostFree(host_status);
+        if (coef_host) (void)hipHostFree(coef_host);
+        if (rb) (void)hipHostFree(rb);
         if (st) (void)hipStreamDestroy(st);
     }
 };
@@ -799,6 +838,11 @@ int vdi_create(const vdi_params *p, vdi_ctx **out) {
         VDI_TRY(c->vx.init(VDI_VX, g.m, g.n, 0, 0, g.x_sym, c->zvx, c->F.src, c->st));
         VDI_TRY(c->vy.init(VDI_VY, g.m, g.n, 0, 0, g.x_sym, c->zvy, c->F.src, c->st));
     }
+    VDI_TRY(c->snap.alloc((3 * NP + ng) * sizeof(double)));
+    VDI_TRY(c->coef_dev.alloc(8 * sizeof(double)));
+    VDI_TRY(c->rb_dev.alloc(sizeof(vdi_ctx::Readback)));
+    VDI_HIP(hipHostMalloc((void **)&c->coef_host, 8 * sizeof(double), hipHostMallocDefault));
+    VDI_HIP(hipHostMalloc((void **)&c->rb, sizeof(vdi_ctx::Readback), hipHostMallocDefault));
     c->gas_lds = std::max<size_t>(3 * (size_t)g.m, 4 * (size_t)g.me) * sizeof(double);
     if (c->gas_lds > 64 * 1024)
         VDI_HIP(hipFuncSetAttribute((const void *)k_gas, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->gas_lds));
@@ -817,6 +861,8 @@ int vdi_set_solver(vdi_ctx *c, int nu1, int nu2, double rel_tol, int max_cycles)
     if (!c || nu1 < 0 || nu2 < 0 || nu1 + nu2 == 0 || nu1 > kMaxNu || nu2 > kMaxNu || !(rel_tol > 0) ||
         max_cycles < 1)
         return VDI_ERR_ARG;
+    VDI_HIP(hipStreamSynchronize(c->st));
+    c->drop_graphs();
     MG *gs[4] = {&c->mac, &c->fem, &c->vx, &c->vy};
     for (MG *g : gs) { g->nu1 = nu1; g->nu2 = nu2; }
     c->rel_tol = rel_tol;
@@ -826,6 +872,7 @@ int vdi_set_solver(vdi_ctx *c, int nu1, int nu2, double rel_tol, int max_cycles)
 
 int vdi_set_form(vdi_ctx *c, int form) {
     if (!c || (form != VDI_FORM_FP64 && form != VDI_FORM_MIXED)) return VDI_ERR_ARG;
+    VDI_HIP(hipStreamSynchronize(c->st));
     c->form = form;
     MG *gs[4] = {&c->mac, &c->fem, &c->vx, &c->vy};
     for (MG *g : gs) g->form = form;
@@ -864,13 +911,18 @@ int vdi_init_fields(vdi_ctx *c) {
     return VDI_OK;
 }
 
-static int gas_update(vdi_ctx *c, double dt) {
+static int enqueue_gas(vdi_ctx *c, double dt) {
     const int slot = c->timer.begin(VDI_KCLASS_GAS, c->st);
     hipLaunchKernelGGL(k_gas, dim3(1), dim3(kGasThreads), c->gas_lds, c->st, c->g, c->F, c->G, c->K, dt, c->Vframe);
     VDI_LAUNCHED();
     // reads u,v rows 0,-1, h, ht, fbd, pg, p, tau; writes p_old, tau_old, h, ht, fbd, hx, pg, pg_old, p, tau
     c->timer.end(slot, c->st, 8.0 * (4.0 * c->g.m + 3.0 * c->g.m + 3.0 * c->g.me + 2.0 * c->g.ml + 3.0 * c->g.m +
                                       4.0 * c->g.me + 2.0 * c->g.ml));
+    return VDI_OK;
+}
+
+static int gas_update(vdi_ctx *c, double dt) {
+    VDI_TRY(enqueue_gas(c, dt));
     VDI_HIP(hipMemcpyAsync(c->host_status, c->G.status, 2 * sizeof(int), hipMemcpyDeviceToHost, c->st));
     VDI_HIP(hipStreamSynchronize(c->st));
     c->last_newton = c->host_status[1];
@@ -893,58 +945,219 @@ int vdi_mg_solve_v_cycle(vdi_ctx *c, int which, double t, int *vcycles) {
     return solve_mg(c, which == VDI_MAC ? c->mac : c->fem, t, vcycles);
 }
 
-static int visco_solve(vdi_ctx *c, MG &g, double vfx, double vfy, double t, int *cyc) {
-    // visco_impl.cc:21-39
+// visco_impl.cc:21-39: re-set the hierarchy when the coupling changed.
+static void visco_setup(MG &g, double vfx, double vfy) {
     if (std::fabs(vfx - g.vfx) > 1e2 * std::numeric_limits<double>::epsilon() * g.vfx) {
         g.vfx = vfx;
         g.vfy = vfy;
         g.set_coeffs(vfx, vfy);
     }
-    return solve_mg(c, g, t, cyc);
 }
 
 int vdi_visco_setup_and_solve(vdi_ctx *c, int which, double vfx, double vfy, double t, int *vcycles) {
     if (!c || !c->P.implicit_visc || (which != VDI_VX && which != VDI_VY)) return VDI_ERR_ARG;
-    return visco_solve(c, which == VDI_VX ? c->vx : c->vy, vfx, vfy, t, vcycles);
+    MG &g = which == VDI_VX ? c->vx : c->vy;
+    visco_setup(g, vfx, vfy);
+    return solve_mg(c, g, t, vcycles);
 }
 
-int vdi_step_forward(vdi_ctx *c, double dt) {
-    if (!c || !(dt > 0)) return VDI_ERR_ARG;
+// Solves in step order: MAC, VX, VY, FEM.
+static MG *step_solver(vdi_ctx *c, int k) {
+    MG *gs[4] = {&c->mac, &c->vx, &c->vy, &c->fem};
+    return gs[k];
+}
+
+// How a solve inside the step body is carried out.
+enum SolveMode {
+    SOLVE_DIRECT,  // initial guess + solve with host synchronisation
+    SOLVE_AGAIN,   // solve from the current z (redo of a step), no extrapolation
+    SOLVE_FIXED    // device-coefficient extrapolation + fixed budget, no host synchronisation
+};
+
+static int body_solve(vdi_ctx *c, int k, double t, SolveMode mode) {
+    MG &g = *step_solver(c, k);
+    if (mode == SOLVE_FIXED) {
+        VDI_TRY(g.enqueue_extrapolate(c->coef_dev.as<double>() + 2 * k));
+        return g.enqueue_fixed(c->rel_tol, c->budget[k]);
+    }
+    int cyc = 0;
+    if (mode == SOLVE_DIRECT) VDI_TRY(g.initial_guess(t));
+    VDI_TRY(g.solve(c->rel_tol, c->max_cycles, &cyc));
+    c->budget[k] = std::max(1, cyc);
+    return VDI_OK;
+}
+
+// fluid_2d::step_forward (fluid_2d.cc:93-405) from the Godunov passes on; the
+// gas layer has been enqueued already.  Host times: t0 at the start of the step.
+static int enqueue_body(vdi_ctx *c, double dt, double t0, SolveMode mode) {
     const Geo &g = c->g;
     const dim3 gc = cells_grid(g), gn = nodes_grid(g);
-    // fluid_2d.cc:99-102
-    VDI_TRY(gas_update(c, dt));
-    c->time += 0.5 * dt;
     const double viscfac = c->P.implicit_visc ? g.rhoinv * c->P.mu * 0.5 : g.rhoinv * c->P.mu;
     hipLaunchKernelGGL(k_godunov, dim3(cdiv(g.m, kGX), cdiv(g.n, kGY)), dim3(256), 0, c->st, g, c->F, c->G, dt, viscfac,
                        0.125 * g.rhoinv * dt);
     VDI_LAUNCHED();
-    int cyc;
-    VDI_TRY(solve_mg(c, c->mac, c->time, &cyc));
-    c->time += 0.5 * dt;
+    VDI_TRY(body_solve(c, 0, t0 + 0.5 * dt, mode));
     hipLaunchKernelGGL(k_correct_advect, gc, kB2, 0, c->st, g, c->F, (const double *)c->zmac, dt);
     VDI_LAUNCHED();
     VDI_TRY(enqueue_inviscid_bc(c, dt));
     VDI_TRY(enqueue_viscous_bc(c));
+    const double t1 = t0 + dt;
     if (c->P.implicit_visc) {
         const double vfx = viscfac * g.xxsp * dt, vfy = viscfac * g.yysp * dt;
         hipLaunchKernelGGL(k_visco_rhs, gc, kB2, 0, c->st, g, c->F, c->G, vfx, vfy, 0);
         VDI_LAUNCHED();
-        VDI_TRY(visco_solve(c, c->vx, vfx, vfy, c->time, &cyc));
+        visco_setup(c->vx, vfx, vfy);
+        VDI_TRY(body_solve(c, 1, t1, mode));
         hipLaunchKernelGGL(k_visco_rhs, gc, kB2, 0, c->st, g, c->F, c->G, vfx, vfy, 1);
         VDI_LAUNCHED();
-        VDI_TRY(visco_solve(c, c->vy, vfx, vfy, c->time, &cyc));
+        visco_setup(c->vy, vfx, vfy);
+        VDI_TRY(body_solve(c, 2, t1, mode));
         hipLaunchKernelGGL(k_visco_copy, gc, kB2, 0, c->st, g, c->F, (const double *)c->zvx, (const double *)c->zvy);
         VDI_LAUNCHED();
     }
     hipLaunchKernelGGL(k_fem_rhs, gn, kB2, 0, c->st, g, c->F, (const double *)c->G.p, g.rho / dt, c->fem.fe.fm);
     VDI_LAUNCHED();
-    VDI_TRY(solve_mg(c, c->fem, c->time, &cyc));
+    VDI_TRY(body_solve(c, 3, t1, mode));
     hipLaunchKernelGGL(k_update, gn, kB2, 0, c->st, g, c->F, (const double *)c->zfem, dt);
     VDI_LAUNCHED();
     VDI_TRY(enqueue_viscous_bc(c));
-    c->steps++;
     return VDI_OK;
+}
+
+// One step with host synchronisation after the gas layer and inside every
+// solve (the path of the first steps, and of a redo).
+static int step_direct(vdi_ctx *c, double dt, SolveMode mode) {
+    const double t0 = c->time;
+    VDI_TRY(gas_update(c, dt));
+    VDI_TRY(enqueue_body(c, dt, t0, mode));
+    c->time = t0 + dt;
+    if (c->timer.cls >= 0) {
+        VDI_HIP(hipStreamSynchronize(c->st));
+        c->timer.harvest_direct();
+    }
+    return VDI_OK;
+}
+
+static bool graph_ready(vdi_ctx *c) {
+    if (!c->use_graphs) return false;
+    for (int k = 0; k < 4; k++) {
+        MG *g = step_solver(c, k);
+        if ((k == 1 || k == 2) && !c->P.implicit_visc) continue;
+        if (g->scount < 2 || c->budget[k] < 1) return false;
+    }
+    return true;
+}
+
+static int capture_step(vdi_ctx *c, double dt, vdi_ctx::GraphEntry &e) {
+    const size_t NP = (size_t)c->g.ml * (c->g.n + 4);
+    const size_t ng = c->gasbuf.bytes / sizeof(double);
+    c->timer.pending.clear();
+    hipGraph_t graph;
+    VDI_HIP(hipStreamBeginCapture(c->st, hipStreamCaptureModeRelaxed));
+    int rc = VDI_OK;
+    do {
+        // extrapolation coefficients of this step, written by the host
+        if (hipMemcpyAsync(c->coef_dev.p, c->coef_host, 8 * sizeof(double), hipMemcpyHostToDevice, c->st) !=
+            hipSuccess) { rc = VDI_ERR_HIP; break; }
+        // snapshot for a redo: u, v, p are the first three field arrays
+        hipLaunchKernelGGL(k_copy, dim3(2048), dim3(256), 0, c->st, (const double *)c->F.u, c->snap.as<double>(),
+                           (long)(3 * NP));
+        hipLaunchKernelGGL(k_copy, dim3(cdiv(ng, 256)), dim3(256), 0, c->st, (const double *)c->gasbuf.as<double>(),
+                           c->snap.as<double>() + 3 * NP, (long)ng);
+        if ((rc = enqueue_gas(c, dt)) != VDI_OK) break;
+        if ((rc = enqueue_body(c, dt, c->time, SOLVE_FIXED)) != VDI_OK) break;
+        vdi_ctx::Readback *rbd = c->rb_dev.as<vdi_ctx::Readback>();
+        const bool iv = c->P.implicit_visc;
+        hipLaunchKernelGGL(k_gather_status, dim3(1), dim3(64), 0, c->st, (const MGState *)c->mac.state(),
+                           iv ? (const MGState *)c->vx.state() : nullptr, iv ? (const MGState *)c->vy.state() : nullptr,
+                           (const MGState *)c->fem.state(), (const int *)c->G.status, rbd->st, rbd->gas);
+        if (hipMemcpyAsync(c->rb, rbd, sizeof(vdi_ctx::Readback), hipMemcpyDeviceToHost, c->st) != hipSuccess) {
+            rc = VDI_ERR_HIP;
+            break;
+        }
+    } while (0);
+    hipError_t ec = hipStreamEndCapture(c->st, &graph);
+    if (rc != VDI_OK) return rc;
+    VDI_HIP(ec);
+    hipError_t ei = hipGraphInstantiate(&e.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    VDI_HIP(ei);
+    e.slots = c->timer.pending;
+    c->timer.pending.clear();
+    c->timer.reserved = c->timer.next;
+    return VDI_OK;
+}
+
+// A captured step: every solve gets its predicted cycle budget (the cycles
+// the previous solve of that system needed) and convergence is only checked
+// at the end.  If a solve fell short, the state is restored from the
+// snapshot the graph took and the step is run again on the direct path.
+static int step_graph(vdi_ctx *c, double dt) {
+    const double t0 = c->time;
+    // host half of the initial guesses (MAC at t+dt/2, the others at t+dt)
+    const double ts[4] = {t0 + 0.5 * dt, t0 + dt, t0 + dt, t0 + dt};
+    for (int k = 0; k < 4; k++) {
+        if ((k == 1 || k == 2) && !c->P.implicit_visc) { c->coef_host[2 * k] = 1; c->coef_host[2 * k + 1] = 0; continue; }
+        double bb = 1, cc = 0;
+        if (step_solver(c, k)->guess_coeffs(ts[k], &bb, &cc) != 1) { bb = 1; cc = 0; }
+        c->coef_host[2 * k] = bb;
+        c->coef_host[2 * k + 1] = cc;
+    }
+    vdi_ctx::GraphEntry *e = nullptr;
+    for (auto &x : c->graphs)
+        if (x.dt == dt && x.form == c->form && std::equal(x.budget, x.budget + 4, c->budget)) e = &x;
+    if (!e) {
+        vdi_ctx::GraphEntry ne{};
+        std::copy(c->budget, c->budget + 4, ne.budget);
+        ne.dt = dt;
+        ne.form = c->form;
+        VDI_TRY(capture_step(c, dt, ne));
+        c->graphs.push_back(ne);
+        e = &c->graphs.back();
+    }
+    VDI_HIP(hipGraphLaunch(e->exec, c->st));
+    VDI_HIP(hipStreamSynchronize(c->st));
+    if (c->timer.cls >= 0) c->timer.harvest(e->slots);
+    c->graph_steps++;
+    const vdi_ctx::Readback &r = *c->rb;
+    c->last_newton = r.gas[1];
+    if (r.gas[0] != VDI_OK) return r.gas[0];
+    bool ok = true;
+    for (int k = 0; k < 4; k++) {
+        if ((k == 1 || k == 2) && !c->P.implicit_visc) continue;
+        MG &g = *step_solver(c, k);
+        const MGState &st = r.st[k];
+        g.total_cycles += st.cycles;
+        if (st.done == 1 || st.done == 3) {
+            g.last_cycles = st.cycles;
+            g.solves++;
+            c->budget[k] = std::max(1, st.cycles);
+        } else {
+            ok = false;
+        }
+    }
+    if (ok) {
+        c->time = t0 + dt;
+        return VDI_OK;
+    }
+    // restore the state the step started from and solve again from the
+    // partially converged iterates
+    const size_t NP = (size_t)c->g.ml * (c->g.n + 4);
+    const size_t ng = c->gasbuf.bytes / sizeof(double);
+    hipLaunchKernelGGL(k_copy, dim3(2048), dim3(256), 0, c->st, (const double *)c->snap.as<double>(), c->F.u, (long)(3 * NP));
+    hipLaunchKernelGGL(k_copy, dim3(cdiv(ng, 256)), dim3(256), 0, c->st, (const double *)(c->snap.as<double>() + 3 * NP),
+                       c->gasbuf.as<double>(), (long)ng);
+    VDI_LAUNCHED();
+    c->redo_steps++;
+    c->time = t0;
+    return step_direct(c, dt, SOLVE_AGAIN);
+}
+
+int vdi_step_forward(vdi_ctx *c, double dt) {
+    if (!c || !(dt > 0)) return VDI_ERR_ARG;
+    int rc = graph_ready(c) ? step_graph(c, dt) : step_direct(c, dt, SOLVE_DIRECT);
+    if (rc == VDI_OK) c->steps++;
+    return rc;
 }
 
 static double *array_ptr(vdi_ctx *c, int id, long *len) {
@@ -1049,13 +1262,13 @@ int vdi_synchronize(vdi_ctx *c) {
 int vdi_time_kernels(vdi_ctx *c, int kclass, int max_launches) {
     if (!c || max_launches < 0 || kclass < -1 || kclass > VDI_KCLASS_SIMPSON) return VDI_ERR_ARG;
     VDI_HIP(hipStreamSynchronize(c->st));
+    c->drop_graphs();  // captured graphs hold the previous timer's events
     c->timer.release();
     c->timer.cls = kclass;
     MG *gs[4] = {&c->mac, &c->fem, &c->vx, &c->vy};
     for (MG *g : gs) g->timer = kclass < 0 ? nullptr : &c->timer;
     if (kclass < 0) return VDI_OK;
     c->timer.ev.resize(2 * (size_t)max_launches);
-    c->timer.bytes.assign(max_launches, 0.0);
     for (auto &e : c->timer.ev) VDI_HIP(hipEventCreate(&e));
     return VDI_OK;
 }
@@ -1063,9 +1276,7 @@ int vdi_time_kernels(vdi_ctx *c, int kclass, int max_launches) {
 int vdi_kernel_times(vdi_ctx *c, long *launches, double *total_ms, double *alg_bytes) {
     if (!c || !launches || !total_ms || !alg_bytes) return VDI_ERR_ARG;
     VDI_HIP(hipStreamSynchronize(c->st));
-    std::vector<float> ms(c->timer.used);
-    for (size_t k = 0; k < c->timer.used; k++)
-        VDI_HIP(hipEventElapsedTime(&ms[k], c->timer.ev[2 * k], c->timer.ev[2 * k + 1]));
+    const std::vector<float> &ms = c->timer.dur_ms;
     // Launches that returned at once (issued after their solve had already
     // converged) move no data; they are left out.
     std::vector<float> sorted = ms;
@@ -1082,6 +1293,14 @@ int vdi_kernel_times(vdi_ctx *c, long *launches, double *total_ms, double *alg_b
     *launches = n;
     *total_ms = t;
     *alg_bytes = b;
+    return VDI_OK;
+}
+
+int vdi_set_graphs(vdi_ctx *c, int enable) {
+    if (!c) return VDI_ERR_ARG;
+    VDI_HIP(hipStreamSynchronize(c->st));
+    c->use_graphs = enable ? 1 : 0;
+    if (!enable) c->drop_graphs();
     return VDI_OK;
 }
 
@@ -1271,6 +1490,12 @@ int vdi_mg_vcycles(vdi_mg *h, int k) {
 int vdi_mg_residual(vdi_mg *h, double *rnorm, double *bnorm) {
     if (!h || !rnorm || !bnorm) return VDI_ERR_ARG;
     return h->g.residual(rnorm, bnorm);
+}
+
+int vdi_mg_cycle_bytes(vdi_mg *h, double *bytes) {
+    if (!h || !bytes) return VDI_ERR_ARG;
+    *bytes = h->g.cycle_bytes();
+    return VDI_OK;
 }
 
 int vdi_mg_synchronize(vdi_mg *h) {

@@ -95,29 +95,54 @@ using Level = LevelT<double>;
 using LevelF = LevelT<float>;
 
 // Times every launch of one kernel class with HIP events recorded on the
-// stream that runs it, and accumulates the algorithmic bytes of those
-// launches (arrays read + arrays written, each counted once per launch).
+// stream that runs it, and keeps the algorithmic bytes of each launch (arrays
+// read + arrays written, each counted once).  Works for direct launches and
+// for captured graphs: a graph owns the event pairs recorded while it was
+// captured and the host harvests them after every replay.
 struct KernelTimer {
+    struct Slot {
+        int pair;
+        double bytes;
+    };
     int cls = -1;  // class being timed (VDI_KCLASS_*), -1 = off
     std::vector<hipEvent_t> ev;
+    size_t next = 0, reserved = 0;  // pairs [0, reserved) belong to captured graphs
+    std::vector<Slot> pending;      // recorded since the last harvest / capture start
+    std::vector<float> dur_ms;
     std::vector<double> bytes;
-    size_t used = 0;
     int begin(int c, hipStream_t st) {
-        if (c != cls || 2 * used + 2 > ev.size()) return -1;
-        (void)hipEventRecord(ev[2 * used], st);
-        return (int)used;
+        if (c != cls || 2 * next + 2 > ev.size()) return -1;
+        (void)hipEventRecord(ev[2 * next], st);
+        return (int)next++;
     }
-    void end(int slot, hipStream_t st, double alg_bytes) {
-        if (slot < 0) return;
-        (void)hipEventRecord(ev[2 * slot + 1], st);
-        bytes[slot] = alg_bytes;
-        used++;
+    void end(int pair, hipStream_t st, double alg_bytes) {
+        if (pair < 0) return;
+        (void)hipEventRecord(ev[2 * pair + 1], st);
+        pending.push_back(Slot{pair, alg_bytes});
+    }
+    // Reads the durations of the given (completed) launches.
+    void harvest(const std::vector<Slot> &slots) {
+        for (const Slot &s : slots) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, ev[2 * s.pair], ev[2 * s.pair + 1]) == hipSuccess) {
+                dur_ms.push_back(ms);
+                bytes.push_back(s.bytes);
+            }
+        }
+    }
+    // After a direct (uncaptured) step has completed.
+    void harvest_direct() {
+        harvest(pending);
+        pending.clear();
+        next = reserved;
     }
     void release() {
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
         ev.clear();
+        pending.clear();
+        dur_ms.clear();
         bytes.clear();
-        used = 0;
+        next = reserved = 0;
     }
     ~KernelTimer() { release(); }
 };
