@@ -89,6 +89,7 @@ typedef struct vdi_stats {
 
 typedef struct vdi_ctx vdi_ctx;
 typedef struct vdi_mg vdi_mg;
+typedef struct vdi_batch vdi_batch;
 
 const char *vdi_status_string(int status);
 /* Text of the last HIP error seen by this library (or ""). */
@@ -177,6 +178,42 @@ int vdi_mg_residual(vdi_mg *g, double *rnorm, double *bnorm);
 int vdi_mg_synchronize(vdi_mg *g);
 /* Algorithmic bytes of one V-cycle in the current form (measurement). */
 int vdi_mg_cycle_bytes(vdi_mg *g, double *bytes);
+
+/* ---- tile shapes of the multigrid level kernels ----
+ * No counterpart in the reference (TGMG's sweeps are not tiled).  Shapes are
+ * indexes into a fixed candidate list (vdi_tile_shape gives tx, ty).
+ * prec 0 = FP64 hierarchy, 1 = FP32 hierarchy of the mixed form; which 0 =
+ * down (smooth + restrict), 1 = up (prolong + smooth), 2 = mixed level-0 up. */
+int vdi_tile_shape(int shape, int *tx, int *ty);
+int vdi_mg_set_tile(vdi_mg *g, int prec, int which, int level, int shape);
+/* Times every candidate for every level kernel on the system's own arrays and
+ * keeps the fastest (z and b are restored). */
+int vdi_mg_autotune(vdi_mg *g, int reps);
+/* Same for the four solvers of a context. */
+int vdi_autotune(vdi_ctx *ctx, int reps);
+/* Writes the chosen shapes [prec][which][level] of solver `which_solver`
+ * (VDI_MAC..VDI_VY) into out; returns the number written or a negative value. */
+int vdi_get_tiles(vdi_ctx *ctx, int which_solver, int *out, int n);
+
+/* ---- batches of independent simulations ----
+ * No counterpart in the reference, which runs one simulation per process.
+ * Each member is a full context (its own grid size and parameters; x_sym and
+ * implicit_visc must agree).  A batched step replays one captured graph of
+ * persistent kernels over all members. */
+int vdi_batch_create(const vdi_params *params, int count, int form, vdi_batch **out);
+int vdi_batch_destroy(vdi_batch *b);
+int vdi_batch_size(vdi_batch *b);
+/* Borrowed handle of member k (for vdi_get_array etc.; do not destroy). */
+int vdi_batch_sim(vdi_batch *b, int k, vdi_ctx **out);
+int vdi_batch_init_fields(vdi_batch *b);
+/* Advances every member by its own dt_reg. */
+int vdi_batch_step(vdi_batch *b);
+int vdi_batch_autotune(vdi_batch *b, int reps);
+/* Last batched step: graph time (ms) and each member's workgroup busy time
+ * (ms, summed over its work items; n >= size). */
+int vdi_batch_step_times(vdi_batch *b, double *graph_ms, double *busy_ms, int n);
+int vdi_batch_counters(vdi_batch *b, long *batched_steps, long *redone_sims);
+int vdi_batch_tiles(vdi_batch *b, int *out, int n);
 
 #ifdef __cplusplus
 }

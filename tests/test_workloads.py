@@ -43,3 +43,16 @@ def test_fem_rhs_dirichlet_row_and_sum():
     b = w.fem_rhs(32, 16).reshape(17, 33)
     assert np.all(b[0] == 0)
     assert np.isfinite(b).all()
+
+
+def test_batch_members_draws():
+    cfgs, prms = w.batch_members(64)
+    assert len(prms) == 64
+    sizes = {(c.m, c.n) for c in cfgs}
+    assert sizes <= set(w.BATCH_GRIDS) and len(sizes) == 3
+    for c, p in zip(cfgs, prms):
+        assert 1e-6 <= c.nul <= 100e-6 and 0.3 <= c.V <= 1.0 and 0.5e-3 <= c.R <= 2e-3
+        # h0 = h_min = 5 um for every member (fileinfo.cc:145-146)
+        assert abs(p.h0 - 5e-6) < 1e-15
+        assert p.m == c.m and p.dt_reg > 0
+    assert w.batch_members(64)[0] == cfgs

@@ -99,6 +99,21 @@ ABI = {
     "vdi_mg_residual": (_i, [_vp, _dp, _dp]),
     "vdi_mg_synchronize": (_i, [_vp]),
     "vdi_mg_cycle_bytes": (_i, [_vp, _dp]),
+    "vdi_tile_shape": (_i, [_i, _ip, _ip]),
+    "vdi_mg_set_tile": (_i, [_vp, _i, _i, _i, _i]),
+    "vdi_mg_autotune": (_i, [_vp, _i]),
+    "vdi_autotune": (_i, [_vp, _i]),
+    "vdi_get_tiles": (_i, [_vp, _i, _ip, _i]),
+    "vdi_batch_create": (_i, [ctypes.POINTER(VdiParams), _i, _i, ctypes.POINTER(_vp)]),
+    "vdi_batch_destroy": (_i, [_vp]),
+    "vdi_batch_size": (_i, [_vp]),
+    "vdi_batch_sim": (_i, [_vp, _i, ctypes.POINTER(_vp)]),
+    "vdi_batch_init_fields": (_i, [_vp]),
+    "vdi_batch_step": (_i, [_vp]),
+    "vdi_batch_autotune": (_i, [_vp, _i]),
+    "vdi_batch_step_times": (_i, [_vp, _dp, _dp, _i]),
+    "vdi_batch_counters": (_i, [_vp, ctypes.POINTER(_l), ctypes.POINTER(_l)]),
+    "vdi_batch_tiles": (_i, [_vp, _ip, _i]),
 }
 
 _lib = None
@@ -142,18 +157,23 @@ class Fluid2D:
     """GPU-resident drop-impact liquid solve (the reference's fluid_2d)."""
 
     def __init__(self, params: VdiParams, nu1: int = 2, nu2: int = 2, rel_tol: float = 1e-10,
-                 max_cycles: int = 200):
+                 max_cycles: int = 200, _handle=None):
         self.lib = load_library()
         self.params = params
         self.m, self.n = params.m, params.n
-        h = _vp()
-        _check(self.lib.vdi_create(ctypes.byref(params), ctypes.byref(h)), "vdi_create")
-        self.h = h
-        self.set_solver(nu1, nu2, rel_tol, max_cycles)
+        self._owned = _handle is None
+        if _handle is None:
+            h = _vp()
+            _check(self.lib.vdi_create(ctypes.byref(params), ctypes.byref(h)), "vdi_create")
+            self.h = h
+            self.set_solver(nu1, nu2, rel_tol, max_cycles)
+        else:
+            self.h = _handle
 
     def close(self):
         if getattr(self, "h", None):
-            self.lib.vdi_destroy(self.h)
+            if self._owned:
+                self.lib.vdi_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -253,6 +273,82 @@ class Fluid2D:
         a = np.ascontiguousarray(state, np.float64).ravel()
         _check(self.lib.vdi_upload_state(self.h, _ptr(a), a.size), "upload_state")
 
+    def autotune(self, reps: int = 5):
+        """Picks the fastest tile shape of every multigrid level kernel."""
+        _check(self.lib.vdi_autotune(self.h, reps), "autotune")
+
+    def tiles(self, which: int) -> list:
+        """Chosen tile shapes (tx, ty) of solver `which`: [prec][down/up][level] then the mixed level-0 up."""
+        arr = (ctypes.c_int * 256)()
+        n = self.lib.vdi_get_tiles(self.h, which, arr, 256)
+        if n < 0:
+            raise VdiError(-n, "get_tiles")
+        return [tile_shape(v) for v in list(arr)[:n]]
+
+
+def tile_shape(index: int):
+    lib = load_library()
+    tx, ty = ctypes.c_int(), ctypes.c_int()
+    _check(lib.vdi_tile_shape(index, ctypes.byref(tx), ctypes.byref(ty)), "tile_shape")
+    return tx.value, ty.value
+
+
+class Batch:
+    """Many independent drop simulations advanced by one captured graph of
+    persistent kernels per step (uneven grids and parameters)."""
+
+    def __init__(self, params_list, form: int = FORM_FP64):
+        self.lib = load_library()
+        self.params = list(params_list)
+        arr = (VdiParams * len(self.params))(*self.params)
+        h = _vp()
+        _check(self.lib.vdi_batch_create(arr, len(self.params), form, ctypes.byref(h)), "vdi_batch_create")
+        self.h = h
+        self.size = len(self.params)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.vdi_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sim(self, k: int) -> Fluid2D:
+        """Borrowed view of member k (valid while the batch lives)."""
+        h = _vp()
+        _check(self.lib.vdi_batch_sim(self.h, k, ctypes.byref(h)), "vdi_batch_sim")
+        return Fluid2D(self.params[k], _handle=h)
+
+    def init_fields(self):
+        _check(self.lib.vdi_batch_init_fields(self.h), "vdi_batch_init_fields")
+
+    def step(self):
+        _check(self.lib.vdi_batch_step(self.h), "vdi_batch_step")
+
+    def autotune(self, reps: int = 5):
+        _check(self.lib.vdi_batch_autotune(self.h, reps), "vdi_batch_autotune")
+
+    def step_times(self):
+        """(graph ms of the last batched step, per-member busy ms array)."""
+        g = ctypes.c_double()
+        busy = np.zeros(self.size)
+        _check(self.lib.vdi_batch_step_times(self.h, ctypes.byref(g), _ptr(busy), self.size), "step_times")
+        return g.value, busy
+
+    def counters(self):
+        a, b = ctypes.c_long(), ctypes.c_long()
+        _check(self.lib.vdi_batch_counters(self.h, ctypes.byref(a), ctypes.byref(b)), "counters")
+        return a.value, b.value
+
+    def tiles(self) -> list:
+        arr = (ctypes.c_int * 4096)()
+        n = self.lib.vdi_batch_tiles(self.h, arr, 4096)
+        return [tile_shape(v) for v in list(arr)[:n]]
+
 
 def copy_probe(n: int, reps: int = 10) -> float:
     """Average ms of a plain device copy of n doubles (counter calibration)."""
@@ -343,8 +439,16 @@ class MGSystem:
         _check(self.lib.vdi_mg_cycle_bytes(self.h, ctypes.byref(b)), "vdi_mg_cycle_bytes")
         return b.value
 
+    def set_tile(self, prec: int, which: int, level: int, shape: int):
+        _check(self.lib.vdi_mg_set_tile(self.h, prec, which, level, shape), "vdi_mg_set_tile")
+
+    def autotune(self, reps: int = 5):
+        _check(self.lib.vdi_mg_autotune(self.h, reps), "vdi_mg_autotune")
+
+
+NUM_TILE_SHAPES = 9
 
 __all__ = [
-    "Fluid2D", "MGSystem", "VdiParams", "VdiStats", "VdiError", "PhysicalConfig",
+    "Fluid2D", "MGSystem", "Batch", "tile_shape", "NUM_TILE_SHAPES", "VdiParams", "VdiStats", "VdiError", "PhysicalConfig",
     "drop_impact_params", "h0_nd_for_hmin", "load_library", "LIB_PATH", "ABI",
 ]

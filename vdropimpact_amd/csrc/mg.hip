@@ -13,13 +13,19 @@
 // predicted number of cycles and synchronises with the host once.
 //
 // The down/up kernels are temporally blocked: a workgroup loads its
-// kTX x kTY output tile plus a halo of H cells into LDS, performs the sweeps
+// tx x ty output tile plus a halo of H cells into LDS, performs the sweeps
 // on a region that shrinks by one cell per colour pass (so every update sees
 // exactly the values a global sweep would), and writes back only the tile.
 // Halo: cell-centred red-black H = 2 nu (+1 for a residual); nodal four-
 // colour H = 4 nu (+1 residual, +1 more for full-weighting restriction).
 // Values outside the domain are zero in LDS, which is exactly how both
-// operators drop missing neighbours.
+// operators drop missing neighbours.  The tile shape is a launch argument
+// chosen per level by the autotuner.
+//
+// Workgroup-to-tile order is XCD-aware: gfx950 hands consecutive workgroups
+// of a launch to its 8 XCDs in turn, so tile = xcd_tile(workgroup) gives each
+// XCD one contiguous band of tiles, whose shared halos then hit that XCD's
+// own L2.
 #include "mg.hpp"
 
 #include <algorithm>
@@ -29,10 +35,15 @@
 
 namespace vdi {
 
-constexpr int kTX = 64, kTY = 32;   // output tile of the level kernels
-constexpr int kBX = 32, kBY = 8;    // 256 threads
-constexpr int kCoarsePoints = 3072; // levels this small go to the single-workgroup kernel
-constexpr int kCoarseThreads = 256;
+const TileShape kTileShapes[kNumTileShapes] = {{64, 32}, {32, 8},  {32, 16}, {64, 8},  {64, 16},
+                                               {32, 32}, {128, 16}, {128, 32}, {64, 64}};
+
+// Position b of n work items -> item index such that the items of one XCD
+// (b mod 8) are contiguous.
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+    const int q = n >> 3, r = n & 7, x = b & 7, k = b >> 3;
+    return x * q + min(x, r) + k;
+}
 
 // ------------------------------------------------------------------------
 // Operators on plain arrays (mgs_mac.hh:37-50, visco_impl.hh:32-53,
@@ -120,18 +131,20 @@ __device__ __forceinline__ T fem_inv_acc(const LevelT<T> &L, int i, int j) {
 }
 
 struct Tile {
-    int x0, y0, gx0, gy0, RX, RY, H;
+    int x0, y0, gx0, gy0, RX, RY, H, TX, TY;
 };
 
-__device__ __forceinline__ Tile make_tile(int bx, int by, int H) {
+__device__ __forceinline__ Tile make_tile(int bx, int by, int H, int TX, int TY) {
     Tile t;
     t.H = H;
-    t.x0 = bx * kTX;
-    t.y0 = by * kTY;
+    t.TX = TX;
+    t.TY = TY;
+    t.x0 = bx * TX;
+    t.y0 = by * TY;
     t.gx0 = t.x0 - H;
     t.gy0 = t.y0 - H;
-    t.RX = kTX + 2 * H;
-    t.RY = kTY + 2 * H;
+    t.RX = TX + 2 * H;
+    t.RY = TY + 2 * H;
     return t;
 }
 
@@ -162,37 +175,54 @@ __device__ __forceinline__ T fem_prolong_at(const T *ez, int Mc, int i, int j) {
 
 // Loads zsrc (+ prolongated coarse correction when C is given; zero when
 // zsrc is null) and b over the tile region; outside the domain both are zero.
+// The region is walked as one flat index range, kLoadUnroll elements per
+// thread and round, with all global loads of a round issued before any LDS
+// store so that several HBM round trips overlap.
+constexpr int kLoadUnroll = 4;
+
 template <class T, bool FEM>
 __device__ __forceinline__ void tile_load(const LevelT<T> &F, const T *zsrc, const LevelT<T> *C, const CCOp &op,
                                           T *sz, T *sb, const Tile &t) {
-    for (int ly = threadIdx.y; ly < t.RY; ly += kBY) {
-        const int gy = t.gy0 + ly;
-        const bool yin = gy >= 0 && gy < F.n;
-        for (int lx = threadIdx.x; lx < t.RX; lx += kBX) {
-            const int gx = t.gx0 + lx;
-            const bool in = yin && gx >= 0 && gx < F.m;
-            T zv = 0, bv = 0;
-            if (in) {
-                const size_t g = gx + (size_t)F.m * gy;
-                if (zsrc) zv = zsrc[g];
-                bv = F.b[g];
-                if (C) {
-                    if (FEM) { if (gy > 0) zv += fem_prolong_at<T>(C->z, C->m, gx, gy); }
-                    else zv += cc_prolong_at<T>(op, C->m, C->n, C->z, gx, gy);
+    const int tid = threadIdx.x + kBX * threadIdx.y;
+    const int nel = t.RX * t.RY;
+    for (int k0 = tid; k0 < nel; k0 += 256 * kLoadUnroll) {
+        T zv[kLoadUnroll], bv[kLoadUnroll];
+#pragma unroll
+        for (int u = 0; u < kLoadUnroll; u++) {
+            const int k = k0 + 256 * u;
+            zv[u] = 0;
+            bv[u] = 0;
+            if (k < nel) {
+                const int ly = k / t.RX, lx = k - ly * t.RX;
+                const int gx = t.gx0 + lx, gy = t.gy0 + ly;
+                if (gy >= 0 && gy < F.n && gx >= 0 && gx < F.m) {
+                    const size_t g = gx + (size_t)F.m * gy;
+                    if (zsrc) zv[u] = zsrc[g];
+                    bv[u] = F.b[g];
+                    if (C) {
+                        if (FEM) { if (gy > 0) zv[u] += fem_prolong_at<T>(C->z, C->m, gx, gy); }
+                        else zv[u] += cc_prolong_at<T>(op, C->m, C->n, C->z, gx, gy);
+                    }
                 }
             }
-            sz[ly * t.RX + lx] = zv;
-            sb[ly * t.RX + lx] = bv;
+        }
+#pragma unroll
+        for (int u = 0; u < kLoadUnroll; u++) {
+            const int k = k0 + 256 * u;
+            if (k < nel) {
+                sz[k] = zv[u];
+                sb[k] = bv[u];
+            }
         }
     }
 }
 
 template <class T>
 __device__ __forceinline__ void tile_store(const LevelT<T> &F, T *dst, const T *sz, const Tile &t) {
-    for (int y = threadIdx.y; y < kTY; y += kBY) {
+    for (int y = threadIdx.y; y < t.TY; y += kBY) {
         const int gy = t.y0 + y;
         if (gy >= F.n) break;
-        for (int x = threadIdx.x; x < kTX; x += kBX) {
+        for (int x = threadIdx.x; x < t.TX; x += kBX) {
             const int gx = t.x0 + x;
             if (gx >= F.m) break;
             dst[gx + (size_t)F.m * gy] = sz[(y + t.H) * t.RX + x + t.H];
@@ -204,8 +234,8 @@ __device__ __forceinline__ void tile_store(const LevelT<T> &F, T *dst, const T *
 template <class T>
 __device__ __forceinline__ void cc_tile_sweep(const LevelT<T> &L, const CCOp &op, T *sz, const T *sb, const Tile &t,
                                               int s, int c) {
-    const int X0 = max(0, t.gx0 + s), X1 = min(L.m, t.x0 + kTX + t.H - s);
-    const int Y0 = max(0, t.gy0 + s), Y1 = min(L.n, t.y0 + kTY + t.H - s);
+    const int X0 = max(0, t.gx0 + s), X1 = min(L.m, t.x0 + t.TX + t.H - s);
+    const int Y0 = max(0, t.gy0 + s), Y1 = min(L.n, t.y0 + t.TY + t.H - s);
     const T sgn = T(op.s);
     for (int gy = Y0 + threadIdx.y; gy < Y1; gy += kBY) {
         const int xs = X0 + (((X0 + gy) & 1) ^ c);
@@ -224,8 +254,8 @@ __device__ __forceinline__ void cc_tile_sweep(const LevelT<T> &L, const CCOp &op
 template <class T>
 __device__ __forceinline__ void fem_tile_sweep(const LevelT<T> &L, const FemOp &op, T *sz, const T *sb, const Tile &t,
                                                int s, int ci, int cj) {
-    const int X0 = max(0, t.gx0 + s), X1 = min(L.m, t.x0 + kTX + t.H - s);
-    const int Y0 = max(1, t.gy0 + s), Y1 = min(L.n, t.y0 + kTY + t.H - s);
+    const int X0 = max(0, t.gx0 + s), X1 = min(L.m, t.x0 + t.TX + t.H - s);
+    const int Y0 = max(1, t.gy0 + s), Y1 = min(L.n, t.y0 + t.TY + t.H - s);
     const int ys = Y0 + ((Y0 ^ cj) & 1), xs = X0 + ((X0 ^ ci) & 1);
     for (int gy = ys + 2 * threadIdx.y; gy < Y1; gy += 2 * kBY) {
         const int row = (gy - t.gy0) * t.RX - t.gx0;
@@ -261,24 +291,24 @@ __device__ __forceinline__ double block_sum(double a, double *sh) {
         a = 0;
         for (int k = 0; k < nw; k++) a += sh[k];
     }
+    __syncthreads();
     return a;
 }
 
 // Convergence bookkeeping: the last workgroup to finish sums the per-
 // workgroup partials in index order and sets the flag.
-__device__ __forceinline__ void finish_norm(const MGCtl &ctl, double part, int nblocks, double tol2, double *sh) {
-    __shared__ int last;
+__device__ __forceinline__ void finish_norm(const MGCtl &ctl, double part, int bid, int nblocks, double tol2,
+                                            double *sh, int *last) {
     const int tid = threadIdx.x + blockDim.x * threadIdx.y;
-    const int bid = blockIdx.x + gridDim.x * blockIdx.y;
     part = block_sum(part, sh);
     if (tid == 0) {
         ctl.partial[bid] = part;
         __threadfence();
         unsigned tk = atomicAdd(&ctl.state->ticket, 1u);
-        last = (tk == (unsigned)nblocks - 1);
+        *last = (tk == (unsigned)nblocks - 1);
     }
     __syncthreads();
-    if (last && tid == 0) {
+    if (*last && tid == 0) {
         __threadfence();
         double rr = 0;
         for (int k = 0; k < nblocks; k++) rr += __hip_atomic_load(&ctl.partial[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -288,96 +318,48 @@ __device__ __forceinline__ void finish_norm(const MGCtl &ctl, double part, int n
         st->cycles += 1;
         st->done = !(rr == rr) ? 2 : (rr <= tol2 * st->bb ? 1 : 0);
     }
+    __syncthreads();
 }
 
 // ------------------------------------------------------------------------
-// Level kernels (T = double for FP64 hierarchies, float for the inner
-// cycles of the mixed form)
+// Level kernel bodies (T = double for FP64 hierarchies, float for the inner
+// cycles of the mixed form).  sh/last: workgroup scratch of the reduction.
 // ------------------------------------------------------------------------
 
-template <class T>
-__global__ void __launch_bounds__(256) k_cc_down(LevelT<T> F, LevelT<T> C, CCOp op, int nu, int zero_z,
-                                                 const MGState *state) {
-    extern __shared__ double lds_d[];
-    T *lds = reinterpret_cast<T *>(lds_d);
-    if (state->done) return;
-    const Tile t = make_tile(blockIdx.x, blockIdx.y, 2 * nu + 1);
+template <class T, bool FEM>
+__device__ __forceinline__ void down_body(const LvArgs<T> &a, int tile, T *lds) {
+    const LevelT<T> &F = a.F, &C = a.C;
+    const Tile t = make_tile(tile % a.nbx, tile / a.nbx, FEM ? 4 * a.nu + 2 : 2 * a.nu + 1, a.tx, a.ty);
     T *sz = lds, *sb = lds + t.RX * t.RY;
-    tile_load<T, false>(F, zero_z ? nullptr : F.z, nullptr, op, sz, sb, t);
+    tile_load<T, FEM>(F, a.flag ? nullptr : F.z, nullptr, a.cop, sz, sb, t);
     __syncthreads();
-    for (int s = 1; s <= 2 * nu; s++) {
-        cc_tile_sweep<T>(F, op, sz, sb, t, s, (s - 1) & 1);
-        __syncthreads();
+    if (FEM) {
+        for (int s = 1; s <= 4 * a.nu; s++) {
+            const int c = (s - 1) & 3;
+            fem_tile_sweep<T>(F, a.fop, sz, sb, t, s, c & 1, c >> 1);
+            __syncthreads();
+        }
+    } else {
+        for (int s = 1; s <= 2 * a.nu; s++) {
+            cc_tile_sweep<T>(F, a.cop, sz, sb, t, s, (s - 1) & 1);
+            __syncthreads();
+        }
     }
     tile_store<T>(F, F.zalt, sz, t);
     const int I0 = t.x0 >> 1, J0 = t.y0 >> 1;
-    const int I1 = min(C.m, I0 + kTX / 2), J1 = min(C.n, J0 + kTY / 2);
-    for (int J = J0 + threadIdx.y; J < J1; J += kBY)
-        for (int I = I0 + threadIdx.x; I < I1; I += kBX) {
-            T r = 0;
-            for (int b2 = 0; b2 < 2; b2++)
-                for (int a2 = 0; a2 < 2; a2++) {
-                    const int gx = 2 * I + a2, gy = 2 * J + b2;
-                    const int l = (gy - t.gy0) * t.RX + gx - t.gx0;
-                    r += cc_resid_t<T>(F, op, sz, sb, t.RX, l, gx, gy);
-                }
-            C.b[I + (size_t)C.m * J] = T(0.25) * r;
-        }
-}
-
-template <class T>
-__global__ void __launch_bounds__(256) k_cc_up(LevelT<T> F, LevelT<T> C, CCOp op, int nu, int norm, MGCtl ctl,
-                                               double tol2) {
-    extern __shared__ double lds_d[];
-    T *lds = reinterpret_cast<T *>(lds_d);
-    __shared__ double sh[8];
-    if (ctl.state->done) return;
-    const Tile t = make_tile(blockIdx.x, blockIdx.y, 2 * nu + norm);
-    T *sz = lds, *sb = lds + t.RX * t.RY;
-    tile_load<T, false>(F, F.zalt, &C, op, sz, sb, t);
-    __syncthreads();
-    for (int s = 1; s <= 2 * nu; s++) {
-        cc_tile_sweep<T>(F, op, sz, sb, t, s, (s - 1) & 1);
-        __syncthreads();
-    }
-    tile_store<T>(F, F.z, sz, t);
-    if (!norm) return;
-    double rr = 0;
-    for (int y = threadIdx.y; y < kTY; y += kBY) {
-        const int gy = t.y0 + y;
-        if (gy >= F.n) break;
-        for (int x = threadIdx.x; x < kTX; x += kBX) {
-            const int gx = t.x0 + x;
-            if (gx >= F.m) break;
-            const double r = cc_resid_t<T>(F, op, sz, sb, t.RX, (y + t.H) * t.RX + x + t.H, gx, gy);
-            rr += r * r;
-        }
-    }
-    finish_norm(ctl, rr, gridDim.x * gridDim.y, tol2, sh);
-}
-
-template <class T>
-__global__ void __launch_bounds__(256) k_fem_down(LevelT<T> F, LevelT<T> C, FemOp op, int nu, int zero_z,
-                                                  const MGState *state) {
-    extern __shared__ double lds_d[];
-    T *lds = reinterpret_cast<T *>(lds_d);
-    if (state->done) return;
-    const Tile t = make_tile(blockIdx.x, blockIdx.y, 4 * nu + 2);
-    T *sz = lds, *sb = lds + t.RX * t.RY;
-    tile_load<T, true>(F, zero_z ? nullptr : F.z, nullptr, CCOp{}, sz, sb, t);
-    __syncthreads();
-    for (int s = 1; s <= 4 * nu; s++) {
-        const int c = (s - 1) & 3;
-        fem_tile_sweep<T>(F, op, sz, sb, t, s, c & 1, c >> 1);
-        __syncthreads();
-    }
-    tile_store<T>(F, F.zalt, sz, t);
-    const int I0 = t.x0 >> 1, J0 = t.y0 >> 1;
-    const int I1 = min(C.m, I0 + kTX / 2), J1 = min(C.n, J0 + kTY / 2);
+    const int I1 = min(C.m, I0 + t.TX / 2), J1 = min(C.n, J0 + t.TY / 2);
     for (int J = J0 + threadIdx.y; J < J1; J += kBY)
         for (int I = I0 + threadIdx.x; I < I1; I += kBX) {
             T v = 0;
-            if (J > 0) {
+            if (!FEM) {
+                for (int b2 = 0; b2 < 2; b2++)
+                    for (int a2 = 0; a2 < 2; a2++) {
+                        const int gx = 2 * I + a2, gy = 2 * J + b2;
+                        const int l = (gy - t.gy0) * t.RX + gx - t.gx0;
+                        v += cc_resid_t<T>(F, a.cop, sz, sb, t.RX, l, gx, gy);
+                    }
+                v *= T(0.25);
+            } else if (J > 0) {
                 for (int b2 = -1; b2 <= 1; b2++) {
                     const int gy = 2 * J + b2;
                     if (gy >= F.n) continue;
@@ -386,7 +368,7 @@ __global__ void __launch_bounds__(256) k_fem_down(LevelT<T> F, LevelT<T> C, FemO
                         const int gx = 2 * I + a2;
                         if (gx < 0 || gx >= F.m) continue;
                         const int l = (gy - t.gy0) * t.RX + gx - t.gx0;
-                        sx += (a2 ? T(0.5) : T(1)) * fem_resid_t<T>(op, F.m, F.n, sz, sb, t.RX, l, gx, gy);
+                        sx += (a2 ? T(0.5) : T(1)) * fem_resid_t<T>(a.fop, F.m, F.n, sz, sb, t.RX, l, gx, gy);
                     }
                     v += (b2 ? T(0.5) : T(1)) * sx;
                 }
@@ -395,36 +377,41 @@ __global__ void __launch_bounds__(256) k_fem_down(LevelT<T> F, LevelT<T> C, FemO
         }
 }
 
-template <class T>
-__global__ void __launch_bounds__(256) k_fem_up(LevelT<T> F, LevelT<T> C, FemOp op, int nu, int norm, MGCtl ctl,
-                                                double tol2) {
-    extern __shared__ double lds_d[];
-    T *lds = reinterpret_cast<T *>(lds_d);
-    __shared__ double sh[8];
-    if (ctl.state->done) return;
-    const Tile t = make_tile(blockIdx.x, blockIdx.y, 4 * nu + norm);
+template <class T, bool FEM>
+__device__ __forceinline__ void up_body(const LvArgs<T> &a, int tile, T *lds, double *sh, int *last) {
+    const LevelT<T> &F = a.F;
+    const Tile t = make_tile(tile % a.nbx, tile / a.nbx, (FEM ? 4 : 2) * a.nu + a.flag, a.tx, a.ty);
     T *sz = lds, *sb = lds + t.RX * t.RY;
-    tile_load<T, true>(F, F.zalt, &C, CCOp{}, sz, sb, t);
+    tile_load<T, FEM>(F, F.zalt, &a.C, a.cop, sz, sb, t);
     __syncthreads();
-    for (int s = 1; s <= 4 * nu; s++) {
-        const int c = (s - 1) & 3;
-        fem_tile_sweep<T>(F, op, sz, sb, t, s, c & 1, c >> 1);
-        __syncthreads();
+    if (FEM) {
+        for (int s = 1; s <= 4 * a.nu; s++) {
+            const int c = (s - 1) & 3;
+            fem_tile_sweep<T>(F, a.fop, sz, sb, t, s, c & 1, c >> 1);
+            __syncthreads();
+        }
+    } else {
+        for (int s = 1; s <= 2 * a.nu; s++) {
+            cc_tile_sweep<T>(F, a.cop, sz, sb, t, s, (s - 1) & 1);
+            __syncthreads();
+        }
     }
     tile_store<T>(F, F.z, sz, t);
-    if (!norm) return;
+    if (!a.flag) return;
     double rr = 0;
-    for (int y = threadIdx.y; y < kTY; y += kBY) {
+    for (int y = threadIdx.y; y < t.TY; y += kBY) {
         const int gy = t.y0 + y;
         if (gy >= F.n) break;
-        for (int x = threadIdx.x; x < kTX; x += kBX) {
+        for (int x = threadIdx.x; x < t.TX; x += kBX) {
             const int gx = t.x0 + x;
             if (gx >= F.m) break;
-            const double r = fem_resid_t<T>(op, F.m, F.n, sz, sb, t.RX, (y + t.H) * t.RX + x + t.H, gx, gy);
+            const int l = (y + t.H) * t.RX + x + t.H;
+            const double r = FEM ? fem_resid_t<T>(a.fop, F.m, F.n, sz, sb, t.RX, l, gx, gy)
+                                 : cc_resid_t<T>(F, a.cop, sz, sb, t.RX, l, gx, gy);
             rr += r * r;
         }
     }
-    finish_norm(ctl, rr, gridDim.x * gridDim.y, tol2, sh);
+    finish_norm(a.ctl, rr, tile, a.nb, a.tol2, sh, last);
 }
 
 // Mixed form, level 0, end of an inner FP32 cycle: prolongate and smooth the
@@ -433,97 +420,239 @@ __global__ void __launch_bounds__(256) k_fem_up(LevelT<T> F, LevelT<T> C, FemOp 
 // which is reduced for the convergence test and stored in r_out as the next
 // cycle's right-hand side.  Inputs and outputs are distinct arrays.
 template <bool FEM>
-__global__ void __launch_bounds__(256) k_mx_up0(Level Z, const double *z_in, double *z_out, LevelF E, float *r_out,
-                                                LevelF C, CCOp cop, FemOp fop, int nu, MGCtl ctl, double tol2) {
-    extern __shared__ double lds_d[];
-    __shared__ double sh[8];
-    if (ctl.state->done) return;
-    const Tile t = make_tile(blockIdx.x, blockIdx.y, (FEM ? 4 : 2) * nu + 1);
+__device__ __forceinline__ void mx_body(const MxArgs &a, int tile, double *lds_d, double *sh, int *last) {
+    const Level &Z = a.Z;
+    const Tile t = make_tile(tile % a.nbx, tile / a.nbx, (FEM ? 4 : 2) * a.nu + 1, a.tx, a.ty);
     const int R = t.RX * t.RY;
     double *zd = lds_d;
     float *se = reinterpret_cast<float *>(lds_d + R), *sr = se + R;
-    tile_load<float, FEM>(E, E.zalt, &C, cop, se, sr, t);
+    tile_load<float, FEM>(a.E, a.E.zalt, &a.C, a.cop, se, sr, t);
     __syncthreads();
-    for (int s = 1; s <= (FEM ? 4 : 2) * nu; s++) {
+    for (int s = 1; s <= (FEM ? 4 : 2) * a.nu; s++) {
         if (FEM) {
             const int c = (s - 1) & 3;
-            fem_tile_sweep<float>(E, fop, se, sr, t, s, c & 1, c >> 1);
+            fem_tile_sweep<float>(a.E, a.fop, se, sr, t, s, c & 1, c >> 1);
         } else {
-            cc_tile_sweep<float>(E, cop, se, sr, t, s, (s - 1) & 1);
+            cc_tile_sweep<float>(a.E, a.cop, se, sr, t, s, (s - 1) & 1);
         }
         __syncthreads();
     }
     // z + e on the tile and a one-cell ring (zero outside the domain)
-    for (int ly = t.H - 1 + threadIdx.y; ly < t.H + kTY + 1; ly += kBY) {
-        const int gy = t.gy0 + ly;
-        for (int lx = t.H - 1 + threadIdx.x; lx < t.H + kTX + 1; lx += kBX) {
-            const int gx = t.gx0 + lx;
-            const bool in = gy >= 0 && gy < Z.n && gx >= 0 && gx < Z.m;
-            const int l = ly * t.RX + lx;
-            zd[l] = in ? z_in[gx + (size_t)Z.m * gy] + (double)se[l] : 0.0;
+    const int tid = threadIdx.x + kBX * threadIdx.y;
+    {
+        const int RW = t.TX + 2, nel = RW * (t.TY + 2);
+        for (int k0 = tid; k0 < nel; k0 += 256 * kLoadUnroll) {
+            double zz[kLoadUnroll];
+#pragma unroll
+            for (int u = 0; u < kLoadUnroll; u++) {
+                const int k = k0 + 256 * u;
+                zz[u] = 0;
+                if (k < nel) {
+                    const int ry = k / RW, rx = k - ry * RW;
+                    const int gx = t.x0 - 1 + rx, gy = t.y0 - 1 + ry;
+                    if (gy >= 0 && gy < Z.n && gx >= 0 && gx < Z.m) zz[u] = a.z_in[gx + (size_t)Z.m * gy];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kLoadUnroll; u++) {
+                const int k = k0 + 256 * u;
+                if (k < nel) {
+                    const int ry = k / RW, rx = k - ry * RW;
+                    const int gx = t.x0 - 1 + rx, gy = t.y0 - 1 + ry;
+                    const bool in = gy >= 0 && gy < Z.n && gx >= 0 && gx < Z.m;
+                    const int l = (ry + t.H - 1) * t.RX + rx + t.H - 1;
+                    zd[l] = in ? zz[u] + (double)se[l] : 0.0;
+                }
+            }
         }
     }
     __syncthreads();
     double rr = 0;
-    for (int y = threadIdx.y; y < kTY; y += kBY) {
-        const int gy = t.y0 + y;
-        if (gy >= Z.n) break;
-        for (int x = threadIdx.x; x < kTX; x += kBX) {
-            const int gx = t.x0 + x;
-            if (gx >= Z.m) break;
+    const int nel = t.TX * t.TY;
+    for (int k0 = tid; k0 < nel; k0 += 256 * kLoadUnroll) {
+        double bv[kLoadUnroll];
+#pragma unroll
+        for (int u = 0; u < kLoadUnroll; u++) {
+            const int k = k0 + 256 * u;
+            bv[u] = 0;
+            if (k < nel) {
+                const int y = k / t.TX, x = k - y * t.TX;
+                const int gx = t.x0 + x, gy = t.y0 + y;
+                if (gx < Z.m && gy < Z.n) bv[u] = Z.b[gx + (size_t)Z.m * gy];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kLoadUnroll; u++) {
+            const int k = k0 + 256 * u;
+            if (k >= nel) continue;
+            const int y = k / t.TX, x = k - y * t.TX;
+            const int gx = t.x0 + x, gy = t.y0 + y;
+            if (gx >= Z.m || gy >= Z.n) continue;
             const int l = (y + t.H) * t.RX + x + t.H;
             const size_t g = gx + (size_t)Z.m * gy;
             double az;
             if (FEM) {
-                const double acc = fem_acc<double>(fop, Z.m, Z.n, gx, gy);
-                az = gy == 0 ? acc * zd[l] : acc * zd[l] + fem_off_t<double>(fop, Z.m, Z.n, zd, l, t.RX, gx, gy);
+                const double acc = fem_acc<double>(a.fop, Z.m, Z.n, gx, gy);
+                az = gy == 0 ? acc * zd[l] : acc * zd[l] + fem_off_t<double>(a.fop, Z.m, Z.n, zd, l, t.RX, gx, gy);
             } else {
-                az = cop.s * cc_off_t<double>(Z.cx, Z.cy, zd, l, t.RX) +
-                     cc_diag<double>(cop, Z.m, Z.n, Z.cx, Z.cy, gx, gy) * zd[l];
+                az = a.cop.s * cc_off_t<double>(Z.cx, Z.cy, zd, l, t.RX) +
+                     cc_diag<double>(a.cop, Z.m, Z.n, Z.cx, Z.cy, gx, gy) * zd[l];
             }
-            const double r = Z.b[g] - az;
-            z_out[g] = zd[l];
-            r_out[g] = (float)r;
+            const double r = bv[u] - az;
+            a.z_out[g] = zd[l];
+            a.r_out[g] = (float)r;
             rr += r * r;
         }
     }
-    finish_norm(ctl, rr, gridDim.x * gridDim.y, tol2, sh);
+    finish_norm(a.ctl, rr, tile, a.nb, a.tol2, sh, last);
+}
+
+// ------------------------------------------------------------------------
+// Level kernels: direct (one record, grid = tiles) and batched (records in
+// device memory, a persistent grid walking the work list)
+// ------------------------------------------------------------------------
+
+template <class T, bool FEM>
+__global__ void __launch_bounds__(256) k_down(LvArgs<T> a) {
+    extern __shared__ double lds_d[];
+    if (a.ctl.state->done) return;
+    stamp_begin(a.ts);
+    down_body<T, FEM>(a, xcd_tile(blockIdx.x, a.nb), reinterpret_cast<T *>(lds_d));
+    stamp_end(a.ts);
+}
+
+template <class T, bool FEM>
+__global__ void __launch_bounds__(256) k_down_b(const LvArgs<T> *A, const WorkItem *W, int nw, BatchAux aux) {
+    extern __shared__ double lds_d[];
+    stamp_begin(aux.ts);
+    for (int p = blockIdx.x; p < nw; p += gridDim.x) {
+        const WorkItem w = W[xcd_tile(p, nw)];
+        const LvArgs<T> a = A[w.rec];
+        if (a.ctl.state->done) continue;
+        const unsigned long long t0 = busy_begin(aux.busy);
+        down_body<T, FEM>(a, w.tile, reinterpret_cast<T *>(lds_d));
+        __syncthreads();
+        busy_end(aux.busy, w.sim, t0);
+    }
+    stamp_end(aux.ts);
+}
+
+template <class T, bool FEM>
+__global__ void __launch_bounds__(256) k_up(LvArgs<T> a) {
+    extern __shared__ double lds_d[];
+    __shared__ double sh[8];
+    __shared__ int last;
+    if (a.ctl.state->done) return;
+    stamp_begin(a.ts);
+    up_body<T, FEM>(a, xcd_tile(blockIdx.x, a.nb), reinterpret_cast<T *>(lds_d), sh, &last);
+    stamp_end(a.ts);
+}
+
+template <class T, bool FEM>
+__global__ void __launch_bounds__(256) k_up_b(const LvArgs<T> *A, const WorkItem *W, int nw, BatchAux aux) {
+    extern __shared__ double lds_d[];
+    __shared__ double sh[8];
+    __shared__ int last;
+    stamp_begin(aux.ts);
+    for (int p = blockIdx.x; p < nw; p += gridDim.x) {
+        const WorkItem w = W[xcd_tile(p, nw)];
+        const LvArgs<T> a = A[w.rec];
+        if (a.ctl.state->done) continue;
+        const unsigned long long t0 = busy_begin(aux.busy);
+        up_body<T, FEM>(a, w.tile, reinterpret_cast<T *>(lds_d), sh, &last);
+        __syncthreads();
+        busy_end(aux.busy, w.sim, t0);
+    }
+    stamp_end(aux.ts);
+}
+
+template <bool FEM>
+__global__ void __launch_bounds__(256) k_mx_up0(MxArgs a) {
+    extern __shared__ double lds_d[];
+    __shared__ double sh[8];
+    __shared__ int last;
+    if (a.ctl.state->done) return;
+    stamp_begin(a.ts);
+    mx_body<FEM>(a, xcd_tile(blockIdx.x, a.nb), lds_d, sh, &last);
+    stamp_end(a.ts);
+}
+
+template <bool FEM>
+__global__ void __launch_bounds__(256) k_mx_up0_b(const MxArgs *A, const WorkItem *W, int nw, BatchAux aux) {
+    extern __shared__ double lds_d[];
+    __shared__ double sh[8];
+    __shared__ int last;
+    stamp_begin(aux.ts);
+    for (int p = blockIdx.x; p < nw; p += gridDim.x) {
+        const WorkItem w = W[xcd_tile(p, nw)];
+        const MxArgs a = A[w.rec];
+        if (a.ctl.state->done) continue;
+        const unsigned long long t0 = busy_begin(aux.busy);
+        mx_body<FEM>(a, w.tile, lds_d, sh, &last);
+        __syncthreads();
+        busy_end(aux.busy, w.sim, t0);
+    }
+    stamp_end(aux.ts);
 }
 
 // ------------------------------------------------------------------------
 // Coarse levels: one workgroup, all levels in LDS
 // ------------------------------------------------------------------------
 
+// count sweeps of level l.  Points of one colour are enumerated directly (no
+// skipped threads); interior points multiply by the stored reciprocal
+// diagonal exactly as the tiled kernels do, so a level gives the same bits
+// whichever kernel smooths it.
 template <class T>
 __device__ void coarse_sweeps(bool fem, const CoarseDesc &d, const CCOp &cop, const FemOp &fop, int l, T *lds,
                               int count) {
     const int m = d.m[l], n = d.n[l], N = m * n;
     T *z = lds + d.off[l], *b = z + N;
-    const T cx = T(d.cx[l]), cy = T(d.cy[l]);
-    const int ncol = fem ? 4 : 2;
-    for (int k = 0; k < count; k++)
-        for (int c = 0; c < ncol; c++) {
-            for (int ij = threadIdx.x; ij < N; ij += blockDim.x) {
-                const int i = ij % m, j = ij / m;
-                if (fem) {
-                    if ((i & 1) != (c & 1) || (j & 1) != (c >> 1) || j == 0) continue;
-                    z[ij] = (b[ij] - fem_off<T>(fop, m, n, z, i, j)) / fem_acc<T>(fop, m, n, i, j);
-                } else {
-                    if (((i + j) & 1) != c) continue;
-                    z[ij] = (b[ij] - T(cop.s) * cc_off<T>(m, n, cx, cy, z, i, j)) / cc_diag<T>(cop, m, n, cx, cy, i, j);
+    const T cx = T(d.cx[l]), cy = T(d.cy[l]), inv = T(d.inv[l]);
+    for (int k = 0; k < count; k++) {
+        if (fem) {
+            for (int c = 0; c < 4; c++) {
+                const int ci = c & 1, cj = c >> 1;
+                const int ni = (m - ci + 1) >> 1, nj = (n - cj + 1) >> 1;
+                for (int q = threadIdx.x; q < ni * nj; q += blockDim.x) {
+                    const int jj = q / ni, i = 2 * (q - jj * ni) + ci, j = 2 * jj + cj;
+                    if (j == 0) continue;
+                    const int ij = i + m * j;
+                    const T f = T(((i == 0 || i == m - 1) ? 2.0 : 1.0) * ((j == n - 1) ? 2.0 : 1.0)) * inv;
+                    z[ij] = (b[ij] - fem_off<T>(fop, m, n, z, i, j)) * f;
                 }
+                __syncthreads();
             }
-            __syncthreads();
+        } else if (!(m & 1)) {
+            const int mh = m >> 1;
+            for (int c = 0; c < 2; c++) {
+                for (int q = threadIdx.x; q < mh * n; q += blockDim.x) {
+                    const int j = q / mh, i = 2 * (q - j * mh) + ((j + c) & 1), ij = i + m * j;
+                    const T rhs = b[ij] - T(cop.s) * cc_off<T>(m, n, cx, cy, z, i, j);
+                    const bool in = i > 0 && i < m - 1 && j > 0 && j < n - 1;
+                    z[ij] = in ? rhs * inv : rhs / cc_diag<T>(cop, m, n, cx, cy, i, j);
+                }
+                __syncthreads();
+            }
+        } else {
+            for (int c = 0; c < 2; c++) {
+                for (int ij = threadIdx.x; ij < N; ij += blockDim.x) {
+                    const int j = ij / m, i = ij - j * m;
+                    if (((i + j) & 1) != c) continue;
+                    const T rhs = b[ij] - T(cop.s) * cc_off<T>(m, n, cx, cy, z, i, j);
+                    const bool in = i > 0 && i < m - 1 && j > 0 && j < n - 1;
+                    z[ij] = in ? rhs * inv : rhs / cc_diag<T>(cop, m, n, cx, cy, i, j);
+                }
+                __syncthreads();
+            }
         }
+    }
 }
 
 template <class T>
-__global__ void __launch_bounds__(kCoarseThreads) k_coarse(int fem, CoarseDesc d, CCOp cop, FemOp fop, int nu1,
-                                                            int nu2, int ncs, MGCtl ctl, double tol2) {
-    extern __shared__ double lds_d[];
-    T *lds = reinterpret_cast<T *>(lds_d);
-    __shared__ double sh[8];
-    if (ctl.state->done) return;
+__device__ void coarse_body(const CoarseArgs<T> &a, T *lds, double *sh, int *last) {
+    const CoarseDesc &d = a.d;
+    const bool fem = a.fem;
     const int Tn = blockDim.x;
     const T *gb = reinterpret_cast<const T *>(d.gb);
     T *gz = reinterpret_cast<T *>(d.gz);
@@ -537,7 +666,7 @@ __global__ void __launch_bounds__(kCoarseThreads) k_coarse(int fem, CoarseDesc d
     }
     __syncthreads();
     for (int l = 0; l + 1 < d.nl; l++) {
-        coarse_sweeps<T>(fem, d, cop, fop, l, lds, nu1);
+        coarse_sweeps<T>(fem, d, a.cop, a.fop, l, lds, a.nu1);
         const int m = d.m[l], n = d.n[l], Mc = d.m[l + 1], Nc = d.n[l + 1];
         const T *z = lds + d.off[l], *b = z + m * n;
         T *zc = lds + d.off[l + 1], *bc = zc + Mc * Nc;
@@ -549,7 +678,7 @@ __global__ void __launch_bounds__(kCoarseThreads) k_coarse(int fem, CoarseDesc d
                 for (int b2 = 0; b2 < 2; b2++)
                     for (int a2 = 0; a2 < 2; a2++) {
                         const int i = 2 * I + a2, j = 2 * J + b2;
-                        v += b[i + m * j] - Az_plain<T>(false, cop, fop, m, n, cx, cy, z, i, j);
+                        v += b[i + m * j] - Az_plain<T>(false, a.cop, a.fop, m, n, cx, cy, z, i, j);
                     }
                 v *= T(0.25);
             } else if (J > 0) {
@@ -560,7 +689,8 @@ __global__ void __launch_bounds__(kCoarseThreads) k_coarse(int fem, CoarseDesc d
                     for (int a2 = -1; a2 <= 1; a2++) {
                         const int i = 2 * I + a2;
                         if (i < 0 || i >= m) continue;
-                        sx += (a2 ? T(0.5) : T(1)) * (b[i + m * j] - Az_plain<T>(true, cop, fop, m, n, T(0), T(0), z, i, j));
+                        sx += (a2 ? T(0.5) : T(1)) *
+                              (b[i + m * j] - Az_plain<T>(true, a.cop, a.fop, m, n, T(0), T(0), z, i, j));
                     }
                     v += (b2 ? T(0.5) : T(1)) * sx;
                 }
@@ -570,7 +700,7 @@ __global__ void __launch_bounds__(kCoarseThreads) k_coarse(int fem, CoarseDesc d
         }
         __syncthreads();
     }
-    coarse_sweeps<T>(fem, d, cop, fop, d.nl - 1, lds, ncs);
+    coarse_sweeps<T>(fem, d, a.cop, a.fop, d.nl - 1, lds, a.ncs);
     for (int l = d.nl - 2; l >= 0; l--) {
         const int m = d.m[l], n = d.n[l], Mc = d.m[l + 1], Nc = d.n[l + 1];
         T *z = lds + d.off[l];
@@ -578,77 +708,158 @@ __global__ void __launch_bounds__(kCoarseThreads) k_coarse(int fem, CoarseDesc d
         for (int k = threadIdx.x; k < m * n; k += Tn) {
             const int i = k % m, j = k / m;
             if (fem) { if (j > 0) z[k] += fem_prolong_at<T>(ec, Mc, i, j); }
-            else z[k] += cc_prolong_at<T>(cop, Mc, Nc, ec, i, j);
+            else z[k] += cc_prolong_at<T>(a.cop, Mc, Nc, ec, i, j);
         }
         __syncthreads();
-        coarse_sweeps<T>(fem, d, cop, fop, l, lds, nu2);
+        coarse_sweeps<T>(fem, d, a.cop, a.fop, l, lds, a.nu2);
     }
-    {
-        const int N = d.m[0] * d.n[0];
-        const T *z = lds + d.off[0], *b = z + N;
-        for (int k = threadIdx.x; k < N; k += Tn) gz[k] = z[k];
-        if (!d.top) return;
-        double rr = 0;
-        for (int k = threadIdx.x; k < N; k += Tn) {
-            const int i = k % d.m[0], j = k / d.m[0];
-            const double r = b[k] - Az_plain<T>(fem, cop, fop, d.m[0], d.n[0], T(d.cx[0]), T(d.cy[0]), z, i, j);
-            rr += r * r;
-        }
-        finish_norm(ctl, rr, 1, tol2, sh);
+    const int N = d.m[0] * d.n[0];
+    const T *z = lds + d.off[0], *b = z + N;
+    for (int k = threadIdx.x; k < N; k += Tn) gz[k] = z[k];
+    if (!d.top) return;
+    double rr = 0;
+    for (int k = threadIdx.x; k < N; k += Tn) {
+        const int i = k % d.m[0], j = k / d.m[0];
+        const double r = b[k] - Az_plain<T>(fem, a.cop, a.fop, d.m[0], d.n[0], T(d.cx[0]), T(d.cy[0]), z, i, j);
+        rr += r * r;
     }
+    finish_norm(a.ctl, rr, 0, 1, a.tol2, sh, last);
+}
+
+template <class T>
+__global__ void __launch_bounds__(kCoarseThreads) k_coarse(CoarseArgs<T> a) {
+    extern __shared__ double lds_d[];
+    __shared__ double sh[8];
+    __shared__ int last;
+    if (a.ctl.state->done) return;
+    stamp_begin(a.ts);
+    coarse_body<T>(a, reinterpret_cast<T *>(lds_d), sh, &last);
+    stamp_end(a.ts);
+}
+
+// One workgroup per system.
+template <class T>
+__global__ void __launch_bounds__(kCoarseThreads) k_coarse_b(const CoarseArgs<T> *A, const WorkItem *W, int nw,
+                                                              BatchAux aux) {
+    extern __shared__ double lds_d[];
+    __shared__ double sh[8];
+    __shared__ int last;
+    stamp_begin(aux.ts);
+    for (int p = blockIdx.x; p < nw; p += gridDim.x) {
+        const WorkItem w = W[p];
+        const CoarseArgs<T> a = A[w.rec];
+        if (a.ctl.state->done) continue;
+        const unsigned long long t0 = busy_begin(aux.busy);
+        coarse_body<T>(a, reinterpret_cast<T *>(lds_d), sh, &last);
+        __syncthreads();
+        busy_end(aux.busy, w.sim, t0);
+    }
+    stamp_end(aux.ts);
 }
 
 // ------------------------------------------------------------------------
-// Norms, Dirichlet row, extrapolation, operator application
+// Start of a solve, element-wise operations, operator application
 // ------------------------------------------------------------------------
 
-// ||b - A z||^2 and ||b||^2 of level 0; the last workgroup stores them and
-// sets the flag: 1 converged, 3 zero right-hand side, 0 otherwise.  With r32
-// non-null (mixed form) the residual is also stored in FP32.
-__global__ void __launch_bounds__(256) k_begin(int fem, Level L, CCOp cop, FemOp fop, MGCtl ctl, double tol2,
-                                               float *r32) {
-    __shared__ double sh[8];
-    __shared__ int last;
+// ||b - A z||^2 and ||b||^2 of level 0 over blocks bid, bid + nb, ...; the
+// last block stores them and sets the flag: 1 converged, 3 zero right-hand
+// side, 0 otherwise.  With r32 non-null (mixed form) the residual is also
+// stored in FP32.
+__device__ void begin_body(const BeginArgs &a, int bid, double *sh, int *last) {
+    const Level &L = a.L;
     double rr = 0, bb = 0;
     const long N = (long)L.m * L.n;
-    for (long ij = blockIdx.x * (long)blockDim.x + threadIdx.x; ij < N; ij += (long)gridDim.x * blockDim.x) {
+    for (long ij = bid * 256L + threadIdx.x; ij < N; ij += (long)a.nb * 256) {
         const int i = (int)(ij % L.m), j = (int)(ij / L.m);
-        const double bv = L.b[ij], r = bv - Az_plain<double>(fem, cop, fop, L.m, L.n, L.cx, L.cy, L.z, i, j);
-        if (r32) r32[ij] = (float)r;
+        const double bv = L.b[ij], r = bv - Az_plain<double>(a.fem, a.cop, a.fop, L.m, L.n, L.cx, L.cy, L.z, i, j);
+        if (a.r32) a.r32[ij] = (float)r;
         rr += r * r;
         bb += bv * bv;
     }
     rr = block_sum(rr, sh);
-    __syncthreads();
     bb = block_sum(bb, sh);
     if (threadIdx.x == 0) {
-        ctl.partial[2 * blockIdx.x] = rr;
-        ctl.partial[2 * blockIdx.x + 1] = bb;
+        a.ctl.partial[2 * bid] = rr;
+        a.ctl.partial[2 * bid + 1] = bb;
         __threadfence();
-        unsigned tk = atomicAdd(&ctl.state->ticket, 1u);
-        last = (tk == gridDim.x - 1);
+        unsigned tk = atomicAdd(&a.ctl.state->ticket, 1u);
+        *last = (tk == (unsigned)a.nb - 1);
     }
     __syncthreads();
-    if (last && threadIdx.x == 0) {
+    if (*last && threadIdx.x == 0) {
         __threadfence();
         double R = 0, B = 0;
-        for (int k = 0; k < (int)gridDim.x; k++) {
-            R += __hip_atomic_load(&ctl.partial[2 * k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            B += __hip_atomic_load(&ctl.partial[2 * k + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int k = 0; k < a.nb; k++) {
+            R += __hip_atomic_load(&a.ctl.partial[2 * k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            B += __hip_atomic_load(&a.ctl.partial[2 * k + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        MGState *st = ctl.state;
+        MGState *st = a.ctl.state;
         st->ticket = 0;
         st->rr = R;
         st->bb = B;
         st->cycles = 0;
-        st->done = (B == 0) ? 3 : (!(R == R) ? 2 : (R <= tol2 * B ? 1 : 0));
+        st->done = (B == 0) ? 3 : (!(R == R) ? 2 : (R <= a.tol2 * B ? 1 : 0));
+    }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(256) k_begin(BeginArgs a) {
+    __shared__ double sh[8];
+    __shared__ int last;
+    stamp_begin(a.ts);
+    begin_body(a, blockIdx.x, sh, &last);
+    stamp_end(a.ts);
+}
+
+__global__ void __launch_bounds__(256) k_begin_b(const BeginArgs *A, const WorkItem *W, int nw, BatchAux aux) {
+    __shared__ double sh[8];
+    __shared__ int last;
+    stamp_begin(aux.ts);
+    for (int p = blockIdx.x; p < nw; p += gridDim.x) {
+        const WorkItem w = W[p];
+        const unsigned long long t0 = busy_begin(aux.busy);
+        begin_body(A[w.rec], w.tile, sh, &last);
+        busy_end(aux.busy, w.sim, t0);
+    }
+    stamp_end(aux.ts);
+}
+
+// Element-wise work of block bid (256 entries per block and round):
+//   VEC_EXTRAP    mgs_common.cc:37-44: z <- b z - c z_prev, z_prev <- old z,
+//                 with (b, c) read from device memory so that a captured
+//                 graph can be replayed with new times;
+//   VEC_SOLVE_END end of a solve whose cycle count is only known on the
+//                 device: the mixed form leaves its iterate in zalt after an
+//                 odd number of cycles, and a zero right-hand side means z = 0;
+//   VEC_DIRICHLET FEM Dirichlet row z = b / a_cc (row 0, M entries).
+__device__ __forceinline__ void vec_body(const VecArgs &a, int bid) {
+    const long stride = (long)a.nb * 256;
+    if (a.op == VEC_EXTRAP) {
+        const double bb = a.coef[0], c = a.coef[1];
+        for (long ij = bid * 256L + threadIdx.x; ij < a.N; ij += stride) {
+            const double zz = a.z[ij];
+            a.z[ij] = bb * zz - c * a.zp[ij];
+            a.zp[ij] = zz;
+        }
+    } else if (a.op == VEC_SOLVE_END) {
+        const int done = a.state->done, odd = a.mixed && (a.state->cycles & 1);
+        if (done != 3 && !odd) return;
+        for (long ij = bid * 256L + threadIdx.x; ij < a.N; ij += stride) a.z[ij] = done == 3 ? 0.0 : a.zalt[ij];
+    } else {
+        for (long i = bid * 256L + threadIdx.x; i < a.M; i += stride)
+            a.z[i] = a.b[i] / ((i > 0 && i < a.M - 1 ? 0.5 : 0.25) * a.fm);
     }
 }
 
-__global__ void k_fem_dirichlet(Level L, FemOp op) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= L.m) return;
-    L.z[i] = L.b[i] / fem_acc<double>(op, L.m, L.n, i, 0);
+__global__ void __launch_bounds__(256) k_vec(VecArgs a) { vec_body(a, blockIdx.x); }
+
+__global__ void __launch_bounds__(256) k_vec_b(const VecArgs *A, const WorkItem *W, int nw, BatchAux aux) {
+    stamp_begin(aux.ts);
+    for (int p = blockIdx.x; p < nw; p += gridDim.x) {
+        const WorkItem w = W[p];
+        vec_body(A[w.rec], w.tile);
+    }
+    stamp_end(aux.ts);
 }
 
 __global__ void k_apply(int fem, Level L, CCOp cop, FemOp fop, double *out) {
@@ -658,39 +869,23 @@ __global__ void k_apply(int fem, Level L, CCOp cop, FemOp fop, double *out) {
     out[i + (size_t)L.m * j] = Az_plain<double>(fem, cop, fop, L.m, L.n, L.cx, L.cy, L.z, i, j);
 }
 
-// mgs_common.cc:37-44: z <- b z - c z_prev, z_prev <- old z.
+// mgs_common.cc:37-44 with the coefficients by value (direct path).
 __global__ void k_extrapolate(double *z, double *zp, long N, double bb, double c) {
     long ij = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (ij >= N) return;
-    double zz = z[ij];
+    const double zz = z[ij];
     z[ij] = bb * zz - c * zp[ij];
     zp[ij] = zz;
-}
-
-// Same update with the coefficients (b, c) read from device memory, so a
-// captured step graph can be replayed with new times.
-__global__ void k_extrapolate_dev(double *z, double *zp, long N, const double *coef) {
-    long ij = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (ij >= N) return;
-    const double bb = coef[0], c = coef[1];
-    double zz = z[ij];
-    z[ij] = bb * zz - c * zp[ij];
-    zp[ij] = zz;
-}
-
-// End of a solve whose cycle count is only known on the device: the mixed
-// form leaves its iterate in zalt after an odd number of cycles, and a zero
-// right-hand side means z = 0.
-__global__ void k_solve_end(const MGState *st, double *z, const double *zalt, long N, int mixed) {
-    long ij = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (ij >= N) return;
-    if (st->done == 3) z[ij] = 0;
-    else if (mixed && (st->cycles & 1)) z[ij] = zalt[ij];
 }
 
 __global__ void k_zero(double *z, long N) {
     long ij = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (ij < N) z[ij] = 0;
+}
+
+__global__ void k_copy_d(const double *a, double *b, long N) {
+    long ij = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (ij < N) b[ij] = a[ij];
 }
 
 __global__ void k_reset_state(MGState *st) {
@@ -700,12 +895,241 @@ __global__ void k_reset_state(MGState *st) {
 }
 
 // ------------------------------------------------------------------------
+// Launchers
+// ------------------------------------------------------------------------
+
+static size_t tile_lds_bytes(int H, size_t elem, int tx, int ty) {
+    return 2 * elem * (size_t)(tx + 2 * H) * (ty + 2 * H);
+}
+
+namespace mgk {
+
+static int g_cus = 0, g_max_lds = 0;
+
+int set_attributes() {
+    static bool done = false;
+    if (done) return VDI_OK;
+    int dev = 0;
+    VDI_HIP(hipGetDevice(&dev));
+    VDI_HIP(hipDeviceGetAttribute(&g_max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    VDI_HIP(hipDeviceGetAttribute(&g_cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (g_max_lds < 2 * (int)sizeof(double) * kCoarsePoints * 2) return VDI_ERR_ARG;
+    const void *ks[] = {
+        (const void *)k_down<double, false>, (const void *)k_down<double, true>, (const void *)k_down<float, false>,
+        (const void *)k_down<float, true>, (const void *)k_up<double, false>, (const void *)k_up<double, true>,
+        (const void *)k_up<float, false>, (const void *)k_up<float, true>, (const void *)k_down_b<double, false>,
+        (const void *)k_down_b<double, true>, (const void *)k_down_b<float, false>, (const void *)k_down_b<float, true>,
+        (const void *)k_up_b<double, false>, (const void *)k_up_b<double, true>, (const void *)k_up_b<float, false>,
+        (const void *)k_up_b<float, true>, (const void *)k_coarse<double>, (const void *)k_coarse<float>,
+        (const void *)k_coarse_b<double>, (const void *)k_coarse_b<float>, (const void *)k_mx_up0<false>,
+        (const void *)k_mx_up0<true>, (const void *)k_mx_up0_b<false>, (const void *)k_mx_up0_b<true>};
+    // dynamic LDS may use what the kernel's own static LDS leaves
+    int dyn = g_max_lds;
+    for (const void *k : ks) {
+        hipFuncAttributes fa;
+        VDI_HIP(hipFuncGetAttributes(&fa, k));
+        dyn = std::min(dyn, g_max_lds - (int)fa.sharedSizeBytes);
+    }
+    for (const void *k : ks) VDI_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
+    g_max_lds = dyn;
+    done = true;
+    return VDI_OK;
+}
+
+int max_lds() { return g_max_lds; }
+
+int persistent_grid(const void *kernel, size_t lds, int nw) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, lds) != hipSuccess || per_cu < 1) per_cu = 1;
+    const int full = per_cu * std::max(1, g_cus);
+    const int g = std::min(full, nw);
+    return std::max(8, (g + 7) / 8 * 8);
+}
+
+const void *down_kernel(int prec, bool fem, bool b) {
+    if (prec == 0) return fem ? (b ? (const void *)k_down_b<double, true> : (const void *)k_down<double, true>)
+                              : (b ? (const void *)k_down_b<double, false> : (const void *)k_down<double, false>);
+    return fem ? (b ? (const void *)k_down_b<float, true> : (const void *)k_down<float, true>)
+               : (b ? (const void *)k_down_b<float, false> : (const void *)k_down<float, false>);
+}
+const void *up_kernel(int prec, bool fem, bool b) {
+    if (prec == 0) return fem ? (b ? (const void *)k_up_b<double, true> : (const void *)k_up<double, true>)
+                              : (b ? (const void *)k_up_b<double, false> : (const void *)k_up<double, false>);
+    return fem ? (b ? (const void *)k_up_b<float, true> : (const void *)k_up<float, true>)
+               : (b ? (const void *)k_up_b<float, false> : (const void *)k_up<float, false>);
+}
+const void *coarse_kernel(int prec, bool b) {
+    if (prec == 0) return b ? (const void *)k_coarse_b<double> : (const void *)k_coarse<double>;
+    return b ? (const void *)k_coarse_b<float> : (const void *)k_coarse<float>;
+}
+const void *mx_kernel(bool fem, bool b) {
+    return fem ? (b ? (const void *)k_mx_up0_b<true> : (const void *)k_mx_up0<true>)
+               : (b ? (const void *)k_mx_up0_b<false> : (const void *)k_mx_up0<false>);
+}
+
+template <class T> int down(const LvArgs<T> &a, bool fem, size_t lds, hipStream_t st) {
+    if (fem) hipLaunchKernelGGL((k_down<T, true>), dim3(a.nb), dim3(kBX, kBY), lds, st, a);
+    else hipLaunchKernelGGL((k_down<T, false>), dim3(a.nb), dim3(kBX, kBY), lds, st, a);
+    VDI_LAUNCHED();
+    return VDI_OK;
+}
+template <class T> int up(const LvArgs<T> &a, bool fem, size_t lds, hipStream_t st) {
+    if (fem) hipLaunchKernelGGL((k_up<T, true>), dim3(a.nb), dim3(kBX, kBY), lds, st, a);
+    else hipLaunchKernelGGL((k_up<T, false>), dim3(a.nb), dim3(kBX, kBY), lds, st, a);
+    VDI_LAUNCHED();
+    return VDI_OK;
+}
+template <class T> int coarse(const CoarseArgs<T> &a, size_t lds, hipStream_t st) {
+    hipLaunchKernelGGL(k_coarse<T>, dim3(1), dim3(kCoarseThreads), lds, st, a);
+    VDI_LAUNCHED();
+    return VDI_OK;
+}
+int mx(const MxArgs &a, bool fem, size_t lds, hipStream_t st) {
+    if (fem) hipLaunchKernelGGL(k_mx_up0<true>, dim3(a.nb), dim3(kBX, kBY), lds, st, a);
+    else hipLaunchKernelGGL(k_mx_up0<false>, dim3(a.nb), dim3(kBX, kBY), lds, st, a);
+    VDI_LAUNCHED();
+    return VDI_OK;
+}
+int begin(const BeginArgs &a, hipStream_t st) {
+    hipLaunchKernelGGL(k_begin, dim3(a.nb), dim3(256), 0, st, a);
+    VDI_LAUNCHED();
+    return VDI_OK;
+}
+int vec(const VecArgs &a, hipStream_t st) {
+    hipLaunchKernelGGL(k_vec, dim3(a.nb), dim3(256), 0, st, a);
+    VDI_LAUNCHED();
+    return VDI_OK;
+}
+
+template <class T> int down_b(const LvArgs<T> *A, const WorkItem *w, int nw, bool fem, size_t lds, int grid,
+                              BatchAux aux, hipStream_t st) {
+    if (fem) hipLaunchKernelGGL((k_down_b<T, true>), dim3(grid), dim3(kBX, kBY), lds, st, A, w, nw, aux);
+    else hipLaunchKernelGGL((k_down_b<T, false>), dim3(grid), dim3(kBX, kBY), lds, st, A, w, nw, aux);
+    VDI_LAUNCHED();
+    return VDI_OK;
+}
+template <class T> int up_b(const LvArgs<T> *A, const WorkItem *w, int nw, bool fem, size_t lds, int grid,
+                            BatchAux aux, hipStream_t st) {
+    if (fem) hipLaunchKernelGGL((k_up_b<T, true>), dim3(grid), dim3(kBX, kBY), lds, st, A, w, nw, aux);
+    else hipLaunchKernelGGL((k_up_b<T, false>), dim3(grid), dim3(kBX, kBY), lds, st, A, w, nw, aux);
+    VDI_LAUNCHED();
+    return VDI_OK;
+}
+template <class T> int coarse_b(const CoarseArgs<T> *A, const WorkItem *w, int nw, size_t lds, int grid,
+                                BatchAux aux, hipStream_t st) {
+    hipLaunchKernelGGL(k_coarse_b<T>, dim3(grid), dim3(kCoarseThreads), lds, st, A, w, nw, aux);
+    VDI_LAUNCHED();
+    return VDI_OK;
+}
+int mx_b(const MxArgs *A, const WorkItem *w, int nw, bool fem, size_t lds, int grid, BatchAux aux, hipStream_t st) {
+    if (fem) hipLaunchKernelGGL(k_mx_up0_b<true>, dim3(grid), dim3(kBX, kBY), lds, st, A, w, nw, aux);
+    else hipLaunchKernelGGL(k_mx_up0_b<false>, dim3(grid), dim3(kBX, kBY), lds, st, A, w, nw, aux);
+    VDI_LAUNCHED();
+    return VDI_OK;
+}
+int begin_b(const BeginArgs *A, const WorkItem *w, int nw, int grid, BatchAux aux, hipStream_t st) {
+    hipLaunchKernelGGL(k_begin_b, dim3(grid), dim3(256), 0, st, A, w, nw, aux);
+    VDI_LAUNCHED();
+    return VDI_OK;
+}
+int vec_b(const VecArgs *A, const WorkItem *w, int nw, int grid, BatchAux aux, hipStream_t st) {
+    hipLaunchKernelGGL(k_vec_b, dim3(grid), dim3(256), 0, st, A, w, nw, aux);
+    VDI_LAUNCHED();
+    return VDI_OK;
+}
+
+template int down<double>(const LvArgs<double> &, bool, size_t, hipStream_t);
+template int down<float>(const LvArgs<float> &, bool, size_t, hipStream_t);
+template int up<double>(const LvArgs<double> &, bool, size_t, hipStream_t);
+template int up<float>(const LvArgs<float> &, bool, size_t, hipStream_t);
+template int coarse<double>(const CoarseArgs<double> &, size_t, hipStream_t);
+template int coarse<float>(const CoarseArgs<float> &, size_t, hipStream_t);
+template int down_b<double>(const LvArgs<double> *, const WorkItem *, int, bool, size_t, int, BatchAux, hipStream_t);
+template int down_b<float>(const LvArgs<float> *, const WorkItem *, int, bool, size_t, int, BatchAux, hipStream_t);
+template int up_b<double>(const LvArgs<double> *, const WorkItem *, int, bool, size_t, int, BatchAux, hipStream_t);
+template int up_b<float>(const LvArgs<float> *, const WorkItem *, int, bool, size_t, int, BatchAux, hipStream_t);
+template int coarse_b<double>(const CoarseArgs<double> *, const WorkItem *, int, size_t, int, BatchAux, hipStream_t);
+template int coarse_b<float>(const CoarseArgs<float> *, const WorkItem *, int, size_t, int, BatchAux, hipStream_t);
+
+}  // namespace mgk
+
+// ------------------------------------------------------------------------
+// KernelTimer (declared in vdi_internal.hpp)
+// ------------------------------------------------------------------------
+
+__global__ void k_stamp_reset(Stamp *s, int n) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) {
+        s[k].t0 = ~0ull;
+        s[k].t1 = 0;
+    }
+}
+
+int KernelTimer::setup(int c, int capacity) {
+    release();
+    cls = c;
+    if (c < 0) return VDI_OK;
+    cap = capacity;
+    VDI_TRY(stamps.alloc(sizeof(Stamp) * (size_t)cap));
+    VDI_HIP(hipHostMalloc((void **)&host, sizeof(Stamp) * (size_t)cap, hipHostMallocDefault));
+    slot_bytes.assign(cap, 0.0);
+    int dev = 0, khz = 0;
+    VDI_HIP(hipGetDevice(&dev));
+    VDI_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    ticks_per_ms = khz > 0 ? (double)khz : 1e5;
+    return VDI_OK;
+}
+
+int KernelTimer::reset(int a, int b, hipStream_t st) {
+    if (b <= a) return VDI_OK;
+    hipLaunchKernelGGL(k_stamp_reset, dim3(cdiv(b - a, 256)), dim3(256), 0, st, stamps.as<Stamp>() + a, b - a);
+    VDI_LAUNCHED();
+    return VDI_OK;
+}
+
+int KernelTimer::harvest(int a, int b, hipStream_t st) {
+    if (b <= a) return VDI_OK;
+    VDI_HIP(hipMemcpyAsync(host + a, stamps.as<Stamp>() + a, sizeof(Stamp) * (size_t)(b - a), hipMemcpyDeviceToHost, st));
+    VDI_HIP(hipStreamSynchronize(st));
+    for (int k = a; k < b; k++) {
+        const Stamp &s = host[k];
+        if (s.t0 == ~0ull || s.t1 < s.t0) continue;  // every workgroup returned at once
+        dur_ms.push_back((float)((double)(s.t1 - s.t0) / ticks_per_ms));
+        bytes.push_back(slot_bytes[k]);
+    }
+    return VDI_OK;
+}
+
+void KernelTimer::release() {
+    stamps.release();
+    if (host) (void)hipHostFree(host);
+    host = nullptr;
+    cap = next = reserved = 0;
+    slot_bytes.clear();
+    dur_ms.clear();
+    bytes.clear();
+    cls = -1;
+}
+
+// ------------------------------------------------------------------------
 // Driver
 // ------------------------------------------------------------------------
 
-static size_t tile_lds_bytes(int H, size_t elem) { return 2 * elem * (size_t)(kTX + 2 * H) * (kTY + 2 * H); }
+static int halo_down(bool fem, int nu) { return fem ? 4 * nu + 2 : 2 * nu + 1; }
+static int halo_up(bool fem, int nu, int norm) { return (fem ? 4 : 2) * nu + norm; }
+
+// Largest default shape that still gives every CU a tile.
+static int default_tile(int m, int n) {
+    const int prefs[] = {0, 4, 2, 1};
+    for (int s : prefs) {
+        const TileShape &t = kTileShapes[s];
+        if ((long)cdiv(m, t.tx) * cdiv(n, t.ty) >= 256) return s;
+    }
+    return 1;
+}
 
 int MG::init(int kind_, int m, int n, double cx, double cy, int x_sym, double *z0, double *b0, hipStream_t stream) {
+    VDI_TRY(mgk::set_attributes());
     kind = kind_;
     fem = kind == VDI_FEM;
     st = stream;
@@ -741,7 +1165,7 @@ int MG::init(int kind_, int m, int n, double cx, double cy, int x_sym, double *z
     }
     nl = (int)lm.size();
     lc = nl - 1;
-    while (lc > 0 && (long)lm[lc - 1] * ln[lc - 1] <= kCoarsePoints) lc--;
+    while (lc > 0 && (long)lm[lc - 1] * ln[lc - 1] <= coarse_pts) lc--;
     if (nl - lc > kMaxCoarse) return VDI_ERR_ARG;
     size_t coarse = 0;
     for (int l = 1; l < nl; l++) coarse += (size_t)lm[l] * ln[l];
@@ -780,28 +1204,32 @@ int MG::init(int kind_, int m, int n, double cx, double cy, int x_sym, double *z
         }
     }
     set_coeffs(cx, cy);
-    long maxblk = std::max<long>(1024, (long)cdiv(lm[0], kTX) * cdiv(ln[0], kTY));
+    for (int pr = 0; pr < 2; pr++)
+        for (int w = 0; w < 2; w++) {
+            tile[pr][w].assign(nl, 0);
+            for (int l = 0; l < nl; l++) tile[pr][w][l] = default_tile(lm[l], ln[l]);
+        }
+    tile_mx = tile[1][1][0];
+    // partial sums: one per tile of the smallest shape, two per begin block
+    const TileShape &small = kTileShapes[1];
+    long maxblk = std::max<long>(2048, (long)cdiv(lm[0], small.tx) * cdiv(ln[0], small.ty));
     VDI_TRY(partial_.alloc(2 * maxblk * sizeof(double)));
     VDI_TRY(state_.alloc(sizeof(MGState)));
     if (!host_state_) VDI_HIP(hipHostMalloc((void **)&host_state_, sizeof(MGState), hipHostMallocDefault));
     coarse_lds_ = 0;
     for (int l = lc; l < nl; l++) coarse_lds_ += 2 * sizeof(double) * (size_t)lm[l] * ln[l];
-    static bool attr_set = false;
-    if (!attr_set) {
-        int maxlds = 0;
-        VDI_HIP(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, 0));
-        const size_t big = std::max<size_t>(tile_lds_bytes(4 * kMaxNu + 2, sizeof(double)),
-                                            2 * sizeof(double) * kCoarsePoints * 2);
-        if ((size_t)maxlds < big) return VDI_ERR_ARG;
-        const void *ks[] = {(const void *)k_cc_down<double>, (const void *)k_cc_up<double>,
-                            (const void *)k_fem_down<double>, (const void *)k_fem_up<double>,
-                            (const void *)k_coarse<double>, (const void *)k_cc_down<float>,
-                            (const void *)k_cc_up<float>, (const void *)k_fem_down<float>,
-                            (const void *)k_fem_up<float>, (const void *)k_coarse<float>,
-                            (const void *)k_mx_up0<false>, (const void *)k_mx_up0<true>};
-        for (const void *k : ks) VDI_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)big));
-        attr_set = true;
-    }
+    return VDI_OK;
+}
+
+int MG::set_coarse_points(int pts) {
+    if (pts < 1 || pts > kCoarsePoints) return VDI_ERR_ARG;
+    int l2 = nl - 1;
+    while (l2 > 0 && (long)lm[l2 - 1] * ln[l2 - 1] <= pts) l2--;
+    if (nl - l2 > kMaxCoarse) return VDI_ERR_ARG;
+    coarse_pts = pts;
+    lc = l2;
+    coarse_lds_ = 0;
+    for (int l = lc; l < nl; l++) coarse_lds_ += 2 * sizeof(double) * (size_t)lm[l] * ln[l];
     return VDI_OK;
 }
 
@@ -833,75 +1261,213 @@ MGCtl MG::ctl() const {
     return c;
 }
 
+int MG::coarse_points() const {
+    int s = 0;
+    for (int l = lc; l < nl; l++) s += lm[l] * ln[l];
+    return s;
+}
+
+size_t MG::down_lds(int prec, int l) const {
+    const TileShape &t = kTileShapes[tile[prec][0][l]];
+    return tile_lds_bytes(halo_down(fem, nu1), prec ? sizeof(float) : sizeof(double), t.tx, t.ty);
+}
+size_t MG::up_lds(int prec, int l) const {
+    const TileShape &t = kTileShapes[tile[prec][1][l]];
+    return tile_lds_bytes(halo_up(fem, nu2, l == 0), prec ? sizeof(float) : sizeof(double), t.tx, t.ty);
+}
+size_t MG::mx_lds() const {
+    const TileShape &t = kTileShapes[tile_mx];
+    const int H = halo_up(fem, nu2, 1);
+    return (size_t)(t.tx + 2 * H) * (t.ty + 2 * H) * (sizeof(double) + 2 * sizeof(float));
+}
+
 template <class T>
-static CoarseDesc coarse_desc(const std::vector<LevelT<T>> &Lv, int lc, int nl) {
-    CoarseDesc d{};
+LvArgs<T> MG::down_args(const std::vector<LevelT<T>> &H, int l, int zero_z) {
+    const int prec = sizeof(T) == 4;
+    LvArgs<T> a{};
+    a.F = H[l];
+    a.C = H[l + 1];
+    a.cop = cc;
+    a.fop = fe;
+    a.nu = nu1;
+    a.flag = zero_z;
+    const TileShape &t = kTileShapes[tile[prec][0][l]];
+    a.tx = t.tx;
+    a.ty = t.ty;
+    if (fem) {
+        a.nbx = cdiv(a.C.m, t.tx / 2);
+        a.nb = a.nbx * cdiv(a.C.n, t.ty / 2);
+    } else {
+        a.nbx = cdiv(a.F.m, t.tx);
+        a.nb = a.nbx * cdiv(a.F.n, t.ty);
+    }
+    a.ctl = ctl();
+    a.tol2 = 0;
+    a.ts = nullptr;
+    return a;
+}
+
+template <class T>
+LvArgs<T> MG::up_args(const std::vector<LevelT<T>> &H, int l, int norm, double tol2) {
+    const int prec = sizeof(T) == 4;
+    LvArgs<T> a{};
+    a.F = H[l];
+    a.C = H[l + 1];
+    a.cop = cc;
+    a.fop = fe;
+    a.nu = nu2;
+    a.flag = norm;
+    const TileShape &t = kTileShapes[tile[prec][1][l]];
+    a.tx = t.tx;
+    a.ty = t.ty;
+    a.nbx = cdiv(a.F.m, t.tx);
+    a.nb = a.nbx * cdiv(a.F.n, t.ty);
+    a.ctl = ctl();
+    a.tol2 = tol2;
+    a.ts = nullptr;
+    return a;
+}
+
+template <class T>
+CoarseArgs<T> MG::coarse_args(const std::vector<LevelT<T>> &H, double tol2) {
+    CoarseArgs<T> a{};
+    CoarseDesc &d = a.d;
     d.nl = nl - lc;
     int off = 0;
     for (int k = 0; k < d.nl; k++) {
-        const LevelT<T> &v = Lv[lc + k];
+        const LevelT<T> &v = H[lc + k];
         d.m[k] = v.m;
         d.n[k] = v.n;
         d.cx[k] = (double)v.cx;
         d.cy[k] = (double)v.cy;
+        d.inv[k] = (double)v.inv_int;
         d.off[k] = off;
         off += 2 * v.m * v.n;
     }
-    d.gz = reinterpret_cast<double *>(Lv[lc].z);
-    d.gb = reinterpret_cast<double *>(Lv[lc].b);
+    d.gz = reinterpret_cast<double *>(H[lc].z);
+    d.gb = reinterpret_cast<double *>(H[lc].b);
     d.top = lc == 0;
-    return d;
+    a.cop = cc;
+    a.fop = fe;
+    a.fem = fem;
+    a.nu1 = nu1;
+    a.nu2 = nu2;
+    a.ncs = ncs;
+    a.ctl = ctl();
+    a.tol2 = tol2;
+    a.ts = nullptr;
+    return a;
+}
+
+// Mixed form: one FP32 V-cycle on A e = r32 from e = 0, then z += e and the
+// FP64 residual (k_mx_up0).  z and r32 alternate between two buffers from
+// one cycle to the next; parity = cycles enqueued so far mod 2.
+LvArgs<float> MG::mx_down0_args(int parity) {
+    LvArgs<float> a = down_args<float>(LF, 0, 1);
+    a.F.b = r32_[parity & 1];
+    return a;
+}
+
+MxArgs MG::mx_args(int parity, double tol2) {
+    const int par = parity & 1;
+    MxArgs a{};
+    a.Z = L[0];
+    a.z_in = par ? L[0].zalt : L[0].z;
+    a.z_out = par ? L[0].z : L[0].zalt;
+    a.E = LF[0];
+    a.E.b = r32_[par];
+    a.r_out = r32_[par ^ 1];
+    a.C = LF[1];
+    a.cop = cc;
+    a.fop = fe;
+    a.nu = nu2;
+    const TileShape &t = kTileShapes[tile_mx];
+    a.tx = t.tx;
+    a.ty = t.ty;
+    a.nbx = cdiv(L[0].m, t.tx);
+    a.nb = a.nbx * cdiv(L[0].n, t.ty);
+    a.ctl = ctl();
+    a.tol2 = tol2;
+    a.ts = nullptr;
+    return a;
+}
+
+BeginArgs MG::begin_args(double tol2) {
+    BeginArgs a{};
+    a.fem = fem;
+    a.L = L[0];
+    a.cop = cc;
+    a.fop = fe;
+    a.ctl = ctl();
+    a.tol2 = tol2;
+    a.r32 = mixed_active() ? r32_[0] : nullptr;
+    a.nb = (int)std::min<long>(1024, cdiv((long)L[0].m * L[0].n, 256));
+    a.ts = nullptr;
+    return a;
+}
+
+VecArgs MG::vec_args(int op, const double *coef) {
+    VecArgs a{};
+    a.op = op;
+    a.mixed = mixed_active();
+    a.N = (long)L[0].m * L[0].n;
+    a.z = L[0].z;
+    a.zp = zprev_.as<double>();
+    a.zalt = L[0].zalt;
+    a.b = L[0].b;
+    a.coef = coef;
+    a.state = state_.as<MGState>();
+    a.fm = fe.fm;
+    a.M = L[0].m;
+    const long work = op == VEC_DIRICHLET ? a.M : a.N;
+    a.nb = (int)std::min<long>(2048, std::max<long>(1, cdiv(work, 256)));
+    return a;
+}
+
+double MG::down_bytes(int prec, int l, int zero_z) const {
+    const double e = prec ? 4.0 : 8.0;
+    return e * ((zero_z ? 2.0 : 3.0) * lm[l] * ln[l] + 1.0 * lm[l + 1] * ln[l + 1]);
+}
+double MG::up_bytes(int prec, int l) const {
+    const double e = prec ? 4.0 : 8.0;
+    return e * (3.0 * lm[l] * ln[l] + 1.0 * lm[l + 1] * ln[l + 1]);
+}
+double MG::coarse_bytes(int prec) const {
+    const double e = prec ? 4.0 : 8.0;
+    return e * (lc == 0 ? 3.0 : 2.0) * lm[lc] * ln[lc];
+}
+// reads e (zalt), r32, coarse e, z64, b64; writes z64, r32
+double MG::mx_bytes() const {
+    const double N0 = (double)lm[0] * ln[0];
+    return 4.0 * (3.0 * N0 + (double)lm[1] * ln[1]) + 8.0 * 3.0 * N0;
 }
 
 // Enqueues the down kernel of level l of hierarchy H (FP64 or FP32).
 template <class T>
 int MG::launch_down(const std::vector<LevelT<T>> &H, int l, int zero_z) {
-    const LevelT<T> &F = H[l], &C = H[l + 1];
-    const dim3 blk(kBX, kBY);
-    const int slot = timer ? timer->begin(l == 0 ? VDI_KCLASS_MG_DOWN0 : VDI_KCLASS_MG_MID, st) : -1;
-    if (!fem) {
-        dim3 g(cdiv(F.m, kTX), cdiv(F.n, kTY));
-        hipLaunchKernelGGL(k_cc_down<T>, g, blk, tile_lds_bytes(2 * nu1 + 1, sizeof(T)), st, F, C, cc, nu1, zero_z,
-                           (const MGState *)state_.as<MGState>());
-    } else {
-        dim3 g(cdiv(C.m, kTX / 2), cdiv(C.n, kTY / 2));
-        hipLaunchKernelGGL(k_fem_down<T>, g, blk, tile_lds_bytes(4 * nu1 + 2, sizeof(T)), st, F, C, fe, nu1, zero_z,
-                           (const MGState *)state_.as<MGState>());
-    }
-    VDI_LAUNCHED();
-    // reads z (unless the level starts from zero) and b; writes zalt and coarse b
-    if (timer) timer->end(slot, st, sizeof(T) * ((zero_z ? 2.0 : 3.0) * F.m * F.n + 1.0 * C.m * C.n));
-    return VDI_OK;
+    const int prec = sizeof(T) == 4;
+    LvArgs<T> a = down_args<T>(H, l, zero_z);
+    if (timer) a.ts = timer->slot(l == 0 ? VDI_KCLASS_MG_DOWN0 : VDI_KCLASS_MG_MID, down_bytes(prec, l, zero_z));
+    return mgk::down<T>(a, fem, down_lds(prec, l), st);
 }
 
 template <class T>
 int MG::launch_up(const std::vector<LevelT<T>> &H, int l, int norm, double tol2) {
-    const LevelT<T> &F = H[l], &C = H[l + 1];
-    const dim3 blk(kBX, kBY);
-    const int slot = timer ? timer->begin(l == 0 ? VDI_KCLASS_MG_UP0 : VDI_KCLASS_MG_MID, st) : -1;
-    dim3 g(cdiv(F.m, kTX), cdiv(F.n, kTY));
-    if (!fem)
-        hipLaunchKernelGGL(k_cc_up<T>, g, blk, tile_lds_bytes(2 * nu2 + norm, sizeof(T)), st, F, C, cc, nu2, norm,
-                           ctl(), tol2);
-    else
-        hipLaunchKernelGGL(k_fem_up<T>, g, blk, tile_lds_bytes(4 * nu2 + norm, sizeof(T)), st, F, C, fe, nu2, norm,
-                           ctl(), tol2);
-    VDI_LAUNCHED();
-    // reads zalt, b, coarse correction; writes z
-    if (timer) timer->end(slot, st, sizeof(T) * (3.0 * F.m * F.n + 1.0 * C.m * C.n));
-    return VDI_OK;
+    const int prec = sizeof(T) == 4;
+    LvArgs<T> a = up_args<T>(H, l, norm, tol2);
+    if (timer) a.ts = timer->slot(l == 0 ? VDI_KCLASS_MG_UP0 : VDI_KCLASS_MG_MID, up_bytes(prec, l));
+    return mgk::up<T>(a, fem, up_lds(prec, l), st);
 }
 
 template <class T>
 int MG::launch_coarse(const std::vector<LevelT<T>> &H, double tol2) {
-    CoarseDesc d = coarse_desc<T>(H, lc, nl);
-    const int slot = timer ? timer->begin(VDI_KCLASS_MG_COARSE, st) : -1;
-    hipLaunchKernelGGL(k_coarse<T>, dim3(1), dim3(kCoarseThreads), coarse_lds_ / sizeof(double) * sizeof(T), st,
-                       (int)fem, d, cc, fe, nu1, nu2, ncs, ctl(), tol2);
-    VDI_LAUNCHED();
-    if (timer) timer->end(slot, st, sizeof(T) * (d.top ? 3.0 : 2.0) * d.m[0] * d.n[0]);
-    return VDI_OK;
+    const int prec = sizeof(T) == 4;
+    CoarseArgs<T> a = coarse_args<T>(H, tol2);
+    if (timer) a.ts = timer->slot(VDI_KCLASS_MG_COARSE, coarse_bytes(prec));
+    return mgk::coarse<T>(a, coarse_lds(prec), st);
 }
+
+int MG::launch_vec(const VecArgs &a) { return mgk::vec(a, st); }
 
 bool MG::mixed_active() const { return form == VDI_FORM_MIXED && lc > 0; }
 
@@ -913,31 +1479,18 @@ int MG::launch_cycle(double tol2) {
         for (int l = lc - 1; l >= 0; l--) VDI_TRY(launch_up<double>(L, l, l == 0, tol2));
         return VDI_OK;
     }
-    // Mixed form: one FP32 V-cycle on A e = r32 from e = 0, then z += e and
-    // the FP64 residual (k_mx_up0).  z and r32 alternate between two
-    // buffers from one cycle to the next.
     const int par = mx_enqueued_ & 1;
-    double *z_in = par ? L[0].zalt : L[0].z, *z_out = par ? L[0].z : L[0].zalt;
-    float *r_in = r32_[par], *r_out = r32_[par ^ 1];
-    LF[0].b = r_in;
-    for (int l = 0; l < lc; l++) VDI_TRY(launch_down<float>(LF, l, 1));
+    {
+        LvArgs<float> a = mx_down0_args(par);
+        if (timer) a.ts = timer->slot(VDI_KCLASS_MG_DOWN0, down_bytes(1, 0, 1));
+        VDI_TRY(mgk::down<float>(a, fem, down_lds(1, 0), st));
+    }
+    for (int l = 1; l < lc; l++) VDI_TRY(launch_down<float>(LF, l, 1));
     VDI_TRY(launch_coarse<float>(LF, tol2));
     for (int l = lc - 1; l >= 1; l--) VDI_TRY(launch_up<float>(LF, l, 0, tol2));
-    const dim3 blk(kBX, kBY);
-    const int H = (fem ? 4 : 2) * nu2 + 1;
-    const size_t lds = (size_t)(kTX + 2 * H) * (kTY + 2 * H) * (sizeof(double) + 2 * sizeof(float));
-    const int slot = timer ? timer->begin(VDI_KCLASS_MG_UP0, st) : -1;
-    dim3 g(cdiv(L[0].m, kTX), cdiv(L[0].n, kTY));
-    if (fem)
-        hipLaunchKernelGGL(k_mx_up0<true>, g, blk, lds, st, L[0], (const double *)z_in, z_out, LF[0], r_out, LF[1], cc,
-                           fe, nu2, ctl(), tol2);
-    else
-        hipLaunchKernelGGL(k_mx_up0<false>, g, blk, lds, st, L[0], (const double *)z_in, z_out, LF[0], r_out, LF[1], cc,
-                           fe, nu2, ctl(), tol2);
-    VDI_LAUNCHED();
-    // reads e (zalt), r32, coarse e, z64, b64; writes z64, r32
-    const double N0 = (double)L[0].m * L[0].n;
-    if (timer) timer->end(slot, st, 4.0 * (3.0 * N0 + (double)LF[1].m * LF[1].n) + 8.0 * 3.0 * N0);
+    MxArgs a = mx_args(par, tol2);
+    if (timer) a.ts = timer->slot(VDI_KCLASS_MG_UP0, mx_bytes());
+    VDI_TRY(mgk::mx(a, fem, mx_lds(), st));
     mx_enqueued_++;
     return VDI_OK;
 }
@@ -951,37 +1504,27 @@ int MG::mixed_finish(int executed) {
 }
 
 double MG::cycle_bytes() const {
-    auto N = [&](int l) { return (double)lm[l] * ln[l]; };
     double b = 0;
     if (!mixed_active()) {
-        for (int l = 0; l < lc; l++) b += 8.0 * ((l == 0 ? 3.0 : 2.0) * N(l) + N(l + 1));
-        b += 8.0 * (lc == 0 ? 3.0 : 2.0) * N(lc);
-        for (int l = 0; l < lc; l++) b += 8.0 * (3.0 * N(l) + N(l + 1));
+        for (int l = 0; l < lc; l++) b += down_bytes(0, l, l > 0) + up_bytes(0, l);
+        b += coarse_bytes(0);
     } else {
-        for (int l = 0; l < lc; l++) b += 4.0 * (2.0 * N(l) + N(l + 1));
-        b += 4.0 * 2.0 * N(lc);
-        for (int l = 1; l < lc; l++) b += 4.0 * (3.0 * N(l) + N(l + 1));
-        b += 4.0 * (3.0 * N(0) + N(1)) + 8.0 * 3.0 * N(0);
+        for (int l = 0; l < lc; l++) b += down_bytes(1, l, 1);
+        for (int l = 1; l < lc; l++) b += up_bytes(1, l);
+        b += coarse_bytes(1) + mx_bytes();
     }
     return b;
 }
 
 int MG::enforce_dirichlet() {
     if (!fem) return VDI_OK;
-    hipLaunchKernelGGL(k_fem_dirichlet, dim3(cdiv(L[0].m, 256)), dim3(256), 0, st, L[0], fe);
-    VDI_LAUNCHED();
-    return VDI_OK;
+    return launch_vec(vec_args(VEC_DIRICHLET, nullptr));
 }
 
 int MG::begin(double tol) {
     VDI_TRY(enforce_dirichlet());
     mx_enqueued_ = 0;
-    const long N = (long)L[0].m * L[0].n;
-    const int nb = (int)std::min<long>(1024, (N + 255) / 256);
-    hipLaunchKernelGGL(k_begin, dim3(nb), dim3(256), 0, st, (int)fem, L[0], cc, fe, ctl(), tol * tol,
-                       mixed_active() ? r32_[0] : (float *)nullptr);
-    VDI_LAUNCHED();
-    return VDI_OK;
+    return mgk::begin(begin_args(tol * tol), st);
 }
 
 int MG::read_state(MGState *out) {
@@ -1042,26 +1585,15 @@ int MG::solve(double tol, int max_cycles, int *cycles) {
 }
 
 int MG::initial_guess(double t) {
-    long N = unknowns();
-    if (scount < 2) {
-        if (scount == 1) {
-            VDI_HIP(hipMemcpyAsync(zprev_.p, L[0].z, N * sizeof(double), hipMemcpyDeviceToDevice, st));
-            t0 = t;
-        } else {
-            t_prev = t;
-        }
-        scount++;
-        return VDI_OK;
-    }
-    if (std::fabs(t0 - t_prev) < std::numeric_limits<double>::epsilon() * 10) {
-        fputs("Multigrid routine called for duplicate times\n", stderr);
-        return VDI_OK;
-    }
-    double a = 1. / (t0 - t_prev), bb = (t - t_prev) * a, c = (t - t0) * a;
+    const long N = unknowns();
+    const int was = scount;
+    double bb = 1, c = 0;
+    const int r = guess_coeffs(t, &bb, &c);
+    if (r == 0 && was == 1)
+        VDI_HIP(hipMemcpyAsync(zprev_.p, L[0].z, N * sizeof(double), hipMemcpyDeviceToDevice, st));
+    if (r != 1) return VDI_OK;
     hipLaunchKernelGGL(k_extrapolate, dim3(cdiv(N, 256)), dim3(256), 0, st, L[0].z, zprev_.as<double>(), N, bb, c);
     VDI_LAUNCHED();
-    t_prev = t0;
-    t0 = t;
     return VDI_OK;
 }
 
@@ -1084,21 +1616,12 @@ int MG::guess_coeffs(double t, double *bb, double *c) {
     return 1;
 }
 
-int MG::enqueue_extrapolate(const double *coef) {
-    const long N = unknowns();
-    hipLaunchKernelGGL(k_extrapolate_dev, dim3(cdiv(N, 256)), dim3(256), 0, st, L[0].z, zprev_.as<double>(), N, coef);
-    VDI_LAUNCHED();
-    return VDI_OK;
-}
+int MG::enqueue_extrapolate(const double *coef) { return launch_vec(vec_args(VEC_EXTRAP, coef)); }
 
 int MG::enqueue_fixed(double tol, int budget) {
     VDI_TRY(begin(tol));
     for (int c = 0; c < budget; c++) VDI_TRY(launch_cycle(tol * tol));
-    const long N = unknowns();
-    hipLaunchKernelGGL(k_solve_end, dim3(cdiv(N, 256)), dim3(256), 0, st, (const MGState *)state_.as<MGState>(),
-                       L[0].z, (const double *)L[0].zalt, N, (int)mixed_active());
-    VDI_LAUNCHED();
-    return VDI_OK;
+    return launch_vec(vec_args(VEC_SOLVE_END, nullptr));
 }
 
 int MG::apply(const double *z, double *out) {
@@ -1109,5 +1632,119 @@ int MG::apply(const double *z, double *out) {
     VDI_LAUNCHED();
     return VDI_OK;
 }
+
+int MG::set_tile(int prec, int which, int level, int shape) {
+    if (prec < 0 || prec > 1 || which < 0 || which > 2 || shape < 0 || shape >= kNumTileShapes) return VDI_ERR_ARG;
+    if (which == 2) {
+        if (level != 0) return VDI_ERR_ARG;
+        const int old = tile_mx;
+        tile_mx = shape;
+        if (mx_lds() > (size_t)mgk::max_lds()) { tile_mx = old; return VDI_ERR_ARG; }
+        return VDI_OK;
+    }
+    if (level < 0 || level >= lc) return VDI_ERR_ARG;
+    const int old = tile[prec][which][level];
+    tile[prec][which][level] = shape;
+    const size_t need = which == 0 ? down_lds(prec, level) : up_lds(prec, level);
+    if (need > (size_t)mgk::max_lds()) { tile[prec][which][level] = old; return VDI_ERR_ARG; }
+    return VDI_OK;
+}
+
+// Times reps launches of one kernel (after one untimed launch) with events.
+template <class F>
+static int time_launches(hipStream_t st, int reps, F &&launch, float *ms) {
+    hipEvent_t e0, e1;
+    VDI_HIP(hipEventCreate(&e0));
+    VDI_HIP(hipEventCreate(&e1));
+    int rc = launch();
+    if (rc == VDI_OK) {
+        (void)hipEventRecord(e0, st);
+        for (int r = 0; r < reps && rc == VDI_OK; r++) rc = launch();
+        (void)hipEventRecord(e1, st);
+        (void)hipEventSynchronize(e1);
+        (void)hipEventElapsedTime(ms, e0, e1);
+        *ms /= reps;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipGetLastError();
+    return rc;
+}
+
+int MG::autotune(int reps) {
+    if (reps < 1) reps = 1;
+    const long N0 = (long)lm[0] * ln[0];
+    DevBuf save;
+    VDI_TRY(save.alloc(2 * N0 * sizeof(double)));
+    VDI_HIP(hipMemcpyAsync(save.p, L[0].z, N0 * sizeof(double), hipMemcpyDeviceToDevice, st));
+    VDI_HIP(hipMemcpyAsync(save.as<double>() + N0, L[0].b, N0 * sizeof(double), hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_reset_state, dim3(1), dim3(1), 0, st, state_.as<MGState>());
+    VDI_LAUNCHED();
+    KernelTimer *tsave = timer;
+    timer = nullptr;
+    int rc = VDI_OK;
+    // tile every level that any hand-over candidate below leaves tiled
+    (void)set_coarse_points(128);
+    for (int prec = 0; prec < 2 && rc == VDI_OK; prec++) {
+        for (int which = 0; which < 3 && rc == VDI_OK; which++) {
+            for (int l = 0; l < lc && rc == VDI_OK; l++) {
+                if (which == 2 && (prec == 0 || l > 0)) continue;
+                if (which == 1 && prec == 1 && l == 0) continue;  // the mixed form's level-0 up is k_mx_up0
+                int best = -1;
+                float best_ms = 0;
+                for (int s = 0; s < kNumTileShapes && rc == VDI_OK; s++) {
+                    if (set_tile(prec, which, l, s) != VDI_OK) continue;
+                    float ms = 0;
+                    auto launch = [&]() -> int {
+                        if (which == 2) return mgk::mx(mx_args(0, -1.0), fem, mx_lds(), st);
+                        if (prec == 0) {
+                            if (which == 0) return mgk::down<double>(down_args<double>(L, l, l > 0), fem, down_lds(0, l), st);
+                            return mgk::up<double>(up_args<double>(L, l, l == 0, -1.0), fem, up_lds(0, l), st);
+                        }
+                        if (which == 0) {
+                            LvArgs<float> a = l == 0 ? mx_down0_args(0) : down_args<float>(LF, l, 1);
+                            return mgk::down<float>(a, fem, down_lds(1, l), st);
+                        }
+                        return mgk::up<float>(up_args<float>(LF, l, 0, -1.0), fem, up_lds(1, l), st);
+                    };
+                    rc = time_launches(st, reps, launch, &ms);
+                    if (rc == VDI_OK && (best < 0 || ms < best_ms)) { best = s; best_ms = ms; }
+                }
+                if (best >= 0) (void)set_tile(prec, which, l, best);
+            }
+        }
+    }
+    // Where the single-workgroup coarse solve takes over: whole V-cycles
+    // (tol2 < 0: never converge) for each candidate size.
+    if (rc == VDI_OK) {
+        const int cands[] = {kCoarsePoints, 1024, 512, 128};
+        int best = coarse_pts;
+        float best_ms = -1;
+        for (int pts : cands) {
+            if (set_coarse_points(pts) != VDI_OK) continue;
+            hipLaunchKernelGGL(k_reset_state, dim3(1), dim3(1), 0, st, state_.as<MGState>());
+            float ms = 0;
+            auto launch = [&]() -> int { return launch_cycle(-1.0); };
+            rc = time_launches(st, reps, launch, &ms);
+            if (rc != VDI_OK) break;
+            if (best_ms < 0 || ms < best_ms) { best = pts; best_ms = ms; }
+        }
+        (void)set_coarse_points(best);
+    }
+    timer = tsave;
+    VDI_HIP(hipMemcpyAsync(L[0].z, save.p, N0 * sizeof(double), hipMemcpyDeviceToDevice, st));
+    VDI_HIP(hipMemcpyAsync(L[0].b, save.as<double>() + N0, N0 * sizeof(double), hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_reset_state, dim3(1), dim3(1), 0, st, state_.as<MGState>());
+    VDI_LAUNCHED();
+    VDI_HIP(hipStreamSynchronize(st));
+    return rc;
+}
+
+template LvArgs<double> MG::down_args<double>(const std::vector<Level> &, int, int);
+template LvArgs<float> MG::down_args<float>(const std::vector<LevelF> &, int, int);
+template LvArgs<double> MG::up_args<double>(const std::vector<Level> &, int, int, double);
+template LvArgs<float> MG::up_args<float>(const std::vector<LevelF> &, int, int, double);
+template CoarseArgs<double> MG::coarse_args<double>(const std::vector<Level> &, double);
+template CoarseArgs<float> MG::coarse_args<float>(const std::vector<LevelF> &, double);
 
 }  // namespace vdi

@@ -12,6 +12,11 @@
 //     transpose of bilinear prolongation) so that the coarse Q1 stiffness is
 //     the Galerkin one, Dirichlet row kept at zero correction;
 //   coarsest level: a fixed number of smoothing sweeps.
+//
+// Every kernel takes one argument record.  A direct launch passes it by
+// value; a batched launch passes an array of records (one per simulation)
+// and a work list of (record, tile) pairs that a persistent grid walks, so
+// one launch advances the same stage of many independent systems.
 #pragma once
 
 #include "vdi_internal.hpp"
@@ -20,6 +25,9 @@ namespace vdi {
 
 constexpr int kMaxCoarse = 12;  // levels handled by the single-workgroup kernel
 constexpr int kMaxNu = 3;       // largest nu1/nu2 the tiled kernels accept
+constexpr int kBX = 32, kBY = 8;      // 256 threads per level-kernel workgroup
+constexpr int kCoarseThreads = 256;
+constexpr int kCoarsePoints = 3072;   // largest level the single-workgroup kernel takes
 
 // Device-side solve state: convergence flag (0 running, 1 converged,
 // 2 NaN, 3 zero right-hand side), V-cycles run, ticket of the last-
@@ -37,13 +45,104 @@ struct MGCtl {
     double *partial;  // one (or two) doubles per workgroup
 };
 
-// Levels lc..L-1 for the single-workgroup kernel (LDS offsets in doubles;
+// Output tile of a level kernel (cells or nodes); both even.
+struct TileShape {
+    int tx, ty;
+};
+
+// Tile shapes the autotuner chooses from.
+constexpr int kNumTileShapes = 9;
+extern const TileShape kTileShapes[kNumTileShapes];
+
+// Down / up kernel of one level.
+template <class T>
+struct LvArgs {
+    LevelT<T> F, C;
+    CCOp cop;
+    FemOp fop;
+    int nu;
+    int flag;          // down: start from z = 0; up: compute ||b - A z||^2 (level 0)
+    int tx, ty, nbx, nb;  // tile shape, tiles per row, tiles in the launch
+    MGCtl ctl;
+    double tol2;
+    Stamp *ts;
+};
+
+// Mixed form, level 0 up kernel (see k_mx_up0).
+struct MxArgs {
+    Level Z;
+    const double *z_in;
+    double *z_out;
+    LevelF E;
+    float *r_out;
+    LevelF C;
+    CCOp cop;
+    FemOp fop;
+    int nu, tx, ty, nbx, nb;
+    MGCtl ctl;
+    double tol2;
+    Stamp *ts;
+};
+
+// Levels lc..L-1 for the single-workgroup kernel (LDS offsets in elements;
 // z at off, b at off + m*n).
 struct CoarseDesc {
     int nl, top;
     int m[kMaxCoarse], n[kMaxCoarse], off[kMaxCoarse];
-    double cx[kMaxCoarse], cy[kMaxCoarse];
+    double cx[kMaxCoarse], cy[kMaxCoarse], inv[kMaxCoarse];
     double *gz, *gb;
+};
+
+template <class T>
+struct CoarseArgs {
+    CoarseDesc d;
+    CCOp cop;
+    FemOp fop;
+    int fem, nu1, nu2, ncs;
+    MGCtl ctl;
+    double tol2;
+    Stamp *ts;
+};
+
+// Start of a solve: ||b - A z||^2, ||b||^2 and the flag.
+struct BeginArgs {
+    int fem;
+    Level L;
+    CCOp cop;
+    FemOp fop;
+    MGCtl ctl;
+    double tol2;
+    float *r32;  // mixed form: FP32 copy of the residual
+    int nb;
+    Stamp *ts;
+};
+
+// Element-wise operations on level-0 arrays.
+enum { VEC_EXTRAP = 0, VEC_SOLVE_END = 1, VEC_DIRICHLET = 2 };
+struct VecArgs {
+    int op;
+    int mixed;
+    long N;
+    double *z, *zp;
+    const double *zalt, *b;
+    const double *coef;     // VEC_EXTRAP: (b, c) in device memory
+    const MGState *state;   // VEC_SOLVE_END
+    double fm;              // VEC_DIRICHLET: FEM diagonal scale (a_cc = fm/4 at corners, fm/2 along the row)
+    int M;
+    int nb;
+};
+
+// Work item of a batched launch: record index, tile (or block) index and
+// the simulation it belongs to (for per-simulation busy time).
+struct WorkItem {
+    int rec, tile, sim;
+};
+
+// Per-launch extras of a batched launch: the launch stamp and, optionally,
+// per-simulation workgroup busy ticks (atomically accumulated).
+struct BatchAux {
+    Stamp *ts;
+    unsigned long long *busy;
 };
 
 class MG {
@@ -58,8 +157,12 @@ class MG {
     CCOp cc{};
     FemOp fe{};
     int nu1 = 2, nu2 = 2, ncs = 8;
+    int coarse_pts = kCoarsePoints;  // levels this small go to the single-workgroup kernel
     int form = VDI_FORM_FP64;
     hipStream_t st = nullptr;
+    // tile shape index (into kTileShapes) per level: [precision][down/up][level]
+    std::vector<int> tile[2][2];
+    int tile_mx = 0;
 
     // time-extrapolation history (mgs_common.hh:21-38, mgs_common.cc:17-50)
     int scount = 0;
@@ -80,6 +183,20 @@ class MG {
     double *b() const { return L[0].b; }
     double *z_prev() const { return zprev_.as<double>(); }
     MGCtl ctl() const;
+
+    // Argument records (shared by direct and batched launches).
+    template <class T> LvArgs<T> down_args(const std::vector<LevelT<T>> &H, int l, int zero_z);
+    template <class T> LvArgs<T> up_args(const std::vector<LevelT<T>> &H, int l, int norm, double tol2);
+    template <class T> CoarseArgs<T> coarse_args(const std::vector<LevelT<T>> &H, double tol2);
+    MxArgs mx_args(int parity, double tol2);
+    LvArgs<float> mx_down0_args(int parity);
+    BeginArgs begin_args(double tol2);
+    VecArgs vec_args(int op, const double *coef);
+    // Byte counts of the launches above (for timing).
+    double down_bytes(int prec, int l, int zero_z) const;
+    double up_bytes(int prec, int l) const;
+    double coarse_bytes(int prec) const;
+    double mx_bytes() const;
 
     // FEM Dirichlet row z = b / a_cc on level 0 (no-op for other kinds).
     int enforce_dirichlet();
@@ -114,11 +231,27 @@ class MG {
     MGState *state() const { return state_.as<MGState>(); }
     // out = A z for level-0 device arrays
     int apply(const double *z, double *out);
+    // Times every candidate tile shape of every tiled level kernel on this
+    // system's arrays and keeps the fastest; level-0 z and b are restored.
+    int autotune(int reps);
+    // Sets one tile choice: prec 0 FP64 / 1 FP32, which 0 down / 1 up /
+    // 2 mixed level-0 up; shape indexes kTileShapes.
+    int set_tile(int prec, int which, int level, int shape);
+    // LDS bytes of a launch.
+    size_t down_lds(int prec, int l) const;
+    size_t up_lds(int prec, int l) const;
+    size_t mx_lds() const;
+    size_t coarse_lds(int prec) const { return coarse_lds_ / sizeof(double) * (prec ? sizeof(float) : sizeof(double)); }
+    int coarse_points() const;
+    // Moves the hand-over to the single-workgroup kernel (pts <= kCoarsePoints);
+    // the cycle computes the same values either way.
+    int set_coarse_points(int pts);
 
   private:
     template <class T> int launch_down(const std::vector<LevelT<T>> &H, int l, int zero_z);
     template <class T> int launch_up(const std::vector<LevelT<T>> &H, int l, int norm, double tol2);
     template <class T> int launch_coarse(const std::vector<LevelT<T>> &H, double tol2);
+    int launch_vec(const VecArgs &a);
     DevBuf store_;  // coarse z/b
     DevBuf store_f_;
     float *r32_[2] = {nullptr, nullptr};  // FP32 residual, ping-pong
@@ -133,5 +266,36 @@ class MG {
   public:
     ~MG();
 };
+
+// Kernel launchers shared with the batched step (mg.hip).
+namespace mgk {
+// Direct launches.
+template <class T> int down(const LvArgs<T> &a, bool fem, size_t lds, hipStream_t st);
+template <class T> int up(const LvArgs<T> &a, bool fem, size_t lds, hipStream_t st);
+template <class T> int coarse(const CoarseArgs<T> &a, size_t lds, hipStream_t st);
+int mx(const MxArgs &a, bool fem, size_t lds, hipStream_t st);
+int begin(const BeginArgs &a, hipStream_t st);
+int vec(const VecArgs &a, hipStream_t st);
+// Batched launches: args[k] are records in device memory, work items pick
+// (record, tile); grid = persistent workgroup count (a multiple of 8).
+template <class T> int down_b(const LvArgs<T> *args, const WorkItem *w, int nw, bool fem, size_t lds, int grid,
+                              BatchAux aux, hipStream_t st);
+template <class T> int up_b(const LvArgs<T> *args, const WorkItem *w, int nw, bool fem, size_t lds, int grid,
+                            BatchAux aux, hipStream_t st);
+template <class T> int coarse_b(const CoarseArgs<T> *args, const WorkItem *w, int nw, size_t lds, int grid,
+                                BatchAux aux, hipStream_t st);
+int mx_b(const MxArgs *args, const WorkItem *w, int nw, bool fem, size_t lds, int grid, BatchAux aux, hipStream_t st);
+int begin_b(const BeginArgs *args, const WorkItem *w, int nw, int grid, BatchAux aux, hipStream_t st);
+int vec_b(const VecArgs *args, const WorkItem *w, int nw, int grid, BatchAux aux, hipStream_t st);
+const void *coarse_kernel(int prec, bool batched);
+// Workgroups of a persistent grid for a kernel (occupancy x CUs, multiple of 8).
+int persistent_grid(const void *kernel, size_t lds, int nw);
+const void *down_kernel(int prec, bool fem, bool batched);
+const void *up_kernel(int prec, bool fem, bool batched);
+const void *mx_kernel(bool fem, bool batched);
+// One-time kernel attributes (dynamic LDS up to the device limit).
+int set_attributes();
+int max_lds();
+}  // namespace mgk
 
 }  // namespace vdi

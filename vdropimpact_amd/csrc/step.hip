@@ -12,7 +12,7 @@
 #include <memory>
 #include <algorithm>
 
-#include "mg.hpp"
+#include "step.hpp"
 
 namespace vdi {
 
@@ -23,33 +23,6 @@ void set_error(const char *what, hipError_t e, const char *file, int line) {
     snprintf(buf, sizeof buf, "%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
     g_last_error = buf;
 }
-
-// Grid constants (fluid_2d.hh:59-80).
-struct Geo {
-    int m, n, ml, me, ne, x_sym;
-    double dx, dy, xsp, ysp, xxsp, yysp;
-    double ax, bx, ay, by;
-    double rho, rhoinv, mu, muinv;
-};
-
-// Device pointers of the cell fields (fields.hh:7-19, one array each).
-struct Fields {
-    double *u, *v, *p;
-    double *ux, *uy, *vx, *vy;
-    double *ud, *vd, *ul, *vl, *ur, *vr, *uu, *vu;
-    double *c0, *c1, *c2, *c3;
-    double *src;
-};
-
-// Gas-layer arrays (gas_layer.hh:26-34, gas_layer_solve.hh:163-173).
-struct Gas {
-    double *p, *tau, *p_old, *tau_old, *h, *hx, *ht, *fbd, *pg, *pg_old;
-    int *status;  // [0] = status code, [1] = Newton steps
-};
-
-struct GasConst {
-    double alpha, mug, Pamb, h0, R, V, sigma, gxsp, gxxsp;
-};
 
 #define IX(i, j) (((size_t)(j) + 2) * (size_t)g.ml + (size_t)((i) + 2))
 
@@ -88,10 +61,14 @@ __device__ __forceinline__ void godunov_set_tang(double &u0, double &u1, double 
     u1 = 0;
 }
 
-#define CELL_IDX                                              \
-    int i = blockIdx.x * blockDim.x + threadIdx.x;            \
-    int j = blockIdx.y * blockDim.y + threadIdx.y;            \
-    if (i >= g.m || j >= g.n) return;
+// Element-wise kernels: 256 threads as 64 x 4, tile (bx, by).
+#define ELEM_IDX                                   \
+    const int i = bx * kEX + (threadIdx.x & 63);   \
+    const int j = by * kEY + (threadIdx.x >> 6);
+#define REC_REFS                                   \
+    const Geo &g = a.g;                            \
+    const Fields &F = a.F;                         \
+    (void)F;
 
 // ------------------------------------------------------------------------
 // Fused Godunov passes K1..K6 on LDS tiles
@@ -107,7 +84,6 @@ __device__ __forceinline__ void godunov_set_tang(double &u0, double &u1, double 
 // Only the final face states, the viscous force c0/c1 and src are written:
 // about 10 words of traffic per cell instead of the ~75 of one pass per
 // reference loop.  Boundary faces follow fluid_2d.cc:141-148 and :210-213.
-constexpr int kGX = 32, kGY = 8;
 constexpr int kGW = kGX + 8, kGH = kGY + 8;  // LDS frame: tile + 4 on each side
 
 struct GTile {
@@ -123,7 +99,11 @@ __device__ __forceinline__ void tang3(double u0, double u1, double v0, double v1
     ou0 = u0; ou1 = u1; ov1 = v1;
 }
 
-__global__ void __launch_bounds__(256) k_godunov(Geo g, Fields F, Gas G, double dt, double viscfac, double macfac) {
+__device__ void godunov_body(const StepRec &a, int bx, int by) {
+    const Geo &g = a.g;
+    const Fields &F = a.F;
+    const Gas &G = a.G;
+    const double dt = a.dt, viscfac = a.viscfac, macfac = a.macfac;
     __shared__ double su[kGW * kGH], sv[kGW * kGH];
     // K1 edges, later K4 edges: ul, vl, ur, vr, ud, vd, uu, vu
     __shared__ double se[8][kGW * kGH];
@@ -132,8 +112,8 @@ __global__ void __launch_bounds__(256) k_godunov(Geo g, Fields F, Gas G, double 
     enum { UL, VL, UR, VR, UD, VD, UU, VU };
     const int tid = threadIdx.x;
     GTile t;
-    t.x0 = blockIdx.x * kGX;
-    t.y0 = blockIdx.y * kGY;
+    t.x0 = bx * kGX;
+    t.y0 = by * kGY;
     const int m = g.m, n = g.n;
     // u, v on T+4 within the padded array
     for (int k = tid; k < kGW * kGH; k += 256) {
@@ -178,19 +158,19 @@ __global__ void __launch_bounds__(256) k_godunov(Geo g, Fields F, Gas G, double 
         const int i = t.x0 - 1 + k % (kGX + 2), j = t.y0 - 1 + k / (kGX + 2);
         if (i < bx0 || i >= bx1 || j < by0 || j >= by1) continue;
         const int l = t.L(i, j);
-        double a, b, c;
+        double ta, tb, tc;
         // x faces: left face (i-1/2) gives UR2(i-1), UL2(i), VL2(i); right face gives UR2(i), UL2(i+1), VL2(i+1)
         double urm, ul0, vl0, ur0, ulp, vlp;
         if (i == 0) { ul0 = se[UL][l]; vl0 = se[VL][l]; urm = ul0; }
-        else { tang3(se[UR][l - 1], se[UL][l], se[VR][l - 1], se[VL][l], a, b, c); urm = a; ul0 = b; vl0 = c; }
+        else { tang3(se[UR][l - 1], se[UL][l], se[VR][l - 1], se[VL][l], ta, tb, tc); urm = ta; ul0 = tb; vl0 = tc; }
         if (i == m - 1) { ur0 = se[UR][l]; ulp = se[UR][l]; vlp = se[VR][l]; }
-        else { tang3(se[UR][l], se[UL][l + 1], se[VR][l], se[VL][l + 1], a, b, c); ur0 = a; ulp = b; vlp = c; }
+        else { tang3(se[UR][l], se[UL][l + 1], se[VR][l], se[VL][l + 1], ta, tb, tc); ur0 = ta; ulp = tb; vlp = tc; }
         // y faces
         double vum, vd0, ud0, vu0, vdp, udp;
         if (j == 0) { vd0 = se[VD][l]; ud0 = se[UD][l]; vum = vd0; }
-        else { tang3(se[VU][l - kGW], se[VD][l], se[UU][l - kGW], se[UD][l], a, b, c); vum = a; vd0 = b; ud0 = c; }
+        else { tang3(se[VU][l - kGW], se[VD][l], se[UU][l - kGW], se[UD][l], ta, tb, tc); vum = ta; vd0 = tb; ud0 = tc; }
         if (j == n - 1) { vu0 = se[VU][l]; vdp = se[VU][l]; udp = se[UU][l]; }
-        else { tang3(se[VU][l], se[VD][l + kGW], se[UU][l], se[UD][l + kGW], a, b, c); vu0 = a; vdp = b; udp = c; }
+        else { tang3(se[VU][l], se[VD][l + kGW], se[UU][l], se[UD][l + kGW], ta, tb, tc); vu0 = ta; vdp = tb; udp = tc; }
         const double sx = 0.5 * g.xsp * (ulp + ul0), sy = 0.5 * g.ysp * (vdp + vd0);
         sc[0][l] = sx * (ur0 - urm);
         sc[1][l] = sx * (vlp - vl0);
@@ -272,8 +252,11 @@ __global__ void __launch_bounds__(256) k_godunov(Geo g, Fields F, Gas G, double 
 // K7 + K8 (fluid_2d.cc:236-261): the MAC gradient correction is applied to
 // the four faces of the cell on the fly and the advective update is formed;
 // the corrected faces are not needed afterwards, so they are not stored.
-__global__ void k_correct_advect(Geo g, Fields F, const double *z, double dt) {
-    CELL_IDX
+__device__ void correct_body(const StepRec &a, int bx, int by) {
+    REC_REFS
+    ELEM_IDX
+    if (i >= g.m || j >= g.n) return;
+    const double *z = a.zmac, dt = a.dt;
     const int m = g.m, n = g.n;
     const size_t q = i + (size_t)m * j, c = IX(i, j), r = IX(i + 1, j), t = IX(i, j + 1);
     const double *sbc = F.src + (size_t)m * n;
@@ -311,8 +294,10 @@ __device__ __forceinline__ void simpson_term(int x_sym, double xms, double xps, 
 // two right ghost cells of row j; waves [n, n+ml) the two top ghost cells of
 // column c = w-n-2.  Composite Simpson over the gas pressure nodes, then the
 // forward-Euler accumulation (fluid_2d_sio.cc:383-384, quirk Q4).
-__global__ void __launch_bounds__(256) k_simpson(Geo g, Fields F, const double *gp, double prefac) {
-    int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+__device__ void simpson_body(const StepRec &a, int blk) {
+    REC_REFS
+    const double *gp = a.G.p, prefac = a.prefac;
+    int w = (blk * 256 + threadIdx.x) >> 6, lane = threadIdx.x & 63;
     if (w >= g.n + g.ml) return;
     double x1, x2, y1, y2;
     bool right = w < g.n;
@@ -358,7 +343,9 @@ __global__ void __launch_bounds__(256) k_simpson(Geo g, Fields F, const double *
 // set_viscous_boundaries (fluid_2d_sio.cc:294-343), phases in the
 // reference's order (quirk Q5) with workgroup barriers between them.  The
 // reference's ghost-row ux scratch stores are dead and are not made.
-__global__ void __launch_bounds__(1024) k_viscous_bc(Geo g, Fields F, const double *tau) {
+__device__ void viscous_bc_body(const StepRec &a) {
+    REC_REFS
+    const double *tau = a.G.tau;
     double *u = F.u, *v = F.v;
     const int T = blockDim.x;
     if (g.x_sym)
@@ -388,8 +375,12 @@ __global__ void __launch_bounds__(1024) k_viscous_bc(Geo g, Fields F, const doub
 // Implicit viscosity RHS (fluid_2d.cc:272-334) and FEM projection
 // ------------------------------------------------------------------------
 
-__global__ void k_visco_rhs(Geo g, Fields F, Gas G, double vfx, double vfy, int comp) {
-    CELL_IDX
+__device__ void visco_rhs_body(const StepRec &a, int bx, int by, int comp) {
+    REC_REFS
+    ELEM_IDX
+    if (i >= g.m || j >= g.n) return;
+    const Gas &G = a.G;
+    const double vfx = a.vfx, vfy = a.vfy;
     const double *w = comp ? F.v : F.u;
     double r = comp ? F.c1[IX(i, j)] : F.c0[IX(i, j)];
     if (!g.x_sym && i == 0) r += vfx * w[IX(-1, j)];
@@ -407,8 +398,11 @@ __global__ void k_visco_rhs(Geo g, Fields F, Gas G, double vfx, double vfy, int 
     F.src[i + (size_t)g.m * j] = r;
 }
 
-__global__ void k_visco_copy(Geo g, Fields F, const double *zx, const double *zy) {
-    CELL_IDX
+__device__ void visco_copy_body(const StepRec &a, int bx, int by) {
+    REC_REFS
+    ELEM_IDX
+    if (i >= g.m || j >= g.n) return;
+    const double *zx = a.zvx, *zy = a.zvy;
     size_t q = i + (size_t)g.m * j;
     F.c0[IX(i, j)] = zx[q];
     F.c1[IX(i, j)] = zy[q];
@@ -416,10 +410,11 @@ __global__ void k_visco_copy(Geo g, Fields F, const double *zx, const double *zy
 
 // K14 (fluid_2d.cc:338-377): nodal weak divergence, boundary fluxes, and the
 // Dirichlet bottom row src = a_cc p_gas.
-__global__ void k_fem_rhs(Geo g, Fields F, const double *gp, double rdt, double fm) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
+__device__ void fem_rhs_body(const StepRec &a, int bx, int by) {
+    REC_REFS
+    ELEM_IDX
     if (i >= g.me || j >= g.ne) return;
+    const double *gp = a.G.p, rdt = a.rdt, fm = a.fm;
     const int m = g.m, n = g.n;
     if (j == 0) {
         F.src[i] = (i > 0 && i < g.me - 1 ? 1.0 : 0.5) * 0.5 * fm * gp[i];
@@ -447,10 +442,11 @@ __global__ void k_fem_rhs(Geo g, Fields F, const double *gp, double rdt, double 
 }
 
 // K15 (fluid_2d.cc:383-396)
-__global__ void k_update(Geo g, Fields F, const double *zf, double dt) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
+__device__ void update_body(const StepRec &a, int bx, int by) {
+    REC_REFS
+    ELEM_IDX
     if (i >= g.me || j >= g.ne) return;
+    const double *zf = a.zfem, dt = a.dt;
     const int me = g.me;
     const double *q = zf + i + (size_t)me * j;
     if (i < g.m && j < g.n) {
@@ -471,10 +467,12 @@ __device__ __forceinline__ double eno2(double p0, double p1, double p2, double p
 
 // gas_layer_solve.cc:133-162, run by one lane.  The reference updates h in
 // place while i sweeps upward, so the stencil at i sees new h[i-1], h[i-2]
-// and old h[i..i+2]; the lane keeps exactly those five values in registers.
+// and old h[i..i+2]; the lane keeps exactly those five values in registers
+// and only stores to global memory (the fbd update, which does not take part
+// in the recurrence, is done by all threads afterwards).
 __device__ void gas_height_serial(int m, int x_sym, double dt, double Vframe, const double *__restrict__ HO,
                                   const double *__restrict__ UC, const double *__restrict__ VC,
-                                  double *__restrict__ h, double *__restrict__ ht, double *__restrict__ fbd) {
+                                  double *__restrict__ h, double *__restrict__ ht) {
     double hn1 = 0, hn2 = 0;
     double o0 = HO[0], o1 = HO[1], o2 = HO[2];
     for (int i = 0; i < m; i++) {
@@ -496,7 +494,6 @@ __device__ void gas_height_serial(int m, int x_sym, double dt, double Vframe, co
         double hnew = o0 + dt * htv;
         ht[i] = htv;
         h[i] = hnew;
-        fbd[i] += dt * vc;
         hn2 = hn1; hn1 = hnew;
         o0 = o1; o1 = o2; o2 = o3;
     }
@@ -518,12 +515,12 @@ __device__ double wg_sum(double x, double *red) {
     return red[32];
 }
 
-constexpr int kGasThreads = 1024;
-constexpr int kGasRows = 5;  // rows per thread: me <= 5120
-
-__global__ void __launch_bounds__(kGasThreads) k_gas(Geo g, Fields F, Gas G, GasConst K, double dt, double Vframe) {
-    extern __shared__ double lds[];
-    __shared__ double red[33];
+__device__ void gas_body(const StepRec &a, double *lds, double *red) {
+    const Geo &g = a.g;
+    const Fields &F = a.F;
+    const Gas &G = a.G;
+    const GasConst &K = a.K;
+    const double dt = a.dt, Vframe = a.Vframe;
     const int m = g.m, me = g.me, ml = g.ml, T = blockDim.x, tid = threadIdx.x;
     const int x_sym = g.x_sym;
 
@@ -539,7 +536,9 @@ __global__ void __launch_bounds__(kGasThreads) k_gas(Geo g, Fields F, Gas G, Gas
         VC[i] = 0.5 * (F.v[IX(i, 0)] + F.v[IX(i, -1)]);
     }
     __syncthreads();
-    if (tid == 0) gas_height_serial(m, x_sym, dt, Vframe, HO, UC, VC, G.h, G.ht, G.fbd);
+    if (tid == 0) gas_height_serial(m, x_sym, dt, Vframe, HO, UC, VC, G.h, G.ht);
+    // fbd += dt v_c does not take part in the in-place sweep
+    for (int i = tid; i < m; i += T) G.fbd[i] += dt * VC[i];
     __syncthreads();
     // calculate_hx (gas_layer_solve.cc:58-62)
     for (int i = tid; i <= m; i += T) {
@@ -685,19 +684,155 @@ __global__ void __launch_bounds__(1024) k_gas_init(Geo g, Gas G, GasConst K) {
 #undef IX
 
 // Gathers the four solve states and the gas-layer status for one readback.
-__global__ void k_gather_status(const MGState *s0, const MGState *s1, const MGState *s2, const MGState *s3,
-                                const int *gas, MGState *out_st, int *out_gas) {
+__device__ void gather_body(const StepRec &a) {
     if (threadIdx.x != 0) return;
-    const MGState *src[4] = {s0, s1, s2, s3};
     for (int k = 0; k < 4; k++)
-        if (src[k]) out_st[k] = *src[k];
-    out_gas[0] = gas[0];
-    out_gas[1] = gas[1];
+        if (a.mgst[k]) a.rb_st[k] = *a.mgst[k];
+    a.rb_gas[0] = a.G.status[0];
+    a.rb_gas[1] = a.G.status[1];
 }
+
+// Snapshot for a redo (u, v, p are the first three field arrays, then the
+// gas arrays) and its restore; 2048 values per block.
+__device__ void snapshot_body(const StepRec &a, int blk, bool restore) {
+    const long n3 = 3 * a.np, tot = n3 + a.ng;
+    for (long k = blk * 2048L + threadIdx.x; k < tot && k < (blk + 1) * 2048L; k += 256) {
+        double *live = k < n3 ? a.F.u + k : a.gasbase + (k - n3);
+        if (restore) *live = a.snap[k];
+        else a.snap[k] = *live;
+    }
+}
+
+// ------------------------------------------------------------------------
+// Kernels: direct (one record, grid = tiles) and batched (records in device
+// memory, a persistent grid walking the (record, tile) work list)
+// ------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(256) k_godunov(StepRec a) { godunov_body(a, blockIdx.x, blockIdx.y); }
+__global__ void __launch_bounds__(256) k_correct_advect(StepRec a) { correct_body(a, blockIdx.x, blockIdx.y); }
+__global__ void __launch_bounds__(256) k_simpson(StepRec a, Stamp *ts) {
+    stamp_begin(ts);
+    simpson_body(a, blockIdx.x);
+    stamp_end(ts);
+}
+__global__ void __launch_bounds__(1024) k_viscous_bc(StepRec a) { viscous_bc_body(a); }
+__global__ void __launch_bounds__(256) k_visco_rhs(StepRec a, int comp) {
+    visco_rhs_body(a, blockIdx.x, blockIdx.y, comp);
+}
+__global__ void __launch_bounds__(256) k_visco_copy(StepRec a) { visco_copy_body(a, blockIdx.x, blockIdx.y); }
+__global__ void __launch_bounds__(256) k_fem_rhs(StepRec a) { fem_rhs_body(a, blockIdx.x, blockIdx.y); }
+__global__ void __launch_bounds__(256) k_update(StepRec a) { update_body(a, blockIdx.x, blockIdx.y); }
+__global__ void __launch_bounds__(kGasThreads) k_gas(StepRec a, Stamp *ts) {
+    extern __shared__ double lds[];
+    __shared__ double red[33];
+    stamp_begin(ts);
+    gas_body(a, lds, red);
+    stamp_end(ts);
+}
+__global__ void __launch_bounds__(256) k_snapshot(StepRec a, int restore) { snapshot_body(a, blockIdx.x, restore); }
+__global__ void __launch_bounds__(64) k_gather_status(StepRec a) { gather_body(a); }
 
 __global__ void k_copy(const double *__restrict__ a, double *__restrict__ b, long n) {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) b[i] = a[i];
 }
+
+// Batched: tiles of kernel K for record a.
+template <int K>
+__device__ __forceinline__ void step_item(const StepRec &a, int tile, double *lds, double *red) {
+    if constexpr (K == SK_GODUNOV) {
+        const int nbx = (a.g.m + kGX - 1) / kGX;
+        godunov_body(a, tile % nbx, tile / nbx);
+    } else if constexpr (K == SK_CORRECT || K == SK_VISCO_RHS_U || K == SK_VISCO_RHS_V || K == SK_VISCO_COPY) {
+        const int nbx = (a.g.m + kEX - 1) / kEX, bx = tile % nbx, by = tile / nbx;
+        if constexpr (K == SK_CORRECT) correct_body(a, bx, by);
+        else if constexpr (K == SK_VISCO_RHS_U) visco_rhs_body(a, bx, by, 0);
+        else if constexpr (K == SK_VISCO_RHS_V) visco_rhs_body(a, bx, by, 1);
+        else visco_copy_body(a, bx, by);
+    } else if constexpr (K == SK_FEM_RHS || K == SK_UPDATE) {
+        const int nbx = (a.g.me + kEX - 1) / kEX, bx = tile % nbx, by = tile / nbx;
+        if constexpr (K == SK_FEM_RHS) fem_rhs_body(a, bx, by);
+        else update_body(a, bx, by);
+    } else if constexpr (K == SK_SIMPSON) {
+        simpson_body(a, tile);
+    } else if constexpr (K == SK_VISCOUS_BC) {
+        viscous_bc_body(a);
+    } else if constexpr (K == SK_GAS) {
+        gas_body(a, lds, red);
+    } else if constexpr (K == SK_SNAPSHOT || K == SK_RESTORE) {
+        snapshot_body(a, tile, K == SK_RESTORE);
+    } else if constexpr (K == SK_GATHER) {
+        gather_body(a);
+    }
+}
+
+template <int K, int NT>
+__global__ void __launch_bounds__(NT) k_step_b(const StepRec *A, const WorkItem *W, int nw, BatchAux aux) {
+    extern __shared__ double lds[];
+    __shared__ double red[33];
+    stamp_begin(aux.ts);
+    for (int p = blockIdx.x; p < nw; p += gridDim.x) {
+        const WorkItem w = W[p];
+        const unsigned long long t0 = busy_begin(aux.busy);
+        step_item<K>(A[w.rec], w.tile, lds, red);
+        __syncthreads();
+        busy_end(aux.busy, w.sim, t0);
+    }
+    stamp_end(aux.ts);
+}
+
+namespace stepk {
+
+int threads(int k) { return (k == SK_VISCOUS_BC || k == SK_GAS) ? 1024 : 256; }
+
+int tiles(int k, const Geo &g) {
+    switch (k) {
+    case SK_GODUNOV: return (int)(cdiv(g.m, kGX) * cdiv(g.n, kGY));
+    case SK_CORRECT:
+    case SK_VISCO_RHS_U:
+    case SK_VISCO_RHS_V:
+    case SK_VISCO_COPY: return (int)(cdiv(g.m, kEX) * cdiv(g.n, kEY));
+    case SK_FEM_RHS:
+    case SK_UPDATE: return (int)(cdiv(g.me, kEX) * cdiv(g.ne, kEY));
+    case SK_SIMPSON: return (int)cdiv(g.n + g.ml, 4);
+    case SK_SNAPSHOT:
+    case SK_RESTORE: {
+        const long np = (long)g.ml * (g.n + 4), ng = 5L * g.me + 2L * g.ml + 3L * g.m;
+        return (int)cdiv(3 * np + ng, 2048);
+    }
+    default: return 1;
+    }
+}
+
+size_t gas_lds(const Geo &g) { return std::max<size_t>(3 * (size_t)g.m, 4 * (size_t)g.me) * sizeof(double); }
+
+#define SK_CASES(X)                                                                                                \
+    X(SK_GODUNOV, 256) X(SK_CORRECT, 256) X(SK_SIMPSON, 256) X(SK_VISCOUS_BC, 1024) X(SK_VISCO_RHS_U, 256)        \
+    X(SK_VISCO_RHS_V, 256) X(SK_VISCO_COPY, 256) X(SK_FEM_RHS, 256) X(SK_UPDATE, 256) X(SK_GAS, 1024)             \
+    X(SK_SNAPSHOT, 256) X(SK_RESTORE, 256) X(SK_GATHER, 256)
+
+const void *kernel_b(int k) {
+    switch (k) {
+#define X(K, NT) \
+    case K: return (const void *)k_step_b<K, NT>;
+        SK_CASES(X)
+#undef X
+    default: return nullptr;
+    }
+}
+
+int launch_b(int k, const StepRec *recs, const WorkItem *w, int nw, int grid, size_t lds, BatchAux aux, hipStream_t st) {
+    switch (k) {
+#define X(K, NT) \
+    case K: hipLaunchKernelGGL((k_step_b<K, NT>), dim3(grid), dim3(NT), lds, st, recs, w, nw, aux); break;
+        SK_CASES(X)
+#undef X
+    default: return VDI_ERR_ARG;
+    }
+    VDI_LAUNCHED();
+    return VDI_OK;
+}
+
+}  // namespace stepk
 
 }  // namespace vdi
 
@@ -707,64 +842,62 @@ __global__ void k_copy(const double *__restrict__ a, double *__restrict__ b, lon
 
 using namespace vdi;
 
-struct vdi_ctx {
-    vdi_params P{};
-    Geo g{};
-    GasConst K{};
-    hipStream_t st = nullptr;
-    double time = 0, Vframe = 0;
-    double rel_tol = 1e-10;
-    int max_cycles = 200;
-    int form = VDI_FORM_FP64;
-    long steps = 0;
-    int last_newton = 0;
-    DevBuf fields;  // 19 padded arrays
-    DevBuf srcbuf, gasbuf, zbuf, statusbuf;
-    Fields F{};
-    Gas G{};
-    double *zmac = nullptr, *zfem = nullptr, *zvx = nullptr, *zvy = nullptr;
-    MG mac, fem, vx, vy;
-    int *host_status = nullptr;
-    size_t gas_lds = 0;
-    KernelTimer timer;
-    // Captured steps (one graph per cycle budget, time step and form).
-    struct GraphEntry {
-        int budget[4];
-        double dt;
-        int form;
-        hipGraphExec_t exec;
-        std::vector<KernelTimer::Slot> slots;
-    };
-    struct Readback {
-        MGState st[4];
-        int gas[2];
-        int pad[2];
-    };
-    int use_graphs = 1;
-    int budget[4] = {0, 0, 0, 0};
-    std::vector<GraphEntry> graphs;
-    DevBuf snap, coef_dev, rb_dev;
-    double *coef_host = nullptr;  // pinned, 2 extrapolation coefficients per solver
-    Readback *rb = nullptr;       // pinned
-    long graph_steps = 0, redo_steps = 0;
-    void drop_graphs() {
-        for (auto &e : graphs) (void)hipGraphExecDestroy(e.exec);
-        graphs.clear();
-    }
-    ~vdi_ctx() {
-        if (st) (void)hipStreamSynchronize(st);
-        drop_graphs();
-        if (host_status) (void)hipHostFree(host_status);
-        if (coef_host) (void)hipHostFree(coef_host);
-        if (rb) (void)hipHostFree(rb);
-        if (st) (void)hipStreamDestroy(st);
-    }
-};
+namespace vdi {
 
-static const dim3 kB2(64, 4);
+static dim3 cells_grid(const Geo &g) { return dim3(cdiv(g.m, kEX), cdiv(g.n, kEY)); }
+static dim3 nodes_grid(const Geo &g) { return dim3(cdiv(g.me, kEX), cdiv(g.ne, kEY)); }
 
-static dim3 cells_grid(const Geo &g) { return dim3(cdiv(g.m, kB2.x), cdiv(g.n, kB2.y)); }
-static dim3 nodes_grid(const Geo &g) { return dim3(cdiv(g.me, kB2.x), cdiv(g.ne, kB2.y)); }
+double half_time(double t0, double dt) { return t0 + 0.5 * dt; }
+double end_time(double t0, double dt) { return half_time(t0, dt) + 0.5 * dt; }
+
+MG *step_solver(vdi_ctx *c, int k) {
+    MG *gs[4] = {&c->mac, &c->vx, &c->vy, &c->fem};
+    return gs[k];
+}
+
+bool solver_used(vdi_ctx *c, int k) { return c->P.implicit_visc || k == 0 || k == 3; }
+
+StepRec step_record(vdi_ctx *c, double dt) {
+    StepRec a{};
+    const Geo &g = c->g;
+    a.g = g;
+    a.F = c->F;
+    a.G = c->G;
+    a.K = c->K;
+    a.dt = dt;
+    a.viscfac = c->P.implicit_visc ? g.rhoinv * c->P.mu * 0.5 : g.rhoinv * c->P.mu;
+    a.macfac = 0.125 * g.rhoinv * dt;
+    a.prefac = g.dx * g.rhoinv * dt / (3 * M_PI);
+    a.vfx = a.viscfac * g.xxsp * dt;
+    a.vfy = a.viscfac * g.yysp * dt;
+    a.rdt = g.rho / dt;
+    a.fm = c->fem.fe.fm;
+    a.Vframe = c->Vframe;
+    a.zmac = c->zmac;
+    a.zvx = c->zvx;
+    a.zvy = c->zvy;
+    a.zfem = c->zfem;
+    a.snap = c->snap.as<double>();
+    a.gasbase = c->gasbuf.as<double>();
+    a.np = (long)g.ml * (g.n + 4);
+    a.ng = (long)(c->gasbuf.bytes / sizeof(double));
+    for (int k = 0; k < 4; k++) a.mgst[k] = solver_used(c, k) ? step_solver(c, k)->state() : nullptr;
+    vdi_ctx::Readback *rbd = c->rb_dev.as<vdi_ctx::Readback>();
+    a.rb_st = rbd->st;
+    a.rb_gas = rbd->gas;
+    return a;
+}
+
+// visco_impl.cc:21-39: re-set the hierarchy when the coupling changed.
+void visco_setup(MG &g, double vfx, double vfy) {
+    if (std::fabs(vfx - g.vfx) > 1e2 * std::numeric_limits<double>::epsilon() * g.vfx) {
+        g.vfx = vfx;
+        g.vfy = vfy;
+        g.set_coeffs(vfx, vfy);
+    }
+}
+
+}  // namespace vdi
 
 extern "C" {
 
@@ -843,9 +976,14 @@ int vdi_create(const vdi_params *p, vdi_ctx **out) {
     VDI_TRY(c->rb_dev.alloc(sizeof(vdi_ctx::Readback)));
     VDI_HIP(hipHostMalloc((void **)&c->coef_host, 8 * sizeof(double), hipHostMallocDefault));
     VDI_HIP(hipHostMalloc((void **)&c->rb, sizeof(vdi_ctx::Readback), hipHostMallocDefault));
-    c->gas_lds = std::max<size_t>(3 * (size_t)g.m, 4 * (size_t)g.me) * sizeof(double);
-    if (c->gas_lds > 64 * 1024)
+    c->gas_lds = stepk::gas_lds(g);
+    static size_t gas_attr = 0;
+    if (c->gas_lds > 64 * 1024 && c->gas_lds > gas_attr) {
         VDI_HIP(hipFuncSetAttribute((const void *)k_gas, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->gas_lds));
+        VDI_HIP(hipFuncSetAttribute(stepk::kernel_b(SK_GAS), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)c->gas_lds));
+        gas_attr = c->gas_lds;
+    }
     *out = c.release();
     return VDI_OK;
 }
@@ -879,24 +1017,24 @@ int vdi_set_form(vdi_ctx *c, int form) {
     return VDI_OK;
 }
 
-static int enqueue_viscous_bc(vdi_ctx *c) {
-    hipLaunchKernelGGL(k_viscous_bc, dim3(1), dim3(1024), 0, c->st, c->g, c->F, (const double *)c->G.tau);
+}  // extern "C"
+
+static int enqueue_viscous_bc(vdi_ctx *c, const StepRec &a) {
+    hipLaunchKernelGGL(k_viscous_bc, dim3(1), dim3(1024), 0, c->st, a);
     VDI_LAUNCHED();
     return VDI_OK;
 }
 
-static int enqueue_inviscid_bc(vdi_ctx *c, double dt) {
-    double prefac = c->g.dx * c->g.rhoinv * dt / (3 * M_PI);
-    int waves = c->g.n + c->g.ml;
-    const int slot = c->timer.begin(VDI_KCLASS_SIMPSON, c->st);
-    hipLaunchKernelGGL(k_simpson, dim3(cdiv(waves, 4)), dim3(256), 0, c->st, c->g, c->F, (const double *)c->G.p, prefac);
-    VDI_LAUNCHED();
+static int enqueue_inviscid_bc(vdi_ctx *c, const StepRec &a) {
+    const int waves = c->g.n + c->g.ml;
     // reads p once; reads and writes two ghost cells of u and v per line
-    c->timer.end(slot, c->st, 8.0 * (c->g.me + 8.0 * waves));
+    Stamp *ts = c->timer.slot(VDI_KCLASS_SIMPSON, 8.0 * (c->g.me + 8.0 * waves));
+    hipLaunchKernelGGL(k_simpson, dim3(cdiv(waves, 4)), dim3(256), 0, c->st, a, ts);
+    VDI_LAUNCHED();
     return VDI_OK;
 }
 
-int vdi_init_fields(vdi_ctx *c) {
+extern "C" int vdi_init_fields(vdi_ctx *c) {
     if (!c) return VDI_ERR_ARG;
     c->time = 0;
     size_t NP = (size_t)c->g.ml * (c->g.n + 4);
@@ -905,34 +1043,35 @@ int vdi_init_fields(vdi_ctx *c) {
     VDI_HIP(hipMemsetAsync(c->F.p, 0, NP * sizeof(double), c->st));
     hipLaunchKernelGGL(k_gas_init, dim3(1), dim3(1024), 0, c->st, c->g, c->G, c->K);
     VDI_LAUNCHED();
-    VDI_TRY(enqueue_inviscid_bc(c, 0));
-    VDI_TRY(enqueue_viscous_bc(c));
+    const StepRec a = step_record(c, 0.0);
+    VDI_TRY(enqueue_inviscid_bc(c, a));
+    VDI_TRY(enqueue_viscous_bc(c, a));
     VDI_HIP(hipStreamSynchronize(c->st));
     return VDI_OK;
 }
 
-static int enqueue_gas(vdi_ctx *c, double dt) {
-    const int slot = c->timer.begin(VDI_KCLASS_GAS, c->st);
-    hipLaunchKernelGGL(k_gas, dim3(1), dim3(kGasThreads), c->gas_lds, c->st, c->g, c->F, c->G, c->K, dt, c->Vframe);
-    VDI_LAUNCHED();
+static int enqueue_gas(vdi_ctx *c, const StepRec &a) {
+    const Geo &g = c->g;
     // reads u,v rows 0,-1, h, ht, fbd, pg, p, tau; writes p_old, tau_old, h, ht, fbd, hx, pg, pg_old, p, tau
-    c->timer.end(slot, c->st, 8.0 * (4.0 * c->g.m + 3.0 * c->g.m + 3.0 * c->g.me + 2.0 * c->g.ml + 3.0 * c->g.m +
-                                      4.0 * c->g.me + 2.0 * c->g.ml));
+    Stamp *ts = c->timer.slot(VDI_KCLASS_GAS, 8.0 * (4.0 * g.m + 3.0 * g.m + 3.0 * g.me + 2.0 * g.ml + 3.0 * g.m +
+                                                      4.0 * g.me + 2.0 * g.ml));
+    hipLaunchKernelGGL(k_gas, dim3(1), dim3(kGasThreads), c->gas_lds, c->st, a, ts);
+    VDI_LAUNCHED();
     return VDI_OK;
 }
 
-static int gas_update(vdi_ctx *c, double dt) {
-    VDI_TRY(enqueue_gas(c, dt));
+static int gas_update(vdi_ctx *c, const StepRec &a) {
+    VDI_TRY(enqueue_gas(c, a));
     VDI_HIP(hipMemcpyAsync(c->host_status, c->G.status, 2 * sizeof(int), hipMemcpyDeviceToHost, c->st));
     VDI_HIP(hipStreamSynchronize(c->st));
     c->last_newton = c->host_status[1];
     return c->host_status[0];
 }
 
-int vdi_gas_layer_update(vdi_ctx *c, double t, double dt) {
+extern "C" int vdi_gas_layer_update(vdi_ctx *c, double t, double dt) {
     (void)t;
     if (!c || !(dt > 0)) return VDI_ERR_ARG;
-    return gas_update(c, dt);
+    return gas_update(c, step_record(c, dt));
 }
 
 static int solve_mg(vdi_ctx *c, MG &g, double t, int *cyc) {
@@ -940,31 +1079,16 @@ static int solve_mg(vdi_ctx *c, MG &g, double t, int *cyc) {
     return g.solve(c->rel_tol, c->max_cycles, cyc);
 }
 
-int vdi_mg_solve_v_cycle(vdi_ctx *c, int which, double t, int *vcycles) {
+extern "C" int vdi_mg_solve_v_cycle(vdi_ctx *c, int which, double t, int *vcycles) {
     if (!c || (which != VDI_MAC && which != VDI_FEM)) return VDI_ERR_ARG;
     return solve_mg(c, which == VDI_MAC ? c->mac : c->fem, t, vcycles);
 }
 
-// visco_impl.cc:21-39: re-set the hierarchy when the coupling changed.
-static void visco_setup(MG &g, double vfx, double vfy) {
-    if (std::fabs(vfx - g.vfx) > 1e2 * std::numeric_limits<double>::epsilon() * g.vfx) {
-        g.vfx = vfx;
-        g.vfy = vfy;
-        g.set_coeffs(vfx, vfy);
-    }
-}
-
-int vdi_visco_setup_and_solve(vdi_ctx *c, int which, double vfx, double vfy, double t, int *vcycles) {
+extern "C" int vdi_visco_setup_and_solve(vdi_ctx *c, int which, double vfx, double vfy, double t, int *vcycles) {
     if (!c || !c->P.implicit_visc || (which != VDI_VX && which != VDI_VY)) return VDI_ERR_ARG;
     MG &g = which == VDI_VX ? c->vx : c->vy;
     visco_setup(g, vfx, vfy);
     return solve_mg(c, g, t, vcycles);
-}
-
-// Solves in step order: MAC, VX, VY, FEM.
-static MG *step_solver(vdi_ctx *c, int k) {
-    MG *gs[4] = {&c->mac, &c->vx, &c->vy, &c->fem};
-    return gs[k];
 }
 
 // How a solve inside the step body is carried out.
@@ -988,39 +1112,39 @@ static int body_solve(vdi_ctx *c, int k, double t, SolveMode mode) {
 }
 
 // fluid_2d::step_forward (fluid_2d.cc:93-405) from the Godunov passes on; the
-// gas layer has been enqueued already.  Host times: t0 at the start of the step.
-static int enqueue_body(vdi_ctx *c, double dt, double t0, SolveMode mode) {
+// gas layer has been enqueued already.  Host times: t0 at the start of the
+// step; the clock advances in two half steps, as fluid_2d.cc:102 and :251 do,
+// so that solve times match bit for bit.
+static int enqueue_body(vdi_ctx *c, const StepRec &a, double t0, SolveMode mode) {
     const Geo &g = c->g;
     const dim3 gc = cells_grid(g), gn = nodes_grid(g);
-    const double viscfac = c->P.implicit_visc ? g.rhoinv * c->P.mu * 0.5 : g.rhoinv * c->P.mu;
-    hipLaunchKernelGGL(k_godunov, dim3(cdiv(g.m, kGX), cdiv(g.n, kGY)), dim3(256), 0, c->st, g, c->F, c->G, dt, viscfac,
-                       0.125 * g.rhoinv * dt);
+    const double dt = a.dt;
+    hipLaunchKernelGGL(k_godunov, dim3(cdiv(g.m, kGX), cdiv(g.n, kGY)), dim3(256), 0, c->st, a);
     VDI_LAUNCHED();
-    VDI_TRY(body_solve(c, 0, t0 + 0.5 * dt, mode));
-    hipLaunchKernelGGL(k_correct_advect, gc, kB2, 0, c->st, g, c->F, (const double *)c->zmac, dt);
+    VDI_TRY(body_solve(c, 0, half_time(t0, dt), mode));
+    hipLaunchKernelGGL(k_correct_advect, gc, dim3(256), 0, c->st, a);
     VDI_LAUNCHED();
-    VDI_TRY(enqueue_inviscid_bc(c, dt));
-    VDI_TRY(enqueue_viscous_bc(c));
-    const double t1 = t0 + dt;
+    VDI_TRY(enqueue_inviscid_bc(c, a));
+    VDI_TRY(enqueue_viscous_bc(c, a));
+    const double t1 = end_time(t0, dt);
     if (c->P.implicit_visc) {
-        const double vfx = viscfac * g.xxsp * dt, vfy = viscfac * g.yysp * dt;
-        hipLaunchKernelGGL(k_visco_rhs, gc, kB2, 0, c->st, g, c->F, c->G, vfx, vfy, 0);
+        hipLaunchKernelGGL(k_visco_rhs, gc, dim3(256), 0, c->st, a, 0);
         VDI_LAUNCHED();
-        visco_setup(c->vx, vfx, vfy);
+        visco_setup(c->vx, a.vfx, a.vfy);
         VDI_TRY(body_solve(c, 1, t1, mode));
-        hipLaunchKernelGGL(k_visco_rhs, gc, kB2, 0, c->st, g, c->F, c->G, vfx, vfy, 1);
+        hipLaunchKernelGGL(k_visco_rhs, gc, dim3(256), 0, c->st, a, 1);
         VDI_LAUNCHED();
-        visco_setup(c->vy, vfx, vfy);
+        visco_setup(c->vy, a.vfx, a.vfy);
         VDI_TRY(body_solve(c, 2, t1, mode));
-        hipLaunchKernelGGL(k_visco_copy, gc, kB2, 0, c->st, g, c->F, (const double *)c->zvx, (const double *)c->zvy);
+        hipLaunchKernelGGL(k_visco_copy, gc, dim3(256), 0, c->st, a);
         VDI_LAUNCHED();
     }
-    hipLaunchKernelGGL(k_fem_rhs, gn, kB2, 0, c->st, g, c->F, (const double *)c->G.p, g.rho / dt, c->fem.fe.fm);
+    hipLaunchKernelGGL(k_fem_rhs, gn, dim3(256), 0, c->st, a);
     VDI_LAUNCHED();
     VDI_TRY(body_solve(c, 3, t1, mode));
-    hipLaunchKernelGGL(k_update, gn, kB2, 0, c->st, g, c->F, (const double *)c->zfem, dt);
+    hipLaunchKernelGGL(k_update, gn, dim3(256), 0, c->st, a);
     VDI_LAUNCHED();
-    VDI_TRY(enqueue_viscous_bc(c));
+    VDI_TRY(enqueue_viscous_bc(c, a));
     return VDI_OK;
 }
 
@@ -1028,50 +1152,45 @@ static int enqueue_body(vdi_ctx *c, double dt, double t0, SolveMode mode) {
 // solve (the path of the first steps, and of a redo).
 static int step_direct(vdi_ctx *c, double dt, SolveMode mode) {
     const double t0 = c->time;
-    VDI_TRY(gas_update(c, dt));
-    VDI_TRY(enqueue_body(c, dt, t0, mode));
-    c->time = t0 + dt;
-    if (c->timer.cls >= 0) {
-        VDI_HIP(hipStreamSynchronize(c->st));
-        c->timer.harvest_direct();
-    }
+    const StepRec a = step_record(c, dt);
+    if (c->timer.cls >= 0) VDI_TRY(c->timer.reset(c->timer.reserved, c->timer.cap, c->st));
+    c->timer.next = c->timer.reserved;
+    VDI_TRY(gas_update(c, a));
+    VDI_TRY(enqueue_body(c, a, t0, mode));
+    c->time = end_time(t0, dt);
+    if (c->timer.cls >= 0) VDI_TRY(c->timer.harvest(c->timer.reserved, c->timer.next, c->st));
     return VDI_OK;
 }
 
 static bool graph_ready(vdi_ctx *c) {
     if (!c->use_graphs) return false;
     for (int k = 0; k < 4; k++) {
-        MG *g = step_solver(c, k);
-        if ((k == 1 || k == 2) && !c->P.implicit_visc) continue;
-        if (g->scount < 2 || c->budget[k] < 1) return false;
+        if (!solver_used(c, k)) continue;
+        if (step_solver(c, k)->scount < 2 || c->budget[k] < 1) return false;
     }
     return true;
 }
 
 static int capture_step(vdi_ctx *c, double dt, vdi_ctx::GraphEntry &e) {
-    const size_t NP = (size_t)c->g.ml * (c->g.n + 4);
-    const size_t ng = c->gasbuf.bytes / sizeof(double);
-    c->timer.pending.clear();
+    const StepRec a = step_record(c, dt);
+    const Geo &g = c->g;
     hipGraph_t graph;
+    e.slot_a = c->timer.next;
     VDI_HIP(hipStreamBeginCapture(c->st, hipStreamCaptureModeRelaxed));
     int rc = VDI_OK;
     do {
         // extrapolation coefficients of this step, written by the host
         if (hipMemcpyAsync(c->coef_dev.p, c->coef_host, 8 * sizeof(double), hipMemcpyHostToDevice, c->st) !=
             hipSuccess) { rc = VDI_ERR_HIP; break; }
-        // snapshot for a redo: u, v, p are the first three field arrays
-        hipLaunchKernelGGL(k_copy, dim3(2048), dim3(256), 0, c->st, (const double *)c->F.u, c->snap.as<double>(),
-                           (long)(3 * NP));
-        hipLaunchKernelGGL(k_copy, dim3(cdiv(ng, 256)), dim3(256), 0, c->st, (const double *)c->gasbuf.as<double>(),
-                           c->snap.as<double>() + 3 * NP, (long)ng);
-        if ((rc = enqueue_gas(c, dt)) != VDI_OK) break;
-        if ((rc = enqueue_body(c, dt, c->time, SOLVE_FIXED)) != VDI_OK) break;
-        vdi_ctx::Readback *rbd = c->rb_dev.as<vdi_ctx::Readback>();
-        const bool iv = c->P.implicit_visc;
-        hipLaunchKernelGGL(k_gather_status, dim3(1), dim3(64), 0, c->st, (const MGState *)c->mac.state(),
-                           iv ? (const MGState *)c->vx.state() : nullptr, iv ? (const MGState *)c->vy.state() : nullptr,
-                           (const MGState *)c->fem.state(), (const int *)c->G.status, rbd->st, rbd->gas);
-        if (hipMemcpyAsync(c->rb, rbd, sizeof(vdi_ctx::Readback), hipMemcpyDeviceToHost, c->st) != hipSuccess) {
+        // the slots this graph will use are cleared at its start; their
+        // count is known only after capture, so clear the rest of the pool
+        if (c->timer.cls >= 0 && (rc = c->timer.reset(e.slot_a, c->timer.cap, c->st)) != VDI_OK) break;
+        hipLaunchKernelGGL(k_snapshot, dim3(stepk::tiles(SK_SNAPSHOT, g)), dim3(256), 0, c->st, a, 0);
+        if ((rc = enqueue_gas(c, a)) != VDI_OK) break;
+        if ((rc = enqueue_body(c, a, c->time, SOLVE_FIXED)) != VDI_OK) break;
+        hipLaunchKernelGGL(k_gather_status, dim3(1), dim3(64), 0, c->st, a);
+        if (hipMemcpyAsync(c->rb, c->rb_dev.p, sizeof(vdi_ctx::Readback), hipMemcpyDeviceToHost, c->st) !=
+            hipSuccess) {
             rc = VDI_ERR_HIP;
             break;
         }
@@ -1082,11 +1201,21 @@ static int capture_step(vdi_ctx *c, double dt, vdi_ctx::GraphEntry &e) {
     hipError_t ei = hipGraphInstantiate(&e.exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     VDI_HIP(ei);
-    e.slots = c->timer.pending;
-    c->timer.pending.clear();
+    e.slot_b = c->timer.next;
     c->timer.reserved = c->timer.next;
     return VDI_OK;
 }
+
+namespace vdi {
+int redo_step(vdi_ctx *c, double dt, double t0) {
+    const StepRec a = step_record(c, dt);
+    hipLaunchKernelGGL(k_snapshot, dim3(stepk::tiles(SK_RESTORE, c->g)), dim3(256), 0, c->st, a, 1);
+    VDI_LAUNCHED();
+    c->redo_steps++;
+    c->time = t0;
+    return step_direct(c, dt, SOLVE_AGAIN);
+}
+}  // namespace vdi
 
 // A captured step: every solve gets its predicted cycle budget (the cycles
 // the previous solve of that system needed) and convergence is only checked
@@ -1095,9 +1224,9 @@ static int capture_step(vdi_ctx *c, double dt, vdi_ctx::GraphEntry &e) {
 static int step_graph(vdi_ctx *c, double dt) {
     const double t0 = c->time;
     // host half of the initial guesses (MAC at t+dt/2, the others at t+dt)
-    const double ts[4] = {t0 + 0.5 * dt, t0 + dt, t0 + dt, t0 + dt};
+    const double ts[4] = {half_time(t0, dt), end_time(t0, dt), end_time(t0, dt), end_time(t0, dt)};
     for (int k = 0; k < 4; k++) {
-        if ((k == 1 || k == 2) && !c->P.implicit_visc) { c->coef_host[2 * k] = 1; c->coef_host[2 * k + 1] = 0; continue; }
+        if (!solver_used(c, k)) { c->coef_host[2 * k] = 1; c->coef_host[2 * k + 1] = 0; continue; }
         double bb = 1, cc = 0;
         if (step_solver(c, k)->guess_coeffs(ts[k], &bb, &cc) != 1) { bb = 1; cc = 0; }
         c->coef_host[2 * k] = bb;
@@ -1117,14 +1246,14 @@ static int step_graph(vdi_ctx *c, double dt) {
     }
     VDI_HIP(hipGraphLaunch(e->exec, c->st));
     VDI_HIP(hipStreamSynchronize(c->st));
-    if (c->timer.cls >= 0) c->timer.harvest(e->slots);
+    if (c->timer.cls >= 0) VDI_TRY(c->timer.harvest(e->slot_a, e->slot_b, c->st));
     c->graph_steps++;
     const vdi_ctx::Readback &r = *c->rb;
     c->last_newton = r.gas[1];
     if (r.gas[0] != VDI_OK) return r.gas[0];
     bool ok = true;
     for (int k = 0; k < 4; k++) {
-        if ((k == 1 || k == 2) && !c->P.implicit_visc) continue;
+        if (!solver_used(c, k)) continue;
         MG &g = *step_solver(c, k);
         const MGState &st = r.st[k];
         g.total_cycles += st.cycles;
@@ -1137,21 +1266,15 @@ static int step_graph(vdi_ctx *c, double dt) {
         }
     }
     if (ok) {
-        c->time = t0 + dt;
+        c->time = end_time(t0, dt);
         return VDI_OK;
     }
     // restore the state the step started from and solve again from the
     // partially converged iterates
-    const size_t NP = (size_t)c->g.ml * (c->g.n + 4);
-    const size_t ng = c->gasbuf.bytes / sizeof(double);
-    hipLaunchKernelGGL(k_copy, dim3(2048), dim3(256), 0, c->st, (const double *)c->snap.as<double>(), c->F.u, (long)(3 * NP));
-    hipLaunchKernelGGL(k_copy, dim3(cdiv(ng, 256)), dim3(256), 0, c->st, (const double *)(c->snap.as<double>() + 3 * NP),
-                       c->gasbuf.as<double>(), (long)ng);
-    VDI_LAUNCHED();
-    c->redo_steps++;
-    c->time = t0;
-    return step_direct(c, dt, SOLVE_AGAIN);
+    return redo_step(c, dt, t0);
 }
+
+extern "C" {
 
 int vdi_step_forward(vdi_ctx *c, double dt) {
     if (!c || !(dt > 0)) return VDI_ERR_ARG;
@@ -1262,14 +1385,10 @@ int vdi_synchronize(vdi_ctx *c) {
 int vdi_time_kernels(vdi_ctx *c, int kclass, int max_launches) {
     if (!c || max_launches < 0 || kclass < -1 || kclass > VDI_KCLASS_SIMPSON) return VDI_ERR_ARG;
     VDI_HIP(hipStreamSynchronize(c->st));
-    c->drop_graphs();  // captured graphs hold the previous timer's events
-    c->timer.release();
-    c->timer.cls = kclass;
+    c->drop_graphs();  // captured graphs own slots of the previous pool
+    VDI_TRY(c->timer.setup(kclass, max_launches));
     MG *gs[4] = {&c->mac, &c->fem, &c->vx, &c->vy};
     for (MG *g : gs) g->timer = kclass < 0 ? nullptr : &c->timer;
-    if (kclass < 0) return VDI_OK;
-    c->timer.ev.resize(2 * (size_t)max_launches);
-    for (auto &e : c->timer.ev) VDI_HIP(hipEventCreate(&e));
     return VDI_OK;
 }
 
@@ -1502,6 +1621,49 @@ int vdi_mg_synchronize(vdi_mg *h) {
     if (!h) return VDI_ERR_ARG;
     VDI_HIP(hipStreamSynchronize(h->st));
     return VDI_OK;
+}
+
+int vdi_tile_shape(int shape, int *tx, int *ty) {
+    if (shape < 0 || shape >= kNumTileShapes || !tx || !ty) return VDI_ERR_ARG;
+    *tx = kTileShapes[shape].tx;
+    *ty = kTileShapes[shape].ty;
+    return VDI_OK;
+}
+
+int vdi_mg_set_tile(vdi_mg *h, int prec, int which, int level, int shape) {
+    if (!h) return VDI_ERR_ARG;
+    VDI_HIP(hipStreamSynchronize(h->st));
+    return h->g.set_tile(prec, which, level, shape);
+}
+
+int vdi_mg_autotune(vdi_mg *h, int reps) {
+    if (!h || reps < 1) return VDI_ERR_ARG;
+    return h->g.autotune(reps);
+}
+
+int vdi_autotune(vdi_ctx *c, int reps) {
+    if (!c || reps < 1) return VDI_ERR_ARG;
+    VDI_HIP(hipStreamSynchronize(c->st));
+    c->drop_graphs();
+    for (int k = 0; k < 4; k++)
+        if (solver_used(c, k)) VDI_TRY(step_solver(c, k)->autotune(reps));
+    return VDI_OK;
+}
+
+int vdi_get_tiles(vdi_ctx *c, int which_solver, int *out, int n) {
+    if (!c || !out || which_solver < VDI_MAC || which_solver > VDI_VY) return VDI_ERR_ARG;
+    MG *gs[4] = {&c->mac, &c->fem, &c->vx, &c->vy};
+    MG &g = *gs[which_solver];
+    if (!g.nl) return VDI_ERR_ARG;
+    int q = 0;
+    for (int pr = 0; pr < 2; pr++)
+        for (int w = 0; w < 2; w++)
+            for (int l = 0; l < g.lc; l++) {
+                if (q < n) out[q] = g.tile[pr][w][l];
+                q++;
+            }
+    if (q < n) out[q] = g.tile_mx;
+    return q + 1;
 }
 
 }  // extern "C"

@@ -1,0 +1,162 @@
+// step.hpp -- state of one drop-impact simulation on the device, the
+// argument record of the step kernels and their launchers (shared by the
+// single-simulation context in step.hip and the batch in batch.hip).
+#pragma once
+
+#include <vector>
+
+#include "mg.hpp"
+
+namespace vdi {
+
+// Grid constants (fluid_2d.hh:59-80).
+struct Geo {
+    int m, n, ml, me, ne, x_sym;
+    double dx, dy, xsp, ysp, xxsp, yysp;
+    double ax, bx, ay, by;
+    double rho, rhoinv, mu, muinv;
+};
+
+// Device pointers of the cell fields (fields.hh:7-19, one array each).
+struct Fields {
+    double *u, *v, *p;
+    double *ux, *uy, *vx, *vy;
+    double *ud, *vd, *ul, *vl, *ur, *vr, *uu, *vu;
+    double *c0, *c1, *c2, *c3;
+    double *src;
+};
+
+// Gas-layer arrays (gas_layer.hh:26-34, gas_layer_solve.hh:163-173).
+struct Gas {
+    double *p, *tau, *p_old, *tau_old, *h, *hx, *ht, *fbd, *pg, *pg_old;
+    int *status;  // [0] = status code, [1] = Newton steps
+};
+
+struct GasConst {
+    double alpha, mug, Pamb, h0, R, V, sigma, gxsp, gxxsp;
+};
+
+// Everything a step kernel needs about one simulation for one step.
+struct StepRec {
+    Geo g;
+    Fields F;
+    Gas G;
+    GasConst K;
+    double dt, viscfac, macfac, prefac, vfx, vfy, rdt, fm, Vframe;
+    const double *zmac, *zvx, *zvy, *zfem;
+    double *snap;   // snapshot: u, v, p (3 NP) then the gas arrays (ng)
+    double *gasbase;
+    long np, ng;
+    const MGState *mgst[4];  // MAC, VX, VY, FEM (null when absent)
+    MGState *rb_st;          // readback: four states, then two gas ints
+    int *rb_gas;
+};
+
+// Tiles of the step kernels: Godunov 32 x 8 cells (256 threads, 1D);
+// element-wise kernels 64 x 4 threads.
+constexpr int kGX = 32, kGY = 8;
+constexpr int kEX = 64, kEY = 4;
+constexpr int kGasThreads = 1024;
+constexpr int kGasRows = 5;  // rows per thread: me <= 5120
+
+// Kernel identifiers for the batched launcher.
+enum StepKernel {
+    SK_GODUNOV = 0,
+    SK_CORRECT,
+    SK_SIMPSON,
+    SK_VISCOUS_BC,
+    SK_VISCO_RHS_U,
+    SK_VISCO_RHS_V,
+    SK_VISCO_COPY,
+    SK_FEM_RHS,
+    SK_UPDATE,
+    SK_GAS,
+    SK_SNAPSHOT,
+    SK_RESTORE,
+    SK_GATHER,
+    SK_COUNT
+};
+
+namespace stepk {
+// Work items (record, tile) of kernel k for one simulation.
+int tiles(int k, const Geo &g);
+// Batched launch of kernel k over the work list.
+int launch_b(int k, const StepRec *recs, const WorkItem *w, int nw, int grid, size_t lds, BatchAux aux,
+             hipStream_t st);
+int threads(int k);
+const void *kernel_b(int k);
+size_t gas_lds(const Geo &g);
+}  // namespace stepk
+
+}  // namespace vdi
+
+// One simulation: device state, the four solvers and the captured steps.
+struct vdi_ctx {
+    vdi_params P{};
+    vdi::Geo g{};
+    vdi::GasConst K{};
+    hipStream_t st = nullptr;
+    double time = 0, Vframe = 0;
+    double rel_tol = 1e-10;
+    int max_cycles = 200;
+    int form = VDI_FORM_FP64;
+    long steps = 0;
+    int last_newton = 0;
+    vdi::DevBuf fields;  // 19 padded arrays
+    vdi::DevBuf srcbuf, gasbuf, zbuf, statusbuf;
+    vdi::Fields F{};
+    vdi::Gas G{};
+    double *zmac = nullptr, *zfem = nullptr, *zvx = nullptr, *zvy = nullptr;
+    vdi::MG mac, fem, vx, vy;
+    int *host_status = nullptr;
+    size_t gas_lds = 0;
+    vdi::KernelTimer timer;
+    // Captured steps (one graph per cycle budget, time step and form).
+    struct GraphEntry {
+        int budget[4];
+        double dt;
+        int form;
+        hipGraphExec_t exec;
+        int slot_a, slot_b;  // timer slots owned by the graph
+    };
+    struct Readback {
+        vdi::MGState st[4];
+        int gas[2];
+        int pad[2];
+    };
+    int use_graphs = 1;
+    int budget[4] = {0, 0, 0, 0};
+    std::vector<GraphEntry> graphs;
+    vdi::DevBuf snap, coef_dev, rb_dev;
+    double *coef_host = nullptr;  // pinned, 2 extrapolation coefficients per solver
+    Readback *rb = nullptr;       // pinned
+    long graph_steps = 0, redo_steps = 0;
+    void drop_graphs() {
+        for (auto &e : graphs) (void)hipGraphExecDestroy(e.exec);
+        graphs.clear();
+        timer.next = timer.reserved = 0;
+    }
+    ~vdi_ctx() {
+        if (st) (void)hipStreamSynchronize(st);
+        drop_graphs();
+        if (host_status) (void)hipHostFree(host_status);
+        if (coef_host) (void)hipHostFree(coef_host);
+        if (rb) (void)hipHostFree(rb);
+        if (st) (void)hipStreamDestroy(st);
+    }
+};
+
+namespace vdi {
+// Record of one step of c with time step dt (step.hip).
+StepRec step_record(vdi_ctx *c, double dt);
+// Solves in step order: MAC, VX, VY, FEM.
+MG *step_solver(vdi_ctx *c, int k);
+bool solver_used(vdi_ctx *c, int k);
+// Time of the half and full step (two half steps, fluid_2d.cc:102, :251).
+double half_time(double t0, double dt);
+double end_time(double t0, double dt);
+// Restores the snapshot and runs the step on the direct path (a redo).
+int redo_step(vdi_ctx *c, double dt, double t0);
+// visco_impl.cc:26-30
+void visco_setup(MG &g, double vfx, double vfy);
+}  // namespace vdi

@@ -94,56 +94,59 @@ struct LevelT {
 using Level = LevelT<double>;
 using LevelF = LevelT<float>;
 
-// Times every launch of one kernel class with HIP events recorded on the
-// stream that runs it, and keeps the algorithmic bytes of each launch (arrays
-// read + arrays written, each counted once).  Works for direct launches and
-// for captured graphs: a graph owns the event pairs recorded while it was
-// captured and the host harvests them after every replay.
+// Launch timing stamp: the wall-clock ticks at which the first workgroup of
+// a launch started and the last one finished (atomicMin / atomicMax).  A
+// launch whose workgroups all returned at once (a converged solve) leaves
+// t0 = ~0 and is not counted.
+struct Stamp {
+    unsigned long long t0, t1;
+};
+
+#ifdef __HIPCC__
+__device__ __forceinline__ void stamp_begin(Stamp *ts) {
+    if (ts && threadIdx.x == 0 && threadIdx.y == 0) atomicMin(&ts->t0, (unsigned long long)wall_clock64());
+}
+__device__ __forceinline__ void stamp_end(Stamp *ts) {
+    if (!ts) return;
+    __syncthreads();
+    if (threadIdx.x == 0 && threadIdx.y == 0) atomicMax(&ts->t1, (unsigned long long)wall_clock64());
+}
+// Workgroup busy time of one work item of a batched launch, per simulation.
+__device__ __forceinline__ unsigned long long busy_begin(const unsigned long long *busy) {
+    return busy ? (unsigned long long)wall_clock64() : 0ull;
+}
+__device__ __forceinline__ void busy_end(unsigned long long *busy, int sim, unsigned long long t0) {
+    if (busy && threadIdx.x == 0 && threadIdx.y == 0) atomicAdd(&busy[sim], (unsigned long long)wall_clock64() - t0);
+}
+#endif
+
+// Times every launch of one kernel class with stamps written by the kernels
+// themselves, and keeps the algorithmic bytes of each launch (arrays read +
+// arrays written, each counted once).  Works for direct launches and for
+// captured graphs: a graph owns the contiguous slot range handed out while it
+// was captured, clears it at its start and the host reads it after every
+// replay.
 struct KernelTimer {
-    struct Slot {
-        int pair;
-        double bytes;
-    };
-    int cls = -1;  // class being timed (VDI_KCLASS_*), -1 = off
-    std::vector<hipEvent_t> ev;
-    size_t next = 0, reserved = 0;  // pairs [0, reserved) belong to captured graphs
-    std::vector<Slot> pending;      // recorded since the last harvest / capture start
+    int cls = -1;       // class being timed (VDI_KCLASS_*), -1 = off
+    DevBuf stamps;      // cap slots
+    Stamp *host = nullptr;  // pinned mirror
+    int cap = 0;
+    int next = 0, reserved = 0;  // slots [0, reserved) belong to captured graphs
+    double ticks_per_ms = 1e5;
+    std::vector<double> slot_bytes;
     std::vector<float> dur_ms;
     std::vector<double> bytes;
-    int begin(int c, hipStream_t st) {
-        if (c != cls || 2 * next + 2 > ev.size()) return -1;
-        (void)hipEventRecord(ev[2 * next], st);
-        return (int)next++;
+    Stamp *slot(int c, double alg_bytes) {
+        if (c != cls || next >= cap) return nullptr;
+        slot_bytes[next] = alg_bytes;
+        return stamps.as<Stamp>() + next++;
     }
-    void end(int pair, hipStream_t st, double alg_bytes) {
-        if (pair < 0) return;
-        (void)hipEventRecord(ev[2 * pair + 1], st);
-        pending.push_back(Slot{pair, alg_bytes});
-    }
-    // Reads the durations of the given (completed) launches.
-    void harvest(const std::vector<Slot> &slots) {
-        for (const Slot &s : slots) {
-            float ms = 0;
-            if (hipEventElapsedTime(&ms, ev[2 * s.pair], ev[2 * s.pair + 1]) == hipSuccess) {
-                dur_ms.push_back(ms);
-                bytes.push_back(s.bytes);
-            }
-        }
-    }
-    // After a direct (uncaptured) step has completed.
-    void harvest_direct() {
-        harvest(pending);
-        pending.clear();
-        next = reserved;
-    }
-    void release() {
-        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
-        ev.clear();
-        pending.clear();
-        dur_ms.clear();
-        bytes.clear();
-        next = reserved = 0;
-    }
+    int setup(int c, int capacity);
+    // Enqueues the reset of slots [a, b) (a kernel, so it can be captured).
+    int reset(int a, int b, hipStream_t st);
+    // Reads slots [a, b) after their launches have completed.
+    int harvest(int a, int b, hipStream_t st);
+    void release();
     ~KernelTimer() { release(); }
 };
 

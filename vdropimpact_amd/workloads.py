@@ -129,5 +129,31 @@ def operator_case(kind: int, m: int, n: int, seed: int = SEED):
     return m, n, VISCO_COUPLING, VISCO_COUPLING, helmholtz_rhs(m, n, seed)
 
 
-__all__ = ["SplitMix64", "fourier_modes", "velocity", "mac_rhs", "fem_rhs", "helmholtz_rhs", "operator_case",
+BATCH_GRIDS = [(256, 128), (512, 256), (1024, 512)]
+
+
+def batch_members(count: int = 64, seed: int = SEED, grids=None):
+    """Parameters of the batched workload: member s draws, from SplitMix64
+    seeded with seed + s, a grid uniformly from `grids` (default 256x128,
+    512x256, 1024x512), a kinematic viscosity log-uniform over 1-100 cSt, an
+    impact speed uniform over 0.3-1.0 m/s and a drop radius uniform over
+    0.5-2 mm; h0_nd is recomputed so that h_min = 5 um.  Other physics as the
+    single-drop workload.  Returns (PhysicalConfig list, VdiParams list)."""
+    from .params import PhysicalConfig, h0_nd_for_hmin
+    grids = grids or BATCH_GRIDS
+    cfgs, prms = [], []
+    for s in range(count):
+        rng = SplitMix64(seed + s)
+        m, n = grids[rng.integer(0, len(grids) - 1)]
+        nu_cst = 10.0 ** (2.0 * rng.uniform())
+        V = 0.3 + 0.7 * rng.uniform()
+        R = 0.5e-3 + 1.5e-3 * rng.uniform()
+        h0_nd = h0_nd_for_hmin(5e-6, 1.8e-5, 1000.0, V, R)
+        cfg = PhysicalConfig(m=m, n=n, nul=nu_cst * 1e-6, V=V, R=R, h0_nd=h0_nd)
+        cfgs.append(cfg)
+        prms.append(cfg.params())
+    return cfgs, prms
+
+
+__all__ = ["SplitMix64", "fourier_modes", "batch_members", "BATCH_GRIDS", "velocity", "mac_rhs", "fem_rhs", "helmholtz_rhs", "operator_case",
            "SEED", "VISCO_COUPLING"]
