@@ -90,6 +90,7 @@ typedef struct vdi_stats {
 typedef struct vdi_ctx vdi_ctx;
 typedef struct vdi_mg vdi_mg;
 typedef struct vdi_batch vdi_batch;
+typedef struct vdi_nn vdi_nn;
 
 const char *vdi_status_string(int status);
 /* Text of the last HIP error seen by this library (or ""). */
@@ -214,6 +215,31 @@ int vdi_batch_autotune(vdi_batch *b, int reps);
 int vdi_batch_step_times(vdi_batch *b, double *graph_ms, double *busy_ms, int n);
 int vdi_batch_counters(vdi_batch *b, long *batched_steps, long *redone_sims);
 int vdi_batch_tiles(vdi_batch *b, int *out, int n);
+
+/* ---- learned initial guess of the MAC solve ----
+ * No counterpart in the reference, whose guess is the time extrapolation of
+ * mgs_common.cc:17-50.  A 4-level BF16 U-Net (MFMA, FP32 accumulation) maps
+ * the extrapolated guess's residual and the guess itself to a correction; the
+ * solve still iterates to the same tolerance, and keeps the extrapolated
+ * guess whenever the network's guess has the larger residual.
+ * m and n must be multiples of 64. */
+int vdi_guess_attach(vdi_ctx *ctx, int capacity, unsigned long long seed);
+/* mode 0 off, 1 record training pairs (direct steps), 2 use the network */
+int vdi_guess_mode(vdi_ctx *ctx, int mode);
+int vdi_guess_train(vdi_ctx *ctx, int epochs, int batch, double lr, unsigned long long seed, double *final_loss);
+int vdi_guess_stats(vdi_ctx *ctx, long *pairs, long *accepted, long *rejected, double *kappa);
+/* Copies the trained network (same grid) from src to dst. */
+int vdi_guess_copy(vdi_ctx *dst, vdi_ctx *src);
+
+/* The network alone (gradient checks): H x W pixels (multiples of 64),
+ * inputs planar [2][H*W]. */
+int vdi_nn_create(int H, int W, unsigned long long seed, vdi_nn **out);
+int vdi_nn_destroy(vdi_nn *nn);
+long vdi_nn_nparams(vdi_nn *nn);
+int vdi_nn_get_params(vdi_nn *nn, float *out, long n);
+int vdi_nn_set_params(vdi_nn *nn, const float *in, long n);
+int vdi_nn_forward(vdi_nn *nn, const float *x2, float *y);
+int vdi_nn_gradient(vdi_nn *nn, const float *x2, const float *t, float *grad, double *loss);
 
 #ifdef __cplusplus
 }

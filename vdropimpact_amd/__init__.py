@@ -114,6 +114,19 @@ ABI = {
     "vdi_batch_step_times": (_i, [_vp, _dp, _dp, _i]),
     "vdi_batch_counters": (_i, [_vp, ctypes.POINTER(_l), ctypes.POINTER(_l)]),
     "vdi_batch_tiles": (_i, [_vp, _ip, _i]),
+    "vdi_guess_attach": (_i, [_vp, _i, ctypes.c_ulonglong]),
+    "vdi_guess_mode": (_i, [_vp, _i]),
+    "vdi_guess_train": (_i, [_vp, _i, _i, _d, ctypes.c_ulonglong, _dp]),
+    "vdi_guess_stats": (_i, [_vp, ctypes.POINTER(_l), ctypes.POINTER(_l), ctypes.POINTER(_l), _dp]),
+    "vdi_guess_copy": (_i, [_vp, _vp]),
+    "vdi_nn_create": (_i, [_i, _i, ctypes.c_ulonglong, ctypes.POINTER(_vp)]),
+    "vdi_nn_destroy": (_i, [_vp]),
+    "vdi_nn_nparams": (_l, [_vp]),
+    "vdi_nn_get_params": (_i, [_vp, ctypes.POINTER(ctypes.c_float), _l]),
+    "vdi_nn_set_params": (_i, [_vp, ctypes.POINTER(ctypes.c_float), _l]),
+    "vdi_nn_forward": (_i, [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
+    "vdi_nn_gradient": (_i, [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                             ctypes.POINTER(ctypes.c_float), _dp]),
 }
 
 _lib = None
@@ -286,6 +299,82 @@ class Fluid2D:
         return [tile_shape(v) for v in list(arr)[:n]]
 
 
+    # ---- learned initial guess of the MAC solve ----
+    def guess_attach(self, capacity: int = 0, seed: int = 1):
+        _check(self.lib.vdi_guess_attach(self.h, capacity, seed), "guess_attach")
+
+    def guess_mode(self, mode: int):
+        """0 off, 1 record training pairs, 2 use the network."""
+        _check(self.lib.vdi_guess_mode(self.h, mode), "guess_mode")
+
+    def guess_train(self, epochs: int, batch: int = 4, lr: float = 1e-3, seed: int = 7) -> float:
+        loss = ctypes.c_double()
+        _check(self.lib.vdi_guess_train(self.h, epochs, batch, lr, seed, ctypes.byref(loss)), "guess_train")
+        return loss.value
+
+    def guess_stats(self) -> dict:
+        a, b, c, k = ctypes.c_long(), ctypes.c_long(), ctypes.c_long(), ctypes.c_double()
+        _check(self.lib.vdi_guess_stats(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(k)),
+               "guess_stats")
+        return {"pairs": a.value, "accepted": b.value, "rejected": c.value, "kappa": k.value}
+
+    def guess_copy_from(self, other: "Fluid2D"):
+        _check(self.lib.vdi_guess_copy(self.h, other.h), "guess_copy")
+
+
+_fp = ctypes.POINTER(ctypes.c_float)
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(_fp)
+
+
+class UNet:
+    """The learned-guess network alone (H x W pixels), for gradient checks."""
+
+    def __init__(self, H: int, W: int, seed: int = 1):
+        self.lib = load_library()
+        self.H, self.W = H, W
+        h = _vp()
+        _check(self.lib.vdi_nn_create(H, W, seed, ctypes.byref(h)), "vdi_nn_create")
+        self.h = h
+        self.nparams = self.lib.vdi_nn_nparams(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.vdi_nn_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self) -> np.ndarray:
+        out = np.empty(self.nparams, np.float32)
+        _check(self.lib.vdi_nn_get_params(self.h, _fptr(out), out.size), "vdi_nn_get_params")
+        return out
+
+    def set_params(self, p):
+        a = np.ascontiguousarray(p, np.float32)
+        _check(self.lib.vdi_nn_set_params(self.h, _fptr(a), a.size), "vdi_nn_set_params")
+
+    def forward(self, x2) -> np.ndarray:
+        x = np.ascontiguousarray(x2, np.float32).reshape(2, -1)
+        y = np.empty(self.H * self.W, np.float32)
+        _check(self.lib.vdi_nn_forward(self.h, _fptr(x), _fptr(y)), "vdi_nn_forward")
+        return y
+
+    def gradient(self, x2, t):
+        x = np.ascontiguousarray(x2, np.float32).reshape(2, -1)
+        tt = np.ascontiguousarray(t, np.float32).ravel()
+        g = np.empty(self.nparams, np.float32)
+        loss = ctypes.c_double()
+        _check(self.lib.vdi_nn_gradient(self.h, _fptr(x), _fptr(tt), _fptr(g), ctypes.byref(loss)), "vdi_nn_gradient")
+        return loss.value, g
+
+
 def tile_shape(index: int):
     lib = load_library()
     tx, ty = ctypes.c_int(), ctypes.c_int()
@@ -449,6 +538,6 @@ class MGSystem:
 NUM_TILE_SHAPES = 9
 
 __all__ = [
-    "Fluid2D", "MGSystem", "Batch", "tile_shape", "NUM_TILE_SHAPES", "VdiParams", "VdiStats", "VdiError", "PhysicalConfig",
+    "Fluid2D", "MGSystem", "Batch", "UNet", "tile_shape", "NUM_TILE_SHAPES", "VdiParams", "VdiStats", "VdiError", "PhysicalConfig",
     "drop_impact_params", "h0_nd_for_hmin", "load_library", "LIB_PATH", "ABI",
 ]

@@ -295,8 +295,18 @@ __device__ __forceinline__ double block_sum(double a, double *sh) {
     return a;
 }
 
+// Sum of n partials by the whole workgroup in a fixed order (thread t adds
+// entries t, t+T, ...; then the fixed-order block_sum), so the result does
+// not depend on which workgroup finished last.  Valid in thread 0.
+__device__ __forceinline__ double wg_sum_partials(const double *p, int n, int stride, double *sh) {
+    const int tid = threadIdx.x + blockDim.x * threadIdx.y, T = blockDim.x * blockDim.y;
+    double a = 0;
+    for (int k = tid; k < n; k += T) a += __hip_atomic_load(&p[(size_t)k * stride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return block_sum(a, sh);
+}
+
 // Convergence bookkeeping: the last workgroup to finish sums the per-
-// workgroup partials in index order and sets the flag.
+// workgroup partials and sets the flag.
 __device__ __forceinline__ void finish_norm(const MGCtl &ctl, double part, int bid, int nblocks, double tol2,
                                             double *sh, int *last) {
     const int tid = threadIdx.x + blockDim.x * threadIdx.y;
@@ -308,15 +318,16 @@ __device__ __forceinline__ void finish_norm(const MGCtl &ctl, double part, int b
         *last = (tk == (unsigned)nblocks - 1);
     }
     __syncthreads();
-    if (*last && tid == 0) {
+    if (*last) {
         __threadfence();
-        double rr = 0;
-        for (int k = 0; k < nblocks; k++) rr += __hip_atomic_load(&ctl.partial[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        MGState *st = ctl.state;
-        st->rr = rr;
-        st->ticket = 0;
-        st->cycles += 1;
-        st->done = !(rr == rr) ? 2 : (rr <= tol2 * st->bb ? 1 : 0);
+        const double rr = wg_sum_partials(ctl.partial, nblocks, 1, sh);
+        if (tid == 0) {
+            MGState *st = ctl.state;
+            st->rr = rr;
+            st->ticket = 0;
+            st->cycles += 1;
+            st->done = !(rr == rr) ? 2 : (rr <= tol2 * st->bb ? 1 : 0);
+        }
     }
     __syncthreads();
 }
@@ -768,13 +779,20 @@ __global__ void __launch_bounds__(kCoarseThreads) k_coarse_b(const CoarseArgs<T>
 __device__ void begin_body(const BeginArgs &a, int bid, double *sh, int *last) {
     const Level &L = a.L;
     double rr = 0, bb = 0;
-    const long N = (long)L.m * L.n;
-    for (long ij = bid * 256L + threadIdx.x; ij < N; ij += (long)a.nb * 256) {
-        const int i = (int)(ij % L.m), j = (int)(ij / L.m);
+    const int N = L.m * L.n;
+    // row-major walk without 64-bit division: (i, j) advance with ij
+    int ij = bid * 256 + threadIdx.x;
+    const int stride = a.nb * 256;
+    int j = ij / L.m, i = ij - j * L.m;
+    const int sj = stride / L.m, si = stride - sj * L.m;
+    for (; ij < N; ij += stride) {
         const double bv = L.b[ij], r = bv - Az_plain<double>(a.fem, a.cop, a.fop, L.m, L.n, L.cx, L.cy, L.z, i, j);
         if (a.r32) a.r32[ij] = (float)r;
         rr += r * r;
         bb += bv * bv;
+        i += si;
+        j += sj;
+        if (i >= L.m) { i -= L.m; j++; }
     }
     rr = block_sum(rr, sh);
     bb = block_sum(bb, sh);
@@ -786,19 +804,18 @@ __device__ void begin_body(const BeginArgs &a, int bid, double *sh, int *last) {
         *last = (tk == (unsigned)a.nb - 1);
     }
     __syncthreads();
-    if (*last && threadIdx.x == 0) {
+    if (*last) {
         __threadfence();
-        double R = 0, B = 0;
-        for (int k = 0; k < a.nb; k++) {
-            R += __hip_atomic_load(&a.ctl.partial[2 * k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            B += __hip_atomic_load(&a.ctl.partial[2 * k + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const double R = wg_sum_partials(a.ctl.partial, a.nb, 2, sh);
+        const double B = wg_sum_partials(a.ctl.partial + 1, a.nb, 2, sh);
+        if (threadIdx.x == 0) {
+            MGState *st = a.ctl.state;
+            st->ticket = 0;
+            st->rr = R;
+            st->bb = B;
+            st->cycles = 0;
+            st->done = (B == 0) ? 3 : (!(R == R) ? 2 : (R <= a.tol2 * B ? 1 : 0));
         }
-        MGState *st = a.ctl.state;
-        st->ticket = 0;
-        st->rr = R;
-        st->bb = B;
-        st->cycles = 0;
-        st->done = (B == 0) ? 3 : (!(R == R) ? 2 : (R <= a.tol2 * B ? 1 : 0));
     }
     __syncthreads();
 }
@@ -876,6 +893,23 @@ __global__ void k_extrapolate(double *z, double *zp, long N, double bb, double c
     const double zz = z[ij];
     z[ij] = bb * zz - c * zp[ij];
     zp[ij] = zz;
+}
+
+// r = b - A z for a level-0 iterate z (any array), and per-block partial
+// sums of r^2 (block order, summed later in index order).
+__global__ void __launch_bounds__(256) k_resid_out(int fem, Level L, CCOp cop, FemOp fop, const double *z, double *r,
+                                                   double *part) {
+    __shared__ double sh[8];
+    const int N = L.m * L.n;
+    double rr = 0;
+    for (int ij = blockIdx.x * 256 + threadIdx.x; ij < N; ij += gridDim.x * 256) {
+        const int j = ij / L.m, i = ij - j * L.m;
+        const double v = L.b[ij] - Az_plain<double>(fem, cop, fop, L.m, L.n, L.cx, L.cy, z, i, j);
+        if (r) r[ij] = v;
+        rr += v * v;
+    }
+    rr = block_sum(rr, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = rr;
 }
 
 __global__ void k_zero(double *z, long N) {
@@ -1622,6 +1656,12 @@ int MG::enqueue_fixed(double tol, int budget) {
     VDI_TRY(begin(tol));
     for (int c = 0; c < budget; c++) VDI_TRY(launch_cycle(tol * tol));
     return launch_vec(vec_args(VEC_SOLVE_END, nullptr));
+}
+
+int MG::residual_out(const double *z, double *r, double *part, int nb) {
+    hipLaunchKernelGGL(k_resid_out, dim3(nb), dim3(256), 0, st, (int)fem, L[0], cc, fe, z, r, part);
+    VDI_LAUNCHED();
+    return VDI_OK;
 }
 
 int MG::apply(const double *z, double *out) {

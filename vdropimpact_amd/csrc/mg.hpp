@@ -26,7 +26,7 @@ namespace vdi {
 constexpr int kMaxCoarse = 12;  // levels handled by the single-workgroup kernel
 constexpr int kMaxNu = 3;       // largest nu1/nu2 the tiled kernels accept
 constexpr int kBX = 32, kBY = 8;      // 256 threads per level-kernel workgroup
-constexpr int kCoarseThreads = 256;
+constexpr int kCoarseThreads = 512;
 constexpr int kCoarsePoints = 3072;   // largest level the single-workgroup kernel takes
 
 // Device-side solve state: convergence flag (0 running, 1 converged,
@@ -231,6 +231,9 @@ class MG {
     MGState *state() const { return state_.as<MGState>(); }
     // out = A z for level-0 device arrays
     int apply(const double *z, double *out);
+    // r = b - A z (r may be null) and nb per-block partial sums of r^2 in
+    // part[0..nb), on this system's stream (capturable).
+    int residual_out(const double *z, double *r, double *part, int nb);
     // Times every candidate tile shape of every tiled level kernel on this
     // system's arrays and keeps the fastest; level-0 z and b are restored.
     int autotune(int reps);

@@ -467,36 +467,59 @@ __device__ __forceinline__ double eno2(double p0, double p1, double p2, double p
 
 // gas_layer_solve.cc:133-162, run by one lane.  The reference updates h in
 // place while i sweeps upward, so the stencil at i sees new h[i-1], h[i-2]
-// and old h[i..i+2]; the lane keeps exactly those five values in registers
-// and only stores to global memory (the fbd update, which does not take part
-// in the recurrence, is done by all threads afterwards).
-__device__ void gas_height_serial(int m, int x_sym, double dt, double Vframe, const double *__restrict__ HO,
-                                  const double *__restrict__ UC, const double *__restrict__ VC,
-                                  double *__restrict__ h, double *__restrict__ ht) {
+// and old h[i..i+2]; the lane keeps exactly those five values in registers.
+// It works in LDS only: the new h overwrites HO[i] and h_t overwrites UC[i]
+// once they have been read (HO[i+1..i+3] are still the old values), and the
+// interior iterations, whose stencil never leaves the domain, are straight-
+// line code with the next iteration's LDS reads issued one step ahead.
+__device__ __forceinline__ double gas_href(int k, int i, int m, int x_sym, double hn1, double hn2, double o0, double o1,
+                                           double o2, double hm1, double hm2, double h0, double h1) {
+    // current value of h[k], k in [i-2, i+2]; hm1/hm2 = current h[m-1], h[m-2];
+    // h0/h1 = current h[0], h[1] (h_ref, gas_layer_solve.cc:176-180)
+    if (k >= 0 && k < m) return k == i - 1 ? hn1 : (k == i - 2 ? hn2 : (k == i ? o0 : (k == i + 1 ? o1 : o2)));
+    if (k >= m) return k == m ? 2 * hm1 - hm2 : 3 * hm1 - 2 * hm2;
+    return x_sym ? (k == -1 ? h0 : h1) : (k == -1 ? 2 * h0 - h1 : 3 * h0 - 2 * h1);
+}
+
+__device__ void gas_height_serial(int m, int x_sym, double dt, double Vframe, double *HO, double *UC, const double *VC) {
     double hn1 = 0, hn2 = 0;
-    double o0 = HO[0], o1 = HO[1], o2 = HO[2];
-    for (int i = 0; i < m; i++) {
-        double o3 = (i + 3 < m) ? HO[i + 3] : 0.0;
-        double uc = UC[i], vc = VC[i];
-        // current value of h[k] for k in [i-2, i+2]
-        auto cur = [&](int k) -> double {
-            return k == i - 1 ? hn1 : (k == i - 2 ? hn2 : (k == i ? o0 : (k == i + 1 ? o1 : o2)));
-        };
-        // h_ref (gas_layer_solve.cc:176-180)
-        auto href = [&](int k) -> double {
-            if (k >= 0 && k < m) return cur(k);
-            if (k >= m) return k == m ? 2 * cur(m - 1) - cur(m - 2) : 3 * cur(m - 1) - 2 * cur(m - 2);
-            return x_sym ? (k == -1 ? cur(0) : cur(1)) : (k == -1 ? 2 * cur(0) - cur(1) : 3 * cur(0) - 2 * cur(1));
-        };
-        double dhdx = uc > 0 ? eno2(href(i + 1), o0, href(i - 1), href(i - 2))
-                             : -eno2(href(i - 1), o0, href(i + 1), href(i + 2));
-        double htv = -uc * dhdx + vc - Vframe;
-        double hnew = o0 + dt * htv;
-        ht[i] = htv;
-        h[i] = hnew;
+    double o0 = HO[0], o1 = HO[1], o2 = m > 2 ? HO[2] : 0.0;
+    auto edge = [&](int i) {
+        // current h of the stencil's edge references
+        const double c0 = i == 0 ? o0 : (i == 1 ? hn1 : (i == 2 ? hn2 : HO[0]));
+        const double c1 = i == 0 ? o1 : (i == 1 ? o0 : (i == 2 ? hn1 : (i == 3 ? hn2 : HO[1])));
+        const double cm1 = i == m - 1 ? o0 : (i == m - 2 ? o1 : (i == m - 3 ? o2 : HO[m - 1]));
+        const double cm2 = i == m - 1 ? hn1 : (i == m - 2 ? o0 : (i == m - 3 ? o1 : (i == m - 4 ? o2 : HO[m - 2])));
+        auto href = [&](int k) { return gas_href(k, i, m, x_sym, hn1, hn2, o0, o1, o2, cm1, cm2, c0, c1); };
+        const double uc = UC[i], vc = VC[i];
+        const double dhdx = uc > 0 ? eno2(href(i + 1), o0, href(i - 1), href(i - 2))
+                                   : -eno2(href(i - 1), o0, href(i + 1), href(i + 2));
+        const double htv = -uc * dhdx + vc - Vframe;
+        const double hnew = o0 + dt * htv;
+        HO[i] = hnew;
+        UC[i] = htv;
         hn2 = hn1; hn1 = hnew;
-        o0 = o1; o1 = o2; o2 = o3;
+        o0 = o1; o1 = o2; o2 = (i + 3 < m) ? HO[i + 3] : 0.0;
+    };
+    int i = 0;
+    for (; i < 2 && i < m; i++) edge(i);
+    // interior: i-2 >= 0 and i+2 <= m-1
+    if (i < m - 2) {
+        double uc = UC[i], vc = VC[i], o3 = HO[i + 3 < m ? i + 3 : m - 1];
+        for (; i < m - 2; i++) {
+            const double ucn = UC[i + 1], vcn = VC[i + 1];
+            const double o4 = HO[i + 4 < m ? i + 4 : m - 1];
+            const double dhdx = uc > 0 ? eno2(o1, o0, hn1, hn2) : -eno2(hn1, o0, o1, o2);
+            const double htv = -uc * dhdx + vc - Vframe;
+            const double hnew = o0 + dt * htv;
+            HO[i] = hnew;
+            UC[i] = htv;
+            hn2 = hn1; hn1 = hnew;
+            o0 = o1; o1 = o2; o2 = o3; o3 = o4;
+            uc = ucn; vc = vcn;
+        }
     }
+    for (; i < m; i++) edge(i);
 }
 
 // Deterministic workgroup sum; result broadcast to every thread.
@@ -536,16 +559,20 @@ __device__ void gas_body(const StepRec &a, double *lds, double *red) {
         VC[i] = 0.5 * (F.v[IX(i, 0)] + F.v[IX(i, -1)]);
     }
     __syncthreads();
-    if (tid == 0) gas_height_serial(m, x_sym, dt, Vframe, HO, UC, VC, G.h, G.ht);
+    if (tid == 0) gas_height_serial(m, x_sym, dt, Vframe, HO, UC, VC);
     // fbd += dt v_c does not take part in the in-place sweep
     for (int i = tid; i < m; i += T) G.fbd[i] += dt * VC[i];
     __syncthreads();
+    for (int i = tid; i < m; i += T) {
+        G.h[i] = HO[i];
+        G.ht[i] = UC[i];
+    }
     // calculate_hx (gas_layer_solve.cc:58-62)
     for (int i = tid; i <= m; i += T) {
         double v;
-        if (i >= 1 && i < m) v = K.gxsp * (G.h[i] - G.h[i - 1]);
-        else if (i == 0) v = x_sym ? 0.0 : K.gxsp * (G.h[1] - G.h[0]);
-        else v = K.gxsp * (G.h[m - 1] - G.h[m - 2]);
+        if (i >= 1 && i < m) v = K.gxsp * (HO[i] - HO[i - 1]);
+        else if (i == 0) v = x_sym ? 0.0 : K.gxsp * (HO[1] - HO[0]);
+        else v = K.gxsp * (HO[m - 1] - HO[m - 2]);
         G.hx[i] = v;
     }
     // update_gas_pressure: pg_old = pg
@@ -1100,13 +1127,23 @@ enum SolveMode {
 
 static int body_solve(vdi_ctx *c, int k, double t, SolveMode mode) {
     MG &g = *step_solver(c, k);
+    // the learned guess replaces the MAC system's extrapolated guess when it
+    // lowers the initial residual (mode 2), or records training pairs (mode 1)
+    LearnedGuess *L = (k == 0 && c->guess) ? c->guess : nullptr;
+    const int lmode = guess_mode(L);
     if (mode == SOLVE_FIXED) {
         VDI_TRY(g.enqueue_extrapolate(c->coef_dev.as<double>() + 2 * k));
+        if (lmode == 2) VDI_TRY(guess_infer(*L, g, c->st));
         return g.enqueue_fixed(c->rel_tol, c->budget[k]);
     }
     int cyc = 0;
-    if (mode == SOLVE_DIRECT) VDI_TRY(g.initial_guess(t));
+    if (mode == SOLVE_DIRECT) {
+        VDI_TRY(g.initial_guess(t));
+        if (lmode == 1) VDI_TRY(guess_capture_pre(*L, g, c->st));
+        if (lmode == 2) VDI_TRY(guess_infer(*L, g, c->st));
+    }
     VDI_TRY(g.solve(c->rel_tol, c->max_cycles, &cyc));
+    if (mode == SOLVE_DIRECT && lmode == 1) VDI_TRY(guess_capture_post(*L, g, c->st));
     c->budget[k] = std::max(1, cyc);
     return VDI_OK;
 }
@@ -1163,7 +1200,7 @@ static int step_direct(vdi_ctx *c, double dt, SolveMode mode) {
 }
 
 static bool graph_ready(vdi_ctx *c) {
-    if (!c->use_graphs) return false;
+    if (!c->use_graphs || guess_mode(c->guess) == 1) return false;
     for (int k = 0; k < 4; k++) {
         if (!solver_used(c, k)) continue;
         if (step_solver(c, k)->scount < 2 || c->budget[k] < 1) return false;

@@ -77,6 +77,15 @@ enum StepKernel {
     SK_COUNT
 };
 
+struct LearnedGuess;
+// Learned initial guess of the MAC solve (nn.hip): pairs recorded around a
+// direct solve, and the network's guess in place of the extrapolation.
+int guess_capture_pre(LearnedGuess &L, MG &g, hipStream_t st);
+int guess_capture_post(LearnedGuess &L, MG &g, hipStream_t st);
+int guess_infer(LearnedGuess &L, MG &g, hipStream_t st);
+int guess_mode(const LearnedGuess *L);
+void guess_destroy(LearnedGuess *L);
+
 namespace stepk {
 // Work items (record, tile) of kernel k for one simulation.
 int tiles(int k, const Geo &g);
@@ -131,6 +140,7 @@ struct vdi_ctx {
     double *coef_host = nullptr;  // pinned, 2 extrapolation coefficients per solver
     Readback *rb = nullptr;       // pinned
     long graph_steps = 0, redo_steps = 0;
+    vdi::LearnedGuess *guess = nullptr;  // optional learned MAC initial guess
     void drop_graphs() {
         for (auto &e : graphs) (void)hipGraphExecDestroy(e.exec);
         graphs.clear();
@@ -142,6 +152,7 @@ struct vdi_ctx {
         if (host_status) (void)hipHostFree(host_status);
         if (coef_host) (void)hipHostFree(coef_host);
         if (rb) (void)hipHostFree(rb);
+        if (guess) vdi::guess_destroy(guess);
         if (st) (void)hipStreamDestroy(st);
     }
 };
