@@ -7,8 +7,12 @@ update, Godunov passes, MAC projection, boundary conditions, two implicit
 viscous solves, FEM projection).  Workload: 32 cSt drop, R=1 mm, V=0.5 m/s,
 h_min=5 um, x_sym, implicit viscosity, synthetic (no input data needed).
 
-Prints one JSON line.  The CPU baseline is the FP64 oracle (a C restatement of
-the reference step) timed on the host's cores on a bounded sample.
+`value` is the plain FP64 path.  Beside it the line carries the mixed-
+precision form, the mixed-precision + learned-guess form (network trained in
+the run on pairs the solver records), the batched form (64 simulations of
+uneven sizes), the operator-level solves and the CPU baseline (the FP64
+oracle, a C restatement of the reference step, on a bounded sample).
+Prints one JSON line; progress goes to stderr.
 """
 from __future__ import annotations
 
@@ -22,10 +26,23 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 METRIC = "steps/s at 1024x512 to 1e-10 residual: plain FP64 vs mixed+learned; HBM %"
+PEAK_HBM = 8000.0  # GB/s, MI355X
 
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def rel(a, b):
+    import numpy as np
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
+
+
+def field_errors(vdi, a, b):
+    fields = {"u": vdi.U, "v": vdi.V, "p": vdi.P, "gas_pg": vdi.PG, "gas_h": vdi.H, "gas_p": vdi.GP,
+              "gas_tau": vdi.GTAU}
+    return {name: rel(a.get(aid), b.get(aid)) for name, aid in fields.items()}
 
 
 def cpu_baseline(m, n, seconds, warmup):
@@ -51,7 +68,7 @@ def cpu_baseline(m, n, seconds, warmup):
             break
     s.close()
     return {"value": k / el, "unit": "steps/s", "cores": threads, "kind": "port",
-            "sample": f"{k} steps at {m}x{n} after {warmup} warm-up steps, {el:.1f} s, "
+            "sample": f"{k} steps of the {m}x{n} workload after {warmup} warm-up steps, {el:.1f} s, "
                       f"OMP_NUM_THREADS={threads}"}
 
 
@@ -73,17 +90,173 @@ def operator_level(vdi, reps=3):
             g.set_form(form)
             g.set_b(b)
             zero = np.zeros(M * N)
+            g.set_z(zero)
+            g.autotune(3)
             times, cyc = [], 0
             for _ in range(reps):
                 g.set_z(zero)
+                g.synchronize()
                 t0 = time.perf_counter()
                 cyc = g.solve(1e-10, 100)
                 times.append(time.perf_counter() - t0)
             t = min(times)
             gbs = cyc * g.cycle_bytes() / t / 1e9
             out.append({"system": names[kind], "grid": [m, n], "form": "fp64" if form == 0 else "mixed",
-                        "vcycles": cyc, "ms_per_solve": 1e3 * t, "GB_per_s": gbs, "frac_of_8TBps": gbs / 8000.0})
+                        "vcycles": cyc, "ms_per_solve": 1e3 * t, "GB_per_s": gbs, "frac_of_8TBps": gbs / PEAK_HBM})
             g.close()
+    return out
+
+
+def time_steps(sim, dt, steps):
+    sim.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.step_forward(dt)
+    sim.synchronize()
+    return time.perf_counter() - t0
+
+
+def run_plain(vdi, args, p):
+    sim = vdi.Fluid2D(p, rel_tol=1e-10)
+    if args.no_graphs:
+        sim.set_graphs(False)
+    sim.init_fields()
+    for _ in range(args.warmup):
+        sim.step_forward(p.dt_reg)
+    if args.autotune > 0:
+        sim.autotune(args.autotune)
+        for _ in range(3):
+            sim.step_forward(p.dt_reg)
+    log(f"plain: {args.warmup} warm-up steps, tiles tuned")
+    # Which kernel class dominates the step: every launch of one class is
+    # timed by the kernels' own wall-clock stamps over a few steps.
+    class_ms = {}
+    for kc in vdi.KCLASS_NAMES:
+        sim.time_kernels(kc, 8192)
+        for _ in range(3):
+            sim.step_forward(p.dt_reg)
+        _n, ms, _b = sim.kernel_times()
+        class_ms[kc] = ms / 3.0
+    dominant = max(class_ms, key=class_ms.get)
+    log("kernel classes, ms/step:", {vdi.KCLASS_NAMES[k]: round(v, 4) for k, v in class_ms.items()})
+    # Timed region (the captured-step path) with the dominant class stamped.
+    sim.time_kernels(dominant, 16384)
+    for _ in range(3):
+        sim.step_forward(p.dt_reg)  # step graphs captured with the stamps
+    sim.reset_stats()
+    el = time_steps(sim, p.dt_reg, args.steps)
+    launches, kms, kbytes = sim.kernel_times()
+    st = sim.stats()
+    cyc = {name: st.total_cycles[k] / max(1, st.solves[k]) for k, name in enumerate(["mac", "fem", "vx", "vy"])}
+    sim.time_kernels(-1)
+    for _ in range(3):
+        sim.step_forward(p.dt_reg)
+    el_clean = time_steps(sim, p.dt_reg, args.steps)
+    log(f"plain: {args.steps} steps in {el:.3f} s (stamped), {el_clean:.3f} s (no stamps)")
+    achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
+    sim.close()
+    return {
+        "value": args.steps / el, "ms_per_step": 1e3 * el / args.steps, "steps_per_s_no_stamps": args.steps / el_clean,
+        "avg_vcycles": cyc,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM, "traffic": None,
+                     "kernel": vdi.KCLASS_NAMES[dominant], "launches": launches,
+                     "avg_launch_us": 1e3 * kms / max(1, launches),
+                     "alg_bytes_per_launch": kbytes / max(1, launches),
+                     "class_ms_per_step": {vdi.KCLASS_NAMES[k]: v for k, v in class_ms.items()}},
+    }
+
+
+def run_forms(vdi, args, p):
+    """Mixed precision, and mixed precision + learned guess, each against a
+    plain FP64 simulation run in lockstep from the same initial state (field
+    errors after the same number of steps)."""
+    ref = vdi.Fluid2D(p, rel_tol=1e-10)
+    mix = vdi.Fluid2D(p, rel_tol=1e-10)
+    lrn = vdi.Fluid2D(p, rel_tol=1e-10)
+    for s in (ref, mix, lrn):
+        s.init_fields()
+    mix.set_form(vdi.FORM_MIXED)
+    lrn.set_form(vdi.FORM_MIXED)
+    lrn.guess_attach(capacity=args.pairs, seed=5)
+    lrn.guess_mode(1)
+    for _ in range(args.pairs + 2):
+        for s in (ref, mix, lrn):
+            s.step_forward(p.dt_reg)
+    t0 = time.perf_counter()
+    loss = lrn.guess_train(epochs=args.epochs, batch=4, lr=1e-3)
+    t_train = time.perf_counter() - t0
+    gst = lrn.guess_stats()
+    log(f"learned guess: {gst['pairs']} pairs, {args.epochs} epochs in {t_train:.1f} s, loss {loss:.4g}")
+    lrn.guess_mode(2)
+    if args.autotune > 0:
+        for s in (ref, mix, lrn):
+            s.autotune(args.autotune)
+    for s in (ref, mix, lrn):
+        for _ in range(3):
+            s.step_forward(p.dt_reg)
+        s.reset_stats()
+    g0 = lrn.guess_stats()
+    res = {}
+    for name, s in (("fp64", ref), ("mixed", mix), ("mixed_learned", lrn)):
+        el = time_steps(s, p.dt_reg, args.steps)
+        st = s.stats()
+        res[name] = {"steps_per_s": args.steps / el,
+                     "avg_vcycles": {n_: st.total_cycles[k] / max(1, st.solves[k])
+                                     for k, n_ in enumerate(["mac", "fem", "vx", "vy"])}}
+        log(f"{name}: {args.steps / el:.1f} steps/s")
+    g1 = lrn.guess_stats()
+    out = {
+        "mixed": dict(res["mixed"], field_error_vs_fp64=field_errors(vdi, mix, ref)),
+        "mixed_learned": dict(res["mixed_learned"], field_error_vs_fp64=field_errors(vdi, lrn, ref),
+                              training={"pairs": gst["pairs"], "epochs": args.epochs, "seconds": t_train,
+                                        "final_loss": loss, "grid": [p.m, p.n]},
+                              guess_accepted=g1["accepted"] - g0["accepted"],
+                              guess_rejected=g1["rejected"] - g0["rejected"]),
+        "fp64_lockstep": res["fp64"],
+    }
+    base = res["fp64"]["steps_per_s"]
+    out["speedup_mixed_vs_fp64"] = res["mixed"]["steps_per_s"] / base
+    out["speedup_mixed_learned_vs_fp64"] = res["mixed_learned"]["steps_per_s"] / base
+    out["mac_vcycles_saved_per_step"] = (res["mixed"]["avg_vcycles"]["mac"] -
+                                         res["mixed_learned"]["avg_vcycles"]["mac"])
+    for s in (ref, mix, lrn):
+        s.close()
+    return out
+
+
+def run_batch(vdi, args):
+    import numpy as np
+    from vdropimpact_amd import workloads
+    cfgs, prms = workloads.batch_members(args.batch_sims)
+    b = vdi.Batch(prms, vdi.FORM_MIXED)
+    b.init_fields()
+    for _ in range(4):
+        b.step()
+    b.autotune(2)
+    for _ in range(3):
+        b.step()
+    busy, graph = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.batch_steps):
+        b.step()
+        g, bz = b.step_times()
+        graph.append(g)
+        busy.append(bz)
+    el = time.perf_counter() - t0
+    busy = np.array(busy)
+    _nb, redo = b.counters()
+    sizes = [f"{p.m}x{p.n}" for p in prms]
+    out = {"simulations": len(prms), "steps_each": args.batch_steps, "form": "mixed",
+           "aggregate_steps_per_s": len(prms) * args.batch_steps / el,
+           "batched_step_ms_p50": float(np.percentile(graph, 50)),
+           "batched_step_ms_p99": float(np.percentile(graph, 99)),
+           "member_busy_ms_p50": float(np.percentile(busy, 50)),
+           "member_busy_ms_p99": float(np.percentile(busy, 99)),
+           "grids": {s: sizes.count(s) for s in sorted(set(sizes))},
+           "members_redone": redo}
+    log(f"batch: {out['aggregate_steps_per_s']:.0f} member-steps/s")
+    b.close()
     return out
 
 
@@ -97,9 +270,15 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-operator", action="store_true")
+    ap.add_argument("--no-forms", action="store_true", help="skip the mixed and mixed+learned forms")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-graphs", action="store_true",
                     help="run every step on the direct path (for profilers that cannot follow graph launches)")
     ap.add_argument("--autotune", type=int, default=3, help="autotuner repetitions (0: default tile shapes)")
+    ap.add_argument("--pairs", type=int, default=64, help="training pairs recorded for the learned guess")
+    ap.add_argument("--epochs", type=int, default=12)
+    ap.add_argument("--batch-sims", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=200)
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -111,70 +290,15 @@ def main():
     from vdropimpact_amd.params import drop_impact_params
 
     p = drop_impact_params(args.m, args.n)
-    sim = vdi.Fluid2D(p, rel_tol=1e-10)
-    if args.no_graphs:
-        sim.set_graphs(False)
-    sim.init_fields()
-    for _ in range(args.warmup):
-        sim.step_forward(p.dt_reg)
-    if args.autotune > 0:
-        sim.autotune(args.autotune)
-        for _ in range(3):
-            sim.step_forward(p.dt_reg)
-    log(f"{args.warmup} warm-up steps done")
-
-    # Which kernel class dominates the step: time each class over a few steps.
-    class_ms = {}
-    for kc in vdi.KCLASS_NAMES:
-        sim.time_kernels(kc, 4096)
-        for _ in range(3):
-            sim.step_forward(p.dt_reg)
-        n, ms, _b = sim.kernel_times()
-        class_ms[kc] = ms / 3.0
-    sim.time_kernels(-1)
-    dominant = max(class_ms, key=class_ms.get)
-    log("kernel classes, ms/step:", {vdi.KCLASS_NAMES[k]: round(v, 4) for k, v in class_ms.items()})
-
-    # Timed region: the dominant class's launches are bracketed by HIP events
-    # on the stream that runs them.
-    sim.synchronize()
-    sim.reset_stats()
-    sim.time_kernels(dominant, 8192)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        sim.step_forward(p.dt_reg)
-    sim.synchronize()
-    el = time.perf_counter() - t0
-    launches, kms, kbytes = sim.kernel_times()
-    sim.time_kernels(-1)
-    log(f"timed region: {args.steps} steps in {el:.3f} s")
-    st = sim.stats()
-    cyc = {name: st.total_cycles[k] / max(1, st.solves[k]) for k, name in enumerate(["mac", "fem", "vx", "vy"])}
-
-    # The captured-step path without events: a few untimed steps re-capture
-    # the step graphs, then the timed region.
-    for _ in range(3):
-        sim.step_forward(p.dt_reg)
-    sim.synchronize()
-    t1 = time.perf_counter()
-    for _ in range(args.steps):
-        sim.step_forward(p.dt_reg)
-    sim.synchronize()
-    el_clean = time.perf_counter() - t1
-
-    # The event-timed pass runs the direct (uncaptured) path; the value is
-    # the clean captured-step rate.
-    value = args.steps / el_clean
-    achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
-    peak = 8000.0
+    plain = run_plain(vdi, args, p)
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": plain["value"],
         "unit": "steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * el_clean / args.steps,
+        "ms_per_step": plain["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -183,16 +307,17 @@ def main():
         "config": {"workload": f"single drop {args.m}x{args.n}, 32 cSt, V=0.5 m/s, R=1 mm, h_min=5 um, "
                                "x_sym, implicit viscosity, all solves to rel. residual 1e-10",
                    "grid": [args.m, args.n], "form": "fp64", "rel_tol": 1e-10,
-                   "avg_vcycles": cyc, "steps_per_s_event_timed_direct_path": args.steps / el},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": peak, "unit": "GB/s",
-                     "frac": achieved / peak, "traffic": None,
-                     "kernel": vdi.KCLASS_NAMES[dominant], "launches": launches,
-                     "avg_launch_us": 1e3 * kms / max(1, launches),
-                     "alg_bytes_per_launch": kbytes / max(1, launches),
-                     "class_ms_per_step": {vdi.KCLASS_NAMES[k]: v for k, v in class_ms.items()}},
+                   "avg_vcycles": plain["avg_vcycles"],
+                   "steps_per_s_no_stamps": plain["steps_per_s_no_stamps"]},
+        "roofline": plain["roofline"],
         "cpu_baseline": None,
     }
-    sim.close()
+    if not args.no_forms:
+        log("mixed and mixed+learned forms ...")
+        out["forms"] = run_forms(vdi, args, p)
+    if not args.no_batch:
+        log("batched form ...")
+        out["batched"] = run_batch(vdi, args)
     if not args.no_operator:
         log("operator-level solves ...")
         out["operator_level"] = operator_level(vdi)

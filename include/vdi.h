@@ -112,6 +112,10 @@ int vdi_set_form(vdi_ctx *ctx, int form);
  * at its end and redone on the synchronising path if a solve fell short.
  * No reference counterpart (launch mechanics only; results are the same). */
 int vdi_set_graphs(vdi_ctx *ctx, int enable);
+/* Steps run as captured graphs, and of those the steps redone on the direct
+ * path because a solve needed more cycles than its budget.  No counterpart
+ * in the reference. */
+int vdi_get_counters(vdi_ctx *ctx, long *graph_steps, long *redo_steps);
 /* fluid_2d::init_fields (fluid_2d_sio.cc:14-35) */
 int vdi_init_fields(vdi_ctx *ctx);
 /* fluid_2d::step_forward(double dt) (fluid_2d.cc:93-405) */
@@ -219,7 +223,8 @@ int vdi_batch_tiles(vdi_batch *b, int *out, int n);
 /* ---- learned initial guess of the MAC solve ----
  * No counterpart in the reference, whose guess is the time extrapolation of
  * mgs_common.cc:17-50.  A 4-level BF16 U-Net (MFMA, FP32 accumulation) maps
- * the extrapolated guess's residual and the guess itself to a correction; the
+ * the extrapolated guess's residual, the guess itself and the previous
+ * step's correction to a correction; the
  * solve still iterates to the same tolerance, and keeps the extrapolated
  * guess whenever the network's guess has the larger residual.
  * m and n must be multiples of 64. */
@@ -232,7 +237,7 @@ int vdi_guess_stats(vdi_ctx *ctx, long *pairs, long *accepted, long *rejected, d
 int vdi_guess_copy(vdi_ctx *dst, vdi_ctx *src);
 
 /* The network alone (gradient checks): H x W pixels (multiples of 64),
- * inputs planar [2][H*W]. */
+ * inputs planar [3][H*W]. */
 int vdi_nn_create(int H, int W, unsigned long long seed, vdi_nn **out);
 int vdi_nn_destroy(vdi_nn *nn);
 long vdi_nn_nparams(vdi_nn *nn);

@@ -8,6 +8,8 @@ from __future__ import annotations
 
 import numpy as np
 
+NIN = 3  # input channels: residual, extrapolated guess, previous correction
+
 # (buffer, channels, level) and the operation list, as in nn.hpp
 BUFS = {"X0": (16, 0), "E1A": (16, 0), "CAT1": (48, 0), "P1": (16, 1), "E2A": (32, 1), "CAT2": (128, 1),
         "P2": (32, 2), "E3A": (96, 2), "CAT3": (288, 2), "P3": (96, 3), "B1": (192, 3), "B2": (192, 3),
@@ -100,11 +102,11 @@ class Net:
         raise KeyError(k)
 
     def forward(self, x2):
-        """x2: (2, H*W) planar inputs -> y (H*W,), keeps activations."""
+        """x2: (NIN, H*W) planar inputs -> y (H*W,), keeps activations."""
         H, W = self.H, self.W
         A = {name: np.zeros((H >> lv, W >> lv, c), self.dtype) for name, (c, lv) in BUFS.items()}
-        A["X0"][:, :, 0] = self._r(x2[0].reshape(H, W))
-        A["X0"][:, :, 1] = self._r(x2[1].reshape(H, W))
+        for c in range(NIN):
+            A["X0"][:, :, c] = self._r(x2[c].reshape(H, W))
         for k, (kind, a, b, act) in enumerate(OPS):
             xin = A[a[0]][:, :, a[1]:a[1] + a[2]]
             if kind == "conv":

@@ -33,7 +33,7 @@ def test_gpu_unet_forward_and_gradient_match_cpu(vdi, H, W):
     for k, off, cout, cin in layout:
         p[off + cout * 9 * cin: off + cout * 9 * cin + cout] = rng.uniform(-0.05, 0.05, cout)
     net.set_params(p)
-    x2 = rng.standard_normal((2, H * W)).astype(np.float32)
+    x2 = rng.standard_normal((nn_ref.NIN, H * W)).astype(np.float32)
     t = rng.standard_normal(H * W).astype(np.float32)
     ref = nn_ref.Net(H, W, p.astype(np.float64), emulate_bf16=True)
     y_cpu = ref.forward(x2.astype(np.float64))
@@ -70,7 +70,7 @@ def test_gpu_mfma_conv_exact_on_bf16_values(vdi):
     for k, off, cout, cin in layout:
         p[off:off + cout * 9 * cin] = rng.integers(-2, 3, cout * 9 * cin) / 8.0
     net.set_params(p)
-    x2 = rng.integers(-3, 4, (2, H * W)).astype(np.float32)
+    x2 = rng.integers(-3, 4, (nn_ref.NIN, H * W)).astype(np.float32)
     ref = nn_ref.Net(H, W, p.astype(np.float64), emulate_bf16=True)
     y_cpu = ref.forward(x2.astype(np.float64))
     y_gpu = net.forward(x2)

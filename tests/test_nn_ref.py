@@ -19,7 +19,7 @@ def test_reference_gradient_matches_finite_differences():
     for k, off, cout, cin in layout:
         p[off:off + cout * 9 * cin] = rng.uniform(-1, 1, cout * 9 * cin) * np.sqrt(6 / (9 * cin))
         p[off + cout * 9 * cin: off + cout * 9 * cin + cout] = rng.uniform(-0.1, 0.1, cout)
-    x2 = rng.standard_normal((2, H * W))
+    x2 = rng.standard_normal((nn_ref.NIN, H * W))
     t = rng.standard_normal(H * W)
     net = nn_ref.Net(H, W, p, emulate_bf16=False)
     loss, g = net.loss_and_grad(x2, t)

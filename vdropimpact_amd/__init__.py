@@ -65,6 +65,7 @@ ABI = {
     "vdi_set_solver": (_i, [_vp, _i, _i, _d, _i]),
     "vdi_set_form": (_i, [_vp, _i]),
     "vdi_set_graphs": (_i, [_vp, _i]),
+    "vdi_get_counters": (_i, [_vp, ctypes.POINTER(_l), ctypes.POINTER(_l)]),
     "vdi_init_fields": (_i, [_vp]),
     "vdi_step_forward": (_i, [_vp, _d]),
     "vdi_gas_layer_update": (_i, [_vp, _d, _d]),
@@ -210,6 +211,12 @@ class Fluid2D:
     def set_graphs(self, enable: bool):
         _check(self.lib.vdi_set_graphs(self.h, int(enable)), "vdi_set_graphs")
 
+    def counters(self):
+        """(steps run as captured graphs, of those redone on the direct path)."""
+        a, b = ctypes.c_long(), ctypes.c_long()
+        _check(self.lib.vdi_get_counters(self.h, ctypes.byref(a), ctypes.byref(b)), "vdi_get_counters")
+        return a.value, b.value
+
     def init_fields(self):
         _check(self.lib.vdi_init_fields(self.h), "init_fields")
 
@@ -323,6 +330,7 @@ class Fluid2D:
 
 
 _fp = ctypes.POINTER(ctypes.c_float)
+NN_INPUTS = 3  # network input planes: residual, extrapolated guess, previous correction
 
 
 def _fptr(a: np.ndarray):
@@ -361,13 +369,14 @@ class UNet:
         _check(self.lib.vdi_nn_set_params(self.h, _fptr(a), a.size), "vdi_nn_set_params")
 
     def forward(self, x2) -> np.ndarray:
-        x = np.ascontiguousarray(x2, np.float32).reshape(2, -1)
+        """x2: the NN_INPUTS input planes, (NN_INPUTS, H*W)."""
+        x = np.ascontiguousarray(x2, np.float32).reshape(NN_INPUTS, -1)
         y = np.empty(self.H * self.W, np.float32)
         _check(self.lib.vdi_nn_forward(self.h, _fptr(x), _fptr(y)), "vdi_nn_forward")
         return y
 
     def gradient(self, x2, t):
-        x = np.ascontiguousarray(x2, np.float32).reshape(2, -1)
+        x = np.ascontiguousarray(x2, np.float32).reshape(NN_INPUTS, -1)
         tt = np.ascontiguousarray(t, np.float32).ravel()
         g = np.empty(self.nparams, np.float32)
         loss = ctypes.c_double()

@@ -445,9 +445,14 @@ static int batch_step_graph(vdi_batch *b) {
             co[2 * k + 1] = cc;
         }
     }
+    // Budget: the most any member needed last step, plus one spare cycle --
+    // members that converge earlier skip the rest on their device flags, and
+    // a redo on the direct path costs far more than one spare cycle.
     int bud[4] = {0, 0, 0, 0};
     for (vdi_ctx *c : b->sims)
         for (int k = 0; k < 4; k++) bud[k] = std::max(bud[k], c->budget[k]);
+    for (int k = 0; k < 4; k++)
+        if (bud[k] > 0) bud[k] += 1;
     vdi_batch::GraphEntry *e = nullptr;
     for (auto &x : b->graphs)
         if (std::equal(x.budget, x.budget + 4, bud)) e = &x;

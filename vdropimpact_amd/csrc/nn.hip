@@ -303,13 +303,11 @@ __global__ void k_zero_f(float *p, long n) {
     const long q = blockIdx.x * 256L + threadIdx.x;
     if (q < n) p[q] = 0.f;
 }
-// x0[p] = (a[p], b[p], 0, ..., 0) in BF16
-__global__ void k_expand(const float *a, const float *b, bf16_t *x0, long P) {
+// x0[p] = (in[0][p], ..., in[kNNInputs-1][p], 0, ..., 0) in BF16 (planar inputs)
+__global__ void k_expand(const float *in, bf16_t *x0, long P) {
     const long p = blockIdx.x * 256L + threadIdx.x;
     if (p >= P) return;
-    x0[p * 16] = f2bf(a[p]);
-    x0[p * 16 + 1] = f2bf(b[p]);
-    for (int c = 2; c < 16; c++) x0[p * 16 + c] = 0;
+    for (int c = 0; c < 16; c++) x0[p * 16 + c] = c < kNNInputs ? f2bf(in[c * P + p]) : (bf16_t)0;
 }
 
 static unsigned g1(long n) { return (unsigned)((n + 255) / 256); }
@@ -615,23 +613,35 @@ __global__ void k_g_scales(const double *pr, const double *pz, int nb, int N, do
     stats[2] = r;
 }
 
-__global__ void k_g_prep(const double *r0, const double *zex, const double *stats, int N, bf16_t *x0, float *rh,
-                         float *zh) {
+// Network inputs of the current guess: r0 / RMS(r0), z_ex / RMS(z_ex) and
+// the previous solve's correction in target units, d_prev xxsp / (RMS(r0) dscale).
+__global__ void k_g_prep(const double *r0, const double *zex, const double *dprev, const double *stats, int N,
+                         double xxsp, double dscale, bf16_t *x0, float *slot) {
     const int p = blockIdx.x * 256 + threadIdx.x;
     if (p >= N) return;
     const float a = (float)(r0[p] / stats[0]), b = (float)(zex[p] / stats[1]);
+    const float c = (float)(dprev[p] * xxsp / (stats[0] * dscale));
     if (x0) {
         x0[(size_t)p * 16] = f2bf(a);
         x0[(size_t)p * 16 + 1] = f2bf(b);
-        for (int c = 2; c < 16; c++) x0[(size_t)p * 16 + c] = 0;
+        x0[(size_t)p * 16 + 2] = f2bf(c);
+        for (int k = 3; k < 16; k++) x0[(size_t)p * 16 + k] = 0;
     }
-    if (rh) rh[p] = a;
-    if (zh) zh[p] = b;
+    if (slot) {
+        slot[p] = a;
+        slot[N + p] = b;
+        slot[2 * N + p] = c;
+    }
 }
 
-__global__ void k_g_target(const double *z, const double *zex, const double *stats, int N, double xxsp, float *t) {
+// After a MAC solve: d_prev = z - z_ex, and (capture) the scaled target.
+__global__ void k_g_after(const double *z, const double *zex, const double *stats, int N, double xxsp, double *dprev,
+                          float *t) {
     const int p = blockIdx.x * 256 + threadIdx.x;
-    if (p < N) t[p] = (float)((z[p] - zex[p]) * xxsp / stats[0]);
+    if (p >= N) return;
+    const double d = z[p] - zex[p];
+    dprev[p] = d;
+    if (t) t[p] = (float)(d * xxsp / stats[0]);
 }
 
 __global__ void k_g_cand(const double *zex, const float *yf, const double *stats, int N, double scale, double *z) {
@@ -681,8 +691,9 @@ int LearnedGuess::init(int m_, int n_, double xxsp_, hipStream_t st, unsigned lo
     VDI_TRY(part_.alloc(3 * kGuessBlocks * sizeof(double)));
     VDI_TRY(stats_.alloc(8 * sizeof(double)));
     VDI_TRY(counters_.alloc(2 * sizeof(long long) + 2 * sizeof(int)));
+    VDI_TRY(dprev_.alloc(N * sizeof(double)));
     capacity = std::max(0, cap);
-    if (capacity) VDI_TRY(data_.alloc((size_t)capacity * 3 * N * sizeof(float)));
+    if (capacity) VDI_TRY(data_.alloc((size_t)capacity * kSlot * N * sizeof(float)));
     return VDI_OK;
 }
 
@@ -702,26 +713,29 @@ static int guess_features(LearnedGuess &L, MG &g, hipStream_t st) {
 }
 
 int guess_capture_pre(LearnedGuess &L, MG &g, hipStream_t st) {
-    if (!L.capacity) return VDI_OK;
     VDI_TRY(guess_features(L, g, st));
+    if (!L.capacity) return VDI_OK;
     const int N = L.m * L.n;
-    float *slot = L.data_.as<float>() + (size_t)L.next * 3 * N;
+    float *slot = L.data_.as<float>() + (size_t)L.next * kSlot * N;
     hipLaunchKernelGGL(k_g_prep, dim3(g1(N)), dim3(256), 0, st, (const double *)L.r0_.as<double>(),
-                       (const double *)L.zex_.as<double>(), (const double *)L.stats_.as<double>(), N, (bf16_t *)nullptr,
-                       slot, slot + N);
+                       (const double *)L.zex_.as<double>(), (const double *)L.dprev_.as<double>(),
+                       (const double *)L.stats_.as<double>(), N, L.xxsp, 1.0, (bf16_t *)nullptr, slot);
     VDI_LAUNCHED();
     return VDI_OK;
 }
 
-int guess_capture_post(LearnedGuess &L, MG &g, hipStream_t st) {
-    if (!L.capacity) return VDI_OK;
+int guess_after_solve(LearnedGuess &L, MG &g, hipStream_t st) {
     const int N = L.m * L.n;
-    float *slot = L.data_.as<float>() + (size_t)L.next * 3 * N;
-    hipLaunchKernelGGL(k_g_target, dim3(g1(N)), dim3(256), 0, st, (const double *)g.z(),
-                       (const double *)L.zex_.as<double>(), (const double *)L.stats_.as<double>(), N, L.xxsp, slot + 2 * N);
+    float *t = nullptr;
+    if (L.mode == 1 && L.capacity) t = L.data_.as<float>() + ((size_t)L.next * kSlot + 3) * N;
+    hipLaunchKernelGGL(k_g_after, dim3(g1(N)), dim3(256), 0, st, (const double *)g.z(),
+                       (const double *)L.zex_.as<double>(), (const double *)L.stats_.as<double>(), N, L.xxsp,
+                       L.dprev_.as<double>(), t);
     VDI_LAUNCHED();
-    L.next = (L.next + 1) % L.capacity;
-    L.count = std::min(L.count + 1, L.capacity);
+    if (t) {
+        L.next = (L.next + 1) % L.capacity;
+        L.count = std::min(L.count + 1, L.capacity);
+    }
     return VDI_OK;
 }
 
@@ -729,8 +743,8 @@ int guess_infer(LearnedGuess &L, MG &g, hipStream_t st) {
     const int N = L.m * L.n;
     VDI_TRY(guess_features(L, g, st));
     hipLaunchKernelGGL(k_g_prep, dim3(g1(N)), dim3(256), 0, st, (const double *)L.r0_.as<double>(),
-                       (const double *)L.zex_.as<double>(), (const double *)L.stats_.as<double>(), N, L.net.x0(),
-                       (float *)nullptr, (float *)nullptr);
+                       (const double *)L.zex_.as<double>(), (const double *)L.dprev_.as<double>(),
+                       (const double *)L.stats_.as<double>(), N, L.xxsp, L.kappa, L.net.x0(), (float *)nullptr);
     VDI_LAUNCHED();
     VDI_TRY(L.net.forward());
     hipLaunchKernelGGL(k_g_cand, dim3(g1(N)), dim3(256), 0, st, (const double *)L.zex_.as<double>(),
@@ -766,7 +780,7 @@ static int guess_train(LearnedGuess &L, int epochs, int batch, double lr, unsign
         std::vector<double> h(1024);
         for (int k = 0; k < L.count; k++) {
             hipLaunchKernelGGL(k_g_sumsq_f, dim3(1024), dim3(256), 0, st,
-                               (const float *)(L.data_.as<float>() + ((size_t)k * 3 + 2) * N), N, part.as<double>());
+                               (const float *)(L.data_.as<float>() + ((size_t)k * kSlot + 3) * N), N, part.as<double>());
             VDI_LAUNCHED();
             VDI_HIP(hipMemcpyAsync(h.data(), part.p, 1024 * sizeof(double), hipMemcpyDeviceToHost, st));
             VDI_HIP(hipStreamSynchronize(st));
@@ -775,8 +789,9 @@ static int guess_train(LearnedGuess &L, int epochs, int batch, double lr, unsign
         const double kap = std::sqrt(tot / ((double)N * L.count));
         L.kappa = kap > 0 ? kap : 1.0;
         for (int k = 0; k < L.count; k++) {
-            hipLaunchKernelGGL(k_g_scale_f, dim3(g1(N)), dim3(256), 0, st, L.data_.as<float>() + ((size_t)k * 3 + 2) * N, N,
-                               (float)(1.0 / L.kappa));
+            // the previous correction (input 2) and the target share the scale
+            hipLaunchKernelGGL(k_g_scale_f, dim3(g1(2 * N)), dim3(256), 0, st,
+                               L.data_.as<float>() + ((size_t)k * kSlot + 2) * N, 2 * N, (float)(1.0 / L.kappa));
             VDI_LAUNCHED();
         }
     }
@@ -789,11 +804,11 @@ static int guess_train(LearnedGuess &L, int epochs, int batch, double lr, unsign
         for (int k = 0; k < L.count; k++) order[k] = k;
         for (int k = L.count - 1; k > 0; k--) std::swap(order[k], order[splitmix(s) % (unsigned long long)(k + 1)]);
         for (int q = 0; q < L.count; q++) {
-            const float *slot = L.data_.as<float>() + (size_t)order[q] * 3 * N;
-            hipLaunchKernelGGL(k_expand, dim3(g1(N)), dim3(256), 0, st, slot, slot + N, net.x0(), N);
+            const float *slot = L.data_.as<float>() + (size_t)order[q] * kSlot * N;
+            hipLaunchKernelGGL(k_expand, dim3(g1(N)), dim3(256), 0, st, slot, net.x0(), N);
             VDI_LAUNCHED();
             VDI_TRY(net.forward());
-            VDI_TRY(net.backward(slot + 2 * N));
+            VDI_TRY(net.backward(slot + 3 * N));
             if ((q + 1) % batch == 0 || q == L.count - 1) VDI_TRY(net.adam((float)lr, 0.9f, 0.999f, 1e-8f));
             if (e == epochs - 1) {
                 acc += net.last_loss();
@@ -878,7 +893,7 @@ int vdi_nn_create(int H, int W, unsigned long long seed, vdi_nn **out) {
     std::unique_ptr<vdi_nn> h(new vdi_nn());
     VDI_HIP(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
     VDI_TRY(h->net.init(H, W, h->st, seed));
-    VDI_TRY(h->x2.alloc(2 * (size_t)H * W * sizeof(float)));
+    VDI_TRY(h->x2.alloc(kNNInputs * (size_t)H * W * sizeof(float)));
     VDI_TRY(h->t.alloc((size_t)H * W * sizeof(float)));
     *out = h.release();
     return VDI_OK;
@@ -906,13 +921,12 @@ int vdi_nn_set_params(vdi_nn *h, const float *in, long n) {
     return h->net.sync_bf16();
 }
 
-// x2: the two input channels, planar [2][H*W]; y: channel 0 of the head.
+// x2: the input channels, planar [kNNInputs][H*W]; y: channel 0 of the head.
 int vdi_nn_forward(vdi_nn *h, const float *x2, float *y) {
     if (!h || !x2 || !y) return VDI_ERR_ARG;
     const long P = (long)h->net.pixels(0);
-    VDI_HIP(hipMemcpy(h->x2.p, x2, 2 * P * sizeof(float), hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_expand, dim3(g1(P)), dim3(256), 0, h->st, (const float *)h->x2.as<float>(),
-                       (const float *)(h->x2.as<float>() + P), h->net.x0(), P);
+    VDI_HIP(hipMemcpy(h->x2.p, x2, kNNInputs * P * sizeof(float), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_expand, dim3(g1(P)), dim3(256), 0, h->st, (const float *)h->x2.as<float>(), h->net.x0(), P);
     VDI_LAUNCHED();
     VDI_TRY(h->net.forward());
     VDI_HIP(hipStreamSynchronize(h->st));
@@ -926,10 +940,9 @@ int vdi_nn_forward(vdi_nn *h, const float *x2, float *y) {
 int vdi_nn_gradient(vdi_nn *h, const float *x2, const float *t, float *grad, double *loss) {
     if (!h || !x2 || !t || !grad || !loss) return VDI_ERR_ARG;
     const long P = (long)h->net.pixels(0);
-    VDI_HIP(hipMemcpy(h->x2.p, x2, 2 * P * sizeof(float), hipMemcpyHostToDevice));
+    VDI_HIP(hipMemcpy(h->x2.p, x2, kNNInputs * P * sizeof(float), hipMemcpyHostToDevice));
     VDI_HIP(hipMemcpy(h->t.p, t, P * sizeof(float), hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_expand, dim3(g1(P)), dim3(256), 0, h->st, (const float *)h->x2.as<float>(),
-                       (const float *)(h->x2.as<float>() + P), h->net.x0(), P);
+    hipLaunchKernelGGL(k_expand, dim3(g1(P)), dim3(256), 0, h->st, (const float *)h->x2.as<float>(), h->net.x0(), P);
     VDI_LAUNCHED();
     VDI_TRY(h->net.zero_grads());
     VDI_TRY(h->net.forward());

@@ -1,10 +1,11 @@
 // nn.hpp -- learned initial guess for the MAC projection (no counterpart in
 // the reference, whose guess is the time extrapolation of mgs_common.cc:17-50).
 //
-// A 4-level convolutional U-Net (about 1.0M parameters) maps two input
+// A 4-level convolutional U-Net (about 1.0M parameters) maps three input
 // channels on the m x n MAC grid -- the residual r0 = b - A z_ex of the
-// extrapolated guess z_ex and z_ex itself, each scaled by its RMS -- to the
-// correction delta with A delta ~= r0.  Activations are BF16 NHWC tensors,
+// extrapolated guess z_ex, z_ex itself (each scaled by its RMS), and the
+// previous step's correction in target units -- to the correction delta with
+// A delta = r0, i.e. z = z_ex + delta.  Activations are BF16 NHWC tensors,
 // m and n are multiples of 64 (whole 8 x 8 tiles at 1/8 resolution);
 // every convolution is a 3x3 implicit GEMM on MFMA (bf16 inputs, FP32
 // accumulation); training keeps FP32 master weights, gradients and Adam
@@ -28,6 +29,11 @@
 namespace vdi {
 
 typedef unsigned short bf16_t;
+
+// Network inputs (channels 0..2 of the 16-channel input tensor) and the
+// floats per recorded training pair: the three inputs and the target.
+constexpr int kNNInputs = 3;
+constexpr int kSlot = kNNInputs + 1;
 
 class UNet {
   public:
@@ -88,8 +94,9 @@ struct LearnedGuess {
     int m = 0, n = 0;
     double xxsp = 1;
     double kappa = 1;   // dataset RMS of the scaled target
-    DevBuf zex_, r0_, part_, stats_, cand_;
-    // dataset: per pair r0-hat, z-hat, target (FP32 each), capacity pairs
+    DevBuf zex_, r0_, dprev_, part_, stats_, cand_;
+    // dataset: per pair the kNNInputs input planes and the target (FP32),
+    // capacity pairs
     DevBuf data_;
     int capacity = 0, count = 0, next = 0;
     int mode = 0;       // 0 off, 1 capture, 2 infer

@@ -467,11 +467,12 @@ __device__ __forceinline__ double eno2(double p0, double p1, double p2, double p
 
 // gas_layer_solve.cc:133-162, run by one lane.  The reference updates h in
 // place while i sweeps upward, so the stencil at i sees new h[i-1], h[i-2]
-// and old h[i..i+2]; the lane keeps exactly those five values in registers.
-// It works in LDS only: the new h overwrites HO[i] and h_t overwrites UC[i]
-// once they have been read (HO[i+1..i+3] are still the old values), and the
-// interior iterations, whose stencil never leaves the domain, are straight-
-// line code with the next iteration's LDS reads issued one step ahead.
+// and old h[i..i+2]; the lane carries the two new values in registers.  HO
+// (old h) is only read; the new h replaces UC[i] and h_t replaces VC[i] once
+// they have been read.  Interior cells (stencil inside the domain) go in
+// chunks of 16: the chunk's inputs are loaded into registers first, the
+// recurrence then runs register to register, and the results are stored
+// together -- one LDS round trip per chunk instead of one per cell.
 __device__ __forceinline__ double gas_href(int k, int i, int m, int x_sym, double hn1, double hn2, double o0, double o1,
                                            double o2, double hm1, double hm2, double h0, double h1) {
     // current value of h[k], k in [i-2, i+2]; hm1/hm2 = current h[m-1], h[m-2];
@@ -481,43 +482,69 @@ __device__ __forceinline__ double gas_href(int k, int i, int m, int x_sym, doubl
     return x_sym ? (k == -1 ? h0 : h1) : (k == -1 ? 2 * h0 - h1 : 3 * h0 - 2 * h1);
 }
 
-__device__ void gas_height_serial(int m, int x_sym, double dt, double Vframe, double *HO, double *UC, const double *VC) {
+__device__ void gas_height_serial(int m, int x_sym, double dt, double Vframe, const double *HO, double *UC, double *VC) {
     double hn1 = 0, hn2 = 0;
     double o0 = HO[0], o1 = HO[1], o2 = m > 2 ? HO[2] : 0.0;
     auto edge = [&](int i) {
-        // current h of the stencil's edge references
-        const double c0 = i == 0 ? o0 : (i == 1 ? hn1 : (i == 2 ? hn2 : HO[0]));
-        const double c1 = i == 0 ? o1 : (i == 1 ? o0 : (i == 2 ? hn1 : (i == 3 ? hn2 : HO[1])));
-        const double cm1 = i == m - 1 ? o0 : (i == m - 2 ? o1 : (i == m - 3 ? o2 : HO[m - 1]));
-        const double cm2 = i == m - 1 ? hn1 : (i == m - 2 ? o0 : (i == m - 3 ? o1 : (i == m - 4 ? o2 : HO[m - 2])));
+        // current h of the stencil's edge references (only i < 2 and
+        // i >= m - 2 come here, where they lie inside the window)
+        const double c0 = i == 0 ? o0 : hn1;
+        const double c1 = i == 0 ? o1 : o0;
+        const double cm1 = i == m - 1 ? o0 : o1;
+        const double cm2 = i == m - 1 ? hn1 : o0;
         auto href = [&](int k) { return gas_href(k, i, m, x_sym, hn1, hn2, o0, o1, o2, cm1, cm2, c0, c1); };
         const double uc = UC[i], vc = VC[i];
         const double dhdx = uc > 0 ? eno2(href(i + 1), o0, href(i - 1), href(i - 2))
                                    : -eno2(href(i - 1), o0, href(i + 1), href(i + 2));
         const double htv = -uc * dhdx + vc - Vframe;
         const double hnew = o0 + dt * htv;
-        HO[i] = hnew;
-        UC[i] = htv;
+        UC[i] = hnew;
+        VC[i] = htv;
         hn2 = hn1; hn1 = hnew;
         o0 = o1; o1 = o2; o2 = (i + 3 < m) ? HO[i + 3] : 0.0;
     };
     int i = 0;
     for (; i < 2 && i < m; i++) edge(i);
     // interior: i-2 >= 0 and i+2 <= m-1
-    if (i < m - 2) {
-        double uc = UC[i], vc = VC[i], o3 = HO[i + 3 < m ? i + 3 : m - 1];
-        for (; i < m - 2; i++) {
-            const double ucn = UC[i + 1], vcn = VC[i + 1];
-            const double o4 = HO[i + 4 < m ? i + 4 : m - 1];
-            const double dhdx = uc > 0 ? eno2(o1, o0, hn1, hn2) : -eno2(hn1, o0, o1, o2);
-            const double htv = -uc * dhdx + vc - Vframe;
-            const double hnew = o0 + dt * htv;
-            HO[i] = hnew;
-            UC[i] = htv;
-            hn2 = hn1; hn1 = hnew;
-            o0 = o1; o1 = o2; o2 = o3; o3 = o4;
-            uc = ucn; vc = vcn;
+    constexpr int CH = 16;
+    for (; i + CH <= m - 2; i += CH) {
+        double ho[CH + 2], uc[CH], vc[CH], hn[CH], ht[CH];
+#pragma unroll
+        for (int j = 0; j < CH + 2; j++) ho[j] = HO[i + j];
+#pragma unroll
+        for (int j = 0; j < CH; j++) {
+            uc[j] = UC[i + j];
+            vc[j] = VC[i + j];
         }
+#pragma unroll
+        for (int j = 0; j < CH; j++) {
+            const double p0 = ho[j], p1 = ho[j + 1], p2 = ho[j + 2];
+            const double dhdx = uc[j] > 0 ? eno2(p1, p0, hn1, hn2) : -eno2(hn1, p0, p1, p2);
+            const double htv = -uc[j] * dhdx + vc[j] - Vframe;
+            const double hnew = p0 + dt * htv;
+            hn[j] = hnew;
+            ht[j] = htv;
+            hn2 = hn1; hn1 = hnew;
+        }
+#pragma unroll
+        for (int j = 0; j < CH; j++) {
+            UC[i + j] = hn[j];
+            VC[i + j] = ht[j];
+        }
+    }
+    for (; i < m - 2; i++) {
+        const double p0 = HO[i], p1 = HO[i + 1], p2 = HO[i + 2], uc = UC[i], vc = VC[i];
+        const double dhdx = uc > 0 ? eno2(p1, p0, hn1, hn2) : -eno2(hn1, p0, p1, p2);
+        const double htv = -uc * dhdx + vc - Vframe;
+        const double hnew = p0 + dt * htv;
+        UC[i] = hnew;
+        VC[i] = htv;
+        hn2 = hn1; hn1 = hnew;
+    }
+    if (i < m) {
+        o0 = HO[i];
+        o1 = i + 1 < m ? HO[i + 1] : 0.0;
+        o2 = i + 2 < m ? HO[i + 2] : 0.0;
     }
     for (; i < m; i++) edge(i);
 }
@@ -559,20 +586,22 @@ __device__ void gas_body(const StepRec &a, double *lds, double *red) {
         VC[i] = 0.5 * (F.v[IX(i, 0)] + F.v[IX(i, -1)]);
     }
     __syncthreads();
-    if (tid == 0) gas_height_serial(m, x_sym, dt, Vframe, HO, UC, VC);
     // fbd += dt v_c does not take part in the in-place sweep
     for (int i = tid; i < m; i += T) G.fbd[i] += dt * VC[i];
     __syncthreads();
+    if (tid == 0) gas_height_serial(m, x_sym, dt, Vframe, HO, UC, VC);
+    __syncthreads();
+    // new h is in UC, h_t in VC
     for (int i = tid; i < m; i += T) {
-        G.h[i] = HO[i];
-        G.ht[i] = UC[i];
+        G.h[i] = UC[i];
+        G.ht[i] = VC[i];
     }
     // calculate_hx (gas_layer_solve.cc:58-62)
     for (int i = tid; i <= m; i += T) {
         double v;
-        if (i >= 1 && i < m) v = K.gxsp * (HO[i] - HO[i - 1]);
-        else if (i == 0) v = x_sym ? 0.0 : K.gxsp * (HO[1] - HO[0]);
-        else v = K.gxsp * (HO[m - 1] - HO[m - 2]);
+        if (i >= 1 && i < m) v = K.gxsp * (UC[i] - UC[i - 1]);
+        else if (i == 0) v = x_sym ? 0.0 : K.gxsp * (UC[1] - UC[0]);
+        else v = K.gxsp * (UC[m - 1] - UC[m - 2]);
         G.hx[i] = v;
     }
     // update_gas_pressure: pg_old = pg
@@ -1134,7 +1163,9 @@ static int body_solve(vdi_ctx *c, int k, double t, SolveMode mode) {
     if (mode == SOLVE_FIXED) {
         VDI_TRY(g.enqueue_extrapolate(c->coef_dev.as<double>() + 2 * k));
         if (lmode == 2) VDI_TRY(guess_infer(*L, g, c->st));
-        return g.enqueue_fixed(c->rel_tol, c->budget[k]);
+        VDI_TRY(g.enqueue_fixed(c->rel_tol, c->budget[k]));
+        if (lmode == 2) VDI_TRY(guess_after_solve(*L, g, c->st));
+        return VDI_OK;
     }
     int cyc = 0;
     if (mode == SOLVE_DIRECT) {
@@ -1143,7 +1174,7 @@ static int body_solve(vdi_ctx *c, int k, double t, SolveMode mode) {
         if (lmode == 2) VDI_TRY(guess_infer(*L, g, c->st));
     }
     VDI_TRY(g.solve(c->rel_tol, c->max_cycles, &cyc));
-    if (mode == SOLVE_DIRECT && lmode == 1) VDI_TRY(guess_capture_post(*L, g, c->st));
+    if (mode == SOLVE_DIRECT && lmode) VDI_TRY(guess_after_solve(*L, g, c->st));
     c->budget[k] = std::max(1, cyc);
     return VDI_OK;
 }
@@ -1449,6 +1480,13 @@ int vdi_kernel_times(vdi_ctx *c, long *launches, double *total_ms, double *alg_b
     *launches = n;
     *total_ms = t;
     *alg_bytes = b;
+    return VDI_OK;
+}
+
+int vdi_get_counters(vdi_ctx *c, long *graph_steps, long *redo_steps) {
+    if (!c || !graph_steps || !redo_steps) return VDI_ERR_ARG;
+    *graph_steps = c->graph_steps;
+    *redo_steps = c->redo_steps;
     return VDI_OK;
 }
 

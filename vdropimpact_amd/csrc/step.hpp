@@ -81,7 +81,7 @@ struct LearnedGuess;
 // Learned initial guess of the MAC solve (nn.hip): pairs recorded around a
 // direct solve, and the network's guess in place of the extrapolation.
 int guess_capture_pre(LearnedGuess &L, MG &g, hipStream_t st);
-int guess_capture_post(LearnedGuess &L, MG &g, hipStream_t st);
+int guess_after_solve(LearnedGuess &L, MG &g, hipStream_t st);
 int guess_infer(LearnedGuess &L, MG &g, hipStream_t st);
 int guess_mode(const LearnedGuess *L);
 void guess_destroy(LearnedGuess *L);
