@@ -14,6 +14,11 @@
 #include <string.h>
 
 #define VDO_MAXLEV 24
+/* Coarsening stops at the first level with at most VDO_DENSE_POINTS
+ * unknowns; that level is solved exactly by a dense inverse when it has at
+ * most VDO_DENSE_MAX unknowns (coarsening can stop earlier, on odd sizes). */
+#define VDO_DENSE_POINTS 256
+#define VDO_DENSE_MAX 1024
 
 /* ------------------------------------------------------------------------ */
 /* Multigrid                                                                 */
@@ -46,6 +51,7 @@ struct vdo_mg {
     double t0, t_prev;
     double vfx, vfy; /* visco: coefficients currently set up (visco_impl.cc) */
     int last_cycles;
+    double *inv;     /* dense inverse of the coarsest level (row-major), or NULL */
 };
 
 static double bc_diag_factor(int bc) { return bc == BC_DF ? 2.0 : (bc == BC_DC ? 1.0 : 0.0); }
@@ -73,10 +79,59 @@ static void mg_build_levels(vdo_mg *g, int m, int n, double cx, double cy) {
         L->r = (double *)xcalloc(N, sizeof(double));
         l++;
         if (l >= VDO_MAXLEV) break;
+        if (N <= VDO_DENSE_POINTS) break;
         if (mc % 2 || nc % 2 || mc / 2 < 2 || nc / 2 < 2) break;
         mc /= 2; nc /= 2; cx *= 0.25; cy *= 0.25;
     }
     g->nl = l;
+}
+
+static void level_apply(const vdo_mg *g, const vdo_level *L, const double *z, double *out);
+
+/* Dense inverse of the coarsest level's operator: columns A e_k by applying
+ * the level operator to unit vectors, then Gauss-Jordan elimination with
+ * partial pivoting. */
+static void mg_dense_inverse(vdo_mg *g) {
+    const vdo_level *L = &g->L[g->nl - 1];
+    const int N = L->m * L->n;
+    free(g->inv);
+    g->inv = NULL;
+    if (N > VDO_DENSE_MAX) return;
+    const int W = 2 * N;
+    double *a = (double *)xcalloc((size_t)N * W, sizeof(double)); /* [A | I], row-major */
+    double *e = (double *)xcalloc(N, sizeof(double)), *col = (double *)xcalloc(N, sizeof(double));
+    for (int k = 0; k < N; k++) {
+        e[k] = 1;
+        level_apply(g, L, e, col);
+        e[k] = 0;
+        for (int i = 0; i < N; i++) a[(size_t)i * W + k] = col[i];
+        a[(size_t)k * W + N + k] = 1;
+    }
+    for (int c = 0; c < N; c++) {
+        int piv = c;
+        for (int i = c + 1; i < N; i++)
+            if (fabs(a[(size_t)i * W + c]) > fabs(a[(size_t)piv * W + c])) piv = i;
+        if (piv != c)
+            for (int k = 0; k < W; k++) {
+                double t = a[(size_t)c * W + k];
+                a[(size_t)c * W + k] = a[(size_t)piv * W + k];
+                a[(size_t)piv * W + k] = t;
+            }
+        const double d = 1.0 / a[(size_t)c * W + c];
+        for (int k = 0; k < W; k++) a[(size_t)c * W + k] *= d;
+        for (int i = 0; i < N; i++) {
+            if (i == c) continue;
+            const double f = a[(size_t)i * W + c];
+            if (f == 0) continue;
+            for (int k = 0; k < W; k++) a[(size_t)i * W + k] -= f * a[(size_t)c * W + k];
+        }
+    }
+    g->inv = (double *)xcalloc((size_t)N * N, sizeof(double));
+    for (int i = 0; i < N; i++)
+        for (int k = 0; k < N; k++) g->inv[(size_t)i * N + k] = a[(size_t)i * W + N + k];
+    free(a);
+    free(e);
+    free(col);
 }
 
 vdo_mg *vdo_mg_create(int kind, int m, int n, double cx, double cy, int x_sym) {
@@ -105,6 +160,7 @@ vdo_mg *vdo_mg_create(int kind, int m, int n, double cx, double cy, int x_sym) {
         g->fc = -1. / 6. * (dxdy + dydx);
     }
     mg_build_levels(g, m, n, cx, cy);
+    mg_dense_inverse(g);
     g->z_prev = (double *)xcalloc((size_t)m * n, sizeof(double));
     return g;
 }
@@ -116,6 +172,7 @@ void vdo_mg_destroy(vdo_mg *g) {
         free(g->L[l].r);
     }
     free(g->z_prev);
+    free(g->inv);
     free(g);
 }
 
@@ -125,6 +182,7 @@ int vdo_mg_levels(const vdo_mg *g) { return g->nl; }
 /* Re-scale the cell-centred couplings (visco re-setup, visco_impl.cc:26-30). */
 static void mg_set_coeffs(vdo_mg *g, double cx, double cy) {
     for (int l = 0; l < g->nl; l++) { g->L[l].cx = cx; g->L[l].cy = cy; cx *= 0.25; cy *= 0.25; }
+    mg_dense_inverse(g);
 }
 
 /* ---- cell-centred operator ---- */
@@ -284,7 +342,17 @@ static void level_prolong(const vdo_mg *g, vdo_level *F, double *zf, const vdo_l
 static void mg_vcycle_level(vdo_mg *g, int l, const double *b, double *z) {
     vdo_level *L = &g->L[l];
     if (l == g->nl - 1) {
-        for (int k = 0; k < g->ncs; k++) level_sweep(g, L, b, z);
+        if (g->inv) {
+            /* exact solve z = A^-1 b on the coarsest level */
+            const int N = L->m * L->n;
+            for (int i = 0; i < N; i++) {
+                double v = 0;
+                for (int k = 0; k < N; k++) v += g->inv[(size_t)i * N + k] * b[k];
+                z[i] = v;
+            }
+        } else {
+            for (int k = 0; k < g->ncs; k++) level_sweep(g, L, b, z);
+        }
         return;
     }
     for (int k = 0; k < g->nu1; k++) level_sweep(g, L, b, z);

@@ -50,15 +50,15 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
 // mgs_fem.hh:42-56 + mgs_fem.cc:34-50), in the level's value type T
 // ------------------------------------------------------------------------
 
-__device__ __forceinline__ double bc_diag_factor(int bc) {
+__host__ __device__ __forceinline__ double bc_diag_factor(int bc) {
     return bc == BC_DF ? 2.0 : (bc == BC_DC ? 1.0 : 0.0);
 }
-__device__ __forceinline__ double bc_ghost_factor(int bc) {
+__host__ __device__ __forceinline__ double bc_ghost_factor(int bc) {
     return bc == BC_DF ? -1.0 : (bc == BC_DC ? 0.0 : 1.0);
 }
 
 template <class T>
-__device__ __forceinline__ T cc_diag(const CCOp &op, int m, int n, T cx, T cy, int i, int j) {
+__host__ __device__ __forceinline__ T cc_diag(const CCOp &op, int m, int n, T cx, T cy, int i, int j) {
     T d = 0;
     d += T(i > 0 ? 1.0 : bc_diag_factor(op.bcl)) * cx;
     d += T(i < m - 1 ? 1.0 : bc_diag_factor(op.bcr)) * cx;
@@ -68,7 +68,7 @@ __device__ __forceinline__ T cc_diag(const CCOp &op, int m, int n, T cx, T cy, i
 }
 
 template <class T>
-__device__ __forceinline__ T cc_off(int m, int n, T cx, T cy, const T *z, int i, int j) {
+__host__ __device__ __forceinline__ T cc_off(int m, int n, T cx, T cy, const T *z, int i, int j) {
     const T *w = z + i + (size_t)m * j;
     T sx = 0, sy = 0;
     if (i > 0) sx += w[-1];
@@ -79,12 +79,12 @@ __device__ __forceinline__ T cc_off(int m, int n, T cx, T cy, const T *z, int i,
 }
 
 template <class T>
-__device__ __forceinline__ T fem_acc(const FemOp &op, int M, int N, int i, int j) {
+__host__ __device__ __forceinline__ T fem_acc(const FemOp &op, int M, int N, int i, int j) {
     return T((i > 0 && i < M - 1 ? 1.0 : 0.5) * (j > 0 && j < N - 1 ? 1.0 : 0.5) * op.fm);
 }
 
 template <class T>
-__device__ __forceinline__ T fem_off(const FemOp &op, int M, int N, const T *z, int i, int j) {
+__host__ __device__ __forceinline__ T fem_off(const FemOp &op, int M, int N, const T *z, int i, int j) {
     const T *w = z + i + (size_t)M * j, *wm = w - M;
     const T fc = T(op.fc), fex = T(op.fex), fey = T(op.fey), hex = T(op.hex), hey = T(op.hey);
     if (j == 0) return 0;
@@ -100,7 +100,7 @@ __device__ __forceinline__ T fem_off(const FemOp &op, int M, int N, const T *z, 
 }
 
 template <class T>
-__device__ __forceinline__ T Az_plain(bool fem, const CCOp &cop, const FemOp &fop, int m, int n, T cx, T cy,
+__host__ __device__ __forceinline__ T Az_plain(bool fem, const CCOp &cop, const FemOp &fop, int m, int n, T cx, T cy,
                                      const T *z, int i, int j) {
     if (fem) return fem_acc<T>(fop, m, n, i, j) * z[i + (size_t)m * j] + fem_off<T>(fop, m, n, z, i, j);
     return T(cop.s) * cc_off<T>(m, n, cx, cy, z, i, j) + cc_diag<T>(cop, m, n, cx, cy, i, j) * z[i + (size_t)m * j];
@@ -711,7 +711,22 @@ __device__ void coarse_body(const CoarseArgs<T> &a, T *lds, double *sh, int *las
         }
         __syncthreads();
     }
-    coarse_sweeps<T>(fem, d, a.cop, a.fop, d.nl - 1, lds, a.ncs);
+    if (d.inv_dense) {
+        // exact solve on the coarsest level: z = A^-1 b with the inverse
+        // stored column by column (coalesced over the rows)
+        const int l = d.nl - 1, N = d.m[l] * d.n[l];
+        T *z = lds + d.off[l];
+        const T *b = z + N;
+        const T *inv = reinterpret_cast<const T *>(d.inv_dense);
+        for (int i = threadIdx.x; i < N; i += Tn) {
+            T v = 0;
+            for (int k = 0; k < N; k++) v += inv[(size_t)k * N + i] * b[k];
+            z[i] = v;
+        }
+        __syncthreads();
+    } else {
+        coarse_sweeps<T>(fem, d, a.cop, a.fop, d.nl - 1, lds, a.ncs);
+    }
     for (int l = d.nl - 2; l >= 0; l--) {
         const int m = d.m[l], n = d.n[l], Mc = d.m[l + 1], Nc = d.n[l + 1];
         T *z = lds + d.off[l];
@@ -1193,6 +1208,7 @@ int MG::init(int kind_, int m, int n, double cx, double cy, int x_sym, double *z
         lm.push_back(fem ? mc + 1 : mc);
         ln.push_back(fem ? nc + 1 : nc);
         if ((int)lm.size() >= 24) break;
+        if ((long)lm.back() * ln.back() <= kDensePoints) break;
         if (mc % 2 || nc % 2 || mc / 2 < 2 || nc / 2 < 2) break;
         mc /= 2;
         nc /= 2;
@@ -1235,6 +1251,13 @@ int MG::init(int kind_, int m, int n, double cx, double cy, int x_sym, double *z
             LF[l].z = q; q += Nl;
             LF[l].b = q; q += Nl;
             LF[l].zalt = q; q += Nl;
+        }
+    }
+    {
+        const long Nd = (long)lm[nl - 1] * ln[nl - 1];
+        if (Nd <= kDenseMax) {
+            VDI_TRY(inv64_.alloc(Nd * Nd * sizeof(double)));
+            VDI_TRY(inv32_.alloc(Nd * Nd * sizeof(float)));
         }
     }
     set_coeffs(cx, cy);
@@ -1286,6 +1309,48 @@ void MG::set_coeffs(double cx, double cy) {
         cx *= 0.25;
         cy *= 0.25;
     }
+    (void)dense_inverse();
+}
+
+// Dense inverse of the coarsest level: columns A e_k from the level operator
+// applied to unit vectors, Gauss-Jordan elimination with partial pivoting on
+// the host, stored column-major (inv[k N + i] = (A^-1)_ik) in FP64 and FP32.
+int MG::dense_inverse() {
+    if (!inv64_.p) return VDI_OK;
+    const int l = nl - 1, M = lm[l], Nn = ln[l], N = M * Nn, W = 2 * N;
+    std::vector<double> a((size_t)N * W, 0.0), e(N, 0.0);
+    for (int k = 0; k < N; k++) {
+        e[k] = 1;
+        for (int q = 0; q < N; q++)
+            a[(size_t)q * W + k] = Az_plain<double>(fem, cc, fe, M, Nn, L[l].cx, L[l].cy, e.data(), q % M, q / M);
+        e[k] = 0;
+        a[(size_t)k * W + N + k] = 1;
+    }
+    for (int c = 0; c < N; c++) {
+        int piv = c;
+        for (int i = c + 1; i < N; i++)
+            if (std::fabs(a[(size_t)i * W + c]) > std::fabs(a[(size_t)piv * W + c])) piv = i;
+        if (piv != c)
+            for (int k = 0; k < W; k++) std::swap(a[(size_t)c * W + k], a[(size_t)piv * W + k]);
+        const double d = 1.0 / a[(size_t)c * W + c];
+        for (int k = 0; k < W; k++) a[(size_t)c * W + k] *= d;
+        for (int i = 0; i < N; i++) {
+            if (i == c) continue;
+            const double f = a[(size_t)i * W + c];
+            if (f == 0) continue;
+            for (int k = 0; k < W; k++) a[(size_t)i * W + k] -= f * a[(size_t)c * W + k];
+        }
+    }
+    std::vector<double> t((size_t)N * N);
+    std::vector<float> tf((size_t)N * N);
+    for (int i = 0; i < N; i++)
+        for (int k = 0; k < N; k++) {
+            t[(size_t)k * N + i] = a[(size_t)i * W + N + k];
+            tf[(size_t)k * N + i] = (float)t[(size_t)k * N + i];
+        }
+    VDI_HIP(hipMemcpy(inv64_.p, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+    VDI_HIP(hipMemcpy(inv32_.p, tf.data(), tf.size() * sizeof(float), hipMemcpyHostToDevice));
+    return VDI_OK;
 }
 
 MGCtl MG::ctl() const {
@@ -1381,6 +1446,7 @@ CoarseArgs<T> MG::coarse_args(const std::vector<LevelT<T>> &H, double tol2) {
     d.gz = reinterpret_cast<double *>(H[lc].z);
     d.gb = reinterpret_cast<double *>(H[lc].b);
     d.top = lc == 0;
+    d.inv_dense = sizeof(T) == 8 ? inv64_.p : inv32_.p;
     a.cop = cc;
     a.fop = fe;
     a.fem = fem;

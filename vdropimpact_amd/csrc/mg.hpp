@@ -28,6 +28,12 @@ constexpr int kMaxNu = 3;       // largest nu1/nu2 the tiled kernels accept
 constexpr int kBX = 32, kBY = 8;      // 256 threads per level-kernel workgroup
 constexpr int kCoarseThreads = 512;
 constexpr int kCoarsePoints = 3072;   // largest level the single-workgroup kernel takes
+// Coarsening stops at the first level with at most kDensePoints unknowns,
+// which is solved exactly with a dense inverse when it has at most kDenseMax
+// unknowns (coarsening can also stop earlier, on odd sizes; then the coarsest
+// level gets ncs smoothing sweeps instead).
+constexpr int kDensePoints = 256;
+constexpr int kDenseMax = 1024;
 
 // Device-side solve state: convergence flag (0 running, 1 converged,
 // 2 NaN, 3 zero right-hand side), V-cycles run, ticket of the last-
@@ -91,6 +97,7 @@ struct CoarseDesc {
     int m[kMaxCoarse], n[kMaxCoarse], off[kMaxCoarse];
     double cx[kMaxCoarse], cy[kMaxCoarse], inv[kMaxCoarse];
     double *gz, *gb;
+    const void *inv_dense;  // column-major inverse of the coarsest level, or null
 };
 
 template <class T>
@@ -249,6 +256,8 @@ class MG {
     // Moves the hand-over to the single-workgroup kernel (pts <= kCoarsePoints);
     // the cycle computes the same values either way.
     int set_coarse_points(int pts);
+    // Recomputes the coarsest level's dense inverse (after a coefficient change).
+    int dense_inverse();
 
   private:
     template <class T> int launch_down(const std::vector<LevelT<T>> &H, int l, int zero_z);
@@ -263,6 +272,7 @@ class MG {
     DevBuf partial_;
     DevBuf state_;
     DevBuf own_z_, own_b_;
+    DevBuf inv64_, inv32_;  // dense inverse of the coarsest level
     MGState *host_state_ = nullptr;  // pinned
     size_t coarse_lds_ = 0;
 
