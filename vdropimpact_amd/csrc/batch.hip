@@ -53,6 +53,7 @@ struct SolvePlan {
     bool used = false, fem = false, mixed = false;
     int maxlc = 0;
     BLaunch extrap, dirichlet, begin, solve_end;
+    BLaunch nbegin, ncycle;            // one-workgroup norm reductions (records only)
     std::vector<BLaunch> down, up;     // FP64 path, per level
     BLaunch coarse;
     BLaunch down0[2], mx[2];           // mixed path, level 0, by parity
@@ -217,6 +218,21 @@ static BLaunch vec_launch(vdi_batch *b, Blob &B, int k, int op, const double *co
     return L;
 }
 
+// Norm reductions after k_begin (begin = 1) or after a cycle's level-0 up
+// kernel (begin = 0; members whose coarse kernel owns level 0 reduce there).
+static BLaunch norm_launch(vdi_batch *b, Blob &B, int k, int begin, double tol2) {
+    std::vector<NormArgs> recs;
+    std::vector<WorkItem> work;
+    for (int s = 0; s < b->S; s++) {
+        MG &g = *step_solver(b->sims[s], k);
+        if (!begin && g.lc == 0) continue;
+        recs.push_back(g.norm_args(begin, tol2));
+    }
+    BLaunch L = add_launch(B, recs, work, 0, nullptr);
+    L.nw = (int)recs.size();
+    return L;
+}
+
 static BLaunch begin_launch(vdi_batch *b, Blob &B, int k, double tol2) {
     std::vector<BeginArgs> recs;
     std::vector<WorkItem> work;
@@ -277,6 +293,7 @@ static int build_plan(vdi_batch *b) {
         P.extrap = vec_launch(b, B, k, VEC_EXTRAP, coef);
         if (P.fem) P.dirichlet = vec_launch(b, B, k, VEC_DIRICHLET, nullptr);
         P.begin = begin_launch(b, B, k, tol2);
+        P.nbegin = norm_launch(b, B, k, 1, tol2);
         P.solve_end = vec_launch(b, B, k, VEC_SOLVE_END, nullptr);
         if (!P.mixed) {
             for (int l = 0; l < P.maxlc; l++) {
@@ -284,6 +301,7 @@ static int build_plan(vdi_batch *b) {
                 P.up.push_back(level_launch(b, B, k, 0, 1, l, 0, tol2));
             }
             P.coarse = coarse_launch(b, B, k, 0, tol2);
+            P.ncycle = norm_launch(b, B, k, 0, tol2);
         } else {
             for (int par = 0; par < 2; par++) {
                 P.down0[par] = level_launch(b, B, k, 1, 0, 0, par, tol2);
@@ -296,6 +314,7 @@ static int build_plan(vdi_batch *b) {
                 P.upf.push_back(level_launch(b, B, k, 1, 1, l, 0, tol2));
             }
             P.coarsef = coarse_launch(b, B, k, 1, tol2);
+            P.ncycle = norm_launch(b, B, k, 0, tol2);
         }
     }
     VDI_TRY(b->blob.alloc(B.bytes.size()));
@@ -354,14 +373,15 @@ static int launch_cycle(vdi_batch *b, int k, int c) {
         for (int l = 0; l < P.maxlc; l++) VDI_TRY(launch_level(b, P.down[l], 0, 0, P.fem));
         VDI_TRY(launch_coarse(b, P.coarse, 0));
         for (int l = P.maxlc - 1; l >= 0; l--) VDI_TRY(launch_level(b, P.up[l], 0, 1, P.fem));
-        return VDI_OK;
+        return mgk::norm_b(recs_at<NormArgs>(b, P.ncycle), P.ncycle.nw, b->st);
     }
     const int par = c & 1;
     VDI_TRY(launch_level(b, P.down0[par], 1, 0, P.fem));
     for (int l = 1; l < P.maxlc; l++) VDI_TRY(launch_level(b, P.downf[l], 1, 0, P.fem));
     VDI_TRY(launch_coarse(b, P.coarsef, 1));
     for (int l = P.maxlc - 1; l >= 1; l--) VDI_TRY(launch_level(b, P.upf[l], 1, 1, P.fem));
-    return launch_level(b, P.mx[par], 1, 2, P.fem);
+    VDI_TRY(launch_level(b, P.mx[par], 1, 2, P.fem));
+    return mgk::norm_b(recs_at<NormArgs>(b, P.ncycle), P.ncycle.nw, b->st);
 }
 
 static int launch_solve(vdi_batch *b, int k, int budget) {
@@ -370,6 +390,7 @@ static int launch_solve(vdi_batch *b, int k, int budget) {
     if (P.fem) VDI_TRY(launch_vec(b, P.dirichlet));
     VDI_TRY(mgk::begin_b(recs_at<BeginArgs>(b, P.begin), work_at(b, P.begin), P.begin.nw, P.begin.grid, aux_of(b),
                          b->st));
+    VDI_TRY(mgk::norm_b(recs_at<NormArgs>(b, P.nbegin), P.nbegin.nw, b->st));
     for (int c = 0; c < budget; c++) VDI_TRY(launch_cycle(b, k, c));
     return launch_vec(b, P.solve_end);
 }

@@ -295,39 +295,24 @@ __device__ __forceinline__ double block_sum(double a, double *sh) {
     return a;
 }
 
-// Sum of n partials by the whole workgroup in a fixed order (thread t adds
-// entries t, t+T, ...; then the fixed-order block_sum), so the result does
-// not depend on which workgroup finished last.  Valid in thread 0.
-__device__ __forceinline__ double wg_sum_partials(const double *p, int n, int stride, double *sh) {
-    const int tid = threadIdx.x + blockDim.x * threadIdx.y, T = blockDim.x * blockDim.y;
-    double a = 0;
-    for (int k = tid; k < n; k += T) a += __hip_atomic_load(&p[(size_t)k * stride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return block_sum(a, sh);
+// Convergence bookkeeping, first half: each workgroup stores its partial of
+// ||b - A z||^2 (plain store; the one-workgroup k_norm_finish launched next
+// sums them in index order and raises the flag).  Kernel boundaries order
+// the stores, so no device-scope fence is needed here.
+__device__ __forceinline__ void store_partial(const MGCtl &ctl, double part, int bid, double *sh) {
+    part = block_sum(part, sh);
+    if (threadIdx.x + blockDim.x * threadIdx.y == 0) ctl.partial[bid] = part;
 }
 
-// Convergence bookkeeping: the last workgroup to finish sums the per-
-// workgroup partials and sets the flag.
-__device__ __forceinline__ void finish_norm(const MGCtl &ctl, double part, int bid, int nblocks, double tol2,
-                                            double *sh, int *last) {
-    const int tid = threadIdx.x + blockDim.x * threadIdx.y;
-    part = block_sum(part, sh);
-    if (tid == 0) {
-        ctl.partial[bid] = part;
-        __threadfence();
-        unsigned tk = atomicAdd(&ctl.state->ticket, 1u);
-        *last = (tk == (unsigned)nblocks - 1);
-    }
-    __syncthreads();
-    if (*last) {
-        __threadfence();
-        const double rr = wg_sum_partials(ctl.partial, nblocks, 1, sh);
-        if (tid == 0) {
-            MGState *st = ctl.state;
-            st->rr = rr;
-            st->ticket = 0;
-            st->cycles += 1;
-            st->done = !(rr == rr) ? 2 : (rr <= tol2 * st->bb ? 1 : 0);
-        }
+// Single-workgroup variant (the coarse kernel when it owns level 0): the
+// workgroup's own sum is final.
+__device__ __forceinline__ void finish_norm_single(const MGCtl &ctl, double part, double tol2, double *sh) {
+    const double rr = block_sum(part, sh);
+    if (threadIdx.x + blockDim.x * threadIdx.y == 0) {
+        MGState *st = ctl.state;
+        st->rr = rr;
+        st->cycles += 1;
+        st->done = !(rr == rr) ? 2 : (rr <= tol2 * st->bb ? 1 : 0);
     }
     __syncthreads();
 }
@@ -422,7 +407,8 @@ __device__ __forceinline__ void up_body(const LvArgs<T> &a, int tile, T *lds, do
             rr += r * r;
         }
     }
-    finish_norm(a.ctl, rr, tile, a.nb, a.tol2, sh, last);
+    store_partial(a.ctl, rr, tile, sh);
+    (void)last;
 }
 
 // Mixed form, level 0, end of an inner FP32 cycle: prolongate and smooth the
@@ -515,7 +501,8 @@ __device__ __forceinline__ void mx_body(const MxArgs &a, int tile, double *lds_d
             rr += r * r;
         }
     }
-    finish_norm(a.ctl, rr, tile, a.nb, a.tol2, sh, last);
+    store_partial(a.ctl, rr, tile, sh);
+    (void)last;
 }
 
 // ------------------------------------------------------------------------
@@ -749,7 +736,8 @@ __device__ void coarse_body(const CoarseArgs<T> &a, T *lds, double *sh, int *las
         const double r = b[k] - Az_plain<T>(fem, a.cop, a.fop, d.m[0], d.n[0], T(d.cx[0]), T(d.cy[0]), z, i, j);
         rr += r * r;
     }
-    finish_norm(a.ctl, rr, 0, 1, a.tol2, sh, last);
+    finish_norm_single(a.ctl, rr, a.tol2, sh);
+    (void)last;
 }
 
 template <class T>
@@ -814,25 +802,54 @@ __device__ void begin_body(const BeginArgs &a, int bid, double *sh, int *last) {
     if (threadIdx.x == 0) {
         a.ctl.partial[2 * bid] = rr;
         a.ctl.partial[2 * bid + 1] = bb;
-        __threadfence();
-        unsigned tk = atomicAdd(&a.ctl.state->ticket, 1u);
-        *last = (tk == (unsigned)a.nb - 1);
     }
-    __syncthreads();
-    if (*last) {
-        __threadfence();
-        const double R = wg_sum_partials(a.ctl.partial, a.nb, 2, sh);
-        const double B = wg_sum_partials(a.ctl.partial + 1, a.nb, 2, sh);
-        if (threadIdx.x == 0) {
+    (void)last;
+}
+
+// Second half of a reduction: one workgroup sums the partials of the
+// previous launch in a fixed order and updates the solve state.
+//   begin = 1: partials are (||r||^2, ||b||^2) pairs; sets rr, bb, cycles = 0
+//              and the flag (1 converged, 2 NaN, 3 zero right-hand side);
+//   begin = 0: partials are ||r||^2 after a V-cycle; cycles += 1.
+__device__ void norm_finish_body(const NormArgs &a, double *sh) {
+    const int T = blockDim.x * blockDim.y;
+    const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+    if (a.begin) {
+        double r = 0, b = 0;
+        for (int k = tid; k < a.nb; k += T) { r += a.ctl.partial[2 * k]; b += a.ctl.partial[2 * k + 1]; }
+        const double R = block_sum(r, sh), B = block_sum(b, sh);
+        if (tid == 0) {
             MGState *st = a.ctl.state;
-            st->ticket = 0;
             st->rr = R;
             st->bb = B;
             st->cycles = 0;
             st->done = (B == 0) ? 3 : (!(R == R) ? 2 : (R <= a.tol2 * B ? 1 : 0));
         }
+    } else {
+        if (a.ctl.state->done) return;
+        double r = 0;
+        for (int k = tid; k < a.nb; k += T) r += a.ctl.partial[k];
+        const double R = block_sum(r, sh);
+        if (tid == 0) {
+            MGState *st = a.ctl.state;
+            st->rr = R;
+            st->cycles += 1;
+            st->done = !(R == R) ? 2 : (R <= a.tol2 * st->bb ? 1 : 0);
+        }
     }
-    __syncthreads();
+}
+
+__global__ void __launch_bounds__(256) k_norm_finish(NormArgs a) {
+    __shared__ double sh[8];
+    norm_finish_body(a, sh);
+}
+
+__global__ void __launch_bounds__(256) k_norm_finish_b(const NormArgs *A, int n) {
+    __shared__ double sh[8];
+    for (int p = blockIdx.x; p < n; p += gridDim.x) {
+        norm_finish_body(A[p], sh);
+        __syncthreads();
+    }
 }
 
 __global__ void __launch_bounds__(256) k_begin(BeginArgs a) {
@@ -1041,6 +1058,17 @@ int mx(const MxArgs &a, bool fem, size_t lds, hipStream_t st) {
 }
 int begin(const BeginArgs &a, hipStream_t st) {
     hipLaunchKernelGGL(k_begin, dim3(a.nb), dim3(256), 0, st, a);
+    VDI_LAUNCHED();
+    return VDI_OK;
+}
+int norm(const NormArgs &a, hipStream_t st) {
+    hipLaunchKernelGGL(k_norm_finish, dim3(1), dim3(256), 0, st, a);
+    VDI_LAUNCHED();
+    return VDI_OK;
+}
+int norm_b(const NormArgs *A, int n, hipStream_t st) {
+    if (n <= 0) return VDI_OK;
+    hipLaunchKernelGGL(k_norm_finish_b, dim3(n), dim3(256), 0, st, A, n);
     VDI_LAUNCHED();
     return VDI_OK;
 }
@@ -1506,6 +1534,19 @@ BeginArgs MG::begin_args(double tol2) {
     return a;
 }
 
+NormArgs MG::norm_args(int begin, double tol2) {
+    NormArgs a{};
+    a.ctl = ctl();
+    a.begin = begin;
+    a.tol2 = tol2;
+    if (begin) a.nb = begin_args(tol2).nb;
+    else {
+        const TileShape &t = kTileShapes[mixed_active() ? tile_mx : tile[0][1][0]];
+        a.nb = (int)(cdiv(lm[0], t.tx) * cdiv(ln[0], t.ty));
+    }
+    return a;
+}
+
 VecArgs MG::vec_args(int op, const double *coef) {
     VecArgs a{};
     a.op = op;
@@ -1577,6 +1618,9 @@ int MG::launch_cycle(double tol2) {
         for (int l = 0; l < lc; l++) VDI_TRY(launch_down<double>(L, l, l > 0));
         VDI_TRY(launch_coarse<double>(L, tol2));
         for (int l = lc - 1; l >= 0; l--) VDI_TRY(launch_up<double>(L, l, l == 0, tol2));
+        // the level-0 norm is reduced by its own one-workgroup kernel (the
+        // coarse kernel reduces it itself when it owns level 0)
+        if (lc > 0) VDI_TRY(mgk::norm(norm_args(0, tol2), st));
         return VDI_OK;
     }
     const int par = mx_enqueued_ & 1;
@@ -1591,6 +1635,7 @@ int MG::launch_cycle(double tol2) {
     MxArgs a = mx_args(par, tol2);
     if (timer) a.ts = timer->slot(VDI_KCLASS_MG_UP0, mx_bytes());
     VDI_TRY(mgk::mx(a, fem, mx_lds(), st));
+    VDI_TRY(mgk::norm(norm_args(0, tol2), st));
     mx_enqueued_++;
     return VDI_OK;
 }
@@ -1624,7 +1669,8 @@ int MG::enforce_dirichlet() {
 int MG::begin(double tol) {
     VDI_TRY(enforce_dirichlet());
     mx_enqueued_ = 0;
-    return mgk::begin(begin_args(tol * tol), st);
+    VDI_TRY(mgk::begin(begin_args(tol * tol), st));
+    return mgk::norm(norm_args(1, tol * tol), st);
 }
 
 int MG::read_state(MGState *out) {

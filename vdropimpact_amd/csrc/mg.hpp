@@ -139,6 +139,13 @@ struct VecArgs {
     int nb;
 };
 
+// Second half of the norm reductions (k_norm_finish).
+struct NormArgs {
+    MGCtl ctl;
+    int nb, begin;
+    double tol2;
+};
+
 // Work item of a batched launch: record index, tile (or block) index and
 // the simulation it belongs to (for per-simulation busy time).
 struct WorkItem {
@@ -199,6 +206,8 @@ class MG {
     LvArgs<float> mx_down0_args(int parity);
     BeginArgs begin_args(double tol2);
     VecArgs vec_args(int op, const double *coef);
+    // begin = 1: after k_begin; 0: after a V-cycle's level-0 up kernel.
+    NormArgs norm_args(int begin, double tol2);
     // Byte counts of the launches above (for timing).
     double down_bytes(int prec, int l, int zero_z) const;
     double up_bytes(int prec, int l) const;
@@ -289,6 +298,8 @@ template <class T> int coarse(const CoarseArgs<T> &a, size_t lds, hipStream_t st
 int mx(const MxArgs &a, bool fem, size_t lds, hipStream_t st);
 int begin(const BeginArgs &a, hipStream_t st);
 int vec(const VecArgs &a, hipStream_t st);
+int norm(const NormArgs &a, hipStream_t st);
+int norm_b(const NormArgs *args, int n, hipStream_t st);
 // Batched launches: args[k] are records in device memory, work items pick
 // (record, tile); grid = persistent workgroup count (a multiple of 8).
 template <class T> int down_b(const LvArgs<T> *args, const WorkItem *w, int nw, bool fem, size_t lds, int grid,
