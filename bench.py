@@ -141,6 +141,8 @@ def run_plain(vdi, args, p):
     sim = vdi.Fluid2D(p, rel_tol=1e-10)
     if args.no_graphs:
         sim.set_graphs(False)
+    if args.serial_visco:
+        sim.set_concurrency(False)
     sim.init_fields()
     for _ in range(args.warmup):
         sim.step_forward(p.dt_reg)
@@ -297,6 +299,8 @@ def main():
     ap.add_argument("--no-operator", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="skip the mixed and mixed+learned forms")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--serial-visco", action="store_true",
+                    help="captured steps run the two viscous solves one after the other")
     ap.add_argument("--no-graphs", action="store_true",
                     help="run every step on the direct path (for profilers that cannot follow graph launches)")
     ap.add_argument("--autotune", type=int, default=3, help="autotuner repetitions (0: default tile shapes)")

@@ -112,6 +112,10 @@ int vdi_set_form(vdi_ctx *ctx, int form);
  * at its end and redone on the synchronising path if a solve fell short.
  * No reference counterpart (launch mechanics only; results are the same). */
 int vdi_set_graphs(vdi_ctx *ctx, int enable);
+/* Run the u- and v-velocity Helmholtz solves of a captured step as two
+ * parallel graph branches (default on; the solves are independent).  No
+ * reference counterpart (scheduling only; results are the same). */
+int vdi_set_concurrency(vdi_ctx *ctx, int enable);
 /* Steps run as captured graphs, and of those the steps redone on the direct
  * path because a solve needed more cycles than its budget.  No counterpart
  * in the reference. */

@@ -65,6 +65,7 @@ ABI = {
     "vdi_set_solver": (_i, [_vp, _i, _i, _d, _i]),
     "vdi_set_form": (_i, [_vp, _i]),
     "vdi_set_graphs": (_i, [_vp, _i]),
+    "vdi_set_concurrency": (_i, [_vp, _i]),
     "vdi_get_counters": (_i, [_vp, ctypes.POINTER(_l), ctypes.POINTER(_l)]),
     "vdi_init_fields": (_i, [_vp]),
     "vdi_step_forward": (_i, [_vp, _d]),
@@ -210,6 +211,10 @@ class Fluid2D:
 
     def set_graphs(self, enable: bool):
         _check(self.lib.vdi_set_graphs(self.h, int(enable)), "vdi_set_graphs")
+
+    def set_concurrency(self, enable: bool):
+        """Parallel graph branches for the two viscous solves (default on)."""
+        _check(self.lib.vdi_set_concurrency(self.h, int(enable)), "vdi_set_concurrency")
 
     def counters(self):
         """(steps run as captured graphs, of those redone on the direct path)."""

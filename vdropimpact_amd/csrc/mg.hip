@@ -150,7 +150,9 @@ __device__ __forceinline__ Tile make_tile(int bx, int by, int H, int TX, int TY)
 
 // Bilinear cell-centred prolongation of the coarse correction at fine (i,j).
 template <class T>
-__device__ __forceinline__ T cc_prolong_at(const CCOp &op, int Mc, int Nc, const T *e, int i, int j) {
+__device__ __forceinline__ T cc_prolong_at(const CCOp &op, int Mc, int Nc, const T *e, int i, int j,
+                                           size_t pitch = 0) {
+    if (!pitch) pitch = Mc;
     int I = i >> 1, J = j >> 1;
     int I2 = (i & 1) ? I + 1 : I - 1, J2 = (j & 1) ? J + 1 : J - 1;
     T fx = 1, fy = 1;
@@ -158,8 +160,8 @@ __device__ __forceinline__ T cc_prolong_at(const CCOp &op, int Mc, int Nc, const
     else if (I2 >= Mc) { fx = T(bc_ghost_factor(op.bcr)); I2 = I; }
     if (J2 < 0) { fy = T(bc_ghost_factor(op.bcb)); J2 = J; }
     else if (J2 >= Nc) { fy = T(bc_ghost_factor(op.bct)); J2 = J; }
-    T e00 = e[I + (size_t)Mc * J], e10 = fx * e[I2 + (size_t)Mc * J];
-    T e01 = fy * e[I + (size_t)Mc * J2], e11 = fx * fy * e[I2 + (size_t)Mc * J2];
+    T e00 = e[I + pitch * J], e10 = fx * e[I2 + pitch * J];
+    T e01 = fy * e[I + pitch * J2], e11 = fx * fy * e[I2 + pitch * J2];
     return T(0.5625) * e00 + T(0.1875) * (e10 + e01) + T(0.0625) * e11;
 }
 
@@ -173,47 +175,106 @@ __device__ __forceinline__ T fem_prolong_at(const T *ez, int Mc, int i, int j) {
     return T(0.25) * (q[0] + q[1] + q[Mc] + q[Mc + 1]);
 }
 
+// Nodal bilinear prolongation at fine node (i,j) with every load made
+// unconditionally (the neighbours a parity does not use are clamped into the
+// level and ignored), so that a round of tile loads issues all its loads
+// before the first one is waited for.  Same value as fem_prolong_at.
+template <class T>
+__device__ __forceinline__ T fem_prolong_all(const T *ez, int Mc, int Nc, int i, int j, size_t pitch = 0) {
+    if (!pitch) pitch = Mc;
+    const int I = i >> 1, J = j >> 1, I1 = min(I + 1, Mc - 1), J1 = min(J + 1, Nc - 1);
+    const T q00 = ez[I + pitch * J], q10 = ez[I1 + pitch * J];
+    const T q01 = ez[I + pitch * J1], q11 = ez[I1 + pitch * J1];
+    // Power-of-two weights by parity: (1,0,0,0), (1/2,1/2,0,0), (1/2,0,1/2,0)
+    // or (1/4,1/4,1/4,1/4).  Scaling by a power of two is exact, so the
+    // weighted sum is bit for bit the parity formula (up to the sign of a
+    // zero); written this way no load sits behind a branch.
+    const T wx = (i & 1) ? T(0.5) : T(1), wy = (j & 1) ? T(0.5) : T(1);
+    const T w10 = (i & 1) ? wx * wy : T(0), w01 = (j & 1) ? wx * wy : T(0), w11 = ((i & 1) && (j & 1)) ? T(0.25) : T(0);
+    return wx * wy * q00 + w10 * q10 + w01 * q01 + w11 * q11;
+}
+
+// q = a / b, r = a % b for 0 <= a < 2^22, 1 <= b < 2^16, from the FP32
+// reciprocal: a * rcp(b) is within one of a / b, and one correction each way
+// makes it exact (an integer division is a long dependent chain).
+__device__ __forceinline__ void small_divmod(int a, int b, int &q, int &r) {
+    q = (int)((float)a * __builtin_amdgcn_rcpf((float)b));
+    r = a - q * b;
+    if (r < 0) {
+        r += b;
+        q--;
+    }
+    if (r >= b) {
+        r -= b;
+        q++;
+    }
+}
+
+// (P, r) = (q / W, q % W) for q = q0, q0 + step, ..., advanced without a
+// division per point.  1 <= W < 2^16, q < 2^22.
+struct Walk {
+    int q, P, r, W, dP, dr;
+    __device__ __forceinline__ Walk(int q0, int w, int step) {
+        W = w;
+        q = q0;
+        small_divmod(q0, w, P, r);
+        small_divmod(step, w, dP, dr);
+    }
+    __device__ __forceinline__ void next(int step) {
+        q += step;
+        P += dP;
+        r += dr;
+        if (r >= W) {
+            r -= W;
+            P++;
+        }
+    }
+};
+
 // Loads zsrc (+ prolongated coarse correction when C is given; zero when
 // zsrc is null) and b over the tile region; outside the domain both are zero.
 // The region is walked as one flat index range, kLoadUnroll elements per
-// thread and round, with all global loads of a round issued before any LDS
-// store so that several HBM round trips overlap.
+// thread and round.  Every load is unconditional -- points outside the domain
+// read the nearest domain point and the value is dropped -- so the compiler
+// issues all loads of a round before waiting for any of them, and a round
+// costs one memory round trip instead of one per element.
 constexpr int kLoadUnroll = 4;
 
 template <class T, bool FEM>
 __device__ __forceinline__ void tile_load(const LevelT<T> &F, const T *zsrc, const LevelT<T> *C, const CCOp &op,
                                           T *sz, T *sb, const Tile &t) {
-    const int tid = threadIdx.x + kBX * threadIdx.y;
     const int nel = t.RX * t.RY;
-    for (int k0 = tid; k0 < nel; k0 += 256 * kLoadUnroll) {
+    Walk w(threadIdx.x + kBX * threadIdx.y, t.RX, 256);
+    while (w.q < nel) {
         T zv[kLoadUnroll], bv[kLoadUnroll];
+        int li[kLoadUnroll];
 #pragma unroll
         for (int u = 0; u < kLoadUnroll; u++) {
-            const int k = k0 + 256 * u;
-            zv[u] = 0;
-            bv[u] = 0;
-            if (k < nel) {
-                const int ly = k / t.RX, lx = k - ly * t.RX;
-                const int gx = t.gx0 + lx, gy = t.gy0 + ly;
-                if (gy >= 0 && gy < F.n && gx >= 0 && gx < F.m) {
-                    const size_t g = gx + (size_t)F.m * gy;
-                    if (zsrc) zv[u] = zsrc[g];
-                    bv[u] = F.b[g];
-                    if (C) {
-                        if (FEM) { if (gy > 0) zv[u] += fem_prolong_at<T>(C->z, C->m, gx, gy); }
-                        else zv[u] += cc_prolong_at<T>(op, C->m, C->n, C->z, gx, gy);
-                    }
+            const int gx = t.gx0 + w.r, gy = t.gy0 + w.P;
+            const bool in = gy >= 0 && gy < F.n && gx >= 0 && gx < F.m;
+            const int cx = min(max(gx, 0), F.m - 1), cy = min(max(gy, 0), F.n - 1);
+            const size_t g = cx + (size_t)F.m * cy;
+            T z = zsrc ? zsrc[g] : T(0);
+            const T b = F.b[g];
+            if (C) {
+                if (FEM) {
+                    const T e = fem_prolong_all<T>(C->z, C->m, C->n, cx, cy);
+                    if (cy > 0) z += e;
+                } else {
+                    z += cc_prolong_at<T>(op, C->m, C->n, C->z, cx, cy);
                 }
             }
+            zv[u] = in ? z : T(0);
+            bv[u] = in ? b : T(0);
+            li[u] = w.q < nel ? w.q : -1;
+            w.next(256);
         }
 #pragma unroll
-        for (int u = 0; u < kLoadUnroll; u++) {
-            const int k = k0 + 256 * u;
-            if (k < nel) {
-                sz[k] = zv[u];
-                sb[k] = bv[u];
+        for (int u = 0; u < kLoadUnroll; u++)
+            if (li[u] >= 0) {
+                sz[li[u]] = zv[u];
+                sb[li[u]] = bv[u];
             }
-        }
     }
 }
 
@@ -231,17 +292,17 @@ __device__ __forceinline__ void tile_store(const LevelT<T> &F, T *dst, const T *
 }
 
 // Red-black half sweep of colour c on the region shrunk by s cells.
-template <class T>
+template <class T, int BX = kBX, int BY = kBY>
 __device__ __forceinline__ void cc_tile_sweep(const LevelT<T> &L, const CCOp &op, T *sz, const T *sb, const Tile &t,
                                               int s, int c) {
     const int X0 = max(0, t.gx0 + s), X1 = min(L.m, t.x0 + t.TX + t.H - s);
     const int Y0 = max(0, t.gy0 + s), Y1 = min(L.n, t.y0 + t.TY + t.H - s);
     const T sgn = T(op.s);
-    for (int gy = Y0 + threadIdx.y; gy < Y1; gy += kBY) {
+    for (int gy = Y0 + threadIdx.y; gy < Y1; gy += BY) {
         const int xs = X0 + (((X0 + gy) & 1) ^ c);
         const int row = (gy - t.gy0) * t.RX - t.gx0;
         const bool yint = gy > 0 && gy < L.n - 1;
-        for (int gx = xs + 2 * threadIdx.x; gx < X1; gx += 2 * kBX) {
+        for (int gx = xs + 2 * threadIdx.x; gx < X1; gx += 2 * BX) {
             const int l = row + gx;
             const T rhs = sb[l] - sgn * cc_off_t<T>(L.cx, L.cy, sz, l, t.RX);
             const bool in = yint && gx > 0 && gx < L.m - 1;
@@ -251,15 +312,15 @@ __device__ __forceinline__ void cc_tile_sweep(const LevelT<T> &L, const CCOp &op
 }
 
 // Four-colour sweep of colour (ci,cj) on the region shrunk by s; row 0 fixed.
-template <class T>
+template <class T, int BX = kBX, int BY = kBY>
 __device__ __forceinline__ void fem_tile_sweep(const LevelT<T> &L, const FemOp &op, T *sz, const T *sb, const Tile &t,
                                                int s, int ci, int cj) {
     const int X0 = max(0, t.gx0 + s), X1 = min(L.m, t.x0 + t.TX + t.H - s);
     const int Y0 = max(1, t.gy0 + s), Y1 = min(L.n, t.y0 + t.TY + t.H - s);
     const int ys = Y0 + ((Y0 ^ cj) & 1), xs = X0 + ((X0 ^ ci) & 1);
-    for (int gy = ys + 2 * threadIdx.y; gy < Y1; gy += 2 * kBY) {
+    for (int gy = ys + 2 * threadIdx.y; gy < Y1; gy += 2 * BY) {
         const int row = (gy - t.gy0) * t.RX - t.gx0;
-        for (int gx = xs + 2 * threadIdx.x; gx < X1; gx += 2 * kBX) {
+        for (int gx = xs + 2 * threadIdx.x; gx < X1; gx += 2 * BX) {
             const int l = row + gx;
             sz[l] = (sb[l] - fem_off_t<T>(op, L.m, L.n, sz, l, t.RX, gx, gy)) * fem_inv_acc<T>(L, gx, gy);
         }
@@ -278,6 +339,50 @@ __device__ __forceinline__ T fem_resid_t(const FemOp &op, int M, int N, const T 
     const T acc = fem_acc<T>(op, M, N, gx, gy);
     if (gy == 0) return sb[l] - acc * sz[l];
     return sb[l] - (acc * sz[l] + fem_off_t<T>(op, M, N, sz, l, RX, gx, gy));
+}
+
+// fem_resid_t off row 0, without the branch (the restriction never weights
+// row 0: coarse row 0 is Dirichlet and rows 2J-1.. of J >= 1 start at 1).
+template <class T>
+__device__ __forceinline__ T fem_resid_nd(const FemOp &op, int M, int N, const T *sz, const T *sb, int RX, int l,
+                                          int gx, int gy) {
+    const T acc = fem_acc<T>(op, M, N, gx, gy);
+    return sb[l] - (acc * sz[l] + fem_off_t<T>(op, M, N, sz, l, RX, gx, gy));
+}
+
+// Restricted residual at coarse point (I, J) from a tile holding the fine
+// level: average of 4 (cell-centred) or full weighting (nodal; coarse row 0
+// is Dirichlet and gets zero).  A fine node outside the domain gets weight
+// zero instead of a branch (it lies in the tile's halo, so its residual is
+// readable), which adds an exact zero.
+template <class T, bool FEM>
+__device__ __forceinline__ T restrict_at(const LevelT<T> &F, const CCOp &cop, const FemOp &fop, const T *sz,
+                                         const T *sb, const Tile &t, int I, int J) {
+    T v = 0;
+    if (!FEM) {
+        for (int b2 = 0; b2 < 2; b2++)
+            for (int a2 = 0; a2 < 2; a2++) {
+                const int gx = 2 * I + a2, gy = 2 * J + b2;
+                const int l = (gy - t.gy0) * t.RX + gx - t.gx0;
+                v += cc_resid_t<T>(F, cop, sz, sb, t.RX, l, gx, gy);
+            }
+        v *= T(0.25);
+    } else {
+        for (int b2 = -1; b2 <= 1; b2++) {
+            const int gy = 2 * J + b2;
+            T sx = 0;
+#pragma unroll
+            for (int a2 = -1; a2 <= 1; a2++) {
+                const int gx = 2 * I + a2;
+                const int l = (gy - t.gy0) * t.RX + gx - t.gx0;
+                const T wa = (gx < 0 || gx >= F.m) ? T(0) : (a2 ? T(0.5) : T(1));
+                sx += wa * fem_resid_nd<T>(fop, F.m, F.n, sz, sb, t.RX, l, gx, gy);
+            }
+            v += (gy >= F.n ? T(0) : (b2 ? T(0.5) : T(1))) * sx;
+        }
+        if (J == 0) v = 0;
+    }
+    return v;
 }
 
 // Workgroup sum (fixed order); result valid in thread 0.
@@ -345,32 +450,8 @@ __device__ __forceinline__ void down_body(const LvArgs<T> &a, int tile, T *lds) 
     const int I0 = t.x0 >> 1, J0 = t.y0 >> 1;
     const int I1 = min(C.m, I0 + t.TX / 2), J1 = min(C.n, J0 + t.TY / 2);
     for (int J = J0 + threadIdx.y; J < J1; J += kBY)
-        for (int I = I0 + threadIdx.x; I < I1; I += kBX) {
-            T v = 0;
-            if (!FEM) {
-                for (int b2 = 0; b2 < 2; b2++)
-                    for (int a2 = 0; a2 < 2; a2++) {
-                        const int gx = 2 * I + a2, gy = 2 * J + b2;
-                        const int l = (gy - t.gy0) * t.RX + gx - t.gx0;
-                        v += cc_resid_t<T>(F, a.cop, sz, sb, t.RX, l, gx, gy);
-                    }
-                v *= T(0.25);
-            } else if (J > 0) {
-                for (int b2 = -1; b2 <= 1; b2++) {
-                    const int gy = 2 * J + b2;
-                    if (gy >= F.n) continue;
-                    T sx = 0;
-                    for (int a2 = -1; a2 <= 1; a2++) {
-                        const int gx = 2 * I + a2;
-                        if (gx < 0 || gx >= F.m) continue;
-                        const int l = (gy - t.gy0) * t.RX + gx - t.gx0;
-                        sx += (a2 ? T(0.5) : T(1)) * fem_resid_t<T>(a.fop, F.m, F.n, sz, sb, t.RX, l, gx, gy);
-                    }
-                    v += (b2 ? T(0.5) : T(1)) * sx;
-                }
-            }
-            C.b[I + (size_t)C.m * J] = v;
-        }
+        for (int I = I0 + threadIdx.x; I < I1; I += kBX)
+            C.b[I + (size_t)C.m * J] = restrict_at<T, FEM>(F, a.cop, a.fop, sz, sb, t, I, J);
 }
 
 template <class T, bool FEM>
@@ -594,149 +675,193 @@ __global__ void __launch_bounds__(256) k_mx_up0_b(const MxArgs *A, const WorkIte
 }
 
 // ------------------------------------------------------------------------
-// Coarse levels: one workgroup, all levels in LDS
-// ------------------------------------------------------------------------
+// Coarse levels: level k is kept in LDS with a one-point zero border, as an
+// (m + 2) x (n + 2) tile for z and one for b, so the sweeps, residuals and
+// restrictions are the tiled kernels' own code on a tile covering the whole
+// level (a level gives the same bits whichever kernel smooths it), and no
+// LDS read sits behind a boundary test.  Threads are laid out 32 x 16.
+__device__ __forceinline__ Tile coarse_tile(int m, int n) {
+    Tile t;
+    t.H = 1;
+    t.TX = m;
+    t.TY = n;
+    t.x0 = t.y0 = 0;
+    t.gx0 = t.gy0 = -1;
+    t.RX = m + 2;
+    t.RY = n + 2;
+    return t;
+}
 
-// count sweeps of level l.  Points of one colour are enumerated directly (no
-// skipped threads); interior points multiply by the stored reciprocal
-// diagonal exactly as the tiled kernels do, so a level gives the same bits
-// whichever kernel smooths it.
 template <class T>
-__device__ void coarse_sweeps(bool fem, const CoarseDesc &d, const CCOp &cop, const FemOp &fop, int l, T *lds,
-                              int count) {
-    const int m = d.m[l], n = d.n[l], N = m * n;
-    T *z = lds + d.off[l], *b = z + N;
-    const T cx = T(d.cx[l]), cy = T(d.cy[l]), inv = T(d.inv[l]);
-    for (int k = 0; k < count; k++) {
+__device__ __forceinline__ LevelT<T> coarse_level(const CoarseDesc &d, int k) {
+    LevelT<T> L{};
+    L.m = d.m[k];
+    L.n = d.n[k];
+    L.cx = T(d.cx[k]);
+    L.cy = T(d.cy[k]);
+    L.inv_int = T(d.inv[k]);
+    return L;
+}
+
+// count sweeps of coarse level k
+template <class T>
+__device__ __forceinline__ void coarse_sweeps(bool fem, const CoarseDesc &d, const CCOp &cop, const FemOp &fop, int k,
+                                              T *lds, int count) {
+    const LevelT<T> L = coarse_level<T>(d, k);
+    const Tile t = coarse_tile(L.m, L.n);
+    T *z = lds + d.off[k], *b = z + t.RX * t.RY;
+    for (int c = 0; c < count; c++) {
         if (fem) {
-            for (int c = 0; c < 4; c++) {
-                const int ci = c & 1, cj = c >> 1;
-                const int ni = (m - ci + 1) >> 1, nj = (n - cj + 1) >> 1;
-                for (int q = threadIdx.x; q < ni * nj; q += blockDim.x) {
-                    const int jj = q / ni, i = 2 * (q - jj * ni) + ci, j = 2 * jj + cj;
-                    if (j == 0) continue;
-                    const int ij = i + m * j;
-                    const T f = T(((i == 0 || i == m - 1) ? 2.0 : 1.0) * ((j == n - 1) ? 2.0 : 1.0)) * inv;
-                    z[ij] = (b[ij] - fem_off<T>(fop, m, n, z, i, j)) * f;
-                }
-                __syncthreads();
-            }
-        } else if (!(m & 1)) {
-            const int mh = m >> 1;
-            for (int c = 0; c < 2; c++) {
-                for (int q = threadIdx.x; q < mh * n; q += blockDim.x) {
-                    const int j = q / mh, i = 2 * (q - j * mh) + ((j + c) & 1), ij = i + m * j;
-                    const T rhs = b[ij] - T(cop.s) * cc_off<T>(m, n, cx, cy, z, i, j);
-                    const bool in = i > 0 && i < m - 1 && j > 0 && j < n - 1;
-                    z[ij] = in ? rhs * inv : rhs / cc_diag<T>(cop, m, n, cx, cy, i, j);
-                }
+            for (int q = 0; q < 4; q++) {
+                fem_tile_sweep<T, kCoarseBX, kCoarseBY>(L, fop, z, b, t, 1, q & 1, q >> 1);
                 __syncthreads();
             }
         } else {
-            for (int c = 0; c < 2; c++) {
-                for (int ij = threadIdx.x; ij < N; ij += blockDim.x) {
-                    const int j = ij / m, i = ij - j * m;
-                    if (((i + j) & 1) != c) continue;
-                    const T rhs = b[ij] - T(cop.s) * cc_off<T>(m, n, cx, cy, z, i, j);
-                    const bool in = i > 0 && i < m - 1 && j > 0 && j < n - 1;
-                    z[ij] = in ? rhs * inv : rhs / cc_diag<T>(cop, m, n, cx, cy, i, j);
-                }
+            for (int q = 0; q < 2; q++) {
+                cc_tile_sweep<T, kCoarseBX, kCoarseBY>(L, cop, z, b, t, 1, q);
                 __syncthreads();
             }
         }
     }
 }
 
-template <class T>
-__device__ void coarse_body(const CoarseArgs<T> &a, T *lds, double *sh, int *last) {
+template <class T, bool FEM>
+__device__ __forceinline__ void coarse_body_t(const CoarseArgs<T> &a, T *lds, double *sh) {
     const CoarseDesc &d = a.d;
-    const bool fem = a.fem;
-    const int Tn = blockDim.x;
+    const int tid = threadIdx.x + kCoarseBX * threadIdx.y, Tn = kCoarseThreads;
     const T *gb = reinterpret_cast<const T *>(d.gb);
     T *gz = reinterpret_cast<T *>(d.gz);
+    const int ld = d.nl - 1, Nd = d.m[ld] * d.n[ld];
+    // zero every level (the borders stay zero), then load the first level
+    for (int k = tid; k < d.red_off; k += Tn) lds[k] = T(0);
+    __syncthreads();
     {
-        const int N = d.m[0] * d.n[0];
-        T *z = lds + d.off[0], *b = z + N;
-        for (int k = threadIdx.x; k < N; k += Tn) {
-            b[k] = gb[k];
-            z[k] = d.top ? gz[k] : T(0);
+        const int m = d.m[0], N = m * d.n[0], RX = m + 2;
+        T *z = lds + d.off[0], *b = z + RX * (d.n[0] + 2);
+        Walk w(tid, m, Tn);
+        while (w.q < N) {
+            T zv[kLoadUnroll], bv[kLoadUnroll];
+            int li[kLoadUnroll];
+#pragma unroll
+            for (int u = 0; u < kLoadUnroll; u++) {
+                const int k = min(w.q, N - 1);
+                li[u] = w.q < N ? (w.P + 1) * RX + w.r + 1 : -1;
+                bv[u] = gb[k];
+                zv[u] = d.top ? gz[k] : T(0);
+                w.next(Tn);
+            }
+#pragma unroll
+            for (int u = 0; u < kLoadUnroll; u++)
+                if (li[u] >= 0) {
+                    b[li[u]] = bv[u];
+                    z[li[u]] = zv[u];
+                }
         }
     }
     __syncthreads();
     for (int l = 0; l + 1 < d.nl; l++) {
-        coarse_sweeps<T>(fem, d, a.cop, a.fop, l, lds, a.nu1);
-        const int m = d.m[l], n = d.n[l], Mc = d.m[l + 1], Nc = d.n[l + 1];
-        const T *z = lds + d.off[l], *b = z + m * n;
-        T *zc = lds + d.off[l + 1], *bc = zc + Mc * Nc;
-        const T cx = T(d.cx[l]), cy = T(d.cy[l]);
-        for (int k = threadIdx.x; k < Mc * Nc; k += Tn) {
-            const int I = k % Mc, J = k / Mc;
-            T v = 0;
-            if (!fem) {
-                for (int b2 = 0; b2 < 2; b2++)
-                    for (int a2 = 0; a2 < 2; a2++) {
-                        const int i = 2 * I + a2, j = 2 * J + b2;
-                        v += b[i + m * j] - Az_plain<T>(false, a.cop, a.fop, m, n, cx, cy, z, i, j);
-                    }
-                v *= T(0.25);
-            } else if (J > 0) {
-                for (int b2 = -1; b2 <= 1; b2++) {
-                    const int j = 2 * J + b2;
-                    if (j >= n) continue;
-                    T sx = 0;
-                    for (int a2 = -1; a2 <= 1; a2++) {
-                        const int i = 2 * I + a2;
-                        if (i < 0 || i >= m) continue;
-                        sx += (a2 ? T(0.5) : T(1)) *
-                              (b[i + m * j] - Az_plain<T>(true, a.cop, a.fop, m, n, T(0), T(0), z, i, j));
-                    }
-                    v += (b2 ? T(0.5) : T(1)) * sx;
-                }
-            }
-            bc[k] = v;
-            zc[k] = 0;
-        }
+        coarse_sweeps<T>(FEM, d, a.cop, a.fop, l, lds, a.nu1);
+        const LevelT<T> F = coarse_level<T>(d, l);
+        const Tile t = coarse_tile(F.m, F.n);
+        const int Mc = d.m[l + 1], Nc = d.n[l + 1], RC = Mc + 2;
+        const T *z = lds + d.off[l], *b = z + t.RX * t.RY;
+        T *bc = lds + d.off[l + 1] + RC * (Nc + 2);
+        for (int J = threadIdx.y; J < Nc; J += kCoarseBY)
+            for (int I = threadIdx.x; I < Mc; I += kCoarseBX)
+                bc[(J + 1) * RC + I + 1] = restrict_at<T, FEM>(F, a.cop, a.fop, z, b, t, I, J);
         __syncthreads();
     }
-    if (d.inv_dense) {
-        // exact solve on the coarsest level: z = A^-1 b with the inverse
-        // stored column by column (coalesced over the rows)
-        const int l = d.nl - 1, N = d.m[l] * d.n[l];
-        T *z = lds + d.off[l];
-        const T *b = z + N;
-        const T *inv = reinterpret_cast<const T *>(d.inv_dense);
-        for (int i = threadIdx.x; i < N; i += Tn) {
+    {
+        const int Md = d.m[ld], RD = Md + 2;
+        T *z = lds + d.off[ld];
+        const T *b = z + RD * (d.n[ld] + 2);
+        if (d.inv_dense && Nd <= kDenseReg) {
+            // exact solve: thread t takes column block p = t / Nd of row
+            // i = t % Nd and loads its entries of the inverse kDenseRegPer at
+            // a time, all in flight together (unused slots load a valid entry
+            // and are zeroed); the right-hand side is gathered into a plain
+            // vector meanwhile; then the blocks' partial products are summed
+            // in order by row
+            const int dP = Tn / Nd, K = (Nd + dP - 1) / dP, di = tid % Nd, dp = tid / Nd;
+            const int dk0 = min(Nd - 1, dp * K), dk1 = dp < dP ? min(Nd, dp * K + K) : dk0;
+            const T *inv = reinterpret_cast<const T *>(d.inv_dense);
+            T *red = lds + d.red_off, *bp = red + kCoarseThreads;
             T v = 0;
-            for (int k = 0; k < N; k++) v += inv[(size_t)k * N + i] * b[k];
-            z[i] = v;
+            for (int k0 = dk0; k0 < dk0 + K; k0 += kDenseRegPer) {
+                T pre[kDenseRegPer];
+#pragma unroll
+                for (int u = 0; u < kDenseRegPer; u++) {
+                    const T e = inv[(size_t)min(k0 + u, Nd - 1) * Nd + di];
+                    pre[u] = k0 + u < dk1 ? e : T(0);
+                }
+                if (k0 == dk0) {
+                    if (tid < Nd) bp[tid] = b[(tid / Md + 1) * RD + tid % Md + 1];
+                    __syncthreads();
+                }
+#pragma unroll
+                for (int u = 0; u < kDenseRegPer; u++) v += pre[u] * bp[min(k0 + u, Nd - 1)];
+            }
+            if (dp < dP) red[dp * Nd + di] = v;
+            __syncthreads();
+            if (tid < Nd) {
+                T v = 0;
+                for (int p = 0; p < dP; p++) v += red[p * Nd + tid];
+                z[(tid / Md + 1) * RD + tid % Md + 1] = v;
+            }
+            __syncthreads();
+        } else if (d.inv_dense) {
+            // exact solve with the inverse read from memory (stored column by
+            // column, coalesced over the rows)
+            const T *inv = reinterpret_cast<const T *>(d.inv_dense);
+            for (int i = tid; i < Nd; i += Tn) {
+                T v = 0;
+                for (int k = 0; k < Nd; k++) v += inv[(size_t)k * Nd + i] * b[(k / Md + 1) * RD + k % Md + 1];
+                z[(i / Md + 1) * RD + i % Md + 1] = v;
+            }
+            __syncthreads();
+        } else {
+            coarse_sweeps<T>(FEM, d, a.cop, a.fop, ld, lds, a.ncs);
         }
-        __syncthreads();
-    } else {
-        coarse_sweeps<T>(fem, d, a.cop, a.fop, d.nl - 1, lds, a.ncs);
     }
     for (int l = d.nl - 2; l >= 0; l--) {
-        const int m = d.m[l], n = d.n[l], Mc = d.m[l + 1], Nc = d.n[l + 1];
+        const int m = d.m[l], n = d.n[l], RX = m + 2, Mc = d.m[l + 1], Nc = d.n[l + 1];
         T *z = lds + d.off[l];
-        const T *ec = lds + d.off[l + 1];
-        for (int k = threadIdx.x; k < m * n; k += Tn) {
-            const int i = k % m, j = k / m;
-            if (fem) { if (j > 0) z[k] += fem_prolong_at<T>(ec, Mc, i, j); }
-            else z[k] += cc_prolong_at<T>(a.cop, Mc, Nc, ec, i, j);
-        }
+        // coarse correction from element (0, 0) of the padded coarse level
+        const T *ec = lds + d.off[l + 1] + (Mc + 2) + 1;
+        for (int j = threadIdx.y; j < n; j += kCoarseBY)
+            for (int i = threadIdx.x; i < m; i += kCoarseBX) {
+                const int k = (j + 1) * RX + i + 1;
+                if (FEM) {
+                    const T e = fem_prolong_all<T>(ec, Mc, Nc, i, j, Mc + 2);
+                    if (j > 0) z[k] += e;
+                } else {
+                    z[k] += cc_prolong_at<T>(a.cop, Mc, Nc, ec, i, j, Mc + 2);
+                }
+            }
         __syncthreads();
-        coarse_sweeps<T>(fem, d, a.cop, a.fop, l, lds, a.nu2);
+        coarse_sweeps<T>(FEM, d, a.cop, a.fop, l, lds, a.nu2);
     }
-    const int N = d.m[0] * d.n[0];
-    const T *z = lds + d.off[0], *b = z + N;
-    for (int k = threadIdx.x; k < N; k += Tn) gz[k] = z[k];
-    if (!d.top) return;
+    const LevelT<T> F = coarse_level<T>(d, 0);
+    const Tile t = coarse_tile(F.m, F.n);
+    const T *z = lds + d.off[0], *b = z + t.RX * t.RY;
     double rr = 0;
-    for (int k = threadIdx.x; k < N; k += Tn) {
-        const int i = k % d.m[0], j = k / d.m[0];
-        const double r = b[k] - Az_plain<T>(fem, a.cop, a.fop, d.m[0], d.n[0], T(d.cx[0]), T(d.cy[0]), z, i, j);
-        rr += r * r;
-    }
-    finish_norm_single(a.ctl, rr, a.tol2, sh);
+    for (int j = threadIdx.y; j < F.n; j += kCoarseBY)
+        for (int i = threadIdx.x; i < F.m; i += kCoarseBX) {
+            const int l0 = (j + 1) * t.RX + i + 1;
+            gz[i + (size_t)F.m * j] = z[l0];
+            if (d.top) {
+                const double r = FEM ? fem_resid_t<T>(a.fop, F.m, F.n, z, b, t.RX, l0, i, j)
+                                     : cc_resid_t<T>(F, a.cop, z, b, t.RX, l0, i, j);
+                rr += r * r;
+            }
+        }
+    if (d.top) finish_norm_single(a.ctl, rr, a.tol2, sh);
+}
+
+template <class T>
+__device__ __forceinline__ void coarse_body(const CoarseArgs<T> &a, T *lds, double *sh, int *last) {
+    if (a.fem) coarse_body_t<T, true>(a, lds, sh);
+    else coarse_body_t<T, false>(a, lds, sh);
     (void)last;
 }
 
@@ -761,7 +886,7 @@ __global__ void __launch_bounds__(kCoarseThreads) k_coarse_b(const CoarseArgs<T>
     stamp_begin(aux.ts);
     for (int p = blockIdx.x; p < nw; p += gridDim.x) {
         const WorkItem w = W[p];
-        const CoarseArgs<T> a = A[w.rec];
+        const CoarseArgs<T> &a = A[w.rec];  // read in place: a copy would go to scratch
         if (a.ctl.state->done) continue;
         const unsigned long long t0 = busy_begin(aux.busy);
         coarse_body<T>(a, reinterpret_cast<T *>(lds_d), sh, &last);
@@ -1046,7 +1171,7 @@ template <class T> int up(const LvArgs<T> &a, bool fem, size_t lds, hipStream_t 
     return VDI_OK;
 }
 template <class T> int coarse(const CoarseArgs<T> &a, size_t lds, hipStream_t st) {
-    hipLaunchKernelGGL(k_coarse<T>, dim3(1), dim3(kCoarseThreads), lds, st, a);
+    hipLaunchKernelGGL(k_coarse<T>, dim3(1), dim3(kCoarseBX, kCoarseBY), lds, st, a);
     VDI_LAUNCHED();
     return VDI_OK;
 }
@@ -1094,7 +1219,7 @@ template <class T> int up_b(const LvArgs<T> *A, const WorkItem *w, int nw, bool 
 }
 template <class T> int coarse_b(const CoarseArgs<T> *A, const WorkItem *w, int nw, size_t lds, int grid,
                                 BatchAux aux, hipStream_t st) {
-    hipLaunchKernelGGL(k_coarse_b<T>, dim3(grid), dim3(kCoarseThreads), lds, st, A, w, nw, aux);
+    hipLaunchKernelGGL(k_coarse_b<T>, dim3(grid), dim3(kCoarseBX, kCoarseBY), lds, st, A, w, nw, aux);
     VDI_LAUNCHED();
     return VDI_OK;
 }
@@ -1301,8 +1426,8 @@ int MG::init(int kind_, int m, int n, double cx, double cy, int x_sym, double *z
     VDI_TRY(partial_.alloc(2 * maxblk * sizeof(double)));
     VDI_TRY(state_.alloc(sizeof(MGState)));
     if (!host_state_) VDI_HIP(hipHostMalloc((void **)&host_state_, sizeof(MGState), hipHostMallocDefault));
-    coarse_lds_ = 0;
-    for (int l = lc; l < nl; l++) coarse_lds_ += 2 * sizeof(double) * (size_t)lm[l] * ln[l];
+    coarse_lds_ = (kCoarseThreads + kDenseReg) * sizeof(double);
+    for (int l = lc; l < nl; l++) coarse_lds_ += 2 * sizeof(double) * (size_t)(lm[l] + 2) * (ln[l] + 2);
     return VDI_OK;
 }
 
@@ -1313,8 +1438,8 @@ int MG::set_coarse_points(int pts) {
     if (nl - l2 > kMaxCoarse) return VDI_ERR_ARG;
     coarse_pts = pts;
     lc = l2;
-    coarse_lds_ = 0;
-    for (int l = lc; l < nl; l++) coarse_lds_ += 2 * sizeof(double) * (size_t)lm[l] * ln[l];
+    coarse_lds_ = (kCoarseThreads + kDenseReg) * sizeof(double);
+    for (int l = lc; l < nl; l++) coarse_lds_ += 2 * sizeof(double) * (size_t)(lm[l] + 2) * (ln[l] + 2);
     return VDI_OK;
 }
 
@@ -1469,8 +1594,9 @@ CoarseArgs<T> MG::coarse_args(const std::vector<LevelT<T>> &H, double tol2) {
         d.cy[k] = (double)v.cy;
         d.inv[k] = (double)v.inv_int;
         d.off[k] = off;
-        off += 2 * v.m * v.n;
+        off += 2 * (v.m + 2) * (v.n + 2);
     }
+    d.red_off = off;
     d.gz = reinterpret_cast<double *>(H[lc].z);
     d.gb = reinterpret_cast<double *>(H[lc].b);
     d.top = lc == 0;

@@ -26,7 +26,8 @@ namespace vdi {
 constexpr int kMaxCoarse = 12;  // levels handled by the single-workgroup kernel
 constexpr int kMaxNu = 3;       // largest nu1/nu2 the tiled kernels accept
 constexpr int kBX = 32, kBY = 8;      // 256 threads per level-kernel workgroup
-constexpr int kCoarseThreads = 512;
+constexpr int kCoarseBX = 32, kCoarseBY = 16;  // coarse-kernel threads, 2-D
+constexpr int kCoarseThreads = kCoarseBX * kCoarseBY;
 constexpr int kCoarsePoints = 3072;   // largest level the single-workgroup kernel takes
 // Coarsening stops at the first level with at most kDensePoints unknowns,
 // which is solved exactly with a dense inverse when it has at most kDenseMax
@@ -34,6 +35,12 @@ constexpr int kCoarsePoints = 3072;   // largest level the single-workgroup kern
 // level gets ncs smoothing sweeps instead).
 constexpr int kDensePoints = 256;
 constexpr int kDenseMax = 1024;
+// The exact solve of a coarsest level of at most kDenseReg unknowns splits
+// the rows into column blocks, one block per thread, and loads a block's
+// entries kDenseRegPer at a time, all in flight together; a larger level
+// reads the inverse entry by entry.
+constexpr int kDenseReg = 256;
+constexpr int kDenseRegPer = 32;
 
 // Device-side solve state: convergence flag (0 running, 1 converged,
 // 2 NaN, 3 zero right-hand side), V-cycles run, ticket of the last-
@@ -95,6 +102,7 @@ struct MxArgs {
 struct CoarseDesc {
     int nl, top;
     int m[kMaxCoarse], n[kMaxCoarse], off[kMaxCoarse];
+    int red_off;  // LDS after the levels: kCoarseThreads partial products, then kDenseReg values
     double cx[kMaxCoarse], cy[kMaxCoarse], inv[kMaxCoarse];
     double *gz, *gb;
     const void *inv_dense;  // column-major inverse of the coarsest level, or null

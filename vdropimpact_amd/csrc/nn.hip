@@ -798,25 +798,41 @@ static int guess_train(LearnedGuess &L, int epochs, int batch, double lr, unsign
     VDI_TRY(net.zero_grads());
     std::vector<int> order(L.count);
     unsigned long long s = seed;
-    double acc = 0;
-    int nacc = 0;
+    // Adam on this small data set can leave a good basin late in training.
+    // The weights at the start of each epoch are saved; the mean of the
+    // per-sample losses seen during the epoch rates them, and the best rated
+    // weights are restored at the end (their rating is reported).
+    DevBuf snap[2];
+    VDI_TRY(snap[0].alloc((size_t)net.nparams * sizeof(float)));
+    VDI_TRY(snap[1].alloc((size_t)net.nparams * sizeof(float)));
+    const size_t wbytes = (size_t)net.nparams * sizeof(float);
+    int ib = 0;  // snap[ib]: best so far; snap[1 - ib]: start of this epoch
+    double best_loss = -1;
     for (int e = 0; e < epochs; e++) {
+        VDI_HIP(hipMemcpyAsync(snap[1 - ib].p, net.params(), wbytes, hipMemcpyDeviceToDevice, st));
         for (int k = 0; k < L.count; k++) order[k] = k;
         for (int k = L.count - 1; k > 0; k--) std::swap(order[k], order[splitmix(s) % (unsigned long long)(k + 1)]);
+        double acc = 0;
         for (int q = 0; q < L.count; q++) {
             const float *slot = L.data_.as<float>() + (size_t)order[q] * kSlot * N;
             hipLaunchKernelGGL(k_expand, dim3(g1(N)), dim3(256), 0, st, slot, net.x0(), N);
             VDI_LAUNCHED();
             VDI_TRY(net.forward());
             VDI_TRY(net.backward(slot + 3 * N));
+            acc += net.last_loss();
             if ((q + 1) % batch == 0 || q == L.count - 1) VDI_TRY(net.adam((float)lr, 0.9f, 0.999f, 1e-8f));
-            if (e == epochs - 1) {
-                acc += net.last_loss();
-                nacc++;
-            }
+        }
+        const double el = acc / L.count;
+        if (!(el >= 0)) return VDI_ERR_ARG;  // NaN loss: the data or the rate is unusable
+        if (best_loss < 0 || el < best_loss) {
+            best_loss = el;
+            ib = 1 - ib;
         }
     }
-    if (loss) *loss = nacc ? acc / nacc : 0.0;
+    VDI_HIP(hipMemcpyAsync(net.params(), snap[ib].p, wbytes, hipMemcpyDeviceToDevice, st));
+    VDI_TRY(net.sync_bf16());
+    VDI_HIP(hipStreamSynchronize(st));
+    if (loss) *loss = best_loss;
     return VDI_OK;
 }
 

@@ -395,7 +395,7 @@ __device__ void visco_rhs_body(const StepRec &a, int bx, int by, int comp) {
             r -= vfy * g.dy * dvdy;
         }
     }
-    F.src[i + (size_t)g.m * j] = r;
+    (comp ? F.src2 : F.src)[i + (size_t)g.m * j] = r;
 }
 
 __device__ void visco_copy_body(const StepRec &a, int bx, int by) {
@@ -991,6 +991,9 @@ int vdi_create(const vdi_params *p, vdi_ctx **out) {
     K.gxxsp = K.gxsp * K.gxsp;
     c->Vframe = p->V;
     VDI_HIP(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+    VDI_HIP(hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking));
+    VDI_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    VDI_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     size_t NP = (size_t)g.ml * (g.n + 4);
     VDI_TRY(c->fields.alloc(19 * NP * sizeof(double)));
     double *f = c->fields.as<double>();
@@ -998,8 +1001,9 @@ int vdi_create(const vdi_params *p, vdi_ctx **out) {
                         &c->F.ul, &c->F.vl, &c->F.ur, &c->F.vr, &c->F.uu, &c->F.vu, &c->F.c0, &c->F.c1,
                         &c->F.c2, &c->F.c3};
     for (int k = 0; k < 19; k++) *dst[k] = f + k * NP;
-    VDI_TRY(c->srcbuf.alloc((size_t)g.me * g.ne * sizeof(double)));
+    VDI_TRY(c->srcbuf.alloc(((size_t)g.me * g.ne + (size_t)g.m * g.n) * sizeof(double)));
     c->F.src = c->srcbuf.as<double>();
+    c->F.src2 = c->F.src + (size_t)g.me * g.ne;
     // gas arrays: p[me], tau[ml], p_old[me], tau_old[ml], h[m], hx[me], ht[m], fbd[m], pg[me], pg_old[me]
     size_t ng = 5 * (size_t)g.me + 2 * (size_t)g.ml + 3 * (size_t)g.m;
     VDI_TRY(c->gasbuf.alloc(ng * sizeof(double)));
@@ -1025,7 +1029,7 @@ int vdi_create(const vdi_params *p, vdi_ctx **out) {
     VDI_TRY(c->fem.init(VDI_FEM, g.me, g.ne, g.dx, g.dy, g.x_sym, c->zfem, c->F.src, c->st));
     if (p->implicit_visc) {
         VDI_TRY(c->vx.init(VDI_VX, g.m, g.n, 0, 0, g.x_sym, c->zvx, c->F.src, c->st));
-        VDI_TRY(c->vy.init(VDI_VY, g.m, g.n, 0, 0, g.x_sym, c->zvy, c->F.src, c->st));
+        VDI_TRY(c->vy.init(VDI_VY, g.m, g.n, 0, 0, g.x_sym, c->zvy, c->F.src2, c->st));
     }
     VDI_TRY(c->snap.alloc((3 * NP + ng) * sizeof(double)));
     VDI_TRY(c->coef_dev.alloc(8 * sizeof(double)));
@@ -1199,11 +1203,28 @@ static int enqueue_body(vdi_ctx *c, const StepRec &a, double t0, SolveMode mode)
         hipLaunchKernelGGL(k_visco_rhs, gc, dim3(256), 0, c->st, a, 0);
         VDI_LAUNCHED();
         visco_setup(c->vx, a.vfx, a.vfy);
-        VDI_TRY(body_solve(c, 1, t1, mode));
-        hipLaunchKernelGGL(k_visco_rhs, gc, dim3(256), 0, c->st, a, 1);
-        VDI_LAUNCHED();
         visco_setup(c->vy, a.vfx, a.vfy);
-        VDI_TRY(body_solve(c, 2, t1, mode));
+        if (mode == SOLVE_FIXED && c->par_visco) {
+            // The u and v solves are independent (separate right-hand sides
+            // and solutions): in a captured step the v branch (its right-hand
+            // side and solve) runs on st2 beside the u solve on st.
+            VDI_HIP(hipEventRecord(c->ev_fork, c->st));
+            VDI_HIP(hipStreamWaitEvent(c->st2, c->ev_fork, 0));
+            hipLaunchKernelGGL(k_visco_rhs, gc, dim3(256), 0, c->st2, a, 1);
+            VDI_LAUNCHED();
+            c->vy.st = c->st2;
+            const int rc = body_solve(c, 2, t1, mode);
+            c->vy.st = c->st;
+            VDI_TRY(rc);
+            VDI_TRY(body_solve(c, 1, t1, mode));
+            VDI_HIP(hipEventRecord(c->ev_join, c->st2));
+            VDI_HIP(hipStreamWaitEvent(c->st, c->ev_join, 0));
+        } else {
+            VDI_TRY(body_solve(c, 1, t1, mode));
+            hipLaunchKernelGGL(k_visco_rhs, gc, dim3(256), 0, c->st, a, 1);
+            VDI_LAUNCHED();
+            VDI_TRY(body_solve(c, 2, t1, mode));
+        }
         hipLaunchKernelGGL(k_visco_copy, gc, dim3(256), 0, c->st, a);
         VDI_LAUNCHED();
     }
@@ -1495,6 +1516,14 @@ int vdi_set_graphs(vdi_ctx *c, int enable) {
     VDI_HIP(hipStreamSynchronize(c->st));
     c->use_graphs = enable ? 1 : 0;
     if (!enable) c->drop_graphs();
+    return VDI_OK;
+}
+
+int vdi_set_concurrency(vdi_ctx *c, int enable) {
+    if (!c) return VDI_ERR_ARG;
+    VDI_HIP(hipStreamSynchronize(c->st));
+    c->par_visco = enable ? 1 : 0;
+    c->drop_graphs();
     return VDI_OK;
 }
 

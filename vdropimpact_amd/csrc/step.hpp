@@ -23,7 +23,8 @@ struct Fields {
     double *ux, *uy, *vx, *vy;
     double *ud, *vd, *ul, *vl, *ur, *vr, *uu, *vu;
     double *c0, *c1, *c2, *c3;
-    double *src;
+    double *src;   // right-hand side of the MAC, FEM and u-velocity solves
+    double *src2;  // right-hand side of the v-velocity solve (m x n)
 };
 
 // Gas-layer arrays (gas_layer.hh:26-34, gas_layer_solve.hh:163-173).
@@ -105,6 +106,11 @@ struct vdi_ctx {
     vdi::Geo g{};
     vdi::GasConst K{};
     hipStream_t st = nullptr;
+    // Captured steps run the two viscous solves as parallel graph branches:
+    // the v solve is recorded on st2 between a fork and a join event.
+    hipStream_t st2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    int par_visco = 1;
     double time = 0, Vframe = 0;
     double rel_tol = 1e-10;
     int max_cycles = 200;
@@ -153,6 +159,10 @@ struct vdi_ctx {
         if (coef_host) (void)hipHostFree(coef_host);
         if (rb) (void)hipHostFree(rb);
         if (guess) vdi::guess_destroy(guess);
+        if (st2) (void)hipStreamSynchronize(st2);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        if (st2) (void)hipStreamDestroy(st2);
         if (st) (void)hipStreamDestroy(st);
     }
 };
