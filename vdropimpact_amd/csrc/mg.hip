@@ -125,6 +125,30 @@ __device__ __forceinline__ T fem_off_t(const FemOp &op, int M, int N, const T *s
            wi * T(op.fey) * (s[l - RX] + s[l + RX]) + wj * T(op.fex) * (s[l - 1] + s[l + 1]);
 }
 
+// Boundary class of a cell-centred point: xc + 3 yc with xc, yc = 0 inside,
+// 1 on the first and 2 on the last column / row.
+template <class T>
+__device__ __forceinline__ int cc_class(const LevelT<T> &L, int i, int j) {
+    return (i == 0 ? 1 : (i == L.m - 1 ? 2 : 0)) + 3 * (j == 0 ? 1 : (j == L.n - 1 ? 2 : 0));
+}
+
+// Reciprocal of the cell-centred diagonal per boundary class, written to
+// tab[0..8] (LDS) by threads 0..8 (one division each, once per launch);
+// tab[0] is the level's stored interior reciprocal.  Sweeps then multiply
+// every point by tab[class] instead of dividing at boundary points: an FP64
+// division behind a branch taken by every wave with a boundary lane was an
+// eighth of a cell-centred V-cycle at 1024 x 512.
+template <class T>
+__device__ __forceinline__ void cc_inv_table(const LevelT<T> &L, const CCOp &op, T *tab, int tid) {
+    if (tid < 9) {
+        const int xc = tid % 3, yc = tid / 3;
+        const int i = xc == 1 ? 0 : (xc == 2 ? L.m - 1 : min(1, L.m - 1));
+        const int j = yc == 1 ? 0 : (yc == 2 ? L.n - 1 : min(1, L.n - 1));
+        tab[tid] = tid == 0 ? L.inv_int : T(1) / cc_diag<T>(op, L.m, L.n, L.cx, L.cy, i, j);
+    }
+}
+constexpr int kInvTab = 16;  // LDS entries reserved for the table
+
 template <class T>
 __device__ __forceinline__ T fem_inv_acc(const LevelT<T> &L, int i, int j) {
     return T(((i == 0 || i == L.m - 1) ? 2.0 : 1.0) * ((j == 0 || j == L.n - 1) ? 2.0 : 1.0)) * L.inv_int;
@@ -294,19 +318,17 @@ __device__ __forceinline__ void tile_store(const LevelT<T> &F, T *dst, const T *
 // Red-black half sweep of colour c on the region shrunk by s cells.
 template <class T, int BX = kBX, int BY = kBY>
 __device__ __forceinline__ void cc_tile_sweep(const LevelT<T> &L, const CCOp &op, T *sz, const T *sb, const Tile &t,
-                                              int s, int c) {
+                                              int s, int c, const T *itab) {
     const int X0 = max(0, t.gx0 + s), X1 = min(L.m, t.x0 + t.TX + t.H - s);
     const int Y0 = max(0, t.gy0 + s), Y1 = min(L.n, t.y0 + t.TY + t.H - s);
     const T sgn = T(op.s);
     for (int gy = Y0 + threadIdx.y; gy < Y1; gy += BY) {
         const int xs = X0 + (((X0 + gy) & 1) ^ c);
         const int row = (gy - t.gy0) * t.RX - t.gx0;
-        const bool yint = gy > 0 && gy < L.n - 1;
         for (int gx = xs + 2 * threadIdx.x; gx < X1; gx += 2 * BX) {
             const int l = row + gx;
             const T rhs = sb[l] - sgn * cc_off_t<T>(L.cx, L.cy, sz, l, t.RX);
-            const bool in = yint && gx > 0 && gx < L.m - 1;
-            sz[l] = in ? rhs * L.inv_int : rhs / cc_diag<T>(op, L.m, L.n, L.cx, L.cy, gx, gy);
+            sz[l] = rhs * itab[cc_class<T>(L, gx, gy)];
         }
     }
 }
@@ -431,8 +453,9 @@ template <class T, bool FEM>
 __device__ __forceinline__ void down_body(const LvArgs<T> &a, int tile, T *lds) {
     const LevelT<T> &F = a.F, &C = a.C;
     const Tile t = make_tile(tile % a.nbx, tile / a.nbx, FEM ? 4 * a.nu + 2 : 2 * a.nu + 1, a.tx, a.ty);
-    T *sz = lds, *sb = lds + t.RX * t.RY;
+    T *sz = lds, *sb = lds + t.RX * t.RY, *itab = sb + t.RX * t.RY;
     tile_load<T, FEM>(F, a.flag ? nullptr : F.z, nullptr, a.cop, sz, sb, t);
+    if (!FEM) cc_inv_table<T>(F, a.cop, itab, threadIdx.x + kBX * threadIdx.y);
     __syncthreads();
     if (FEM) {
         for (int s = 1; s <= 4 * a.nu; s++) {
@@ -442,7 +465,7 @@ __device__ __forceinline__ void down_body(const LvArgs<T> &a, int tile, T *lds) 
         }
     } else {
         for (int s = 1; s <= 2 * a.nu; s++) {
-            cc_tile_sweep<T>(F, a.cop, sz, sb, t, s, (s - 1) & 1);
+            cc_tile_sweep<T>(F, a.cop, sz, sb, t, s, (s - 1) & 1, itab);
             __syncthreads();
         }
     }
@@ -458,8 +481,9 @@ template <class T, bool FEM>
 __device__ __forceinline__ void up_body(const LvArgs<T> &a, int tile, T *lds, double *sh, int *last) {
     const LevelT<T> &F = a.F;
     const Tile t = make_tile(tile % a.nbx, tile / a.nbx, (FEM ? 4 : 2) * a.nu + a.flag, a.tx, a.ty);
-    T *sz = lds, *sb = lds + t.RX * t.RY;
+    T *sz = lds, *sb = lds + t.RX * t.RY, *itab = sb + t.RX * t.RY;
     tile_load<T, FEM>(F, F.zalt, &a.C, a.cop, sz, sb, t);
+    if (!FEM) cc_inv_table<T>(F, a.cop, itab, threadIdx.x + kBX * threadIdx.y);
     __syncthreads();
     if (FEM) {
         for (int s = 1; s <= 4 * a.nu; s++) {
@@ -469,7 +493,7 @@ __device__ __forceinline__ void up_body(const LvArgs<T> &a, int tile, T *lds, do
         }
     } else {
         for (int s = 1; s <= 2 * a.nu; s++) {
-            cc_tile_sweep<T>(F, a.cop, sz, sb, t, s, (s - 1) & 1);
+            cc_tile_sweep<T>(F, a.cop, sz, sb, t, s, (s - 1) & 1, itab);
             __syncthreads();
         }
     }
@@ -503,15 +527,16 @@ __device__ __forceinline__ void mx_body(const MxArgs &a, int tile, double *lds_d
     const Tile t = make_tile(tile % a.nbx, tile / a.nbx, (FEM ? 4 : 2) * a.nu + 1, a.tx, a.ty);
     const int R = t.RX * t.RY;
     double *zd = lds_d;
-    float *se = reinterpret_cast<float *>(lds_d + R), *sr = se + R;
+    float *se = reinterpret_cast<float *>(lds_d + R), *sr = se + R, *itab = sr + R;
     tile_load<float, FEM>(a.E, a.E.zalt, &a.C, a.cop, se, sr, t);
+    if (!FEM) cc_inv_table<float>(a.E, a.cop, itab, threadIdx.x + kBX * threadIdx.y);
     __syncthreads();
     for (int s = 1; s <= (FEM ? 4 : 2) * a.nu; s++) {
         if (FEM) {
             const int c = (s - 1) & 3;
             fem_tile_sweep<float>(a.E, a.fop, se, sr, t, s, c & 1, c >> 1);
         } else {
-            cc_tile_sweep<float>(a.E, a.cop, se, sr, t, s, (s - 1) & 1);
+            cc_tile_sweep<float>(a.E, a.cop, se, sr, t, s, (s - 1) & 1, itab);
         }
         __syncthreads();
     }
@@ -703,6 +728,11 @@ __device__ __forceinline__ LevelT<T> coarse_level(const CoarseDesc &d, int k) {
     return L;
 }
 
+// LDS of the coarse kernel after the levels: kCoarseThreads partial
+// products, kDenseReg right-hand-side values, then the reciprocal tables of
+// the levels (9 each).
+__device__ __forceinline__ int coarse_tab_off(const CoarseDesc &d) { return d.red_off + kCoarseThreads + kDenseReg; }
+
 // count sweeps of coarse level k
 template <class T>
 __device__ __forceinline__ void coarse_sweeps(bool fem, const CoarseDesc &d, const CCOp &cop, const FemOp &fop, int k,
@@ -710,6 +740,7 @@ __device__ __forceinline__ void coarse_sweeps(bool fem, const CoarseDesc &d, con
     const LevelT<T> L = coarse_level<T>(d, k);
     const Tile t = coarse_tile(L.m, L.n);
     T *z = lds + d.off[k], *b = z + t.RX * t.RY;
+    const T *itab = lds + coarse_tab_off(d) + 9 * k;
     for (int c = 0; c < count; c++) {
         if (fem) {
             for (int q = 0; q < 4; q++) {
@@ -718,7 +749,7 @@ __device__ __forceinline__ void coarse_sweeps(bool fem, const CoarseDesc &d, con
             }
         } else {
             for (int q = 0; q < 2; q++) {
-                cc_tile_sweep<T, kCoarseBX, kCoarseBY>(L, cop, z, b, t, 1, q);
+                cc_tile_sweep<T, kCoarseBX, kCoarseBY>(L, cop, z, b, t, 1, q, itab);
                 __syncthreads();
             }
         }
@@ -732,8 +763,10 @@ __device__ __forceinline__ void coarse_body_t(const CoarseArgs<T> &a, T *lds, do
     const T *gb = reinterpret_cast<const T *>(d.gb);
     T *gz = reinterpret_cast<T *>(d.gz);
     const int ld = d.nl - 1, Nd = d.m[ld] * d.n[ld];
-    // zero every level (the borders stay zero), then load the first level
+    // zero every level (the borders stay zero), then load the first level;
+    // the levels' reciprocal tables meanwhile
     for (int k = tid; k < d.red_off; k += Tn) lds[k] = T(0);
+    if (!FEM && tid < 9 * d.nl) cc_inv_table<T>(coarse_level<T>(d, tid / 9), a.cop, lds + coarse_tab_off(d) + 9 * (tid / 9), tid % 9);
     __syncthreads();
     {
         const int m = d.m[0], N = m * d.n[0], RX = m + 2;
@@ -1090,7 +1123,7 @@ __global__ void k_reset_state(MGState *st) {
 // ------------------------------------------------------------------------
 
 static size_t tile_lds_bytes(int H, size_t elem, int tx, int ty) {
-    return 2 * elem * (size_t)(tx + 2 * H) * (ty + 2 * H);
+    return 2 * elem * (size_t)(tx + 2 * H) * (ty + 2 * H) + kInvTab * elem;
 }
 
 namespace mgk {
@@ -1426,7 +1459,7 @@ int MG::init(int kind_, int m, int n, double cx, double cy, int x_sym, double *z
     VDI_TRY(partial_.alloc(2 * maxblk * sizeof(double)));
     VDI_TRY(state_.alloc(sizeof(MGState)));
     if (!host_state_) VDI_HIP(hipHostMalloc((void **)&host_state_, sizeof(MGState), hipHostMallocDefault));
-    coarse_lds_ = (kCoarseThreads + kDenseReg) * sizeof(double);
+    coarse_lds_ = (kCoarseThreads + kDenseReg + 9 * kMaxCoarse) * sizeof(double);
     for (int l = lc; l < nl; l++) coarse_lds_ += 2 * sizeof(double) * (size_t)(lm[l] + 2) * (ln[l] + 2);
     return VDI_OK;
 }
@@ -1438,7 +1471,7 @@ int MG::set_coarse_points(int pts) {
     if (nl - l2 > kMaxCoarse) return VDI_ERR_ARG;
     coarse_pts = pts;
     lc = l2;
-    coarse_lds_ = (kCoarseThreads + kDenseReg) * sizeof(double);
+    coarse_lds_ = (kCoarseThreads + kDenseReg + 9 * kMaxCoarse) * sizeof(double);
     for (int l = lc; l < nl; l++) coarse_lds_ += 2 * sizeof(double) * (size_t)(lm[l] + 2) * (ln[l] + 2);
     return VDI_OK;
 }
@@ -1530,7 +1563,7 @@ size_t MG::up_lds(int prec, int l) const {
 size_t MG::mx_lds() const {
     const TileShape &t = kTileShapes[tile_mx];
     const int H = halo_up(fem, nu2, 1);
-    return (size_t)(t.tx + 2 * H) * (t.ty + 2 * H) * (sizeof(double) + 2 * sizeof(float));
+    return (size_t)(t.tx + 2 * H) * (t.ty + 2 * H) * (sizeof(double) + 2 * sizeof(float)) + kInvTab * sizeof(float);
 }
 
 template <class T>

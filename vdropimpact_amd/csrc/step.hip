@@ -240,7 +240,7 @@ __device__ void godunov_body(const StepRec &a, int bx, int by) {
         if (i == m - 1) { F.ul[IX(m, j)] = se[UR][l]; F.vl[IX(m, j)] = se[VR][l]; }
         if (j == n - 1) { F.vd[IX(i, n)] = se[VU][l]; F.ud[IX(i, n)] = se[UU][l]; }
         double s = g.xsp * (ulR - ulL) + g.ysp * (vdT - vdB);
-        if (j == 0) {
+        if (j == 0 && !a.defer_sbc) {
             const double sbc = macfac * (G.p_old[i] + G.p_old[i + 1] + G.p[i] + G.p[i + 1]);
             F.src[(size_t)m * n + i] = sbc;
             s -= 2 * g.yysp * sbc;
@@ -765,6 +765,22 @@ __device__ void snapshot_body(const StepRec &a, int blk, bool restore) {
 // ------------------------------------------------------------------------
 
 __global__ void __launch_bounds__(256) k_godunov(StepRec a) { godunov_body(a, blockIdx.x, blockIdx.y); }
+// The gas-pressure term of the bottom MAC source row (the j = 0 branch of
+// godunov_body, same operations in the same order) when Godunov ran with
+// defer_sbc, beside the gas update that produces p.
+__global__ void __launch_bounds__(256) k_mac_bottom(StepRec a) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const Geo &g = a.g;
+    if (i >= g.m) return;
+    const Gas &G = a.G;
+    const size_t mn = (size_t)g.m * g.n;
+    const double sbc = a.macfac * (G.p_old[i] + G.p_old[i + 1] + G.p[i] + G.p[i + 1]);
+    a.F.src[mn + i] = sbc;
+    double s = a.F.src[i];
+    s -= 2 * g.yysp * sbc;
+    a.F.src[i] = s;
+}
+
 __global__ void __launch_bounds__(256) k_correct_advect(StepRec a) { correct_body(a, blockIdx.x, blockIdx.y); }
 __global__ void __launch_bounds__(256) k_simpson(StepRec a, Stamp *ts) {
     stamp_begin(ts);
@@ -994,6 +1010,8 @@ int vdi_create(const vdi_params *p, vdi_ctx **out) {
     VDI_HIP(hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking));
     VDI_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
     VDI_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+    VDI_HIP(hipEventCreateWithFlags(&c->ev_gfork, hipEventDisableTiming));
+    VDI_HIP(hipEventCreateWithFlags(&c->ev_gjoin, hipEventDisableTiming));
     size_t NP = (size_t)g.ml * (g.n + 4);
     VDI_TRY(c->fields.alloc(19 * NP * sizeof(double)));
     double *f = c->fields.as<double>();
@@ -1110,12 +1128,12 @@ extern "C" int vdi_init_fields(vdi_ctx *c) {
     return VDI_OK;
 }
 
-static int enqueue_gas(vdi_ctx *c, const StepRec &a) {
+static int enqueue_gas(vdi_ctx *c, const StepRec &a, hipStream_t st = nullptr) {
     const Geo &g = c->g;
     // reads u,v rows 0,-1, h, ht, fbd, pg, p, tau; writes p_old, tau_old, h, ht, fbd, hx, pg, pg_old, p, tau
     Stamp *ts = c->timer.slot(VDI_KCLASS_GAS, 8.0 * (4.0 * g.m + 3.0 * g.m + 3.0 * g.me + 2.0 * g.ml + 3.0 * g.m +
                                                       4.0 * g.me + 2.0 * g.ml));
-    hipLaunchKernelGGL(k_gas, dim3(1), dim3(kGasThreads), c->gas_lds, c->st, a, ts);
+    hipLaunchKernelGGL(k_gas, dim3(1), dim3(kGasThreads), c->gas_lds, st ? st : c->st, a, ts);
     VDI_LAUNCHED();
     return VDI_OK;
 }
@@ -1187,12 +1205,29 @@ static int body_solve(vdi_ctx *c, int k, double t, SolveMode mode) {
 // gas layer has been enqueued already.  Host times: t0 at the start of the
 // step; the clock advances in two half steps, as fluid_2d.cc:102 and :251 do,
 // so that solve times match bit for bit.
-static int enqueue_body(vdi_ctx *c, const StepRec &a, double t0, SolveMode mode) {
+static int enqueue_body(vdi_ctx *c, const StepRec &a, double t0, SolveMode mode, bool gas_branch = false) {
     const Geo &g = c->g;
     const dim3 gc = cells_grid(g), gn = nodes_grid(g);
     const double dt = a.dt;
-    hipLaunchKernelGGL(k_godunov, dim3(cdiv(g.m, kGX), cdiv(g.n, kGY)), dim3(256), 0, c->st, a);
-    VDI_LAUNCHED();
+    if (gas_branch) {
+        // The gas update (st2) and the Godunov passes (st) share no data but
+        // the gas pressure of the bottom MAC source row, which is added once
+        // both are done.
+        VDI_HIP(hipEventRecord(c->ev_gfork, c->st));
+        VDI_HIP(hipStreamWaitEvent(c->st2, c->ev_gfork, 0));
+        VDI_TRY(enqueue_gas(c, a, c->st2));
+        VDI_HIP(hipEventRecord(c->ev_gjoin, c->st2));
+        StepRec ad = a;
+        ad.defer_sbc = 1;
+        hipLaunchKernelGGL(k_godunov, dim3(cdiv(g.m, kGX), cdiv(g.n, kGY)), dim3(256), 0, c->st, ad);
+        VDI_LAUNCHED();
+        VDI_HIP(hipStreamWaitEvent(c->st, c->ev_gjoin, 0));
+        hipLaunchKernelGGL(k_mac_bottom, dim3(cdiv(g.m, 256)), dim3(256), 0, c->st, a);
+        VDI_LAUNCHED();
+    } else {
+        hipLaunchKernelGGL(k_godunov, dim3(cdiv(g.m, kGX), cdiv(g.n, kGY)), dim3(256), 0, c->st, a);
+        VDI_LAUNCHED();
+    }
     VDI_TRY(body_solve(c, 0, half_time(t0, dt), mode));
     hipLaunchKernelGGL(k_correct_advect, gc, dim3(256), 0, c->st, a);
     VDI_LAUNCHED();
@@ -1275,8 +1310,12 @@ static int capture_step(vdi_ctx *c, double dt, vdi_ctx::GraphEntry &e) {
         // count is known only after capture, so clear the rest of the pool
         if (c->timer.cls >= 0 && (rc = c->timer.reset(e.slot_a, c->timer.cap, c->st)) != VDI_OK) break;
         hipLaunchKernelGGL(k_snapshot, dim3(stepk::tiles(SK_SNAPSHOT, g)), dim3(256), 0, c->st, a, 0);
-        if ((rc = enqueue_gas(c, a)) != VDI_OK) break;
-        if ((rc = enqueue_body(c, a, c->time, SOLVE_FIXED)) != VDI_OK) break;
+        if (c->par_visco) {
+            if ((rc = enqueue_body(c, a, c->time, SOLVE_FIXED, true)) != VDI_OK) break;
+        } else {
+            if ((rc = enqueue_gas(c, a)) != VDI_OK) break;
+            if ((rc = enqueue_body(c, a, c->time, SOLVE_FIXED)) != VDI_OK) break;
+        }
         hipLaunchKernelGGL(k_gather_status, dim3(1), dim3(64), 0, c->st, a);
         if (hipMemcpyAsync(c->rb, c->rb_dev.p, sizeof(vdi_ctx::Readback), hipMemcpyDeviceToHost, c->st) !=
             hipSuccess) {

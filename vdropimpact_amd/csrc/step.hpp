@@ -51,6 +51,9 @@ struct StepRec {
     const MGState *mgst[4];  // MAC, VX, VY, FEM (null when absent)
     MGState *rb_st;          // readback: four states, then two gas ints
     int *rb_gas;
+    // Godunov leaves the gas-pressure term of the bottom MAC source row to
+    // k_mac_bottom, so that it need not wait for the gas update.
+    int defer_sbc;
 };
 
 // Tiles of the step kernels: Godunov 32 x 8 cells (256 threads, 1D);
@@ -106,10 +109,11 @@ struct vdi_ctx {
     vdi::Geo g{};
     vdi::GasConst K{};
     hipStream_t st = nullptr;
-    // Captured steps run the two viscous solves as parallel graph branches:
-    // the v solve is recorded on st2 between a fork and a join event.
+    // Captured steps run the gas update beside the Godunov passes, and the
+    // two viscous solves beside each other, as parallel graph branches: the
+    // second branch is recorded on st2 between a fork and a join event.
     hipStream_t st2 = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_gfork = nullptr, ev_gjoin = nullptr;
     int par_visco = 1;
     double time = 0, Vframe = 0;
     double rel_tol = 1e-10;
@@ -162,6 +166,8 @@ struct vdi_ctx {
         if (st2) (void)hipStreamSynchronize(st2);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (ev_join) (void)hipEventDestroy(ev_join);
+        if (ev_gfork) (void)hipEventDestroy(ev_gfork);
+        if (ev_gjoin) (void)hipEventDestroy(ev_gjoin);
         if (st2) (void)hipStreamDestroy(st2);
         if (st) (void)hipStreamDestroy(st);
     }
