@@ -153,15 +153,17 @@ def run_plain(vdi, args, p):
     log(f"plain: {args.warmup} warm-up steps, tiles tuned")
     # Which kernel class dominates the step: every launch of one class is
     # timed by the kernels' own wall-clock stamps over a few steps.
-    class_ms = {}
+    class_ms, class_n = {}, {}
     for kc in vdi.KCLASS_NAMES:
         sim.time_kernels(kc, 8192)
         for _ in range(3):
             sim.step_forward(p.dt_reg)
-        _n, ms, _b = sim.kernel_times()
+        n_, ms, _b = sim.kernel_times()
         class_ms[kc] = ms / 3.0
+        class_n[kc] = n_ / 3.0
     dominant = max(class_ms, key=class_ms.get)
     log("kernel classes, ms/step:", {vdi.KCLASS_NAMES[k]: round(v, 4) for k, v in class_ms.items()})
+    log("kernel classes, launches/step:", {vdi.KCLASS_NAMES[k]: round(v, 2) for k, v in class_n.items()})
     # Timed region (the captured-step path) with the dominant class stamped.
     sim.time_kernels(dominant, 16384)
     for _ in range(3):
@@ -174,7 +176,9 @@ def run_plain(vdi, args, p):
     sim.time_kernels(-1)
     for _ in range(3):
         sim.step_forward(p.dt_reg)
+    g0, r0 = sim.counters()
     el_clean = time_steps(sim, p.dt_reg, args.steps)
+    g1, r1 = sim.counters()
     log(f"plain: {args.steps} steps in {el:.3f} s (stamped), {el_clean:.3f} s (no stamps)")
     achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
     sim.close()
@@ -185,6 +189,7 @@ def run_plain(vdi, args, p):
         "value": args.steps / el_clean, "ms_per_step": 1e3 * el_clean / args.steps,
         "steps_per_s_stamped": args.steps / el,
         "avg_vcycles": cyc,
+        "captured_steps": g1 - g0, "redone_steps": r1 - r0,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM, "traffic": traffic,
                      "kernel": vdi.KCLASS_NAMES[dominant], "launches": launches,
@@ -342,6 +347,8 @@ def main():
                                "x_sym, implicit viscosity, all solves to rel. residual 1e-10",
                    "grid": [args.m, args.n], "form": "fp64", "rel_tol": 1e-10,
                    "avg_vcycles": plain["avg_vcycles"],
+                   "captured_steps": plain["captured_steps"], "redone_steps": plain["redone_steps"],
+                   "concurrent_branches": not args.serial_visco,
                    "steps_per_s_with_kernel_stamps": plain["steps_per_s_stamped"]},
         "roofline": plain["roofline"],
         "cpu_baseline": None,

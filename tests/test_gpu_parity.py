@@ -362,3 +362,25 @@ def test_gpu_parallel_visco_branches_same_bits(vdi):
         assert np.array_equal(a.get(aid), b.get(aid))
     a.close()
     b.close()
+
+
+def test_gpu_redo_restores_budgets(vdi):
+    """A captured step that falls short is redone on the direct path, and the
+    cycle budgets then cover the whole solve (the captured cycles plus the
+    redo's), so the following captured steps are not redone as well."""
+    say("redo budgets 256x128")
+    p = PhysicalConfig(m=256, n=128).params()
+    a = vdi.Fluid2D(p)
+    a.init_fields()
+    for _ in range(12):
+        a.step_forward(p.dt_reg)
+    g0, r0 = a.counters()
+    # a tighter tolerance needs more cycles than the budgets recorded so far
+    a.set_solver(rel_tol=1e-12)
+    for _ in range(10):
+        a.step_forward(p.dt_reg)
+    g1, r1 = a.counters()
+    say("captured", g1 - g0, "redone", r1 - r0)
+    assert g1 - g0 == 10
+    assert 1 <= r1 - r0 <= 3
+    a.close()
