@@ -503,10 +503,11 @@ static int batch_step_graph(vdi_batch *b) {
             MG &g = *step_solver(c, k);
             const MGState &stt = r.st[k];
             g.total_cycles += stt.cycles;
+            c->redo_base[k] = stt.cycles;
             if (stt.done == 1 || stt.done == 3) {
                 g.last_cycles = stt.cycles;
                 g.solves++;
-                c->budget[k] = std::max(1, stt.cycles);
+                c->record_cycles(k, stt.cycles);
             } else {
                 ok = false;
             }

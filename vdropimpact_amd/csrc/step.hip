@@ -1197,7 +1197,9 @@ static int body_solve(vdi_ctx *c, int k, double t, SolveMode mode) {
     }
     VDI_TRY(g.solve(c->rel_tol, c->max_cycles, &cyc));
     if (mode == SOLVE_DIRECT && lmode) VDI_TRY(guess_after_solve(*L, g, c->st));
-    c->budget[k] = std::max(1, cyc);
+    // a redo continues from the captured step's iterates: the whole solve
+    // took those cycles plus the redo's
+    c->record_cycles(k, (mode == SOLVE_AGAIN ? c->redo_base[k] : 0) + cyc);
     return VDI_OK;
 }
 
@@ -1385,10 +1387,11 @@ static int step_graph(vdi_ctx *c, double dt) {
         MG &g = *step_solver(c, k);
         const MGState &st = r.st[k];
         g.total_cycles += st.cycles;
+        c->redo_base[k] = st.cycles;
         if (st.done == 1 || st.done == 3) {
             g.last_cycles = st.cycles;
             g.solves++;
-            c->budget[k] = std::max(1, st.cycles);
+            c->record_cycles(k, st.cycles);
         } else {
             ok = false;
         }

@@ -145,6 +145,25 @@ struct vdi_ctx {
     };
     int use_graphs = 1;
     int budget[4] = {0, 0, 0, 0};
+    // cycles each solve ran in a captured step that is being redone: the
+    // redo's solve starts from those iterates, so its budget becomes these
+    // plus the cycles the redo needed
+    int redo_base[4] = {0, 0, 0, 0};
+    // Cycles of the last kHist solves of each system; a captured step's
+    // budget is their maximum.  A count that alternates (3, 4, 3, ...) would
+    // otherwise send every rise through a redo, which costs a whole step on
+    // the direct path, while a cycle too many costs only its launches
+    // (their kernels return at once on the converged flag).
+    static constexpr int kHist = 4;
+    int hist[4][kHist] = {};
+    int hist_n[4] = {0, 0, 0, 0};
+    void record_cycles(int k, int cycles) {
+        hist[k][hist_n[k] % kHist] = cycles;
+        hist_n[k]++;
+        int b = 1;
+        for (int q = 0; q < std::min(hist_n[k], kHist); q++) b = std::max(b, hist[k][q]);
+        budget[k] = b;
+    }
     std::vector<GraphEntry> graphs;
     vdi::DevBuf snap, coef_dev, rb_dev;
     double *coef_host = nullptr;  // pinned, 2 extrapolation coefficients per solver
