@@ -984,15 +984,19 @@ __device__ void norm_finish_body(const NormArgs &a, double *sh) {
             st->done = (B == 0) ? 3 : (!(R == R) ? 2 : (R <= a.tol2 * B ? 1 : 0));
         }
     } else {
-        if (a.ctl.state->done) return;
+        // the partials are read before the flag (not after it) so that the
+        // two memory round trips overlap; a solve that had already converged
+        // leaves its state alone
         double r = 0;
         for (int k = tid; k < a.nb; k += T) r += a.ctl.partial[k];
         const double R = block_sum(r, sh);
         if (tid == 0) {
             MGState *st = a.ctl.state;
-            st->rr = R;
-            st->cycles += 1;
-            st->done = !(R == R) ? 2 : (R <= a.tol2 * st->bb ? 1 : 0);
+            if (!st->done) {
+                st->rr = R;
+                st->cycles += 1;
+                st->done = !(R == R) ? 2 : (R <= a.tol2 * st->bb ? 1 : 0);
+            }
         }
     }
 }

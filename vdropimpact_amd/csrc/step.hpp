@@ -154,7 +154,7 @@ struct vdi_ctx {
     // otherwise send every rise through a redo, which costs a whole step on
     // the direct path, while a cycle too many costs only its launches
     // (their kernels return at once on the converged flag).
-    static constexpr int kHist = 4;
+    static constexpr int kHist = 16;
     int hist[4][kHist] = {};
     int hist_n[4] = {0, 0, 0, 0};
     void record_cycles(int k, int cycles) {
