@@ -7,8 +7,9 @@
 //   coarse               : a complete V-cycle on levels lc..L-1 inside one
 //                          workgroup's LDS
 //   up(l), l = lc-1..0   : prolongation + correction, nu2 sweeps; at level 0
-//                          also ||b - A z||^2, reduced by the last workgroup,
-//                          which raises the device-side convergence flag.
+//                          also per-workgroup partials of ||b - A z||^2
+//   norm                 : one workgroup sums the partials in a fixed order
+//                          and raises the device-side convergence flag.
 // Every kernel returns at once when the flag is set, so a solve enqueues the
 // predicted number of cycles and synchronises with the host once.
 //
@@ -933,10 +934,10 @@ __global__ void __launch_bounds__(kCoarseThreads) k_coarse_b(const CoarseArgs<T>
 // Start of a solve, element-wise operations, operator application
 // ------------------------------------------------------------------------
 
-// ||b - A z||^2 and ||b||^2 of level 0 over blocks bid, bid + nb, ...; the
-// last block stores them and sets the flag: 1 converged, 3 zero right-hand
-// side, 0 otherwise.  With r32 non-null (mixed form) the residual is also
-// stored in FP32.
+// Partials of ||b - A z||^2 and ||b||^2 of level 0 over blocks bid, bid + nb,
+// ...; k_norm_finish (begin = 1) sums them and sets the flag: 1 converged,
+// 2 NaN, 3 zero right-hand side, 0 otherwise.  With r32 non-null (mixed form)
+// the residual is also stored in FP32.
 __device__ void begin_body(const BeginArgs &a, int bid, double *sh, int *last) {
     const Level &L = a.L;
     double rr = 0, bb = 0;
@@ -1119,7 +1120,6 @@ __global__ void k_copy_d(const double *a, double *b, long N) {
 __global__ void k_reset_state(MGState *st) {
     st->done = 0;
     st->cycles = 0;
-    st->ticket = 0;
 }
 
 // ------------------------------------------------------------------------

@@ -43,13 +43,11 @@ constexpr int kDenseReg = 256;
 constexpr int kDenseRegPer = 32;
 
 // Device-side solve state: convergence flag (0 running, 1 converged,
-// 2 NaN, 3 zero right-hand side), V-cycles run, ticket of the last-
-// workgroup reduction, ||b-Az||^2 and ||b||^2.
+// 2 NaN, 3 zero right-hand side), V-cycles run, ||b-Az||^2 and ||b||^2.
 struct MGState {
     int done;
     int cycles;
-    unsigned ticket;
-    int pad;
+    int pad[2];
     double rr, bb;
 };
 
