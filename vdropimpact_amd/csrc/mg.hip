@@ -58,8 +58,14 @@ __host__ __device__ __forceinline__ double bc_ghost_factor(int bc) {
     return bc == BC_DF ? -1.0 : (bc == BC_DC ? 0.0 : 1.0);
 }
 
+// The cell-centred tile formulas below fix every rounding in the source (no
+// contraction left to the compiler): the compiler's choice of which product
+// to fuse into an add can depend on the context a formula is inlined into,
+// and the level kernels, their batched forms and the coarse kernel must give
+// the same bits for the same values.
 template <class T>
 __host__ __device__ __forceinline__ T cc_diag(const CCOp &op, int m, int n, T cx, T cy, int i, int j) {
+#pragma clang fp contract(off)
     T d = 0;
     d += T(i > 0 ? 1.0 : bc_diag_factor(op.bcl)) * cx;
     d += T(i < m - 1 ? 1.0 : bc_diag_factor(op.bcr)) * cx;
@@ -113,7 +119,8 @@ __host__ __device__ __forceinline__ T Az_plain(bool fem, const CCOp &cop, const 
 
 template <class T>
 __device__ __forceinline__ T cc_off_t(T cx, T cy, const T *s, int l, int RX) {
-    return cx * (s[l - 1] + s[l + 1]) + cy * (s[l - RX] + s[l + RX]);
+#pragma clang fp contract(off)
+    return fma(cx, s[l - 1] + s[l + 1], cy * (s[l - RX] + s[l + RX]));
 }
 
 // 9-point Q1 stencil with missing neighbours zero: edge columns halve the
@@ -177,6 +184,7 @@ __device__ __forceinline__ Tile make_tile(int bx, int by, int H, int TX, int TY)
 template <class T>
 __device__ __forceinline__ T cc_prolong_at(const CCOp &op, int Mc, int Nc, const T *e, int i, int j,
                                            size_t pitch = 0) {
+#pragma clang fp contract(off)
     if (!pitch) pitch = Mc;
     int I = i >> 1, J = j >> 1;
     int I2 = (i & 1) ? I + 1 : I - 1, J2 = (j & 1) ? J + 1 : J - 1;
@@ -187,7 +195,7 @@ __device__ __forceinline__ T cc_prolong_at(const CCOp &op, int Mc, int Nc, const
     else if (J2 >= Nc) { fy = T(bc_ghost_factor(op.bct)); J2 = J; }
     T e00 = e[I + pitch * J], e10 = fx * e[I2 + pitch * J];
     T e01 = fy * e[I + pitch * J2], e11 = fx * fy * e[I2 + pitch * J2];
-    return T(0.5625) * e00 + T(0.1875) * (e10 + e01) + T(0.0625) * e11;
+    return fma(T(0.0625), e11, fma(T(0.5625), e00, T(0.1875) * (e10 + e01)));
 }
 
 // Nodal bilinear prolongation at fine node (i,j), j >= 1.
@@ -320,6 +328,7 @@ __device__ __forceinline__ void tile_store(const LevelT<T> &F, T *dst, const T *
 template <class T, int BX = kBX, int BY = kBY>
 __device__ __forceinline__ void cc_tile_sweep(const LevelT<T> &L, const CCOp &op, T *sz, const T *sb, const Tile &t,
                                               int s, int c, const T *itab) {
+#pragma clang fp contract(off)
     const int X0 = max(0, t.gx0 + s), X1 = min(L.m, t.x0 + t.TX + t.H - s);
     const int Y0 = max(0, t.gy0 + s), Y1 = min(L.n, t.y0 + t.TY + t.H - s);
     const T sgn = T(op.s);
@@ -328,7 +337,7 @@ __device__ __forceinline__ void cc_tile_sweep(const LevelT<T> &L, const CCOp &op
         const int row = (gy - t.gy0) * t.RX - t.gx0;
         for (int gx = xs + 2 * threadIdx.x; gx < X1; gx += 2 * BX) {
             const int l = row + gx;
-            const T rhs = sb[l] - sgn * cc_off_t<T>(L.cx, L.cy, sz, l, t.RX);
+            const T rhs = fma(-sgn, cc_off_t<T>(L.cx, L.cy, sz, l, t.RX), sb[l]);
             sz[l] = rhs * itab[cc_class<T>(L, gx, gy)];
         }
     }
@@ -353,7 +362,8 @@ __device__ __forceinline__ void fem_tile_sweep(const LevelT<T> &L, const FemOp &
 template <class T>
 __device__ __forceinline__ T cc_resid_t(const LevelT<T> &L, const CCOp &op, const T *sz, const T *sb, int RX, int l,
                                         int gx, int gy) {
-    return sb[l] - (T(op.s) * cc_off_t<T>(L.cx, L.cy, sz, l, RX) + cc_diag<T>(op, L.m, L.n, L.cx, L.cy, gx, gy) * sz[l]);
+#pragma clang fp contract(off)
+    return sb[l] - fma(T(op.s), cc_off_t<T>(L.cx, L.cy, sz, l, RX), cc_diag<T>(op, L.m, L.n, L.cx, L.cy, gx, gy) * sz[l]);
 }
 
 template <class T>
@@ -383,6 +393,7 @@ __device__ __forceinline__ T restrict_at(const LevelT<T> &F, const CCOp &cop, co
                                          const T *sb, const Tile &t, int I, int J) {
     T v = 0;
     if (!FEM) {
+#pragma clang fp contract(off)
         for (int b2 = 0; b2 < 2; b2++)
             for (int a2 = 0; a2 < 2; a2++) {
                 const int gx = 2 * I + a2, gy = 2 * J + b2;
