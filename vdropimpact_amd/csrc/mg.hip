@@ -277,7 +277,7 @@ template <class T, bool FEM>
 __device__ __forceinline__ void tile_load(const LevelT<T> &F, const T *zsrc, const LevelT<T> *C, const CCOp &op,
                                           T *sz, T *sb, const Tile &t) {
     const int nel = t.RX * t.RY;
-    Walk w(threadIdx.x + kBX * threadIdx.y, t.RX, 256);
+    Walk w(threadIdx.x + kBX * threadIdx.y, t.RX, kLvThreads);
     while (w.q < nel) {
         T zv[kLoadUnroll], bv[kLoadUnroll];
         int li[kLoadUnroll];
@@ -300,7 +300,7 @@ __device__ __forceinline__ void tile_load(const LevelT<T> &F, const T *zsrc, con
             zv[u] = in ? z : T(0);
             bv[u] = in ? b : T(0);
             li[u] = w.q < nel ? w.q : -1;
-            w.next(256);
+            w.next(kLvThreads);
         }
 #pragma unroll
         for (int u = 0; u < kLoadUnroll; u++)
@@ -418,6 +418,9 @@ __device__ __forceinline__ T restrict_at(const LevelT<T> &F, const CCOp &cop, co
     }
     return v;
 }
+
+// sh holds one partial per wave: 8 entries in every kernel that calls this.
+static_assert(kLvThreads <= 8 * 64 && kCoarseThreads <= 8 * 64, "block_sum takes at most eight waves");
 
 // Workgroup sum (fixed order); result valid in thread 0.
 __device__ __forceinline__ double block_sum(double a, double *sh) {
@@ -556,11 +559,11 @@ __device__ __forceinline__ void mx_body(const MxArgs &a, int tile, double *lds_d
     const int tid = threadIdx.x + kBX * threadIdx.y;
     {
         const int RW = t.TX + 2, nel = RW * (t.TY + 2);
-        for (int k0 = tid; k0 < nel; k0 += 256 * kLoadUnroll) {
+        for (int k0 = tid; k0 < nel; k0 += kLvThreads * kLoadUnroll) {
             double zz[kLoadUnroll];
 #pragma unroll
             for (int u = 0; u < kLoadUnroll; u++) {
-                const int k = k0 + 256 * u;
+                const int k = k0 + kLvThreads * u;
                 zz[u] = 0;
                 if (k < nel) {
                     const int ry = k / RW, rx = k - ry * RW;
@@ -570,7 +573,7 @@ __device__ __forceinline__ void mx_body(const MxArgs &a, int tile, double *lds_d
             }
 #pragma unroll
             for (int u = 0; u < kLoadUnroll; u++) {
-                const int k = k0 + 256 * u;
+                const int k = k0 + kLvThreads * u;
                 if (k < nel) {
                     const int ry = k / RW, rx = k - ry * RW;
                     const int gx = t.x0 - 1 + rx, gy = t.y0 - 1 + ry;
@@ -584,11 +587,11 @@ __device__ __forceinline__ void mx_body(const MxArgs &a, int tile, double *lds_d
     __syncthreads();
     double rr = 0;
     const int nel = t.TX * t.TY;
-    for (int k0 = tid; k0 < nel; k0 += 256 * kLoadUnroll) {
+    for (int k0 = tid; k0 < nel; k0 += kLvThreads * kLoadUnroll) {
         double bv[kLoadUnroll];
 #pragma unroll
         for (int u = 0; u < kLoadUnroll; u++) {
-            const int k = k0 + 256 * u;
+            const int k = k0 + kLvThreads * u;
             bv[u] = 0;
             if (k < nel) {
                 const int y = k / t.TX, x = k - y * t.TX;
@@ -598,7 +601,7 @@ __device__ __forceinline__ void mx_body(const MxArgs &a, int tile, double *lds_d
         }
 #pragma unroll
         for (int u = 0; u < kLoadUnroll; u++) {
-            const int k = k0 + 256 * u;
+            const int k = k0 + kLvThreads * u;
             if (k >= nel) continue;
             const int y = k / t.TX, x = k - y * t.TX;
             const int gx = t.x0 + x, gy = t.y0 + y;
@@ -629,7 +632,7 @@ __device__ __forceinline__ void mx_body(const MxArgs &a, int tile, double *lds_d
 // ------------------------------------------------------------------------
 
 template <class T, bool FEM>
-__global__ void __launch_bounds__(256) k_down(LvArgs<T> a) {
+__global__ void __launch_bounds__(kLvThreads) k_down(LvArgs<T> a) {
     extern __shared__ double lds_d[];
     if (a.ctl.state->done) return;
     stamp_begin(a.ts);
@@ -638,7 +641,7 @@ __global__ void __launch_bounds__(256) k_down(LvArgs<T> a) {
 }
 
 template <class T, bool FEM>
-__global__ void __launch_bounds__(256) k_down_b(const LvArgs<T> *A, const WorkItem *W, int nw, BatchAux aux) {
+__global__ void __launch_bounds__(kLvThreads) k_down_b(const LvArgs<T> *A, const WorkItem *W, int nw, BatchAux aux) {
     extern __shared__ double lds_d[];
     stamp_begin(aux.ts);
     for (int p = blockIdx.x; p < nw; p += gridDim.x) {
@@ -654,7 +657,7 @@ __global__ void __launch_bounds__(256) k_down_b(const LvArgs<T> *A, const WorkIt
 }
 
 template <class T, bool FEM>
-__global__ void __launch_bounds__(256) k_up(LvArgs<T> a) {
+__global__ void __launch_bounds__(kLvThreads) k_up(LvArgs<T> a) {
     extern __shared__ double lds_d[];
     __shared__ double sh[8];
     __shared__ int last;
@@ -665,7 +668,7 @@ __global__ void __launch_bounds__(256) k_up(LvArgs<T> a) {
 }
 
 template <class T, bool FEM>
-__global__ void __launch_bounds__(256) k_up_b(const LvArgs<T> *A, const WorkItem *W, int nw, BatchAux aux) {
+__global__ void __launch_bounds__(kLvThreads) k_up_b(const LvArgs<T> *A, const WorkItem *W, int nw, BatchAux aux) {
     extern __shared__ double lds_d[];
     __shared__ double sh[8];
     __shared__ int last;
@@ -683,7 +686,7 @@ __global__ void __launch_bounds__(256) k_up_b(const LvArgs<T> *A, const WorkItem
 }
 
 template <bool FEM>
-__global__ void __launch_bounds__(256) k_mx_up0(MxArgs a) {
+__global__ void __launch_bounds__(kLvThreads) k_mx_up0(MxArgs a) {
     extern __shared__ double lds_d[];
     __shared__ double sh[8];
     __shared__ int last;
@@ -694,7 +697,7 @@ __global__ void __launch_bounds__(256) k_mx_up0(MxArgs a) {
 }
 
 template <bool FEM>
-__global__ void __launch_bounds__(256) k_mx_up0_b(const MxArgs *A, const WorkItem *W, int nw, BatchAux aux) {
+__global__ void __launch_bounds__(kLvThreads) k_mx_up0_b(const MxArgs *A, const WorkItem *W, int nw, BatchAux aux) {
     extern __shared__ double lds_d[];
     __shared__ double sh[8];
     __shared__ int last;
