@@ -179,6 +179,7 @@ def run_plain(vdi, args, p):
     g0, r0 = sim.counters()
     el_clean = time_steps(sim, p.dt_reg, args.steps)
     g1, r1 = sim.counters()
+    height_passes = sim.stats().last_height_passes
     log(f"plain: {args.steps} steps in {el:.3f} s (stamped), {el_clean:.3f} s (no stamps)")
     achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
     sim.close()
@@ -190,6 +191,7 @@ def run_plain(vdi, args, p):
         "steps_per_s_stamped": args.steps / el,
         "avg_vcycles": cyc,
         "captured_steps": g1 - g0, "redone_steps": r1 - r0,
+        "height_passes_last_step": height_passes,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM, "traffic": traffic,
                      "kernel": vdi.KCLASS_NAMES[dominant], "launches": launches,
@@ -348,6 +350,7 @@ def main():
                    "grid": [args.m, args.n], "form": "fp64", "rel_tol": 1e-10,
                    "avg_vcycles": plain["avg_vcycles"],
                    "captured_steps": plain["captured_steps"], "redone_steps": plain["redone_steps"],
+                   "gas_height_passes_last_step": plain["height_passes_last_step"],
                    "concurrent_branches": not args.serial_visco,
                    "steps_per_s_with_kernel_stamps": plain["steps_per_s_stamped"]},
         "roofline": plain["roofline"],

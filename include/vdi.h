@@ -77,6 +77,9 @@ typedef struct vdi_stats {
     int last_newton;       /* Newton steps of the most recent gas-layer update */
     long steps;
     double time;           /* fluid_2d::time */
+    int last_height_passes; /* Jacobi passes of the most recent height update
+                               (0: the serial sweep ran); no counterpart in the
+                               reference, whose sweep is serial */
 } vdi_stats;
 
 /* Kernel classes for vdi_time_kernels (measurement only). */

@@ -244,9 +244,14 @@ def test_gpu_step_at_bench_tolerance(vdi, oracle):
     gpu.close()
 
 
-def test_gpu_gas_update_matches_oracle(vdi, oracle):
-    say("gas update parity 512x256")
-    cfg = PhysicalConfig(m=512, n=256)
+@pytest.mark.parametrize("m,n", [(512, 256), (2688, 64)])
+def test_gpu_gas_update_matches_oracle(vdi, oracle, m, n):
+    """One gas-layer update from the oracle's state after four steps.  The
+    height update (an in-place upward sweep) runs as Jacobi passes up to
+    m = 2560 and as the serial sweep above; both give the oracle's h and h_t
+    bit for bit (same operations, no contraction)."""
+    say("gas update parity", m, n)
+    cfg = PhysicalConfig(m=m, n=n)
     p, gpu, orc = make_pair(vdi, oracle, cfg)
     for _ in range(4):
         orc.step_forward(p.dt_reg)
@@ -254,6 +259,14 @@ def test_gpu_gas_update_matches_oracle(vdi, oracle):
         gpu.set(aid, orc.get(aid))
     gpu.gas_layer_update(orc.time, p.dt_reg)
     orc.gas_update(orc.time, p.dt_reg)
+    for aid in (14, 16):
+        np.testing.assert_array_equal(gpu.get(aid), orc.get(aid))
+    passes = gpu.stats().last_height_passes
+    say("height update passes", passes)
+    if m <= 2560:
+        assert 1 <= passes <= 48   # the Jacobi form ran, well below m
+    else:
+        assert passes == 0         # serial sweep
     for name, aid in [("h", 14), ("ht", 16), ("hx", 15), ("pg", 18), ("gp", 10), ("gtau", 11), ("fbd", 17)]:
         assert rel(gpu.get(aid), orc.get(aid)) < 1e-11, name
     assert gpu.stats().last_newton == orc.last_newton()

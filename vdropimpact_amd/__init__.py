@@ -54,6 +54,7 @@ class VdiStats(ctypes.Structure):
         ("last_newton", ctypes.c_int),
         ("steps", ctypes.c_long),
         ("time", ctypes.c_double),
+        ("last_height_passes", ctypes.c_int),
     ]
 
 

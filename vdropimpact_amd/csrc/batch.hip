@@ -496,6 +496,7 @@ static int batch_step_graph(vdi_batch *b) {
         vdi_ctx *c = b->sims[s];
         const vdi_ctx::Readback &r = b->rb[s];
         c->last_newton = r.gas[1];
+        c->last_height_passes = r.gas[2];
         if (r.gas[0] != VDI_OK) return r.gas[0];
         bool ok = true;
         for (int k = 0; k < 4; k++) {

@@ -462,91 +462,82 @@ __device__ void update_body(const StepRec &a, int bx, int by) {
 // ------------------------------------------------------------------------
 
 __device__ __forceinline__ double eno2(double p0, double p1, double p2, double p3) {
+#pragma clang fp contract(off)
     return fabs(p0 - 2 * p1 + p2) > fabs(p1 - 2 * p2 + p3) ? 3 * p1 - 4 * p2 + p3 : p0 - p2;
 }
 
-// gas_layer_solve.cc:133-162, run by one lane.  The reference updates h in
-// place while i sweeps upward, so the stencil at i sees new h[i-1], h[i-2]
-// and old h[i..i+2]; the lane carries the two new values in registers.  HO
-// (old h) is only read; the new h replaces UC[i] and h_t replaces VC[i] once
-// they have been read.  Interior cells (stencil inside the domain) go in
-// chunks of 16: the chunk's inputs are loaded into registers first, the
-// recurrence then runs register to register, and the results are stored
-// together -- one LDS round trip per chunk instead of one per cell.
-__device__ __forceinline__ double gas_href(int k, int i, int m, int x_sym, double hn1, double hn2, double o0, double o1,
-                                           double o2, double hm1, double hm2, double h0, double h1) {
-    // current value of h[k], k in [i-2, i+2]; hm1/hm2 = current h[m-1], h[m-2];
-    // h0/h1 = current h[0], h[1] (h_ref, gas_layer_solve.cc:176-180)
-    if (k >= 0 && k < m) return k == i - 1 ? hn1 : (k == i - 2 ? hn2 : (k == i ? o0 : (k == i + 1 ? o1 : o2)));
-    if (k >= m) return k == m ? 2 * hm1 - hm2 : 3 * hm1 - 2 * hm2;
-    return x_sym ? (k == -1 ? h0 : h1) : (k == -1 ? 2 * h0 - h1 : 3 * h0 - 2 * h1);
+// update_height (gas_layer_solve.cc:133-162) updates h in place while i
+// sweeps upward, so cell i sees the new h below i and the old h from i on.
+// gas_href is h_ref (gas_layer_solve.cc:176-180) as cell i sees it: HN holds
+// new values below i, HO the old ones.  gas_cell is one cell of the sweep.
+// Both fix their roundings (no contraction left to the compiler), so every
+// caller computes the same bits.
+__device__ __forceinline__ double gas_href(int k, int i, int m, int x_sym, const double *HN, const double *HO) {
+#pragma clang fp contract(off)
+    auto cur = [&](int q) { return q < i ? HN[q] : HO[q]; };
+    if (k >= 0 && k < m) return cur(k);
+    if (k >= m) return k == m ? 2 * cur(m - 1) - cur(m - 2) : 3 * cur(m - 1) - 2 * cur(m - 2);
+    return x_sym ? cur(k == -1 ? 0 : 1) : (k == -1 ? 2 * cur(0) - cur(1) : 3 * cur(0) - 2 * cur(1));
 }
 
-__device__ void gas_height_serial(int m, int x_sym, double dt, double Vframe, const double *HO, double *UC, double *VC) {
-    double hn1 = 0, hn2 = 0;
-    double o0 = HO[0], o1 = HO[1], o2 = m > 2 ? HO[2] : 0.0;
-    auto edge = [&](int i) {
-        // current h of the stencil's edge references (only i < 2 and
-        // i >= m - 2 come here, where they lie inside the window)
-        const double c0 = i == 0 ? o0 : hn1;
-        const double c1 = i == 0 ? o1 : o0;
-        const double cm1 = i == m - 1 ? o0 : o1;
-        const double cm2 = i == m - 1 ? hn1 : o0;
-        auto href = [&](int k) { return gas_href(k, i, m, x_sym, hn1, hn2, o0, o1, o2, cm1, cm2, c0, c1); };
+__device__ __forceinline__ void gas_cell(int i, int m, int x_sym, double dt, double Vframe, const double *HN,
+                                         const double *HO, double uc, double vc, double &hnew, double &htv) {
+#pragma clang fp contract(off)
+    auto h = [&](int k) { return gas_href(k, i, m, x_sym, HN, HO); };
+    const double dhdx = uc > 0 ? eno2(h(i + 1), HO[i], h(i - 1), h(i - 2)) : -eno2(h(i - 1), HO[i], h(i + 1), h(i + 2));
+    htv = -uc * dhdx + vc - Vframe;
+    hnew = HO[i] + dt * htv;
+}
+
+// The sweep itself, by one lane.  HN may alias UC and HT VC: cell i reads its
+// u_c and v_c before it overwrites them.
+__device__ void gas_height_serial(int m, int x_sym, double dt, double Vframe, const double *HO, const double *UC,
+                                  const double *VC, double *HN, double *HT) {
+    for (int i = 0; i < m; i++) {
         const double uc = UC[i], vc = VC[i];
-        const double dhdx = uc > 0 ? eno2(href(i + 1), o0, href(i - 1), href(i - 2))
-                                   : -eno2(href(i - 1), o0, href(i + 1), href(i + 2));
-        const double htv = -uc * dhdx + vc - Vframe;
-        const double hnew = o0 + dt * htv;
-        UC[i] = hnew;
-        VC[i] = htv;
-        hn2 = hn1; hn1 = hnew;
-        o0 = o1; o1 = o2; o2 = (i + 3 < m) ? HO[i + 3] : 0.0;
-    };
-    int i = 0;
-    for (; i < 2 && i < m; i++) edge(i);
-    // interior: i-2 >= 0 and i+2 <= m-1
-    constexpr int CH = 16;
-    for (; i + CH <= m - 2; i += CH) {
-        double ho[CH + 2], uc[CH], vc[CH], hn[CH], ht[CH];
-#pragma unroll
-        for (int j = 0; j < CH + 2; j++) ho[j] = HO[i + j];
-#pragma unroll
-        for (int j = 0; j < CH; j++) {
-            uc[j] = UC[i + j];
-            vc[j] = VC[i + j];
-        }
-#pragma unroll
-        for (int j = 0; j < CH; j++) {
-            const double p0 = ho[j], p1 = ho[j + 1], p2 = ho[j + 2];
-            const double dhdx = uc[j] > 0 ? eno2(p1, p0, hn1, hn2) : -eno2(hn1, p0, p1, p2);
-            const double htv = -uc[j] * dhdx + vc[j] - Vframe;
-            const double hnew = p0 + dt * htv;
-            hn[j] = hnew;
-            ht[j] = htv;
-            hn2 = hn1; hn1 = hnew;
-        }
-#pragma unroll
-        for (int j = 0; j < CH; j++) {
-            UC[i + j] = hn[j];
-            VC[i + j] = ht[j];
-        }
+        double hn, ht;
+        gas_cell(i, m, x_sym, dt, Vframe, HN, HO, uc, vc, hn, ht);
+        HN[i] = hn;
+        HT[i] = ht;
     }
-    for (; i < m - 2; i++) {
-        const double p0 = HO[i], p1 = HO[i + 1], p2 = HO[i + 2], uc = UC[i], vc = VC[i];
-        const double dhdx = uc > 0 ? eno2(p1, p0, hn1, hn2) : -eno2(hn1, p0, p1, p2);
-        const double htv = -uc * dhdx + vc - Vframe;
-        const double hnew = p0 + dt * htv;
-        UC[i] = hnew;
-        VC[i] = htv;
-        hn2 = hn1; hn1 = hnew;
+}
+
+// The same sweep by Jacobi passes over all cells at once: pass p computes
+// every cell from pass p-1's values (the first from old h).  Cell i depends
+// only on cells below it, so after p passes cells 0..p-1 hold the sweep's
+// values exactly, and a pass that changes no bit has reached them everywhere:
+// the result is the serial sweep's, bit for bit.  The map is contracting
+// (the couplings are about 2 |u| dt / dx), so a few passes suffice instead of
+// m dependent steps; after kGasJacobiPasses the serial sweep takes over.
+// Returns the array holding new h (h_t in HT) and sets passes (0 when the
+// serial sweep ran); all threads call it.
+__device__ double *gas_height_jacobi(int m, int x_sym, double dt, double Vframe, const double *HO, const double *UC,
+                                     const double *VC, double *H0, double *H1, double *HT, int &passes) {
+    const int tid = threadIdx.x, T = blockDim.x;
+    for (int i = tid; i < m; i += T) H0[i] = HO[i];
+    __syncthreads();
+    double *src = H0, *dst = H1;
+    for (int pass = 0; pass < kGasJacobiPasses; pass++) {
+        int changed = 0;
+        for (int i = tid; i < m; i += T) {
+            double hn, ht;
+            gas_cell(i, m, x_sym, dt, Vframe, src, HO, UC[i], VC[i], hn, ht);
+            changed |= __double_as_longlong(hn) != __double_as_longlong(src[i]);
+            dst[i] = hn;
+            HT[i] = ht;
+        }
+        if (!__syncthreads_or(changed)) {
+            passes = pass + 1;
+            return dst;
+        }
+        double *t = src;
+        src = dst;
+        dst = t;
     }
-    if (i < m) {
-        o0 = HO[i];
-        o1 = i + 1 < m ? HO[i + 1] : 0.0;
-        o2 = i + 2 < m ? HO[i + 2] : 0.0;
-    }
-    for (; i < m; i++) edge(i);
+    if (tid == 0) gas_height_serial(m, x_sym, dt, Vframe, HO, UC, VC, H0, HT);
+    __syncthreads();
+    passes = 0;
+    return H0;
 }
 
 // Deterministic workgroup sum; result broadcast to every thread.
@@ -589,19 +580,25 @@ __device__ void gas_body(const StepRec &a, double *lds, double *red) {
     // fbd += dt v_c does not take part in the in-place sweep
     for (int i = tid; i < m; i += T) G.fbd[i] += dt * VC[i];
     __syncthreads();
-    if (tid == 0) gas_height_serial(m, x_sym, dt, Vframe, HO, UC, VC);
-    __syncthreads();
-    // new h is in UC, h_t in VC
+    double *HN = UC, *HT = VC;
+    int passes = 0;
+    if (m <= kGasJacobiM) {
+        HT = lds + 5 * m;
+        HN = gas_height_jacobi(m, x_sym, dt, Vframe, HO, UC, VC, lds + 3 * m, lds + 4 * m, HT, passes);
+    } else {
+        if (tid == 0) gas_height_serial(m, x_sym, dt, Vframe, HO, UC, VC, UC, VC);
+        __syncthreads();
+    }
     for (int i = tid; i < m; i += T) {
-        G.h[i] = UC[i];
-        G.ht[i] = VC[i];
+        G.h[i] = HN[i];
+        G.ht[i] = HT[i];
     }
     // calculate_hx (gas_layer_solve.cc:58-62)
     for (int i = tid; i <= m; i += T) {
         double v;
-        if (i >= 1 && i < m) v = K.gxsp * (UC[i] - UC[i - 1]);
-        else if (i == 0) v = x_sym ? 0.0 : K.gxsp * (UC[1] - UC[0]);
-        else v = K.gxsp * (UC[m - 1] - UC[m - 2]);
+        if (i >= 1 && i < m) v = K.gxsp * (HN[i] - HN[i - 1]);
+        else if (i == 0) v = x_sym ? 0.0 : K.gxsp * (HN[1] - HN[0]);
+        else v = K.gxsp * (HN[m - 1] - HN[m - 2]);
         G.hx[i] = v;
     }
     // update_gas_pressure: pg_old = pg
@@ -701,6 +698,7 @@ __device__ void gas_body(const StepRec &a, double *lds, double *red) {
         tau[ml - 2] = -tau[ml - 3]; tau[ml - 1] = -tau[ml - 4];
         G.status[0] = status;
         G.status[1] = nstep;
+        G.status[2] = passes;
     }
 }
 
@@ -746,6 +744,7 @@ __device__ void gather_body(const StepRec &a) {
         if (a.mgst[k]) a.rb_st[k] = *a.mgst[k];
     a.rb_gas[0] = a.G.status[0];
     a.rb_gas[1] = a.G.status[1];
+    a.rb_gas[2] = a.G.status[2];
 }
 
 // Snapshot for a redo (u, v, p are the first three field arrays, then the
@@ -875,7 +874,10 @@ int tiles(int k, const Geo &g) {
     }
 }
 
-size_t gas_lds(const Geo &g) { return std::max<size_t>(3 * (size_t)g.m, 4 * (size_t)g.me) * sizeof(double); }
+size_t gas_lds(const Geo &g) {
+    const size_t h = (g.m <= kGasJacobiM ? 6 : 3) * (size_t)g.m;  // height update
+    return std::max<size_t>(h, 4 * (size_t)g.me) * sizeof(double);  // Newton rows
+}
 
 #define SK_CASES(X)                                                                                                \
     X(SK_GODUNOV, 256) X(SK_CORRECT, 256) X(SK_SIMPSON, 256) X(SK_VISCOUS_BC, 1024) X(SK_VISCO_RHS_U, 256)        \
@@ -1140,9 +1142,10 @@ static int enqueue_gas(vdi_ctx *c, const StepRec &a, hipStream_t st = nullptr) {
 
 static int gas_update(vdi_ctx *c, const StepRec &a) {
     VDI_TRY(enqueue_gas(c, a));
-    VDI_HIP(hipMemcpyAsync(c->host_status, c->G.status, 2 * sizeof(int), hipMemcpyDeviceToHost, c->st));
+    VDI_HIP(hipMemcpyAsync(c->host_status, c->G.status, 3 * sizeof(int), hipMemcpyDeviceToHost, c->st));
     VDI_HIP(hipStreamSynchronize(c->st));
     c->last_newton = c->host_status[1];
+    c->last_height_passes = c->host_status[2];
     return c->host_status[0];
 }
 
@@ -1380,6 +1383,7 @@ static int step_graph(vdi_ctx *c, double dt) {
     c->graph_steps++;
     const vdi_ctx::Readback &r = *c->rb;
     c->last_newton = r.gas[1];
+    c->last_height_passes = r.gas[2];
     if (r.gas[0] != VDI_OK) return r.gas[0];
     bool ok = true;
     for (int k = 0; k < 4; k++) {
@@ -1496,6 +1500,7 @@ int vdi_get_stats(vdi_ctx *c, vdi_stats *s) {
     s->last_newton = c->last_newton;
     s->steps = c->steps;
     s->time = c->time;
+    s->last_height_passes = c->last_height_passes;
     return VDI_OK;
 }
 

@@ -30,7 +30,7 @@ struct Fields {
 // Gas-layer arrays (gas_layer.hh:26-34, gas_layer_solve.hh:163-173).
 struct Gas {
     double *p, *tau, *p_old, *tau_old, *h, *hx, *ht, *fbd, *pg, *pg_old;
-    int *status;  // [0] = status code, [1] = Newton steps
+    int *status;  // [0] = status code, [1] = Newton steps, [2] = height passes
 };
 
 struct GasConst {
@@ -62,6 +62,10 @@ constexpr int kGX = 32, kGY = 8;
 constexpr int kEX = 64, kEY = 4;
 constexpr int kGasThreads = 1024;
 constexpr int kGasRows = 5;  // rows per thread: me <= 5120
+// Height update by Jacobi passes (m <= kGasJacobiM: six LDS arrays of m
+// doubles); at most kGasJacobiPasses passes before the serial sweep.
+constexpr int kGasJacobiM = 2560;
+constexpr int kGasJacobiPasses = 96;
 
 // Kernel identifiers for the batched launcher.
 enum StepKernel {
@@ -121,6 +125,7 @@ struct vdi_ctx {
     int form = VDI_FORM_FP64;
     long steps = 0;
     int last_newton = 0;
+    int last_height_passes = 0;
     vdi::DevBuf fields;  // 19 padded arrays
     vdi::DevBuf srcbuf, gasbuf, zbuf, statusbuf;
     vdi::Fields F{};
@@ -140,8 +145,8 @@ struct vdi_ctx {
     };
     struct Readback {
         vdi::MGState st[4];
-        int gas[2];
-        int pad[2];
+        int gas[3];
+        int pad;
     };
     int use_graphs = 1;
     int budget[4] = {0, 0, 0, 0};
