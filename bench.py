@@ -45,9 +45,10 @@ def field_errors(vdi, a, b):
     return {name: rel(a.get(aid), b.get(aid)) for name, aid in fields.items()}
 
 
-def cpu_baseline(m, n, seconds, warmup):
+def cpu_baseline(m, n, seconds, warmup, schedules):
     """Times the oracle on the host (test infrastructure used as the checker
-    and as the reported CPU baseline)."""
+    and as the reported CPU baseline), with the GPU run's V(nu1, nu2) per
+    solver."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
     import oracle_lib
     from vdropimpact_amd.params import drop_impact_params
@@ -55,6 +56,8 @@ def cpu_baseline(m, n, seconds, warmup):
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
     p = drop_impact_params(m, n)
     s = oracle_lib.OracleSim(p, rel_tol=1e-10)
+    for k, nu in enumerate(schedules):
+        s.set_schedule(k, *nu)
     s.init_fields()
     for _ in range(warmup):
         s.step_forward(p.dt_reg)
@@ -69,7 +72,7 @@ def cpu_baseline(m, n, seconds, warmup):
     s.close()
     return {"value": k / el, "unit": "steps/s", "cores": threads, "kind": "port",
             "sample": f"{k} steps of the {m}x{n} workload after {warmup} warm-up steps, {el:.1f} s, "
-                      f"OMP_NUM_THREADS={threads}"}
+                      f"OMP_NUM_THREADS={threads}, V(nu1,nu2) per solver {schedules}"}
 
 
 def operator_level(vdi, reps=3):
@@ -144,13 +147,14 @@ def run_plain(vdi, args, p):
     if args.serial_visco:
         sim.set_concurrency(False)
     sim.init_fields()
+    schedules = [list(sim.schedule(k)) for k in (vdi.MAC, vdi.FEM, vdi.VX, vdi.VY)]
     for _ in range(args.warmup):
         sim.step_forward(p.dt_reg)
     if args.autotune > 0:
         sim.autotune(args.autotune)
         for _ in range(3):
             sim.step_forward(p.dt_reg)
-    log(f"plain: {args.warmup} warm-up steps, tiles tuned")
+    log(f"plain: {args.warmup} warm-up steps, tiles tuned, schedules {schedules}")
     # Which kernel class dominates the step: every launch of one class is
     # timed by the kernels' own wall-clock stamps over a few steps.
     class_ms, class_n = {}, {}
@@ -192,6 +196,7 @@ def run_plain(vdi, args, p):
         "avg_vcycles": cyc,
         "captured_steps": g1 - g0, "redone_steps": r1 - r0,
         "height_passes_last_step": height_passes,
+        "schedules": schedules,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM, "traffic": traffic,
                      "kernel": vdi.KCLASS_NAMES[dominant], "launches": launches,
@@ -351,6 +356,7 @@ def main():
                    "avg_vcycles": plain["avg_vcycles"],
                    "captured_steps": plain["captured_steps"], "redone_steps": plain["redone_steps"],
                    "gas_height_passes_last_step": plain["height_passes_last_step"],
+                   "vcycle_schedule_mac_fem_vx_vy": plain["schedules"],
                    "concurrent_branches": not args.serial_visco,
                    "steps_per_s_with_kernel_stamps": plain["steps_per_s_stamped"]},
         "roofline": plain["roofline"],
@@ -367,7 +373,7 @@ def main():
         out["operator_level"] = operator_level(vdi)
     if not args.no_cpu:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(args.m, args.n, args.cpu_seconds, 3)
+        out["cpu_baseline"] = cpu_baseline(args.m, args.n, args.cpu_seconds, 3, plain["schedules"])
     print(json.dumps(out), flush=True)
 
 

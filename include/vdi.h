@@ -104,10 +104,18 @@ const char *vdi_last_error(void);
 int vdi_create(const vdi_params *p, vdi_ctx **out);
 /* fluid_2d::~fluid_2d (fluid_2d.cc:27-36) */
 int vdi_destroy(vdi_ctx *ctx);
-/* Solver schedule and stopping rule: nu1/nu2 smoothing sweeps, relative
- * residual tolerance ||b-Az||/||b||, V-cycle cap (replaces the TGMG
- * constants and tgmg_accuracy, mgs_mac.cc:11, mgs_fem.cc:11, visco_impl.cc:13). */
+/* Solver schedule and stopping rule for all four solvers: nu1/nu2 smoothing
+ * sweeps (nu1 = nu2 = 0 keeps each solver's own), relative residual
+ * tolerance ||b-Az||/||b||, V-cycle cap (replaces the TGMG constants and
+ * tgmg_accuracy, mgs_mac.cc:11, mgs_fem.cc:11, visco_impl.cc:13). */
 int vdi_set_solver(vdi_ctx *ctx, int nu1, int nu2, double rel_tol, int max_cycles);
+/* nu1/nu2 of one solver (VDI_MAC..VDI_VY) only.  No single reference
+ * counterpart: each reference solver has its own TGMG instance and constants
+ * (mgs_mac.cc:13-14, mgs_fem.cc:13-14, visco_impl.cc:13). */
+int vdi_set_schedule(vdi_ctx *ctx, int which_solver, int nu1, int nu2);
+/* The schedule in use (defaults: V(3,2) for MAC and FEM, V(4,2) for the
+ * viscous solves). */
+int vdi_get_schedule(vdi_ctx *ctx, int which_solver, int *nu1, int *nu2);
 /* Arithmetic form of the V-cycles (VDI_FORM_*). */
 int vdi_set_form(vdi_ctx *ctx, int form);
 /* Capture steps in HIP graphs (default on): after the first two steps each

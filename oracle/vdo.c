@@ -638,6 +638,14 @@ void vdo_set_mg(vdo_sim *s, int nu1, int nu2, double rel_tol, int max_cycles) {
     s->rel_tol = rel_tol; s->max_cycles = max_cycles;
 }
 
+int vdo_set_schedule(vdo_sim *s, int which, int nu1, int nu2) {
+    vdo_mg *gs[4] = {s->mac, s->fem, s->vx, s->vy};
+    if (which < 0 || which > 3 || !gs[which] || nu1 < 0 || nu2 < 0 || nu1 + nu2 == 0) return 1;
+    gs[which]->nu1 = nu1;
+    gs[which]->nu2 = nu2;
+    return 0;
+}
+
 double vdo_time(const vdo_sim *s) { return s->time; }
 double vdo_vframe(const vdo_sim *s) { return s->Vframe; }
 int vdo_set_time(vdo_sim *s, double t) { s->time = t; return 0; }

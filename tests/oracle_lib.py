@@ -40,6 +40,8 @@ def lib():
         L.vdo_create.argtypes = [ctypes.c_void_p]
         L.vdo_destroy.argtypes = [vp]
         L.vdo_set_mg.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int]
+        L.vdo_set_schedule.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.vdo_set_schedule.restype = ctypes.c_int
         L.vdo_init_fields.argtypes = [vp]
         L.vdo_step_forward.argtypes = [vp, ctypes.c_double]
         L.vdo_gas_update.argtypes = [vp, ctypes.c_double, ctypes.c_double]
@@ -93,6 +95,10 @@ class OracleSim:
         self.m, self.n = params.m, params.n
         self.h = lib().vdo_create(ctypes.byref(params))
         lib().vdo_set_mg(self.h, nu1, nu2, rel_tol, max_cycles)
+
+    def set_schedule(self, which, nu1, nu2):
+        if lib().vdo_set_schedule(self.h, which, nu1, nu2) != 0:
+            raise ValueError("vdo_set_schedule")
 
     def close(self):
         if self.h:

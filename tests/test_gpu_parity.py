@@ -140,6 +140,11 @@ def make_pair(vdi, oracle, cfg, tol=1e-13):
     p = cfg.params()
     gpu = vdi.Fluid2D(p, rel_tol=tol)
     orc = oracle.OracleSim(p, rel_tol=tol)
+    for k in range(4):   # the oracle runs the step's V(nu1, nu2) per solver
+        try:
+            orc.set_schedule(k, *gpu.schedule(k))
+        except vdi.VdiError:
+            pass         # no viscous solvers (explicit viscosity)
     gpu.init_fields()
     orc.init_fields()
     return p, gpu, orc

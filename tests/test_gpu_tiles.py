@@ -72,3 +72,26 @@ def test_gpu_autotuned_context_same_steps(vdi):
         assert np.linalg.norm(x - y) <= 1e-9 * max(np.linalg.norm(y), 1e-30)
     a.close()
     c.close()
+
+
+def test_gpu_step_schedules(vdi):
+    """Per-solver V(nu1, nu2): the step's defaults, a larger schedule after
+    tuning (tiles whose halo no longer fits LDS move to a default shape), and
+    the argument check."""
+    from vdropimpact_amd.params import PhysicalConfig
+    p = PhysicalConfig(m=256, n=128).params()
+    a = vdi.Fluid2D(p)
+    assert [a.schedule(k) for k in (vdi.MAC, vdi.FEM, vdi.VX, vdi.VY)] == [(3, 2), (3, 2), (4, 2), (4, 2)]
+    a.init_fields()
+    for _ in range(3):
+        a.step_forward(p.dt_reg)
+    a.autotune(3)
+    for k in (vdi.MAC, vdi.FEM, vdi.VX, vdi.VY):
+        a.set_schedule(k, 4, 4)
+        assert a.schedule(k) == (4, 4)
+    for _ in range(3):
+        a.step_forward(p.dt_reg)
+    for bad in [(5, 2), (0, 0), (-1, 2)]:
+        with pytest.raises(vdi.VdiError):
+            a.set_schedule(vdi.MAC, *bad)
+    a.close()

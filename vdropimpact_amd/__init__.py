@@ -64,6 +64,8 @@ ABI = {
     "vdi_create": (_i, [ctypes.POINTER(VdiParams), ctypes.POINTER(_vp)]),
     "vdi_destroy": (_i, [_vp]),
     "vdi_set_solver": (_i, [_vp, _i, _i, _d, _i]),
+    "vdi_set_schedule": (_i, [_vp, _i, _i, _i]),
+    "vdi_get_schedule": (_i, [_vp, _i, _ip, _ip]),
     "vdi_set_form": (_i, [_vp, _i]),
     "vdi_set_graphs": (_i, [_vp, _i]),
     "vdi_set_concurrency": (_i, [_vp, _i]),
@@ -172,7 +174,7 @@ def _ptr(a: np.ndarray):
 class Fluid2D:
     """GPU-resident drop-impact liquid solve (the reference's fluid_2d)."""
 
-    def __init__(self, params: VdiParams, nu1: int = 2, nu2: int = 2, rel_tol: float = 1e-10,
+    def __init__(self, params: VdiParams, nu1: int = 0, nu2: int = 0, rel_tol: float = 1e-10,
                  max_cycles: int = 200, _handle=None):
         self.lib = load_library()
         self.params = params
@@ -204,8 +206,18 @@ class Fluid2D:
     def __exit__(self, *exc):
         self.close()
 
-    def set_solver(self, nu1=2, nu2=2, rel_tol=1e-10, max_cycles=200):
+    def set_solver(self, nu1=0, nu2=0, rel_tol=1e-10, max_cycles=200):
+        """Tolerance, V-cycle cap and (unless both are 0) one V(nu1, nu2) for every solver."""
         _check(self.lib.vdi_set_solver(self.h, nu1, nu2, rel_tol, max_cycles), "vdi_set_solver")
+
+    def set_schedule(self, which: int, nu1: int, nu2: int):
+        """Smoothing sweeps of one solver (MAC, FEM, VX or VY)."""
+        _check(self.lib.vdi_set_schedule(self.h, which, nu1, nu2), "vdi_set_schedule")
+
+    def schedule(self, which: int) -> tuple:
+        a, b = ctypes.c_int(), ctypes.c_int()
+        _check(self.lib.vdi_get_schedule(self.h, which, ctypes.byref(a), ctypes.byref(b)), "vdi_get_schedule")
+        return a.value, b.value
 
     def set_form(self, form: int):
         _check(self.lib.vdi_set_form(self.h, form), "vdi_set_form")

@@ -1979,6 +1979,32 @@ int MG::set_tile(int prec, int which, int level, int shape) {
     return VDI_OK;
 }
 
+int MG::set_schedule(int n1, int n2) {
+    if (n1 < 0 || n2 < 0 || n1 + n2 == 0 || n1 > kMaxNu || n2 > kMaxNu) return VDI_ERR_ARG;
+    nu1 = n1;
+    nu2 = n2;
+    const int fallback[] = {0, 4, 2, 1};
+    for (int prec = 0; prec < 2; prec++)
+        for (int which = 0; which < 2; which++)
+            for (int l = 0; l < (int)tile[prec][which].size(); l++) {
+                auto need = [&]() { return which == 0 ? down_lds(prec, l) : up_lds(prec, l); };
+                if (need() <= (size_t)mgk::max_lds()) continue;
+                for (int s : fallback) {
+                    tile[prec][which][l] = s;
+                    if (need() <= (size_t)mgk::max_lds()) break;
+                }
+                if (need() > (size_t)mgk::max_lds()) return VDI_ERR_ARG;
+            }
+    if (mx_lds() > (size_t)mgk::max_lds()) {
+        for (int s : fallback) {
+            tile_mx = s;
+            if (mx_lds() <= (size_t)mgk::max_lds()) break;
+        }
+        if (mx_lds() > (size_t)mgk::max_lds()) return VDI_ERR_ARG;
+    }
+    return VDI_OK;
+}
+
 // Times reps launches of one kernel (after one untimed launch) with events.
 template <class F>
 static int time_launches(hipStream_t st, int reps, F &&launch, float *ms) {

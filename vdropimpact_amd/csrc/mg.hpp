@@ -24,7 +24,7 @@
 namespace vdi {
 
 constexpr int kMaxCoarse = 12;  // levels handled by the single-workgroup kernel
-constexpr int kMaxNu = 3;       // largest nu1/nu2 the tiled kernels accept
+constexpr int kMaxNu = 4;       // largest nu1/nu2 the tiled kernels accept
 // Level-kernel workgroups: 32 x 16 = 512 threads (eight waves).  With the
 // 64 x 32 tiles of the finest level there is about one tile per CU, and 512
 // threads instead of 256 made the cell-centred V-cycle at 1024 x 512 6 %
@@ -268,6 +268,9 @@ class MG {
     // Sets one tile choice: prec 0 FP64 / 1 FP32, which 0 down / 1 up /
     // 2 mixed level-0 up; shape indexes kTileShapes.
     int set_tile(int prec, int which, int level, int shape);
+    // Smoothing sweeps per level (0..kMaxNu, not both 0).  A tile shape whose
+    // halo no longer fits LDS moves to the largest default shape that does.
+    int set_schedule(int nu1, int nu2);
     // LDS bytes of a launch.
     size_t down_lds(int prec, int l) const;
     size_t up_lds(int prec, int l) const;

@@ -1051,6 +1051,11 @@ int vdi_create(const vdi_params *p, vdi_ctx **out) {
         VDI_TRY(c->vx.init(VDI_VX, g.m, g.n, 0, 0, g.x_sym, c->zvx, c->F.src, c->st));
         VDI_TRY(c->vy.init(VDI_VY, g.m, g.n, 0, 0, g.x_sym, c->zvy, c->F.src2, c->st));
     }
+    {
+        MG *gs[4] = {&c->mac, &c->fem, &c->vx, &c->vy};
+        for (int k = 0; k < 4; k++)
+            if (gs[k]->nl) VDI_TRY(gs[k]->set_schedule(kStepNu[k][0], kStepNu[k][1]));
+    }
     VDI_TRY(c->snap.alloc((3 * NP + ng) * sizeof(double)));
     VDI_TRY(c->coef_dev.alloc(8 * sizeof(double)));
     VDI_TRY(c->rb_dev.alloc(sizeof(vdi_ctx::Readback)));
@@ -1076,16 +1081,36 @@ int vdi_destroy(vdi_ctx *ctx) {
 }
 
 int vdi_set_solver(vdi_ctx *c, int nu1, int nu2, double rel_tol, int max_cycles) {
-    if (!c || nu1 < 0 || nu2 < 0 || nu1 + nu2 == 0 || nu1 > kMaxNu || nu2 > kMaxNu || !(rel_tol > 0) ||
-        max_cycles < 1)
+    const bool keep = nu1 == 0 && nu2 == 0;
+    if (!c || (!keep && (nu1 < 0 || nu2 < 0 || nu1 > kMaxNu || nu2 > kMaxNu)) || !(rel_tol > 0) || max_cycles < 1)
         return VDI_ERR_ARG;
     VDI_HIP(hipStreamSynchronize(c->st));
     c->drop_graphs();
     MG *gs[4] = {&c->mac, &c->fem, &c->vx, &c->vy};
-    for (MG *g : gs) { g->nu1 = nu1; g->nu2 = nu2; }
+    for (MG *g : gs)
+        if (g->nl && !keep) VDI_TRY(g->set_schedule(nu1, nu2));
     c->rel_tol = rel_tol;
     c->max_cycles = max_cycles;
     return VDI_OK;
+}
+
+int vdi_get_schedule(vdi_ctx *c, int which_solver, int *nu1, int *nu2) {
+    if (!c || !nu1 || !nu2 || which_solver < VDI_MAC || which_solver > VDI_VY) return VDI_ERR_ARG;
+    MG *gs[4] = {&c->mac, &c->fem, &c->vx, &c->vy};
+    if (!gs[which_solver]->nl) return VDI_ERR_ARG;
+    *nu1 = gs[which_solver]->nu1;
+    *nu2 = gs[which_solver]->nu2;
+    return VDI_OK;
+}
+
+int vdi_set_schedule(vdi_ctx *c, int which_solver, int nu1, int nu2) {
+    if (!c || which_solver < VDI_MAC || which_solver > VDI_VY) return VDI_ERR_ARG;
+    MG *gs[4] = {&c->mac, &c->fem, &c->vx, &c->vy};
+    MG &g = *gs[which_solver];
+    if (!g.nl) return VDI_ERR_ARG;
+    VDI_HIP(hipStreamSynchronize(c->st));
+    c->drop_graphs();
+    return g.set_schedule(nu1, nu2);
 }
 
 int vdi_set_form(vdi_ctx *c, int form) {
@@ -1700,10 +1725,9 @@ int vdi_mg_destroy(vdi_mg *h) {
 }
 
 int vdi_mg_set_schedule(vdi_mg *h, int nu1, int nu2) {
-    if (!h || nu1 < 0 || nu2 < 0 || nu1 + nu2 == 0 || nu1 > kMaxNu || nu2 > kMaxNu) return VDI_ERR_ARG;
-    h->g.nu1 = nu1;
-    h->g.nu2 = nu2;
-    return VDI_OK;
+    if (!h) return VDI_ERR_ARG;
+    VDI_HIP(hipStreamSynchronize(h->st));
+    return h->g.set_schedule(nu1, nu2);
 }
 
 int vdi_mg_set_form(vdi_mg *h, int form) {

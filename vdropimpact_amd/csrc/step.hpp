@@ -61,6 +61,12 @@ struct StepRec {
 constexpr int kGX = 32, kGY = 8;
 constexpr int kEX = 64, kEY = 4;
 constexpr int kGasThreads = 1024;
+// Default V(nu1, nu2) of the step's solvers [MAC, FEM, VX, VY], measured at
+// the 1024 x 512 single-drop workload (steps/s with every solve to 1e-10):
+// V(2,2) everywhere 922, V(3,3) 1044-1054, these 1104-1116.  A level launch
+// runs all its sweeps in LDS, so a sweep more costs far less than the
+// V-cycle it saves; the Helmholtz solves then converge in one or two cycles.
+constexpr int kStepNu[4][2] = {{3, 2}, {3, 2}, {4, 2}, {4, 2}};
 constexpr int kGasRows = 5;  // rows per thread: me <= 5120
 // Height update by Jacobi passes (m <= kGasJacobiM: six LDS arrays of m
 // doubles); at most kGasJacobiPasses passes before the serial sweep.
