@@ -640,13 +640,17 @@ __device__ void gas_body(const StepRec &a, double *lds, double *red) {
         }
         double ressq = wg_sum(part, red) / me;  // quirk Q7: divided by me
         // parallel cyclic reduction (no pivoting; the Newton matrix is
-        // diagonally dominant)
+        // diagonally dominant).  The last row is the Dirichlet row pg = 0
+        // (a = c = 0, b = 1), which a reduction step leaves as it is, so only
+        // rows 0..me-2 are updated: one row per thread up to me = T + 1 (at
+        // m = 1024 the 1025th row had made the first wave do every step twice).
+        const int nr = me - 1;
         for (int s = 1; s < me; s <<= 1) {
             double na[kGasRows], nb[kGasRows], nc[kGasRows], nd[kGasRows];
 #pragma unroll
             for (int r = 0; r < kGasRows; r++) {
                 int i = tid + r * T;
-                if (i >= me) continue;
+                if (i >= nr) continue;
                 double ai = A[i], bi = B[i], ci = C[i], di = D[i];
                 double k1 = 0, k2 = 0, am = 0, cm = 0, dm = 0, ap = 0, cp = 0, dp = 0;
                 if (i - s >= 0) { k1 = ai / B[i - s]; am = A[i - s]; cm = C[i - s]; dm = D[i - s]; }
@@ -659,7 +663,7 @@ __device__ void gas_body(const StepRec &a, double *lds, double *red) {
 #pragma unroll
             for (int r = 0; r < kGasRows; r++) {
                 int i = tid + r * T;
-                if (i >= me) continue;
+                if (i >= nr) continue;
                 A[i] = na[r]; B[i] = nb[r]; C[i] = nc[r]; D[i] = nd[r];
             }
             __syncthreads();
@@ -1014,6 +1018,8 @@ int vdi_create(const vdi_params *p, vdi_ctx **out) {
     VDI_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     VDI_HIP(hipEventCreateWithFlags(&c->ev_gfork, hipEventDisableTiming));
     VDI_HIP(hipEventCreateWithFlags(&c->ev_gjoin, hipEventDisableTiming));
+    VDI_HIP(hipEventCreateWithFlags(&c->ev_god, hipEventDisableTiming));
+    VDI_HIP(hipEventCreateWithFlags(&c->ev_bcjoin, hipEventDisableTiming));
     size_t NP = (size_t)g.ml * (g.n + 4);
     VDI_TRY(c->fields.alloc(19 * NP * sizeof(double)));
     double *f = c->fields.as<double>();
@@ -1124,17 +1130,17 @@ int vdi_set_form(vdi_ctx *c, int form) {
 
 }  // extern "C"
 
-static int enqueue_viscous_bc(vdi_ctx *c, const StepRec &a) {
-    hipLaunchKernelGGL(k_viscous_bc, dim3(1), dim3(1024), 0, c->st, a);
+static int enqueue_viscous_bc(vdi_ctx *c, const StepRec &a, hipStream_t st = nullptr) {
+    hipLaunchKernelGGL(k_viscous_bc, dim3(1), dim3(1024), 0, st ? st : c->st, a);
     VDI_LAUNCHED();
     return VDI_OK;
 }
 
-static int enqueue_inviscid_bc(vdi_ctx *c, const StepRec &a) {
+static int enqueue_inviscid_bc(vdi_ctx *c, const StepRec &a, hipStream_t st = nullptr) {
     const int waves = c->g.n + c->g.ml;
     // reads p once; reads and writes two ghost cells of u and v per line
     Stamp *ts = c->timer.slot(VDI_KCLASS_SIMPSON, 8.0 * (c->g.me + 8.0 * waves));
-    hipLaunchKernelGGL(k_simpson, dim3(cdiv(waves, 4)), dim3(256), 0, c->st, a, ts);
+    hipLaunchKernelGGL(k_simpson, dim3(cdiv(waves, 4)), dim3(256), 0, st ? st : c->st, a, ts);
     VDI_LAUNCHED();
     return VDI_OK;
 }
@@ -1242,7 +1248,11 @@ static int enqueue_body(vdi_ctx *c, const StepRec &a, double t0, SolveMode mode,
     if (gas_branch) {
         // The gas update (st2) and the Godunov passes (st) share no data but
         // the gas pressure of the bottom MAC source row, which is added once
-        // both are done.
+        // both are done.  The boundary kernels that follow on st2 (far-field
+        // ghosts from the gas pressure, then the viscous ghosts) wait for
+        // Godunov, which reads the old ghosts, and run beside the MAC solve
+        // and the advective update: those read and write no ghost cell of u
+        // or v (only interior u, v, the faces, c0/c1, z and src).
         VDI_HIP(hipEventRecord(c->ev_gfork, c->st));
         VDI_HIP(hipStreamWaitEvent(c->st2, c->ev_gfork, 0));
         VDI_TRY(enqueue_gas(c, a, c->st2));
@@ -1251,6 +1261,11 @@ static int enqueue_body(vdi_ctx *c, const StepRec &a, double t0, SolveMode mode,
         ad.defer_sbc = 1;
         hipLaunchKernelGGL(k_godunov, dim3(cdiv(g.m, kGX), cdiv(g.n, kGY)), dim3(256), 0, c->st, ad);
         VDI_LAUNCHED();
+        VDI_HIP(hipEventRecord(c->ev_god, c->st));
+        VDI_HIP(hipStreamWaitEvent(c->st2, c->ev_god, 0));
+        VDI_TRY(enqueue_inviscid_bc(c, a, c->st2));
+        VDI_TRY(enqueue_viscous_bc(c, a, c->st2));
+        VDI_HIP(hipEventRecord(c->ev_bcjoin, c->st2));
         VDI_HIP(hipStreamWaitEvent(c->st, c->ev_gjoin, 0));
         hipLaunchKernelGGL(k_mac_bottom, dim3(cdiv(g.m, 256)), dim3(256), 0, c->st, a);
         VDI_LAUNCHED();
@@ -1261,8 +1276,12 @@ static int enqueue_body(vdi_ctx *c, const StepRec &a, double t0, SolveMode mode,
     VDI_TRY(body_solve(c, 0, half_time(t0, dt), mode));
     hipLaunchKernelGGL(k_correct_advect, gc, dim3(256), 0, c->st, a);
     VDI_LAUNCHED();
-    VDI_TRY(enqueue_inviscid_bc(c, a));
-    VDI_TRY(enqueue_viscous_bc(c, a));
+    if (gas_branch) {
+        VDI_HIP(hipStreamWaitEvent(c->st, c->ev_bcjoin, 0));
+    } else {
+        VDI_TRY(enqueue_inviscid_bc(c, a));
+        VDI_TRY(enqueue_viscous_bc(c, a));
+    }
     const double t1 = end_time(t0, dt);
     if (c->P.implicit_visc) {
         hipLaunchKernelGGL(k_visco_rhs, gc, dim3(256), 0, c->st, a, 0);

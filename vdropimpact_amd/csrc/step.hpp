@@ -124,6 +124,7 @@ struct vdi_ctx {
     // second branch is recorded on st2 between a fork and a join event.
     hipStream_t st2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_gfork = nullptr, ev_gjoin = nullptr;
+    hipEvent_t ev_god = nullptr, ev_bcjoin = nullptr;
     int par_visco = 1;
     double time = 0, Vframe = 0;
     double rel_tol = 1e-10;
@@ -198,6 +199,8 @@ struct vdi_ctx {
         if (ev_join) (void)hipEventDestroy(ev_join);
         if (ev_gfork) (void)hipEventDestroy(ev_gfork);
         if (ev_gjoin) (void)hipEventDestroy(ev_gjoin);
+        if (ev_god) (void)hipEventDestroy(ev_god);
+        if (ev_bcjoin) (void)hipEventDestroy(ev_bcjoin);
         if (st2) (void)hipStreamDestroy(st2);
         if (st) (void)hipStreamDestroy(st);
     }
