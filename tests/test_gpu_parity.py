@@ -108,6 +108,33 @@ def test_gpu_solve_reaches_tolerance_and_oracle(vdi, oracle, kind, m, n, cx, cy)
     g.close()
 
 
+@pytest.mark.parametrize("kind,m,n,cx,cy,form", [(0, 1024, 512, 1.0, 1.0, 0), (1, 1025, 513, 1.0, 1.0, 0),
+                                                  (0, 1024, 512, 1.0, 1.0, 1)])
+def test_gpu_folded_norms_same_cycles(vdi, kind, m, n, cx, cy, form):
+    """In a run of V-cycles with no host read between them, each cycle's
+    ||b - A z|| is finished by the next cycle's level-0 down kernel instead of
+    a one-workgroup launch.  A solve whose predicted run covers every cycle
+    (the second solve of a system) gives the iterate of plain V-cycles and the
+    cycle count of the unfolded first solve, bit for bit."""
+    say("folded norms", kind, m, n, form)
+    rng = np.random.default_rng(3 + kind)
+    b = rng.standard_normal(m * n)
+    g = vdi.MGSystem(kind, m, n, cx, cy, 1)
+    g.set_form(form)
+    g.set_b(b)
+    g.set_z(np.zeros(m * n))
+    c1 = g.solve(1e-10, 100)       # predicted run of one cycle, then one at a time
+    z1 = g.get_z()
+    g.set_z(np.zeros(m * n))
+    c2 = g.solve(1e-10, 100)       # predicted run of c1 cycles: norms folded
+    assert c2 == c1 and c1 >= 3
+    np.testing.assert_array_equal(g.get_z(), z1)
+    g.set_z(np.zeros(m * n))
+    g.vcycles(c1)
+    np.testing.assert_array_equal(g.get_z(), z1)
+    g.close()
+
+
 def test_gpu_mg_closed_form_solution(vdi):
     """A single smooth eigenmode: the FP64 residual floor is about
     eps * lam_max / lam_mode ~ 3e-13 here, so solve to 1e-12."""

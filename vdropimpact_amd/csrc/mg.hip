@@ -471,13 +471,56 @@ __device__ __forceinline__ void finish_norm_single(const MGCtl &ctl, double part
 // cycles of the mixed form).  sh/last: workgroup scratch of the reduction.
 // ------------------------------------------------------------------------
 
+// The previous V-cycle's norm, finished by every workgroup of the next
+// level-0 down launch (LvArgs::fin): threads 0..255 take the partials in
+// k_norm_finish's order (thread t: t, t + 256, ...) and the workgroup sum is
+// k_norm_finish's, so every workgroup gets the bits that kernel computes;
+// workgroup 0 makes its state update.  pf holds the loads, issued before the
+// tile's own so that the two round trips overlap.  Returns false when the
+// solve stops here (all workgroups agree; one that already sees workgroup
+// 0's new flag stops too, which is what it would have computed).
+constexpr int kFinRegs = kFinMax / 256;
+
+__device__ __forceinline__ void fin_load(const MGCtl &ctl, int np, double *pf) {
+    const int tid = threadIdx.x + kBX * threadIdx.y;
+#pragma unroll
+    for (int u = 0; u < kFinRegs; u++) {
+        const int k = tid + 256 * u;
+        pf[u] = (tid < 256 && k < np) ? ctl.partial[k] : 0.0;
+    }
+}
+
+__device__ __forceinline__ bool fin_norm(const MGCtl &ctl, int np, double tol2, const double *pf, double *sh) {
+    const int tid = threadIdx.x + kBX * threadIdx.y;
+    MGState *st = ctl.state;
+    const double bb = st->bb;
+    double r = 0;
+#pragma unroll
+    for (int u = 0; u < kFinRegs; u++)
+        if (tid < 256 && tid + 256 * u < np) r += pf[u];
+    r = block_sum(r, sh);
+    if (tid == 0) sh[8] = r;
+    __syncthreads();
+    const double R = sh[8];
+    const int nd = !(R == R) ? 2 : (R <= tol2 * bb ? 1 : 0);
+    if (tid == 0 && blockIdx.x == 0 && !st->done) {
+        st->rr = R;
+        st->cycles += 1;
+        st->done = nd;
+    }
+    return nd == 0;
+}
+
 template <class T, bool FEM>
-__device__ __forceinline__ void down_body(const LvArgs<T> &a, int tile, T *lds) {
+__device__ __forceinline__ void down_body(const LvArgs<T> &a, int tile, T *lds, double *sh) {
     const LevelT<T> &F = a.F, &C = a.C;
     const Tile t = make_tile(tile % a.nbx, tile / a.nbx, FEM ? 4 * a.nu + 2 : 2 * a.nu + 1, a.tx, a.ty);
     T *sz = lds, *sb = lds + t.RX * t.RY, *itab = sb + t.RX * t.RY;
+    double pf[kFinRegs];
+    if (a.fin) fin_load(a.ctl, a.fin, pf);
     tile_load<T, FEM>(F, a.flag ? nullptr : F.z, nullptr, a.cop, sz, sb, t);
     if (!FEM) cc_inv_table<T>(F, a.cop, itab, threadIdx.x + kBX * threadIdx.y);
+    if (a.fin && !fin_norm(a.ctl, a.fin, a.tol2, pf, sh)) return;
     __syncthreads();
     if (FEM) {
         for (int s = 1; s <= 4 * a.nu; s++) {
@@ -641,22 +684,24 @@ __device__ __forceinline__ void mx_body(const MxArgs &a, int tile, double *lds_d
 template <class T, bool FEM>
 __global__ void __launch_bounds__(kLvThreads) k_down(LvArgs<T> a) {
     extern __shared__ double lds_d[];
+    __shared__ double sh[9];
     if (a.ctl.state->done) return;
     stamp_begin(a.ts);
-    down_body<T, FEM>(a, xcd_tile(blockIdx.x, a.nb), reinterpret_cast<T *>(lds_d));
+    down_body<T, FEM>(a, xcd_tile(blockIdx.x, a.nb), reinterpret_cast<T *>(lds_d), sh);
     stamp_end(a.ts);
 }
 
 template <class T, bool FEM>
 __global__ void __launch_bounds__(kLvThreads) k_down_b(const LvArgs<T> *A, const WorkItem *W, int nw, BatchAux aux) {
     extern __shared__ double lds_d[];
+    __shared__ double sh[9];  // batched records never fold a norm (fin = 0)
     stamp_begin(aux.ts);
     for (int p = blockIdx.x; p < nw; p += gridDim.x) {
         const WorkItem w = W[xcd_tile(p, nw)];
         const LvArgs<T> a = A[w.rec];
         if (a.ctl.state->done) continue;
         const unsigned long long t0 = busy_begin(aux.busy);
-        down_body<T, FEM>(a, w.tile, reinterpret_cast<T *>(lds_d));
+        down_body<T, FEM>(a, w.tile, reinterpret_cast<T *>(lds_d), sh);
         __syncthreads();
         busy_end(aux.busy, w.sim, t0);
     }
@@ -1769,9 +1814,11 @@ double MG::mx_bytes() const {
 
 // Enqueues the down kernel of level l of hierarchy H (FP64 or FP32).
 template <class T>
-int MG::launch_down(const std::vector<LevelT<T>> &H, int l, int zero_z) {
+int MG::launch_down(const std::vector<LevelT<T>> &H, int l, int zero_z, int fin, double tol2) {
     const int prec = sizeof(T) == 4;
     LvArgs<T> a = down_args<T>(H, l, zero_z);
+    a.fin = fin;
+    a.tol2 = tol2;
     if (timer) a.ts = timer->slot(l == 0 ? VDI_KCLASS_MG_DOWN0 : VDI_KCLASS_MG_MID, down_bytes(prec, l, zero_z));
     return mgk::down<T>(a, fem, down_lds(prec, l), st);
 }
@@ -1796,20 +1843,33 @@ int MG::launch_vec(const VecArgs &a) { return mgk::vec(a, st); }
 
 bool MG::mixed_active() const { return form == VDI_FORM_MIXED && lc > 0; }
 
-int MG::launch_cycle(double tol2) {
+int MG::launch_cycle(double tol2) { return launch_cycle(tol2, 0, 1); }
+
+bool MG::can_fold_norm() const {
+    if (lc <= 0) return false;
+    const TileShape &t = kTileShapes[mixed_active() ? tile_mx : tile[0][1][0]];
+    return (long)cdiv(lm[0], t.tx) * cdiv(ln[0], t.ty) <= kFinMax;
+}
+
+int MG::launch_cycle(double tol2, int fin_prev, int norm) {
+    // partials the level-0 down kernel finishes for the previous cycle
+    const int fin = fin_prev && can_fold_norm() ? norm_args(0, tol2).nb : 0;
     if (!mixed_active()) {
         // coarser levels hold corrections, which start from zero
-        for (int l = 0; l < lc; l++) VDI_TRY(launch_down<double>(L, l, l > 0));
+        for (int l = 0; l < lc; l++) VDI_TRY(launch_down<double>(L, l, l > 0, l == 0 ? fin : 0, tol2));
         VDI_TRY(launch_coarse<double>(L, tol2));
         for (int l = lc - 1; l >= 0; l--) VDI_TRY(launch_up<double>(L, l, l == 0, tol2));
-        // the level-0 norm is reduced by its own one-workgroup kernel (the
-        // coarse kernel reduces it itself when it owns level 0)
-        if (lc > 0) VDI_TRY(mgk::norm(norm_args(0, tol2), st));
+        // the level-0 norm is reduced by its own one-workgroup kernel, or by
+        // the next cycle's level-0 down kernel (the coarse kernel reduces it
+        // itself when it owns level 0)
+        if (lc > 0 && (norm || !can_fold_norm())) VDI_TRY(mgk::norm(norm_args(0, tol2), st));
         return VDI_OK;
     }
     const int par = mx_enqueued_ & 1;
     {
         LvArgs<float> a = mx_down0_args(par);
+        a.fin = fin;
+        a.tol2 = tol2;
         if (timer) a.ts = timer->slot(VDI_KCLASS_MG_DOWN0, down_bytes(1, 0, 1));
         VDI_TRY(mgk::down<float>(a, fem, down_lds(1, 0), st));
     }
@@ -1819,7 +1879,7 @@ int MG::launch_cycle(double tol2) {
     MxArgs a = mx_args(par, tol2);
     if (timer) a.ts = timer->slot(VDI_KCLASS_MG_UP0, mx_bytes());
     VDI_TRY(mgk::mx(a, fem, mx_lds(), st));
-    VDI_TRY(mgk::norm(norm_args(0, tol2), st));
+    if (norm || !can_fold_norm()) VDI_TRY(mgk::norm(norm_args(0, tol2), st));
     mx_enqueued_++;
     return VDI_OK;
 }
@@ -1889,7 +1949,8 @@ int MG::solve(double tol, int max_cycles, int *cycles) {
     const double tol2 = tol * tol;
     VDI_TRY(begin(tol));
     const int pred = std::max(1, std::min(max_cycles, last_cycles));
-    for (int c = 0; c < pred; c++) VDI_TRY(launch_cycle(tol2));
+    // no host read inside this run: each cycle's norm is finished by the next
+    for (int c = 0; c < pred; c++) VDI_TRY(launch_cycle(tol2, c > 0, c == pred - 1));
     MGState s;
     VDI_TRY(read_state(&s));
     while (s.done == 0) {
@@ -1950,7 +2011,9 @@ int MG::enqueue_extrapolate(const double *coef) { return launch_vec(vec_args(VEC
 
 int MG::enqueue_fixed(double tol, int budget) {
     VDI_TRY(begin(tol));
-    for (int c = 0; c < budget; c++) VDI_TRY(launch_cycle(tol * tol));
+    // each cycle's norm is finished by the next cycle's level-0 down kernel,
+    // the last one by its own launch
+    for (int c = 0; c < budget; c++) VDI_TRY(launch_cycle(tol * tol, c > 0, c == budget - 1));
     return launch_vec(vec_args(VEC_SOLVE_END, nullptr));
 }
 

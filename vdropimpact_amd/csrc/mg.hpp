@@ -83,7 +83,12 @@ struct LvArgs {
     MGCtl ctl;
     double tol2;
     Stamp *ts;
+    // level-0 down: number of ||b - A z||^2 partials of the previous V-cycle
+    // to finish (sum, convergence test, state update) before smoothing, in
+    // place of a k_norm_finish launch; 0: none (at most kFinMax)
+    int fin;
 };
+constexpr int kFinMax = 1024;
 
 // Mixed form, level 0 up kernel (see k_mx_up0).
 struct MxArgs {
@@ -233,6 +238,11 @@ class MG {
     // Enqueues one V-cycle; kernels return at once when the device flag says
     // the solve has converged (tol2 < 0: never converge).
     int launch_cycle(double tol2);
+    // fin_prev: the level-0 down kernel finishes the previous cycle's norm
+    // (which that cycle did not launch, norm = 0); only when lc > 0.
+    int launch_cycle(double tol2, int fin_prev, int norm);
+    // Whether a run of cycles can fold each cycle's norm into the next one.
+    bool can_fold_norm() const;
     // Mixed form in use (needs at least one tiled level).
     bool mixed_active() const;
     int mixed_finish(int executed);
@@ -284,7 +294,8 @@ class MG {
     int dense_inverse();
 
   private:
-    template <class T> int launch_down(const std::vector<LevelT<T>> &H, int l, int zero_z);
+    template <class T> int launch_down(const std::vector<LevelT<T>> &H, int l, int zero_z, int fin = 0,
+                                       double tol2 = 0);
     template <class T> int launch_up(const std::vector<LevelT<T>> &H, int l, int norm, double tol2);
     template <class T> int launch_coarse(const std::vector<LevelT<T>> &H, double tol2);
     int launch_vec(const VecArgs &a);
