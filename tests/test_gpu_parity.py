@@ -111,11 +111,12 @@ def test_gpu_solve_reaches_tolerance_and_oracle(vdi, oracle, kind, m, n, cx, cy)
 @pytest.mark.parametrize("kind,m,n,cx,cy,form", [(0, 1024, 512, 1.0, 1.0, 0), (1, 1025, 513, 1.0, 1.0, 0),
                                                   (0, 1024, 512, 1.0, 1.0, 1)])
 def test_gpu_folded_norms_same_cycles(vdi, kind, m, n, cx, cy, form):
-    """In a run of V-cycles with no host read between them, each cycle's
-    ||b - A z|| is finished by the next cycle's level-0 down kernel instead of
-    a one-workgroup launch.  A solve whose predicted run covers every cycle
-    (the second solve of a system) gives the iterate of plain V-cycles and the
-    cycle count of the unfolded first solve, bit for bit."""
+    """In a run of V-cycles with no host read between them, the start's norms
+    and each cycle's ||b - A z|| are finished by the next level-0 down kernel
+    instead of a one-workgroup launch.  The first solve of a system runs one
+    predicted cycle and then one cycle per host check (norms launched); the
+    second runs all its cycles as one folded run.  Both give the cycle count
+    and the iterate of plain V-cycles, bit for bit."""
     say("folded norms", kind, m, n, form)
     rng = np.random.default_rng(3 + kind)
     b = rng.standard_normal(m * n)

@@ -481,31 +481,49 @@ __device__ __forceinline__ void finish_norm_single(const MGCtl &ctl, double part
 // 0's new flag stops too, which is what it would have computed).
 constexpr int kFinRegs = kFinMax / 256;
 
-__device__ __forceinline__ void fin_load(const MGCtl &ctl, int np, double *pf) {
+// With pairs (k_begin's partials: ||b - A z||^2, ||b||^2 per workgroup) the
+// same holds for k_norm_finish's begin branch: pf holds 2 kFinRegs values.
+__device__ __forceinline__ void fin_load(const MGCtl &ctl, int np, int pairs, double *pf) {
     const int tid = threadIdx.x + kBX * threadIdx.y;
 #pragma unroll
     for (int u = 0; u < kFinRegs; u++) {
         const int k = tid + 256 * u;
-        pf[u] = (tid < 256 && k < np) ? ctl.partial[k] : 0.0;
+        const bool in = tid < 256 && k < np;
+        pf[u] = in ? ctl.partial[pairs ? 2 * k : k] : 0.0;
+        pf[kFinRegs + u] = in && pairs ? ctl.partial[2 * k + 1] : 0.0;
     }
 }
 
-__device__ __forceinline__ bool fin_norm(const MGCtl &ctl, int np, double tol2, const double *pf, double *sh) {
+__device__ __forceinline__ bool fin_norm(const MGCtl &ctl, int np, int pairs, double tol2, const double *pf,
+                                         double *sh) {
     const int tid = threadIdx.x + kBX * threadIdx.y;
     MGState *st = ctl.state;
-    const double bb = st->bb;
-    double r = 0;
+    const double bb_state = pairs ? 0.0 : st->bb;
+    double r = 0, b = 0;
 #pragma unroll
     for (int u = 0; u < kFinRegs; u++)
-        if (tid < 256 && tid + 256 * u < np) r += pf[u];
+        if (tid < 256 && tid + 256 * u < np) {
+            r += pf[u];
+            b += pf[kFinRegs + u];
+        }
     r = block_sum(r, sh);
-    if (tid == 0) sh[8] = r;
+    if (pairs) b = block_sum(b, sh);
+    if (tid == 0) {
+        sh[8] = r;
+        sh[9] = b;
+    }
     __syncthreads();
-    const double R = sh[8];
-    const int nd = !(R == R) ? 2 : (R <= tol2 * bb ? 1 : 0);
+    const double R = sh[8], B = pairs ? sh[9] : bb_state;
+    const int nd = pairs ? ((B == 0) ? 3 : (!(R == R) ? 2 : (R <= tol2 * B ? 1 : 0)))
+                         : (!(R == R) ? 2 : (R <= tol2 * B ? 1 : 0));
     if (tid == 0 && blockIdx.x == 0 && !st->done) {
         st->rr = R;
-        st->cycles += 1;
+        if (pairs) {
+            st->bb = B;
+            st->cycles = 0;
+        } else {
+            st->cycles += 1;
+        }
         st->done = nd;
     }
     return nd == 0;
@@ -516,11 +534,11 @@ __device__ __forceinline__ void down_body(const LvArgs<T> &a, int tile, T *lds, 
     const LevelT<T> &F = a.F, &C = a.C;
     const Tile t = make_tile(tile % a.nbx, tile / a.nbx, FEM ? 4 * a.nu + 2 : 2 * a.nu + 1, a.tx, a.ty);
     T *sz = lds, *sb = lds + t.RX * t.RY, *itab = sb + t.RX * t.RY;
-    double pf[kFinRegs];
-    if (a.fin) fin_load(a.ctl, a.fin, pf);
+    double pf[2 * kFinRegs];
+    if (a.fin) fin_load(a.ctl, a.fin, a.fin_begin, pf);
     tile_load<T, FEM>(F, a.flag ? nullptr : F.z, nullptr, a.cop, sz, sb, t);
     if (!FEM) cc_inv_table<T>(F, a.cop, itab, threadIdx.x + kBX * threadIdx.y);
-    if (a.fin && !fin_norm(a.ctl, a.fin, a.tol2, pf, sh)) return;
+    if (a.fin && !fin_norm(a.ctl, a.fin, a.fin_begin, a.tol2, pf, sh)) return;
     __syncthreads();
     if (FEM) {
         for (int s = 1; s <= 4 * a.nu; s++) {
@@ -684,7 +702,7 @@ __device__ __forceinline__ void mx_body(const MxArgs &a, int tile, double *lds_d
 template <class T, bool FEM>
 __global__ void __launch_bounds__(kLvThreads) k_down(LvArgs<T> a) {
     extern __shared__ double lds_d[];
-    __shared__ double sh[9];
+    __shared__ double sh[10];
     if (a.ctl.state->done) return;
     stamp_begin(a.ts);
     down_body<T, FEM>(a, xcd_tile(blockIdx.x, a.nb), reinterpret_cast<T *>(lds_d), sh);
@@ -694,7 +712,7 @@ __global__ void __launch_bounds__(kLvThreads) k_down(LvArgs<T> a) {
 template <class T, bool FEM>
 __global__ void __launch_bounds__(kLvThreads) k_down_b(const LvArgs<T> *A, const WorkItem *W, int nw, BatchAux aux) {
     extern __shared__ double lds_d[];
-    __shared__ double sh[9];  // batched records never fold a norm (fin = 0)
+    __shared__ double sh[10];  // batched records never fold a norm (fin = 0)
     stamp_begin(aux.ts);
     for (int p = blockIdx.x; p < nw; p += gridDim.x) {
         const WorkItem w = W[xcd_tile(p, nw)];
@@ -1086,6 +1104,10 @@ __global__ void __launch_bounds__(256) k_begin(BeginArgs a) {
     __shared__ int last;
     stamp_begin(a.ts);
     begin_body(a, blockIdx.x, sh, &last);
+    if (a.reset && blockIdx.x == 0 && threadIdx.x == 0) {
+        a.ctl.state->done = 0;
+        a.ctl.state->cycles = 0;
+    }
     stamp_end(a.ts);
 }
 
@@ -1814,10 +1836,11 @@ double MG::mx_bytes() const {
 
 // Enqueues the down kernel of level l of hierarchy H (FP64 or FP32).
 template <class T>
-int MG::launch_down(const std::vector<LevelT<T>> &H, int l, int zero_z, int fin, double tol2) {
+int MG::launch_down(const std::vector<LevelT<T>> &H, int l, int zero_z, int fin, double tol2, int fin_begin) {
     const int prec = sizeof(T) == 4;
     LvArgs<T> a = down_args<T>(H, l, zero_z);
     a.fin = fin;
+    a.fin_begin = fin_begin;
     a.tol2 = tol2;
     if (timer) a.ts = timer->slot(l == 0 ? VDI_KCLASS_MG_DOWN0 : VDI_KCLASS_MG_MID, down_bytes(prec, l, zero_z));
     return mgk::down<T>(a, fem, down_lds(prec, l), st);
@@ -1851,12 +1874,23 @@ bool MG::can_fold_norm() const {
     return (long)cdiv(lm[0], t.tx) * cdiv(ln[0], t.ty) <= kFinMax;
 }
 
+int MG::begin_folded(double tol) {
+    if (!can_fold_norm()) return begin(tol);
+    VDI_TRY(enforce_dirichlet());
+    mx_enqueued_ = 0;
+    BeginArgs a = begin_args(tol * tol);
+    a.reset = 1;
+    return mgk::begin(a, st);
+}
+
 int MG::launch_cycle(double tol2, int fin_prev, int norm) {
-    // partials the level-0 down kernel finishes for the previous cycle
-    const int fin = fin_prev && can_fold_norm() ? norm_args(0, tol2).nb : 0;
+    // partials the level-0 down kernel finishes: the previous cycle's
+    // (fin_prev = 1) or k_begin's pairs (fin_prev = 2)
+    const int fin = !fin_prev || !can_fold_norm() ? 0 : (fin_prev == 2 ? norm_args(1, tol2).nb : norm_args(0, tol2).nb);
+    const int fin_begin = fin_prev == 2 && fin > 0;
     if (!mixed_active()) {
         // coarser levels hold corrections, which start from zero
-        for (int l = 0; l < lc; l++) VDI_TRY(launch_down<double>(L, l, l > 0, l == 0 ? fin : 0, tol2));
+        for (int l = 0; l < lc; l++) VDI_TRY(launch_down<double>(L, l, l > 0, l == 0 ? fin : 0, tol2, fin_begin));
         VDI_TRY(launch_coarse<double>(L, tol2));
         for (int l = lc - 1; l >= 0; l--) VDI_TRY(launch_up<double>(L, l, l == 0, tol2));
         // the level-0 norm is reduced by its own one-workgroup kernel, or by
@@ -1869,6 +1903,7 @@ int MG::launch_cycle(double tol2, int fin_prev, int norm) {
     {
         LvArgs<float> a = mx_down0_args(par);
         a.fin = fin;
+        a.fin_begin = fin_begin;
         a.tol2 = tol2;
         if (timer) a.ts = timer->slot(VDI_KCLASS_MG_DOWN0, down_bytes(1, 0, 1));
         VDI_TRY(mgk::down<float>(a, fem, down_lds(1, 0), st));
@@ -1947,10 +1982,11 @@ int MG::vcycles(int k) {
 
 int MG::solve(double tol, int max_cycles, int *cycles) {
     const double tol2 = tol * tol;
-    VDI_TRY(begin(tol));
+    VDI_TRY(begin_folded(tol));
     const int pred = std::max(1, std::min(max_cycles, last_cycles));
-    // no host read inside this run: each cycle's norm is finished by the next
-    for (int c = 0; c < pred; c++) VDI_TRY(launch_cycle(tol2, c > 0, c == pred - 1));
+    // no host read inside this run: the start's norms and each cycle's are
+    // finished by the next level-0 down kernel
+    for (int c = 0; c < pred; c++) VDI_TRY(launch_cycle(tol2, c > 0 ? 1 : 2, c == pred - 1));
     MGState s;
     VDI_TRY(read_state(&s));
     while (s.done == 0) {
@@ -2010,10 +2046,14 @@ int MG::guess_coeffs(double t, double *bb, double *c) {
 int MG::enqueue_extrapolate(const double *coef) { return launch_vec(vec_args(VEC_EXTRAP, coef)); }
 
 int MG::enqueue_fixed(double tol, int budget) {
-    VDI_TRY(begin(tol));
-    // each cycle's norm is finished by the next cycle's level-0 down kernel,
-    // the last one by its own launch
-    for (int c = 0; c < budget; c++) VDI_TRY(launch_cycle(tol * tol, c > 0, c == budget - 1));
+    // the start's norms and each cycle's are finished by the next level-0
+    // down kernel, the last cycle's by its own launch
+    if (budget < 1) {
+        VDI_TRY(begin(tol));
+        return launch_vec(vec_args(VEC_SOLVE_END, nullptr));
+    }
+    VDI_TRY(begin_folded(tol));
+    for (int c = 0; c < budget; c++) VDI_TRY(launch_cycle(tol * tol, c > 0 ? 1 : 2, c == budget - 1));
     return launch_vec(vec_args(VEC_SOLVE_END, nullptr));
 }
 

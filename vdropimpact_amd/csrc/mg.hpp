@@ -87,6 +87,8 @@ struct LvArgs {
     // to finish (sum, convergence test, state update) before smoothing, in
     // place of a k_norm_finish launch; 0: none (at most kFinMax)
     int fin;
+    // 1: those partials are k_begin's (||b - A z||^2, ||b||^2) pairs
+    int fin_begin;
 };
 constexpr int kFinMax = 1024;
 
@@ -139,6 +141,9 @@ struct BeginArgs {
     float *r32;  // mixed form: FP32 copy of the residual
     int nb;
     Stamp *ts;
+    // 1: workgroup 0 clears the flag and the cycle count (the first level-0
+    // down kernel finishes this launch's norms, LvArgs::fin_begin)
+    int reset;
 };
 
 // Element-wise operations on level-0 arrays.
@@ -241,6 +246,9 @@ class MG {
     // fin_prev: the level-0 down kernel finishes the previous cycle's norm
     // (which that cycle did not launch, norm = 0); only when lc > 0.
     int launch_cycle(double tol2, int fin_prev, int norm);
+    // begin() whose norms the first cycle's level-0 down kernel finishes
+    // (launch_cycle with fin_prev = 2); only when can_fold_norm().
+    int begin_folded(double tol);
     // Whether a run of cycles can fold each cycle's norm into the next one.
     bool can_fold_norm() const;
     // Mixed form in use (needs at least one tiled level).
@@ -295,7 +303,7 @@ class MG {
 
   private:
     template <class T> int launch_down(const std::vector<LevelT<T>> &H, int l, int zero_z, int fin = 0,
-                                       double tol2 = 0);
+                                       double tol2 = 0, int fin_begin = 0);
     template <class T> int launch_up(const std::vector<LevelT<T>> &H, int l, int norm, double tol2);
     template <class T> int launch_coarse(const std::vector<LevelT<T>> &H, double tol2);
     int launch_vec(const VecArgs &a);
