@@ -133,6 +133,11 @@ def test_gpu_folded_norms_same_cycles(vdi, kind, m, n, cx, cy, form):
     g.set_z(np.zeros(m * n))
     g.vcycles(c1)
     np.testing.assert_array_equal(g.get_z(), z1)
+    # zero right-hand side: the folded start finds it and the solution is 0
+    g.set_b(np.zeros(m * n))
+    g.set_z(rng.standard_normal(m * n))
+    assert g.solve(1e-10, 100) == 0
+    assert not np.any(g.get_z())
     g.close()
 
 

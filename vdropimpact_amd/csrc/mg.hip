@@ -494,8 +494,8 @@ __device__ __forceinline__ void fin_load(const MGCtl &ctl, int np, int pairs, do
     }
 }
 
-__device__ __forceinline__ bool fin_norm(const MGCtl &ctl, int np, int pairs, double tol2, const double *pf,
-                                         double *sh) {
+__device__ __forceinline__ int fin_norm(const MGCtl &ctl, int np, int pairs, double tol2, const double *pf,
+                                        double *sh) {
     const int tid = threadIdx.x + kBX * threadIdx.y;
     MGState *st = ctl.state;
     const double bb_state = pairs ? 0.0 : st->bb;
@@ -526,7 +526,7 @@ __device__ __forceinline__ bool fin_norm(const MGCtl &ctl, int np, int pairs, do
         }
         st->done = nd;
     }
-    return nd == 0;
+    return nd;
 }
 
 template <class T, bool FEM>
@@ -538,7 +538,17 @@ __device__ __forceinline__ void down_body(const LvArgs<T> &a, int tile, T *lds, 
     if (a.fin) fin_load(a.ctl, a.fin, a.fin_begin, pf);
     tile_load<T, FEM>(F, a.flag ? nullptr : F.z, nullptr, a.cop, sz, sb, t);
     if (!FEM) cc_inv_table<T>(F, a.cop, itab, threadIdx.x + kBX * threadIdx.y);
-    if (a.fin && !fin_norm(a.ctl, a.fin, a.fin_begin, a.tol2, pf, sh)) return;
+    if (a.fin) {
+        const int nd = fin_norm(a.ctl, a.fin, a.fin_begin, a.tol2, pf, sh);
+        if (nd == 3 && a.fin_begin && sizeof(T) == sizeof(double)) {
+            // zero right-hand side: the solution is z = 0 (each workgroup its
+            // tile), which an FP64 solve's end launch would otherwise write
+            for (int y = threadIdx.y; y < t.TY && t.y0 + y < F.n; y += kBY)
+                for (int x = threadIdx.x; x < t.TX && t.x0 + x < F.m; x += kBX)
+                    F.z[t.x0 + x + (size_t)F.m * (t.y0 + y)] = T(0);
+        }
+        if (nd) return;
+    }
     __syncthreads();
     if (FEM) {
         for (int s = 1; s <= 4 * a.nu; s++) {
@@ -2054,6 +2064,9 @@ int MG::enqueue_fixed(double tol, int budget) {
     }
     VDI_TRY(begin_folded(tol));
     for (int c = 0; c < budget; c++) VDI_TRY(launch_cycle(tol * tol, c > 0 ? 1 : 2, c == budget - 1));
+    // FP64 with folded norms: the first level-0 down kernel wrote z = 0 for a
+    // zero right-hand side, and there is no odd-cycle buffer to bring back
+    if (!mixed_active() && can_fold_norm()) return VDI_OK;
     return launch_vec(vec_args(VEC_SOLVE_END, nullptr));
 }
 
