@@ -1,7 +1,7 @@
 """GPU parity: the HIP path (through the C ABI) against the CPU FP64 oracle
 on the same inputs, and against closed forms.
 
-Tolerance: the north-star bar is <= 1e-9 relative L2 per field once every
+Tolerance: the parity bar is <= 1e-9 relative L2 per field once every
 solve has converged to the same residual; parity runs drive both sides'
 multigrid to a relative residual of 1e-13 so that the comparison measures
 the step, not the stopping point of two different iterations.
