@@ -858,18 +858,16 @@ __device__ __forceinline__ void coarse_body_t(const CoarseArgs<T> &a, T *lds, do
     const T *gb = reinterpret_cast<const T *>(d.gb);
     T *gz = reinterpret_cast<T *>(d.gz);
     const int ld = d.nl - 1, Nd = d.m[ld] * d.n[ld];
-    // zero every level (the borders stay zero), then load the first level;
-    // the levels' reciprocal tables meanwhile
-    for (int k = tid; k < d.red_off; k += Tn) lds[k] = T(0);
-    if (!FEM && tid < 9 * d.nl) cc_inv_table<T>(coarse_level<T>(d, tid / 9), a.cop, lds + coarse_tab_off(d) + 9 * (tid / 9), tid % 9);
-    __syncthreads();
+    // load the first level into the padded tiles; the first round of loads
+    // is issued before every level is cleared (the borders stay zero) and the
+    // reciprocal tables are written, so that its round trip overlaps them
     {
         const int m = d.m[0], N = m * d.n[0], RX = m + 2;
         T *z = lds + d.off[0], *b = z + RX * (d.n[0] + 2);
         Walk w(tid, m, Tn);
-        while (w.q < N) {
-            T zv[kLoadUnroll], bv[kLoadUnroll];
-            int li[kLoadUnroll];
+        T zv[kLoadUnroll], bv[kLoadUnroll];
+        int li[kLoadUnroll];
+        auto load_round = [&]() {
 #pragma unroll
             for (int u = 0; u < kLoadUnroll; u++) {
                 const int k = min(w.q, N - 1);
@@ -878,12 +876,24 @@ __device__ __forceinline__ void coarse_body_t(const CoarseArgs<T> &a, T *lds, do
                 zv[u] = d.top ? gz[k] : T(0);
                 w.next(Tn);
             }
+        };
+        auto store_round = [&]() {
 #pragma unroll
             for (int u = 0; u < kLoadUnroll; u++)
                 if (li[u] >= 0) {
                     b[li[u]] = bv[u];
                     z[li[u]] = zv[u];
                 }
+        };
+        load_round();
+        for (int k = tid; k < d.red_off; k += Tn) lds[k] = T(0);
+        if (!FEM && tid < 9 * d.nl)
+            cc_inv_table<T>(coarse_level<T>(d, tid / 9), a.cop, lds + coarse_tab_off(d) + 9 * (tid / 9), tid % 9);
+        __syncthreads();
+        store_round();
+        while (w.q < N) {
+            load_round();
+            store_round();
         }
     }
     __syncthreads();
