@@ -84,11 +84,16 @@ __device__ __forceinline__ void godunov_set_tang(double &u0, double &u1, double 
 // Only the final face states, the viscous force c0/c1 and src are written:
 // about 10 words of traffic per cell instead of the ~75 of one pass per
 // reference loop.  Boundary faces follow fluid_2d.cc:141-148 and :210-213.
-constexpr int kGW = kGX + 8, kGH = kGY + 8;  // LDS frame: tile + 4 on each side
+constexpr int kGW = kGX + 8, kGH = kGY + 8;  // LDS frame of u, v: tile + 4 on each side
+// Frames of the edge states (tile + 2, K1 on T+2) and of c0..c3 (tile + 1);
+// sized to what each array holds so that three workgroups fit a CU's LDS.
+constexpr int kEW = kGX + 4, kEH = kGY + 4, kCW = kGX + 2, kCH = kGY + 2;
 
 struct GTile {
     int x0, y0;
     __device__ int L(int i, int j) const { return (j - y0 + 4) * kGW + (i - x0 + 4); }
+    __device__ int E(int i, int j) const { return (j - y0 + 2) * kEW + (i - x0 + 2); }
+    __device__ int C(int i, int j) const { return (j - y0 + 1) * kCW + (i - x0 + 1); }
 };
 
 // Tangential Godunov select on face between left state a and right state b
@@ -106,8 +111,8 @@ __device__ void godunov_body(const StepRec &a, int bx, int by) {
     const double dt = a.dt, viscfac = a.viscfac, macfac = a.macfac;
     __shared__ double su[kGW * kGH], sv[kGW * kGH];
     // K1 edges, later K4 edges: ul, vl, ur, vr, ud, vd, uu, vu
-    __shared__ double se[8][kGW * kGH];
-    __shared__ double sc[4][kGW * kGH];         // c0..c3 of K3
+    __shared__ double se[8][kEW * kEH];
+    __shared__ double sc[4][kCW * kCH];         // c0..c3 of K3
     __shared__ double sp[(kGX + 3) * (kGY + 3)]; // nodal p on T+1 (+1 node)
     enum { UL, VL, UR, VR, UD, VD, UU, VU };
     const int tid = threadIdx.x;
@@ -135,20 +140,20 @@ __device__ void godunov_body(const StepRec &a, int bx, int by) {
     for (int k = tid; k < (kGX + 4) * (kGY + 4); k += 256) {
         const int i = t.x0 - 2 + k % (kGX + 4), j = t.y0 - 2 + k / (kGX + 4);
         if (i < ax0 || i >= ax1 || j < ay0 || j >= ay1) continue;
-        const int l = t.L(i, j);
+        const int l = t.L(i, j), e = t.E(i, j);
         const double fu = su[l], fv = sv[l];
         const double uy = g.ysp * mono_diff(su[l - 2 * kGW], su[l - kGW], fu, su[l + kGW], su[l + 2 * kGW]);
         const double vy = g.ysp * mono_diff(sv[l - 2 * kGW], sv[l - kGW], fv, sv[l + kGW], sv[l + 2 * kGW]);
         const double ux = g.xsp * mono_diff(su[l - 2], su[l - 1], fu, su[l + 1], su[l + 2]);
         const double vx = g.xsp * mono_diff(sv[l - 2], sv[l - 1], fv, sv[l + 1], sv[l + 2]);
-        se[UD][l] = fu - 0.5 * (g.dy + dt * fv) * uy;
-        se[VD][l] = fv - 0.5 * (g.dy + dt * fv) * vy;
-        se[UL][l] = (g.x_sym && i == 0) ? 0.0 : fu - 0.5 * (g.dx + dt * fu) * ux;
-        se[VL][l] = fv - 0.5 * (g.dx + dt * fu) * vx;
-        se[UR][l] = fu + 0.5 * (g.dx - dt * fu) * ux;
-        se[VR][l] = fv + 0.5 * (g.dx - dt * fu) * vx;
-        se[UU][l] = fu + 0.5 * (g.dy - dt * fv) * uy;
-        se[VU][l] = fv + 0.5 * (g.dy - dt * fv) * vy;
+        se[UD][e] = fu - 0.5 * (g.dy + dt * fv) * uy;
+        se[VD][e] = fv - 0.5 * (g.dy + dt * fv) * vy;
+        se[UL][e] = (g.x_sym && i == 0) ? 0.0 : fu - 0.5 * (g.dx + dt * fu) * ux;
+        se[VL][e] = fv - 0.5 * (g.dx + dt * fu) * vx;
+        se[UR][e] = fu + 0.5 * (g.dx - dt * fu) * ux;
+        se[VR][e] = fv + 0.5 * (g.dx - dt * fu) * vx;
+        se[UU][e] = fu + 0.5 * (g.dy - dt * fv) * uy;
+        se[VU][e] = fv + 0.5 * (g.dy - dt * fv) * vy;
     }
     __syncthreads();
     // post-K2 face states and K3 on T+1 (fluid_2d.cc:136-160)
@@ -157,7 +162,7 @@ __device__ void godunov_body(const StepRec &a, int bx, int by) {
     for (int k = tid; k < (kGX + 2) * (kGY + 2); k += 256) {
         const int i = t.x0 - 1 + k % (kGX + 2), j = t.y0 - 1 + k / (kGX + 2);
         if (i < bx0 || i >= bx1 || j < by0 || j >= by1) continue;
-        const int l = t.L(i, j);
+        const int l = t.E(i, j), lc = t.C(i, j);
         double ta, tb, tc;
         // x faces: left face (i-1/2) gives UR2(i-1), UL2(i), VL2(i); right face gives UR2(i), UL2(i+1), VL2(i+1)
         double urm, ul0, vl0, ur0, ulp, vlp;
@@ -168,21 +173,21 @@ __device__ void godunov_body(const StepRec &a, int bx, int by) {
         // y faces
         double vum, vd0, ud0, vu0, vdp, udp;
         if (j == 0) { vd0 = se[VD][l]; ud0 = se[UD][l]; vum = vd0; }
-        else { tang3(se[VU][l - kGW], se[VD][l], se[UU][l - kGW], se[UD][l], ta, tb, tc); vum = ta; vd0 = tb; ud0 = tc; }
+        else { tang3(se[VU][l - kEW], se[VD][l], se[UU][l - kEW], se[UD][l], ta, tb, tc); vum = ta; vd0 = tb; ud0 = tc; }
         if (j == n - 1) { vu0 = se[VU][l]; vdp = se[VU][l]; udp = se[UU][l]; }
-        else { tang3(se[VU][l], se[VD][l + kGW], se[UU][l], se[UD][l + kGW], ta, tb, tc); vu0 = ta; vdp = tb; udp = tc; }
+        else { tang3(se[VU][l], se[VD][l + kEW], se[UU][l], se[UD][l + kEW], ta, tb, tc); vu0 = ta; vdp = tb; udp = tc; }
         const double sx = 0.5 * g.xsp * (ulp + ul0), sy = 0.5 * g.ysp * (vdp + vd0);
-        sc[0][l] = sx * (ur0 - urm);
-        sc[1][l] = sx * (vlp - vl0);
-        sc[2][l] = sy * (udp - ud0);
-        sc[3][l] = sy * (vu0 - vum);
+        sc[0][lc] = sx * (ur0 - urm);
+        sc[1][lc] = sx * (vlp - vl0);
+        sc[2][lc] = sy * (udp - ud0);
+        sc[3][lc] = sy * (vu0 - vum);
     }
     __syncthreads();
     // K4 on T+1 (fluid_2d.cc:164-201); tile cells also store c0,c1
     for (int k = tid; k < (kGX + 2) * (kGY + 2); k += 256) {
         const int i = t.x0 - 1 + k % (kGX + 2), j = t.y0 - 1 + k / (kGX + 2);
         if (i < bx0 || i >= bx1 || j < by0 || j >= by1) continue;
-        const int l = t.L(i, j);
+        const int l = t.L(i, j), e = t.E(i, j), lc = t.C(i, j);
         const int q = (j - t.y0 + 1) * (kGX + 3) + (i - t.x0 + 1);
         const double px = 0.5 * g.xsp * (sp[q + kGX + 4] + sp[q + 1] - sp[q + kGX + 3] - sp[q]);
         const double py = 0.5 * g.ysp * (sp[q + kGX + 4] - sp[q + 1] + sp[q + kGX + 3] - sp[q]);
@@ -197,16 +202,16 @@ __device__ void godunov_body(const StepRec &a, int bx, int by) {
         const double fvy = g.ysp * mono_diff(sv[l - 2 * kGW], sv[l - kGW], fv, sv[l + kGW], sv[l + 2 * kGW]);
         const double fux = g.xsp * mono_diff(su[l - 2], su[l - 1], fu, su[l + 1], su[l + 2]);
         const double fvx = g.xsp * mono_diff(sv[l - 2], sv[l - 1], fv, sv[l + 1], sv[l + 2]);
-        const double fc0 = sc[0][l], fc1 = sc[1][l], fc2 = sc[2][l], fc3 = sc[3][l];
+        const double fc0 = sc[0][lc], fc1 = sc[1][lc], fc2 = sc[2][lc], fc3 = sc[3][lc];
         const double dx = g.dx, dy = g.dy;
-        se[UD][l] = fu + 0.5 * (-dy * fuy + dt * (-fc0 - fv * fuy + f_x));
-        se[VD][l] = fv + 0.5 * (-dy * fvy + dt * (-fc1 - fv * fvy + f_y));
-        se[UL][l] = (g.x_sym && i == 0) ? 0.0 : fu + 0.5 * (-dx * fux + dt * (-fu * fux - fc2 + f_x));
-        se[VL][l] = fv + 0.5 * (-dx * fvx + dt * (-fu * fvx - fc3 + f_y));
-        se[UR][l] = fu + 0.5 * (dx * fux + dt * (-fu * fux - fc2 + f_x));
-        se[VR][l] = fv + 0.5 * (dx * fvx + dt * (-fu * fvx - fc3 + f_y));
-        se[UU][l] = fu + 0.5 * (dy * fuy + dt * (-fc0 - fv * fuy + f_x));
-        se[VU][l] = fv + 0.5 * (dy * fvy + dt * (-fc1 - fv * fvy + f_y));
+        se[UD][e] = fu + 0.5 * (-dy * fuy + dt * (-fc0 - fv * fuy + f_x));
+        se[VD][e] = fv + 0.5 * (-dy * fvy + dt * (-fc1 - fv * fvy + f_y));
+        se[UL][e] = (g.x_sym && i == 0) ? 0.0 : fu + 0.5 * (-dx * fux + dt * (-fu * fux - fc2 + f_x));
+        se[VL][e] = fv + 0.5 * (-dx * fvx + dt * (-fu * fvx - fc3 + f_y));
+        se[UR][e] = fu + 0.5 * (dx * fux + dt * (-fu * fux - fc2 + f_x));
+        se[VR][e] = fv + 0.5 * (dx * fvx + dt * (-fu * fvx - fc3 + f_y));
+        se[UU][e] = fu + 0.5 * (dy * fuy + dt * (-fc0 - fv * fuy + f_x));
+        se[VU][e] = fv + 0.5 * (dy * fvy + dt * (-fc1 - fv * fvy + f_y));
         if (i >= t.x0 && i < t.x0 + kGX && j >= t.y0 && j < t.y0 + kGY) {
             F.c0[IX(i, j)] = viscfac * (uxx + uyy);
             F.c1[IX(i, j)] = viscfac * (vxx + vyy);
@@ -217,7 +222,7 @@ __device__ void godunov_body(const StepRec &a, int bx, int by) {
     {
         const int i = t.x0 + tid % kGX, j = t.y0 + tid / kGX;
         if (i >= m || j >= n) return;
-        const int l = t.L(i, j);
+        const int l = t.E(i, j);
         double a0, a1, b0, b1;
         // face i-1/2
         double ulL, vlL;
@@ -230,11 +235,11 @@ __device__ void godunov_body(const StepRec &a, int bx, int by) {
         // face j-1/2
         double vdB, udB;
         if (j == 0) { vdB = se[VD][l]; udB = se[UD][l]; }
-        else { a0 = se[VU][l - kGW]; a1 = se[VD][l]; b0 = se[UU][l - kGW]; b1 = se[UD][l]; godunov_set(a0, a1, b0, b1); vdB = a1; udB = b1; }
+        else { a0 = se[VU][l - kEW]; a1 = se[VD][l]; b0 = se[UU][l - kEW]; b1 = se[UD][l]; godunov_set(a0, a1, b0, b1); vdB = a1; udB = b1; }
         // face j+1/2
         double vdT;
         if (j == n - 1) vdT = se[VU][l];
-        else { a0 = se[VU][l]; a1 = se[VD][l + kGW]; b0 = se[UU][l]; b1 = se[UD][l + kGW]; godunov_set(a0, a1, b0, b1); vdT = a1; }
+        else { a0 = se[VU][l]; a1 = se[VD][l + kEW]; b0 = se[UU][l]; b1 = se[UD][l + kEW]; godunov_set(a0, a1, b0, b1); vdT = a1; }
         const size_t c = IX(i, j);
         F.ul[c] = ulL; F.vl[c] = vlL; F.vd[c] = vdB; F.ud[c] = udB;
         if (i == m - 1) { F.ul[IX(m, j)] = se[UR][l]; F.vl[IX(m, j)] = se[VR][l]; }
@@ -640,37 +645,56 @@ __device__ void gas_body(const StepRec &a, double *lds, double *red) {
         }
         double ressq = wg_sum(part, red) / me;  // quirk Q7: divided by me
         // parallel cyclic reduction (no pivoting; the Newton matrix is
-        // diagonally dominant).  The last row is the Dirichlet row pg = 0
-        // (a = c = 0, b = 1), which a reduction step leaves as it is, so only
-        // rows 0..me-2 are updated: one row per thread up to me = T + 1 (at
-        // m = 1024 the 1025th row had made the first wave do every step twice).
+        // diagonally dominant) on rows scaled to b = 1: the elimination
+        // factors are then the row's own a and c, so a step reads a, c, d of
+        // the two partner rows (the own row stays in registers), writes a, c,
+        // d and divides once.  The last row is the Dirichlet row pg = 0
+        // (a = c = 0, b = 1), which a step leaves as it is, so only rows
+        // 0..me-2 are updated: one row per thread up to me = T + 1.
         const int nr = me - 1;
+        double ra[kGasRows], rc[kGasRows], rd[kGasRows];
+#pragma unroll
+        for (int r = 0; r < kGasRows; r++) {
+            const int i = tid + r * T;
+            if (i >= me) continue;
+            const double inv = 1.0 / B[i];
+            ra[r] = A[i] * inv;
+            rc[r] = C[i] * inv;
+            rd[r] = D[i] * inv;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kGasRows; r++) {
+            const int i = tid + r * T;
+            if (i >= me) continue;
+            A[i] = ra[r]; C[i] = rc[r]; D[i] = rd[r];
+        }
+        __syncthreads();
         for (int s = 1; s < me; s <<= 1) {
-            double na[kGasRows], nb[kGasRows], nc[kGasRows], nd[kGasRows];
 #pragma unroll
             for (int r = 0; r < kGasRows; r++) {
-                int i = tid + r * T;
+                const int i = tid + r * T;
                 if (i >= nr) continue;
-                double ai = A[i], bi = B[i], ci = C[i], di = D[i];
                 double k1 = 0, k2 = 0, am = 0, cm = 0, dm = 0, ap = 0, cp = 0, dp = 0;
-                if (i - s >= 0) { k1 = ai / B[i - s]; am = A[i - s]; cm = C[i - s]; dm = D[i - s]; }
-                if (i + s < me) { k2 = ci / B[i + s]; ap = A[i + s]; cp = C[i + s]; dp = D[i + s]; }
-                na[r] = -am * k1; nc[r] = -cp * k2;
-                nb[r] = bi - cm * k1 - ap * k2;
-                nd[r] = di - dm * k1 - dp * k2;
+                if (i - s >= 0) { k1 = ra[r]; am = A[i - s]; cm = C[i - s]; dm = D[i - s]; }
+                if (i + s < me) { k2 = rc[r]; ap = A[i + s]; cp = C[i + s]; dp = D[i + s]; }
+                const double inv = 1.0 / (1.0 - cm * k1 - ap * k2);
+                ra[r] = -am * k1 * inv;
+                rc[r] = -cp * k2 * inv;
+                rd[r] = (rd[r] - dm * k1 - dp * k2) * inv;
             }
             __syncthreads();
 #pragma unroll
             for (int r = 0; r < kGasRows; r++) {
-                int i = tid + r * T;
+                const int i = tid + r * T;
                 if (i >= nr) continue;
-                A[i] = na[r]; B[i] = nb[r]; C[i] = nc[r]; D[i] = nd[r];
+                A[i] = ra[r]; C[i] = rc[r]; D[i] = rd[r];
             }
             __syncthreads();
         }
         double bad = 0;
         for (int i = tid; i < me; i += T) {
-            double x = D[i] / B[i];
+            double x = D[i];
             if (!(x == x) || x - x != 0) bad = 1;
             G.pg[i] += x;
         }
