@@ -644,14 +644,17 @@ __device__ void gas_body(const StepRec &a, double *lds, double *red) {
             A[i] = a; B[i] = b; C[i] = c; D[i] = d;
         }
         double ressq = wg_sum(part, red) / me;  // quirk Q7: divided by me
-        // parallel cyclic reduction (no pivoting; the Newton matrix is
-        // diagonally dominant) on rows scaled to b = 1: the elimination
-        // factors are then the row's own a and c, so a step reads a, c, d of
-        // the two partner rows (the own row stays in registers), writes a, c,
-        // d and divides once.  The last row is the Dirichlet row pg = 0
-        // (a = c = 0, b = 1), which a step leaves as it is, so only rows
-        // 0..me-2 are updated: one row per thread up to me = T + 1.
+        // Tridiagonal solve without pivoting (the Newton matrix is diagonally
+        // dominant), on rows scaled to b = 1: k steps of parallel cyclic
+        // reduction (a step reads a, c, d of the two partner rows, the own row
+        // staying in registers, and divides once) leave P = 2^k independent
+        // systems of rows r, r + P, r + 2P, ... of at most about 16 rows, which
+        // P threads then solve by Thomas elimination.  The last row is the
+        // Dirichlet row pg = 0 (a = c = 0, b = 1), which a reduction step
+        // leaves as it is, so only rows 0..me-2 are reduced.
         const int nr = me - 1;
+        int P = 1;
+        while (me > 16 * P && P < T) P <<= 1;
         double ra[kGasRows], rc[kGasRows], rd[kGasRows];
 #pragma unroll
         for (int r = 0; r < kGasRows; r++) {
@@ -670,7 +673,7 @@ __device__ void gas_body(const StepRec &a, double *lds, double *red) {
             A[i] = ra[r]; C[i] = rc[r]; D[i] = rd[r];
         }
         __syncthreads();
-        for (int s = 1; s < me; s <<= 1) {
+        for (int s = 1; s < P; s <<= 1) {
 #pragma unroll
             for (int r = 0; r < kGasRows; r++) {
                 const int i = tid + r * T;
@@ -692,6 +695,26 @@ __device__ void gas_body(const StepRec &a, double *lds, double *red) {
             }
             __syncthreads();
         }
+        // Thomas on system tid: row i couples to i - P (a) and i + P (c)
+        if (tid < P) {
+            double cq = 0, dq = 0;
+            int last = tid;
+            for (int i = tid; i < me; i += P) {
+                const double a = A[i];
+                const double inv = 1.0 / (1.0 - a * cq);
+                cq = C[i] * inv;
+                dq = (D[i] - a * dq) * inv;
+                C[i] = cq;
+                D[i] = dq;
+                last = i;
+            }
+            double x = 0;
+            for (int i = last; i >= tid; i -= P) {
+                x = D[i] - C[i] * x;
+                D[i] = x;
+            }
+        }
+        __syncthreads();
         double bad = 0;
         for (int i = tid; i < me; i += T) {
             double x = D[i];
