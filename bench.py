@@ -184,6 +184,7 @@ def run_plain(vdi, args, p):
     el_clean = time_steps(sim, p.dt_reg, args.steps)
     g1, r1 = sim.counters()
     height_passes = sim.stats().last_height_passes
+    budgets = sim.budgets()
     log(f"plain: {args.steps} steps in {el:.3f} s (stamped), {el_clean:.3f} s (no stamps)")
     achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
     sim.close()
@@ -196,6 +197,7 @@ def run_plain(vdi, args, p):
         "avg_vcycles": cyc,
         "captured_steps": g1 - g0, "redone_steps": r1 - r0,
         "height_passes_last_step": height_passes,
+        "budgets_mac_vx_vy_fem": budgets,
         "schedules": schedules,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM, "traffic": traffic,
@@ -357,6 +359,7 @@ def main():
                    "captured_steps": plain["captured_steps"], "redone_steps": plain["redone_steps"],
                    "gas_height_passes_last_step": plain["height_passes_last_step"],
                    "vcycle_schedule_mac_fem_vx_vy": plain["schedules"],
+                   "vcycle_budgets_mac_vx_vy_fem_after_run": plain["budgets_mac_vx_vy_fem"],
                    "concurrent_branches": not args.serial_visco,
                    "steps_per_s_with_kernel_stamps": plain["steps_per_s_stamped"]},
         "roofline": plain["roofline"],

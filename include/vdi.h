@@ -131,6 +131,10 @@ int vdi_set_concurrency(vdi_ctx *ctx, int enable);
  * path because a solve needed more cycles than its budget.  No counterpart
  * in the reference. */
 int vdi_get_counters(vdi_ctx *ctx, long *graph_steps, long *redo_steps);
+/* The V-cycle budgets [MAC, VX, VY, FEM] the next captured step enqueues
+ * (the largest count among each system's recent solves).  No counterpart in
+ * the reference. */
+int vdi_get_budgets(vdi_ctx *ctx, int *out);
 /* fluid_2d::init_fields (fluid_2d_sio.cc:14-35) */
 int vdi_init_fields(vdi_ctx *ctx);
 /* fluid_2d::step_forward(double dt) (fluid_2d.cc:93-405) */

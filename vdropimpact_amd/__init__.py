@@ -70,6 +70,7 @@ ABI = {
     "vdi_set_graphs": (_i, [_vp, _i]),
     "vdi_set_concurrency": (_i, [_vp, _i]),
     "vdi_get_counters": (_i, [_vp, ctypes.POINTER(_l), ctypes.POINTER(_l)]),
+    "vdi_get_budgets": (_i, [_vp, _ip]),
     "vdi_init_fields": (_i, [_vp]),
     "vdi_step_forward": (_i, [_vp, _d]),
     "vdi_gas_layer_update": (_i, [_vp, _d, _d]),
@@ -234,6 +235,12 @@ class Fluid2D:
         a, b = ctypes.c_long(), ctypes.c_long()
         _check(self.lib.vdi_get_counters(self.h, ctypes.byref(a), ctypes.byref(b)), "vdi_get_counters")
         return a.value, b.value
+
+    def budgets(self) -> list:
+        """V-cycle budgets [MAC, VX, VY, FEM] of the next captured step."""
+        arr = (ctypes.c_int * 4)()
+        _check(self.lib.vdi_get_budgets(self.h, arr), "vdi_get_budgets")
+        return list(arr)
 
     def init_fields(self):
         _check(self.lib.vdi_init_fields(self.h), "init_fields")

@@ -1649,6 +1649,12 @@ int vdi_get_counters(vdi_ctx *c, long *graph_steps, long *redo_steps) {
     return VDI_OK;
 }
 
+int vdi_get_budgets(vdi_ctx *c, int *out) {
+    if (!c || !out) return VDI_ERR_ARG;
+    for (int k = 0; k < 4; k++) out[k] = c->budget[k];
+    return VDI_OK;
+}
+
 int vdi_set_graphs(vdi_ctx *c, int enable) {
     if (!c) return VDI_ERR_ARG;
     VDI_HIP(hipStreamSynchronize(c->st));
