@@ -427,12 +427,16 @@ def test_gpu_redo_restores_budgets(vdi):
     for _ in range(30):
         a.step_forward(p.dt_reg)
     g0, r0 = a.counters()
+    b0 = a.budgets()
     # a much tighter tolerance needs more cycles than any budget recorded
     a.set_solver(rel_tol=1e-13)
     for _ in range(10):
         a.step_forward(p.dt_reg)
     g1, r1 = a.counters()
-    say("captured", g1 - g0, "redone", r1 - r0)
+    b1 = a.budgets()
+    say("captured", g1 - g0, "redone", r1 - r0, "budgets", b0, "->", b1)
     assert g1 - g0 == 10
     assert 1 <= r1 - r0 <= 3
+    # the budgets (MAC, u, v, FEM) grew to cover the tighter solves
+    assert all(x >= 1 for x in b0) and all(y >= x for x, y in zip(b0, b1)) and sum(b1) > sum(b0)
     a.close()
