@@ -113,9 +113,17 @@ int vdi_set_solver(vdi_ctx *ctx, int nu1, int nu2, double rel_tol, int max_cycle
  * counterpart: each reference solver has its own TGMG instance and constants
  * (mgs_mac.cc:13-14, mgs_fem.cc:13-14, visco_impl.cc:13). */
 int vdi_set_schedule(vdi_ctx *ctx, int which_solver, int nu1, int nu2);
-/* The schedule in use (defaults: V(3,2) for MAC and FEM, V(4,2) for the
- * viscous solves). */
+/* The schedule in use on the finest level (defaults: V(3,2) for MAC and
+ * FEM, V(4,2) for the viscous solves). */
 int vdi_get_schedule(vdi_ctx *ctx, int which_solver, int *nu1, int *nu2);
+/* V(nu1, nu2) on the finest level and V(nu1s, nu2s) on the restricted
+ * levels of one solver (vdi_set_schedule sets both to the same pair).  No
+ * reference counterpart (the TGMG schedule is internal to the absent
+ * library); defaults: MAC V(3,2) everywhere, FEM V(3,2) then V(2,2),
+ * viscous V(4,2) then V(1,1). */
+int vdi_set_schedule_levels(vdi_ctx *ctx, int which_solver, int nu1, int nu2, int nu1s, int nu2s);
+/* out = {nu1, nu2, nu1s, nu2s} of one solver. */
+int vdi_get_schedule_levels(vdi_ctx *ctx, int which_solver, int out[4]);
 /* Arithmetic form of the V-cycles (VDI_FORM_*). */
 int vdi_set_form(vdi_ctx *ctx, int form);
 /* Capture steps in HIP graphs (default on): after the first two steps each

@@ -45,6 +45,7 @@ struct vdo_mg {
     int bc[4];       /* left, right, bottom, top */
     double fm, fex, fey, fc, hex, hey; /* FEM stencil (mgs_fem.cc:8-11) */
     int nu1, nu2, ncs;
+    int nu1s, nu2s;  /* sweeps on the restricted levels 1 .. nl - 2 */
     /* solve history for the time-extrapolated initial guess (mgs_common.hh) */
     double *z_prev;
     int scount;
@@ -139,6 +140,7 @@ vdo_mg *vdo_mg_create(int kind, int m, int n, double cx, double cy, int x_sym) {
     g->kind = kind;
     g->fem = kind == VDO_KIND_FEM;
     g->nu1 = 2; g->nu2 = 2; g->ncs = 8;
+    g->nu1s = 2; g->nu2s = 2;
     if (kind == VDO_KIND_MAC) {
         /* mgs_mac.hh:37-50: Neumann left/right/top, Dirichlet face at bottom */
         g->sigma = 0; g->s = 1;
@@ -176,7 +178,11 @@ void vdo_mg_destroy(vdo_mg *g) {
     free(g);
 }
 
-void vdo_mg_set(vdo_mg *g, int nu1, int nu2, int ncs) { g->nu1 = nu1; g->nu2 = nu2; g->ncs = ncs; }
+void vdo_mg_set(vdo_mg *g, int nu1, int nu2, int ncs) {
+    g->nu1 = g->nu1s = nu1;
+    g->nu2 = g->nu2s = nu2;
+    g->ncs = ncs;
+}
 int vdo_mg_levels(const vdo_mg *g) { return g->nl; }
 
 /* Re-scale the cell-centred couplings (visco re-setup, visco_impl.cc:26-30). */
@@ -355,14 +361,16 @@ static void mg_vcycle_level(vdo_mg *g, int l, const double *b, double *z) {
         }
         return;
     }
-    for (int k = 0; k < g->nu1; k++) level_sweep(g, L, b, z);
+    /* V(nu1, nu2) on the finest level, V(nu1s, nu2s) on the restricted ones */
+    const int n1 = l == 0 ? g->nu1 : g->nu1s, n2 = l == 0 ? g->nu2 : g->nu2s;
+    for (int k = 0; k < n1; k++) level_sweep(g, L, b, z);
     level_residual(g, L, b, z, L->r);
     vdo_level *C = &g->L[l + 1];
     level_restrict(g, L, L->r, C, C->b);
     memset(C->z, 0, sizeof(double) * (size_t)C->m * C->n);
     mg_vcycle_level(g, l + 1, C->b, C->z);
     level_prolong(g, L, z, C, C->z);
-    for (int k = 0; k < g->nu2; k++) level_sweep(g, L, b, z);
+    for (int k = 0; k < n2; k++) level_sweep(g, L, b, z);
 }
 
 /* FEM Dirichlet row (mgs_fem.cc:38-41 + fluid_2d.cc:377): z = b / a_cc. */
@@ -634,15 +642,22 @@ void vdo_destroy(vdo_sim *s) {
 
 void vdo_set_mg(vdo_sim *s, int nu1, int nu2, double rel_tol, int max_cycles) {
     vdo_mg *gs[4] = {s->mac, s->fem, s->vx, s->vy};
-    for (int k = 0; k < 4; k++) if (gs[k]) { gs[k]->nu1 = nu1; gs[k]->nu2 = nu2; }
+    for (int k = 0; k < 4; k++) if (gs[k]) { gs[k]->nu1 = gs[k]->nu1s = nu1; gs[k]->nu2 = gs[k]->nu2s = nu2; }
     s->rel_tol = rel_tol; s->max_cycles = max_cycles;
 }
 
 int vdo_set_schedule(vdo_sim *s, int which, int nu1, int nu2) {
+    return vdo_set_schedule_levels(s, which, nu1, nu2, nu1, nu2);
+}
+
+int vdo_set_schedule_levels(vdo_sim *s, int which, int nu1, int nu2, int nu1s, int nu2s) {
     vdo_mg *gs[4] = {s->mac, s->fem, s->vx, s->vy};
     if (which < 0 || which > 3 || !gs[which] || nu1 < 0 || nu2 < 0 || nu1 + nu2 == 0) return 1;
+    if (nu1s < 0 || nu2s < 0 || nu1s + nu2s == 0) return 1;
     gs[which]->nu1 = nu1;
     gs[which]->nu2 = nu2;
+    gs[which]->nu1s = nu1s;
+    gs[which]->nu2s = nu2s;
     return 0;
 }
 

@@ -70,6 +70,8 @@ void vdo_destroy(vdo_sim *s);
 void vdo_set_mg(vdo_sim *s, int nu1, int nu2, double rel_tol, int max_cycles);
 /* nu1/nu2 of one solver (0 MAC, 1 FEM, 2 VX, 3 VY); nonzero on a bad argument. */
 int vdo_set_schedule(vdo_sim *s, int which, int nu1, int nu2);
+/* V(nu1, nu2) on the finest level, V(nu1s, nu2s) on the restricted levels. */
+int vdo_set_schedule_levels(vdo_sim *s, int which, int nu1, int nu2, int nu1s, int nu2s);
 void vdo_init_fields(vdo_sim *s);
 int vdo_step_forward(vdo_sim *s, double dt);
 int vdo_gas_update(vdo_sim *s, double t, double dt);

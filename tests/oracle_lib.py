@@ -42,6 +42,8 @@ def lib():
         L.vdo_set_mg.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int]
         L.vdo_set_schedule.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.vdo_set_schedule.restype = ctypes.c_int
+        L.vdo_set_schedule_levels.argtypes = [vp] + [ctypes.c_int] * 5
+        L.vdo_set_schedule_levels.restype = ctypes.c_int
         L.vdo_init_fields.argtypes = [vp]
         L.vdo_step_forward.argtypes = [vp, ctypes.c_double]
         L.vdo_gas_update.argtypes = [vp, ctypes.c_double, ctypes.c_double]
@@ -96,9 +98,11 @@ class OracleSim:
         self.h = lib().vdo_create(ctypes.byref(params))
         lib().vdo_set_mg(self.h, nu1, nu2, rel_tol, max_cycles)
 
-    def set_schedule(self, which, nu1, nu2):
-        if lib().vdo_set_schedule(self.h, which, nu1, nu2) != 0:
-            raise ValueError("vdo_set_schedule")
+    def set_schedule(self, which, nu1, nu2, nu1s=None, nu2s=None):
+        nu1s = nu1 if nu1s is None else nu1s
+        nu2s = nu2 if nu2s is None else nu2s
+        if lib().vdo_set_schedule_levels(self.h, which, nu1, nu2, nu1s, nu2s) != 0:
+            raise ValueError("vdo_set_schedule_levels")
 
     def close(self):
         if self.h:

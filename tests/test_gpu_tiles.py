@@ -81,17 +81,20 @@ def test_gpu_step_schedules(vdi):
     from vdropimpact_amd.params import PhysicalConfig
     p = PhysicalConfig(m=256, n=128).params()
     a = vdi.Fluid2D(p)
-    assert [a.schedule(k) for k in (vdi.MAC, vdi.FEM, vdi.VX, vdi.VY)] == [(3, 2), (3, 2), (4, 2), (4, 2)]
+    assert [a.schedule(k) for k in (vdi.MAC, vdi.FEM, vdi.VX, vdi.VY)] == [(3, 2, 3, 2), (3, 2, 2, 2), (4, 2, 1, 1),
+                                                                          (4, 2, 1, 1)]
     a.init_fields()
     for _ in range(3):
         a.step_forward(p.dt_reg)
     a.autotune(3)
     for k in (vdi.MAC, vdi.FEM, vdi.VX, vdi.VY):
         a.set_schedule(k, 4, 4)
-        assert a.schedule(k) == (4, 4)
+        assert a.schedule(k) == (4, 4, 4, 4)
+    a.set_schedule(vdi.VX, 4, 2, 1, 1)
+    assert a.schedule(vdi.VX) == (4, 2, 1, 1)
     for _ in range(3):
         a.step_forward(p.dt_reg)
-    for bad in [(5, 2), (0, 0), (-1, 2)]:
+    for bad in [(5, 2), (0, 0), (-1, 2), (3, 2, 5, 2), (3, 2, 0, 0)]:
         with pytest.raises(vdi.VdiError):
             a.set_schedule(vdi.MAC, *bad)
     a.close()

@@ -66,6 +66,8 @@ ABI = {
     "vdi_set_solver": (_i, [_vp, _i, _i, _d, _i]),
     "vdi_set_schedule": (_i, [_vp, _i, _i, _i]),
     "vdi_get_schedule": (_i, [_vp, _i, _ip, _ip]),
+    "vdi_set_schedule_levels": (_i, [_vp, _i, _i, _i, _i, _i]),
+    "vdi_get_schedule_levels": (_i, [_vp, _i, _ip]),
     "vdi_set_form": (_i, [_vp, _i]),
     "vdi_set_graphs": (_i, [_vp, _i]),
     "vdi_set_concurrency": (_i, [_vp, _i]),
@@ -211,14 +213,19 @@ class Fluid2D:
         """Tolerance, V-cycle cap and (unless both are 0) one V(nu1, nu2) for every solver."""
         _check(self.lib.vdi_set_solver(self.h, nu1, nu2, rel_tol, max_cycles), "vdi_set_solver")
 
-    def set_schedule(self, which: int, nu1: int, nu2: int):
-        """Smoothing sweeps of one solver (MAC, FEM, VX or VY)."""
-        _check(self.lib.vdi_set_schedule(self.h, which, nu1, nu2), "vdi_set_schedule")
+    def set_schedule(self, which: int, nu1: int, nu2: int, nu1s=None, nu2s=None):
+        """Smoothing sweeps of one solver (MAC, FEM, VX or VY): V(nu1, nu2)
+        on the finest level and V(nu1s, nu2s) on the restricted levels
+        (the same pair when these are omitted)."""
+        nu1s = nu1 if nu1s is None else nu1s
+        nu2s = nu2 if nu2s is None else nu2s
+        _check(self.lib.vdi_set_schedule_levels(self.h, which, nu1, nu2, nu1s, nu2s), "vdi_set_schedule_levels")
 
     def schedule(self, which: int) -> tuple:
-        a, b = ctypes.c_int(), ctypes.c_int()
-        _check(self.lib.vdi_get_schedule(self.h, which, ctypes.byref(a), ctypes.byref(b)), "vdi_get_schedule")
-        return a.value, b.value
+        """(nu1, nu2, nu1s, nu2s) of one solver."""
+        out = (ctypes.c_int * 4)()
+        _check(self.lib.vdi_get_schedule_levels(self.h, which, out), "vdi_get_schedule_levels")
+        return tuple(out)
 
     def set_form(self, form: int):
         _check(self.lib.vdi_set_form(self.h, form), "vdi_set_form")

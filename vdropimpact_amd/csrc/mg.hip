@@ -898,7 +898,7 @@ __device__ __forceinline__ void coarse_body_t(const CoarseArgs<T> &a, T *lds, do
     }
     __syncthreads();
     for (int l = 0; l + 1 < d.nl; l++) {
-        coarse_sweeps<T>(FEM, d, a.cop, a.fop, l, lds, a.nu1);
+        coarse_sweeps<T>(FEM, d, a.cop, a.fop, l, lds, d.top && l == 0 ? a.nu1 : a.nu1s);
         const LevelT<T> F = coarse_level<T>(d, l);
         const Tile t = coarse_tile(F.m, F.n);
         const int Mc = d.m[l + 1], Nc = d.n[l + 1], RC = Mc + 2;
@@ -977,7 +977,7 @@ __device__ __forceinline__ void coarse_body_t(const CoarseArgs<T> &a, T *lds, do
                 }
             }
         __syncthreads();
-        coarse_sweeps<T>(FEM, d, a.cop, a.fop, l, lds, a.nu2);
+        coarse_sweeps<T>(FEM, d, a.cop, a.fop, l, lds, d.top && l == 0 ? a.nu2 : a.nu2s);
     }
     const LevelT<T> F = coarse_level<T>(d, 0);
     const Tile t = coarse_tile(F.m, F.n);
@@ -1666,11 +1666,11 @@ int MG::coarse_points() const {
 
 size_t MG::down_lds(int prec, int l) const {
     const TileShape &t = kTileShapes[tile[prec][0][l]];
-    return tile_lds_bytes(halo_down(fem, nu1), prec ? sizeof(float) : sizeof(double), t.tx, t.ty);
+    return tile_lds_bytes(halo_down(fem, l == 0 ? nu1 : nu1s), prec ? sizeof(float) : sizeof(double), t.tx, t.ty);
 }
 size_t MG::up_lds(int prec, int l) const {
     const TileShape &t = kTileShapes[tile[prec][1][l]];
-    return tile_lds_bytes(halo_up(fem, nu2, l == 0), prec ? sizeof(float) : sizeof(double), t.tx, t.ty);
+    return tile_lds_bytes(halo_up(fem, l == 0 ? nu2 : nu2s, l == 0), prec ? sizeof(float) : sizeof(double), t.tx, t.ty);
 }
 size_t MG::mx_lds() const {
     const TileShape &t = kTileShapes[tile_mx];
@@ -1686,7 +1686,7 @@ LvArgs<T> MG::down_args(const std::vector<LevelT<T>> &H, int l, int zero_z) {
     a.C = H[l + 1];
     a.cop = cc;
     a.fop = fe;
-    a.nu = nu1;
+    a.nu = l == 0 ? nu1 : nu1s;
     a.flag = zero_z;
     const TileShape &t = kTileShapes[tile[prec][0][l]];
     a.tx = t.tx;
@@ -1712,7 +1712,7 @@ LvArgs<T> MG::up_args(const std::vector<LevelT<T>> &H, int l, int norm, double t
     a.C = H[l + 1];
     a.cop = cc;
     a.fop = fe;
-    a.nu = nu2;
+    a.nu = l == 0 ? nu2 : nu2s;
     a.flag = norm;
     const TileShape &t = kTileShapes[tile[prec][1][l]];
     a.tx = t.tx;
@@ -1751,6 +1751,8 @@ CoarseArgs<T> MG::coarse_args(const std::vector<LevelT<T>> &H, double tol2) {
     a.fem = fem;
     a.nu1 = nu1;
     a.nu2 = nu2;
+    a.nu1s = nu1s;
+    a.nu2s = nu2s;
     a.ncs = ncs;
     a.ctl = ctl();
     a.tol2 = tol2;
@@ -2112,10 +2114,13 @@ int MG::set_tile(int prec, int which, int level, int shape) {
     return VDI_OK;
 }
 
-int MG::set_schedule(int n1, int n2) {
+int MG::set_schedule(int n1, int n2, int n1s, int n2s) {
     if (n1 < 0 || n2 < 0 || n1 + n2 == 0 || n1 > kMaxNu || n2 > kMaxNu) return VDI_ERR_ARG;
+    if (n1s < 0 || n2s < 0 || n1s + n2s == 0 || n1s > kMaxNu || n2s > kMaxNu) return VDI_ERR_ARG;
     nu1 = n1;
     nu2 = n2;
+    nu1s = n1s;
+    nu2s = n2s;
     const int fallback[] = {0, 4, 2, 1};
     for (int prec = 0; prec < 2; prec++)
         for (int which = 0; which < 2; which++)

@@ -125,6 +125,7 @@ struct CoarseArgs {
     CCOp cop;
     FemOp fop;
     int fem, nu1, nu2, ncs;
+    int nu1s, nu2s;  // sweeps on the restricted levels (nu1, nu2 on a top level held here)
     MGCtl ctl;
     double tol2;
     Stamp *ts;
@@ -193,6 +194,7 @@ class MG {
     CCOp cc{};
     FemOp fe{};
     int nu1 = 2, nu2 = 2, ncs = 8;
+    int nu1s = 2, nu2s = 2;  // V(nu1s, nu2s) on levels 1 .. nl - 2
     int coarse_pts = kCoarsePoints;  // levels this small go to the single-workgroup kernel
     int form = VDI_FORM_FP64;
     hipStream_t st = nullptr;
@@ -288,7 +290,8 @@ class MG {
     int set_tile(int prec, int which, int level, int shape);
     // Smoothing sweeps per level (0..kMaxNu, not both 0).  A tile shape whose
     // halo no longer fits LDS moves to the largest default shape that does.
-    int set_schedule(int nu1, int nu2);
+    int set_schedule(int nu1, int nu2) { return set_schedule(nu1, nu2, nu1, nu2); }
+    int set_schedule(int nu1, int nu2, int nu1s, int nu2s);
     // LDS bytes of a launch.
     size_t down_lds(int prec, int l) const;
     size_t up_lds(int prec, int l) const;

@@ -1107,7 +1107,8 @@ int vdi_create(const vdi_params *p, vdi_ctx **out) {
     {
         MG *gs[4] = {&c->mac, &c->fem, &c->vx, &c->vy};
         for (int k = 0; k < 4; k++)
-            if (gs[k]->nl) VDI_TRY(gs[k]->set_schedule(kStepNu[k][0], kStepNu[k][1]));
+            if (gs[k]->nl)
+                VDI_TRY(gs[k]->set_schedule(kStepNu[k][0], kStepNu[k][1], kStepNu[k][2], kStepNu[k][3]));
     }
     VDI_TRY(c->snap.alloc((3 * NP + ng) * sizeof(double)));
     VDI_TRY(c->coef_dev.alloc(8 * sizeof(double)));
@@ -1157,13 +1158,29 @@ int vdi_get_schedule(vdi_ctx *c, int which_solver, int *nu1, int *nu2) {
 }
 
 int vdi_set_schedule(vdi_ctx *c, int which_solver, int nu1, int nu2) {
+    return vdi_set_schedule_levels(c, which_solver, nu1, nu2, nu1, nu2);
+}
+
+int vdi_set_schedule_levels(vdi_ctx *c, int which_solver, int nu1, int nu2, int nu1s, int nu2s) {
     if (!c || which_solver < VDI_MAC || which_solver > VDI_VY) return VDI_ERR_ARG;
     MG *gs[4] = {&c->mac, &c->fem, &c->vx, &c->vy};
     MG &g = *gs[which_solver];
     if (!g.nl) return VDI_ERR_ARG;
     VDI_HIP(hipStreamSynchronize(c->st));
     c->drop_graphs();
-    return g.set_schedule(nu1, nu2);
+    return g.set_schedule(nu1, nu2, nu1s, nu2s);
+}
+
+int vdi_get_schedule_levels(vdi_ctx *c, int which_solver, int out[4]) {
+    if (!c || !out || which_solver < VDI_MAC || which_solver > VDI_VY) return VDI_ERR_ARG;
+    MG *gs[4] = {&c->mac, &c->fem, &c->vx, &c->vy};
+    const MG &g = *gs[which_solver];
+    if (!g.nl) return VDI_ERR_ARG;
+    out[0] = g.nu1;
+    out[1] = g.nu2;
+    out[2] = g.nu1s;
+    out[3] = g.nu2s;
+    return VDI_OK;
 }
 
 int vdi_set_form(vdi_ctx *c, int form) {
