@@ -120,7 +120,7 @@ int vdi_get_schedule(vdi_ctx *ctx, int which_solver, int *nu1, int *nu2);
  * levels of one solver (vdi_set_schedule sets both to the same pair).  No
  * reference counterpart (the TGMG schedule is internal to the absent
  * library); defaults: MAC V(3,2) everywhere, FEM V(3,2) then V(2,2),
- * viscous V(4,2) then V(1,1). */
+ * viscous V(4,2) then V(0,1). */
 int vdi_set_schedule_levels(vdi_ctx *ctx, int which_solver, int nu1, int nu2, int nu1s, int nu2s);
 /* out = {nu1, nu2, nu1s, nu2s} of one solver. */
 int vdi_get_schedule_levels(vdi_ctx *ctx, int which_solver, int out[4]);
