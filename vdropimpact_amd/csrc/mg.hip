@@ -8,6 +8,8 @@
 //                          workgroup's LDS
 //   up(l), l = lc-1..0   : prolongation + correction, nu2 sweeps; at level 0
 //                          also per-workgroup partials of ||b - A z||^2
+//                          (level 0 takes V(nu1, nu2), every restricted level
+//                          V(nu1s, nu2s), in these kernels and in coarse)
 //   norm                 : one workgroup sums the partials in a fixed order
 //                          and raises the device-side convergence flag.
 // Every kernel returns at once when the flag is set, so a solve enqueues the
