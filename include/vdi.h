@@ -199,12 +199,21 @@ int vdi_mg_levels(vdi_mg *g);
 int vdi_mg_set_b(vdi_mg *g, const double *host, long len);
 int vdi_mg_set_z(vdi_mg *g, const double *host, long len);
 int vdi_mg_get_z(vdi_mg *g, double *host, long len);
+/* Device addresses of the system's level-0 solution z and right-hand side b
+ * (m*n doubles each, owned by the system), for entry points that take device
+ * arrays.  No reference counterpart. */
+int vdi_mg_device_arrays(vdi_mg *g, double **z_dev, double **b_dev);
 /* out = A z (host arrays, for operator checks). */
 int vdi_mg_apply(vdi_mg *g, const double *z, double *out, long len);
 /* tgmg::solve_v_cycle: V-cycles until ||b-Az|| <= rel_tol ||b||. */
 int vdi_mg_solve(vdi_mg *g, double rel_tol, int max_cycles, int *vcycles);
 /* tgmg::v_cycle (mg_test.cc:38): exactly k V-cycles, no convergence test. */
 int vdi_mg_vcycles(vdi_mg *g, int k);
+/* A solve with a fixed budget of V-cycles and no host read between them (the
+ * form a captured step enqueues), then its outcome: *done = 1 converged, 3
+ * zero right-hand side (z = 0), 0 budget spent, 2 NaN; *vcycles = V-cycles
+ * run.  No reference counterpart (TGMG's solve_v_cycle always synchronises). */
+int vdi_mg_solve_fixed(vdi_mg *g, double rel_tol, int budget, int *done, int *vcycles);
 /* ||b - A z||_2 and ||b||_2 of the device arrays. */
 int vdi_mg_residual(vdi_mg *g, double *rnorm, double *bnorm);
 int vdi_mg_synchronize(vdi_mg *g);
@@ -244,6 +253,10 @@ int vdi_batch_autotune(vdi_batch *b, int reps);
 /* Last batched step: graph time (ms) and each member's workgroup busy time
  * (ms, summed over its work items; n >= size). */
 int vdi_batch_step_times(vdi_batch *b, double *graph_ms, double *busy_ms, int n);
+/* Per-member step time of the last batched step (ms, n >= size): from the
+ * start of the member's first work item to the end of its last one, read from
+ * the device clock (wall_clock64) inside the captured step. */
+int vdi_batch_member_spans(vdi_batch *b, double *span_ms, int n);
 int vdi_batch_counters(vdi_batch *b, long *batched_steps, long *redone_sims);
 int vdi_batch_tiles(vdi_batch *b, int *out, int n);
 

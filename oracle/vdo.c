@@ -698,6 +698,16 @@ static double *array_ptr(const vdo_sim *s, int id, long *len) {
     case VDO_Z_FEM: *len = (long)s->me * s->ne; return s->zfem;
     case VDO_Z_VX: *len = s->zvx ? s->mn : 0; return s->zvx;
     case VDO_Z_VY: *len = s->zvy ? s->mn : 0; return s->zvy;
+    case VDO_UX: *len = NP; return s->F[F_UX];
+    case VDO_UY: *len = NP; return s->F[F_UY];
+    case VDO_VXS: *len = NP; return s->F[F_VX];
+    case VDO_VYS: *len = NP; return s->F[F_VY];
+    case VDO_UR: *len = NP; return s->F[F_UR];
+    case VDO_VR: *len = NP; return s->F[F_VR];
+    case VDO_UU: *len = NP; return s->F[F_UU];
+    case VDO_VU: *len = NP; return s->F[F_VU];
+    case VDO_C2: *len = NP; return s->F[F_C2];
+    case VDO_C3: *len = NP; return s->F[F_C3];
     default: return NULL;
     }
 }
@@ -942,24 +952,29 @@ static int mg_solve_sim(vdo_sim *s, vdo_mg *g, double *z, double t) {
     return c < 0 ? 1 : 0;
 }
 
-/* fluid_2d.cc:93-405 */
-int vdo_step_forward(vdo_sim *s, double dt) {
-    const int m = s->m, n = s->n, ml = s->ml, me = s->me, ne = s->ne;
-    const double dx = s->dx, dy = s->dy, xsp = s->xsp, ysp = s->ysp, xxsp = s->xxsp, yysp = s->yysp;
-    const double rhoinv = s->rhoinv, rho = s->P.rho;
-    const int x_sym = s->P.x_sym;
-    double *u = s->F[F_U], *v = s->F[F_V], *p = s->F[F_P];
-    double *ux = s->F[F_UX], *uy = s->F[F_UY], *vx = s->F[F_VX], *vy = s->F[F_VY];
-    double *ud = s->F[F_UD], *vd = s->F[F_VD], *ul = s->F[F_UL], *vl = s->F[F_VL];
-    double *ur = s->F[F_UR], *vr = s->F[F_VR], *uu = s->F[F_UU], *vu = s->F[F_VU];
-    double *c0 = s->F[F_C0], *c1 = s->F[F_C1], *c2 = s->F[F_C2], *c3 = s->F[F_C3];
-    int rc;
+/* ---- the passes of fluid_2d::step_forward, one function each ---- */
 
-    /* fluid_2d.cc:99-102 */
-    if ((rc = vdo_gas_update(s, s->time, dt))) return rc == 2 ? 2 : 3;
-    s->time += 0.5 * dt;
+#define STEP_LOCALS                                                                                 \
+    const int m = s->m, n = s->n, ml = s->ml;                                                       \
+    (void)m; (void)n; (void)ml;                                                                     \
+    double *u = s->F[F_U], *v = s->F[F_V], *p = s->F[F_P];                                          \
+    double *ux = s->F[F_UX], *uy = s->F[F_UY], *vx = s->F[F_VX], *vy = s->F[F_VY];                  \
+    double *ud = s->F[F_UD], *vd = s->F[F_VD], *ul = s->F[F_UL], *vl = s->F[F_VL];                  \
+    double *ur = s->F[F_UR], *vr = s->F[F_VR], *uu = s->F[F_UU], *vu = s->F[F_VU];                  \
+    double *c0 = s->F[F_C0], *c1 = s->F[F_C1], *c2 = s->F[F_C2], *c3 = s->F[F_C3];                  \
+    (void)u; (void)v; (void)p; (void)ux; (void)uy; (void)vx; (void)vy; (void)ud; (void)vd; (void)ul; \
+    (void)vl; (void)ur; (void)vr; (void)uu; (void)vu; (void)c0; (void)c1; (void)c2; (void)c3
 
-    /* K1, fluid_2d.cc:105-132 */
+static double step_viscfac(const vdo_sim *s) {
+    /* fluid_2d.cc:164 */
+    return s->P.implicit_visc ? s->rhoinv * s->P.mu * 0.5 : s->rhoinv * s->P.mu;
+}
+
+/* K1, fluid_2d.cc:105-132: limited slopes and the normal half-step
+ * extrapolation of u, v to the four faces */
+static void pass_k1(vdo_sim *s, double dt) {
+    STEP_LOCALS;
+    const double dx = s->dx, dy = s->dy, xsp = s->xsp, ysp = s->ysp;
 #pragma omp parallel for schedule(static)
     for (int j = 0; j < n; j++)
         for (int i = 0; i < m; i++) {
@@ -977,9 +992,13 @@ int vdo_step_forward(vdo_sim *s, double dt) {
             AT(uu, i, j) = fu + 0.5 * (dy - dt * fv) * AT(uy, i, j);
             AT(vu, i, j) = fv + 0.5 * (dy - dt * fv) * AT(vy, i, j);
         }
+}
 
-    /* K2, fluid_2d.cc:136-150 (zero_edge_velocities :480-482) */
-    if (x_sym) for (int j = 0; j < n; j++) AT(ul, 0, j) = 0;
+/* K2, fluid_2d.cc:136-150 (zero_edge_velocities :480-482): upwind choice at
+ * every face, keeping the tangential-average form godunov_set_tang needs */
+static void pass_k2(vdo_sim *s) {
+    STEP_LOCALS;
+    if (s->P.x_sym) for (int j = 0; j < n; j++) AT(ul, 0, j) = 0;
 #pragma omp parallel for schedule(static)
     for (int j = 0; j < n; j++)
         for (int i = 0; i < m; i++) {
@@ -992,8 +1011,12 @@ int vdo_step_forward(vdo_sim *s, double dt) {
                 if (j == 0) AT(vu, i, -1) = AT(vd, i, 0);
             } else { AT(vd, i, n) = AT(vu, i, n - 1); AT(ud, i, n) = AT(uu, i, n - 1); }
         }
+}
 
-    /* K3, fluid_2d.cc:153-160 */
+/* K3, fluid_2d.cc:153-160: transverse (tangential) derivative terms */
+static void pass_k3(vdo_sim *s) {
+    STEP_LOCALS;
+    const double xsp = s->xsp, ysp = s->ysp;
 #pragma omp parallel for schedule(static)
     for (int j = 0; j < n; j++)
         for (int i = 0; i < m; i++) {
@@ -1002,9 +1025,15 @@ int vdo_step_forward(vdo_sim *s, double dt) {
             AT(c2, i, j) = 0.5 * ysp * (AT(vd, i, j + 1) + AT(vd, i, j)) * (AT(ud, i, j + 1) - AT(ud, i, j));
             AT(c3, i, j) = 0.5 * ysp * (AT(vd, i, j + 1) + AT(vd, i, j)) * (AT(vu, i, j) - AT(vu, i, j - 1));
         }
+}
 
-    /* K4, fluid_2d.cc:164-201 (centered_diff :412-417) */
-    const double viscfac = s->P.implicit_visc ? rhoinv * s->P.mu * 0.5 : rhoinv * s->P.mu;
+/* K4, fluid_2d.cc:164-201 (centered_diff :412-417): full half-step face
+ * predictor with the pressure gradient and the (explicit part of the)
+ * viscous force; c0, c1 keep that force */
+static void pass_k4(vdo_sim *s, double dt) {
+    STEP_LOCALS;
+    const double dx = s->dx, dy = s->dy, xsp = s->xsp, ysp = s->ysp, xxsp = s->xxsp, yysp = s->yysp;
+    const double rhoinv = s->rhoinv, viscfac = step_viscfac(s);
 #pragma omp parallel for schedule(static)
     for (int j = 0; j < n; j++)
         for (int i = 0; i < m; i++) {
@@ -1030,9 +1059,12 @@ int vdo_step_forward(vdo_sim *s, double dt) {
             AT(c0, i, j) = viscfac * (uxx + uyy);
             AT(c1, i, j) = viscfac * (vxx + vyy);
         }
+}
 
-    /* K5, fluid_2d.cc:205-215 */
-    if (x_sym) for (int j = 0; j < n; j++) AT(ul, 0, j) = 0;
+/* K5, fluid_2d.cc:205-215: upwind choice at every face */
+static void pass_k5(vdo_sim *s) {
+    STEP_LOCALS;
+    if (s->P.x_sym) for (int j = 0; j < n; j++) AT(ul, 0, j) = 0;
 #pragma omp parallel for schedule(static)
     for (int j = 0; j < n; j++)
         for (int i = 0; i < m; i++) {
@@ -1041,22 +1073,28 @@ int vdo_step_forward(vdo_sim *s, double dt) {
             if (j < n - 1) vdo_godunov_set(&AT(vu, i, j), &AT(vd, i, j + 1), &AT(uu, i, j), &AT(ud, i, j + 1));
             else { AT(vd, i, n) = AT(vu, i, n - 1); AT(ud, i, n) = AT(uu, i, n - 1); }
         }
+}
 
-    /* K6, fluid_2d.cc:218-232 */
+/* K6, fluid_2d.cc:218-232: MAC right-hand side (face divergence) and the
+ * Dirichlet data of the bottom face from the gas pressure (s_bc, quirk Q8) */
+static void pass_k6(vdo_sim *s, double dt) {
+    STEP_LOCALS;
+    const double xsp = s->xsp, ysp = s->ysp, yysp = s->yysp;
     double *src = s->src, *s_bc = s->s_bc;
 #pragma omp parallel for schedule(static)
     for (int j = 0; j < n; j++)
         for (int i = 0; i < m; i++)
             src[i + (size_t)m * j] = xsp * (AT(ul, i + 1, j) - AT(ul, i, j)) + ysp * (AT(vd, i, j + 1) - AT(vd, i, j));
-    const double macfac = 0.125 * rhoinv * dt;
+    const double macfac = 0.125 * s->rhoinv * dt;
     for (int i = 0; i < m; i++)
         src[i] -= 2 * yysp * (s_bc[i] = macfac * (s->gp_old[i] + s->gp_old[i + 1] + s->gp[i] + s->gp[i + 1]));
+}
 
-    /* MAC solve, fluid_2d.cc:233 */
-    if (mg_solve_sim(s, s->mac, s->zmac, s->time)) return 1;
-
-    /* K7, fluid_2d.cc:236-248 */
-    const double *zm = s->zmac;
+/* K7, fluid_2d.cc:236-248: MAC correction of the face velocities */
+static void pass_k7(vdo_sim *s) {
+    STEP_LOCALS;
+    const double xsp = s->xsp, ysp = s->ysp;
+    const double *zm = s->zmac, *s_bc = s->s_bc;
 #pragma omp parallel for schedule(static)
     for (int j = 0; j < n; j++)
         for (int i = 0; i < m; i++) {
@@ -1064,10 +1102,12 @@ int vdo_step_forward(vdo_sim *s, double dt) {
             if (j != 0) AT(vd, i, j) -= (zm[i + (size_t)m * j] - zm[i + (size_t)m * (j - 1)]) * ysp;
         }
     for (int i = 0; i < m; i++) AT(vd, i, 0) -= 2 * ysp * (zm[i] - s_bc[i]);
+}
 
-    s->time += 0.5 * dt;
-
-    /* K8, fluid_2d.cc:253-261 */
+/* K8, fluid_2d.cc:253-261: advective update into c0, c1 */
+static void pass_k8(vdo_sim *s, double dt) {
+    STEP_LOCALS;
+    const double xsp = s->xsp, ysp = s->ysp;
 #pragma omp parallel for schedule(static)
     for (int j = 0; j < n; j++)
         for (int i = 0; i < m; i++) {
@@ -1076,53 +1116,56 @@ int vdo_step_forward(vdo_sim *s, double dt) {
             AT(c0, i, j) = fu + dt * (-0.5 * (sx * (AT(ul, i + 1, j) - AT(ul, i, j)) + sy * (AT(ud, i, j + 1) - AT(ud, i, j))) + AT(c0, i, j));
             AT(c1, i, j) = fv + dt * (-0.5 * (sx * (AT(vl, i + 1, j) - AT(vl, i, j)) + sy * (AT(vd, i, j + 1) - AT(vd, i, j))) + AT(c1, i, j));
         }
+}
 
-    /* fluid_2d.cc:267-269 -> :485-492 */
-    set_inviscid_boundaries(s, dt);
-    set_viscous_boundaries(s);
-
-    if (s->P.implicit_visc) {
-        /* fluid_2d.cc:272-335 */
-        const double vfx = viscfac * xxsp * dt, vfy = viscfac * yysp * dt;
-        const double muinv = s->muinv;
-        for (int comp = 0; comp < 2; comp++) {
-            const double *w = comp ? v : u, *cc = comp ? c1 : c0;
+/* Implicit viscosity right-hand side, fluid_2d.cc:272-324, component comp
+ * (0: u from c0, 1: v from c1) into src */
+static void pass_visco_rhs(vdo_sim *s, int comp, double dt) {
+    STEP_LOCALS;
+    const double dy = s->dy, xsp = s->xsp, viscfac = step_viscfac(s);
+    const double vfx = viscfac * s->xxsp * dt, vfy = viscfac * s->yysp * dt;
+    const double muinv = s->muinv;
+    const int x_sym = s->P.x_sym;
+    const double *w = comp ? v : u, *cc = comp ? c1 : c0;
+    double *src = s->src;
 #pragma omp parallel for schedule(static)
-            for (int j = 0; j < n; j++)
-                for (int i = 0; i < m; i++) {
-                    double r = AT(cc, i, j);
-                    if (!x_sym && i == 0) r += vfx * AT(w, -1, j);
-                    if (i == m - 1) r += vfx * AT(w, m, j);
-                    if (j == n - 1) r += vfy * AT(w, i, n);
-                    if (j == 0) {
-                        if (!comp) {
-                            double dudy = muinv * 0.5 * (s->gtau_old[i + 2] + s->gtau[i + 2]);
-                            r -= vfy * dy * dudy;
-                        } else {
-                            double dvdy = -0.5 * xsp * (AT(u, i + 1, 0) - AT(u, i - 1, 0));
-                            r -= vfy * dy * dvdy;
-                        }
-                    }
-                    src[i + (size_t)m * j] = r;
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < m; i++) {
+            double r = AT(cc, i, j);
+            if (!x_sym && i == 0) r += vfx * AT(w, -1, j);
+            if (i == m - 1) r += vfx * AT(w, m, j);
+            if (j == n - 1) r += vfy * AT(w, i, n);
+            if (j == 0) {
+                if (!comp) {
+                    double dudy = muinv * 0.5 * (s->gtau_old[i + 2] + s->gtau[i + 2]);
+                    r -= vfy * dy * dudy;
+                } else {
+                    double dvdy = -0.5 * xsp * (AT(u, i + 1, 0) - AT(u, i - 1, 0));
+                    r -= vfy * dy * dvdy;
                 }
-            /* visco_impl.cc:21-39 */
-            vdo_mg *g = comp ? s->vy : s->vx;
-            if (fabs(vfx - g->vfx) > 1e2 * DBL_EPSILON * g->vfx) {
-                g->vfx = vfx; g->vfy = vfy;
-                mg_set_coeffs(g, vfx, vfy);
             }
-            if (mg_solve_sim(s, g, comp ? s->zvy : s->zvx, s->time)) return 1;
+            src[i + (size_t)m * j] = r;
         }
-#pragma omp parallel for schedule(static)
-        for (int j = 0; j < n; j++)
-            for (int i = 0; i < m; i++) {
-                AT(c0, i, j) = s->zvx[i + (size_t)m * j];
-                AT(c1, i, j) = s->zvy[i + (size_t)m * j];
-            }
-    }
+}
 
-    /* K14, fluid_2d.cc:338-377 */
-    const double rdt = rho / dt;
+/* fluid_2d.cc:330-334: the two Helmholtz solutions become c0, c1 */
+static void pass_visco_copy(vdo_sim *s) {
+    STEP_LOCALS;
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < m; i++) {
+            AT(c0, i, j) = s->zvx[i + (size_t)m * j];
+            AT(c1, i, j) = s->zvy[i + (size_t)m * j];
+        }
+}
+
+/* K14, fluid_2d.cc:338-377: nodal weak divergence of (c0, c1) times rho/dt,
+ * boundary flux terms and the Dirichlet bottom row from the gas pressure */
+static void pass_k14(vdo_sim *s, double dt) {
+    STEP_LOCALS;
+    const int me = s->me, ne = s->ne, x_sym = s->P.x_sym;
+    const double dx = s->dx, dy = s->dy, rdt = s->P.rho / dt;
+    double *src = s->src;
 #pragma omp parallel for schedule(static)
     for (int j = 1; j < ne; j++)
         for (int i = 0; i < me; i++) {
@@ -1147,11 +1190,14 @@ int vdo_step_forward(vdo_sim *s, double dt) {
         if (i < m) src[(size_t)n * me + i] -= 0.25 * dx * rdt * (AT(c1, i, n - 1) + AT(v, i, n));
     }
     for (int i = 0; i < me; i++) src[i] = fem_acc(s->fem, &s->fem->L[0], i, 0) * s->gp[i];
+}
 
-    /* FEM solve, fluid_2d.cc:380 */
-    if (mg_solve_sim(s, s->fem, s->zfem, s->time)) return 1;
-
-    /* K15, fluid_2d.cc:383-396 */
+/* K15, fluid_2d.cc:383-396: velocity update from the nodal pressure, and the
+ * nodal pressure copied into the cell array (ghost column m and row n) */
+static void pass_k15(vdo_sim *s, double dt) {
+    STEP_LOCALS;
+    const int me = s->me, ne = s->ne;
+    const double xsp = s->xsp, ysp = s->ysp, rhoinv = s->rhoinv;
     const double *zf = s->zfem;
 #pragma omp parallel for schedule(static)
     for (int j = 0; j < n; j++)
@@ -1162,8 +1208,114 @@ int vdo_step_forward(vdo_sim *s, double dt) {
         }
     for (int j = 0; j < ne; j++)
         for (int i = 0; i < me; i++) AT(p, i, j) = zf[i + (size_t)me * j];
+}
+
+static int step_solve(vdo_sim *s, int which, double t) {
+    vdo_mg *g = which == VDO_KIND_MAC ? s->mac : which == VDO_KIND_FEM ? s->fem : which == VDO_KIND_VX ? s->vx : s->vy;
+    double *z = which == VDO_KIND_MAC ? s->zmac : which == VDO_KIND_FEM ? s->zfem : which == VDO_KIND_VX ? s->zvx : s->zvy;
+    return mg_solve_sim(s, g, z, t);
+}
+
+/* visco_impl.cc:21-39 (the coefficient part): re-set the hierarchy when the
+ * coupling changed */
+static void visco_setup(vdo_mg *g, double vfx, double vfy) {
+    if (fabs(vfx - g->vfx) > 1e2 * DBL_EPSILON * g->vfx) {
+        g->vfx = vfx; g->vfy = vfy;
+        mg_set_coeffs(g, vfx, vfy);
+    }
+}
+
+/* fluid_2d.cc:93-405 */
+int vdo_step_forward(vdo_sim *s, double dt) {
+    int rc;
+
+    /* fluid_2d.cc:99-102 */
+    if ((rc = vdo_gas_update(s, s->time, dt))) return rc == 2 ? 2 : 3;
+    s->time += 0.5 * dt;
+
+    pass_k1(s, dt);
+    pass_k2(s);
+    pass_k3(s);
+    pass_k4(s, dt);
+    pass_k5(s);
+    pass_k6(s, dt);
+    /* MAC solve, fluid_2d.cc:233 */
+    if (step_solve(s, VDO_KIND_MAC, s->time)) return 1;
+    pass_k7(s);
+    s->time += 0.5 * dt;
+    pass_k8(s, dt);
+
+    /* fluid_2d.cc:267-269 -> :485-492 */
+    set_inviscid_boundaries(s, dt);
+    set_viscous_boundaries(s);
+
+    if (s->P.implicit_visc) {
+        /* fluid_2d.cc:272-335 */
+        const double viscfac = step_viscfac(s);
+        const double vfx = viscfac * s->xxsp * dt, vfy = viscfac * s->yysp * dt;
+        for (int comp = 0; comp < 2; comp++) {
+            pass_visco_rhs(s, comp, dt);
+            vdo_mg *g = comp ? s->vy : s->vx;
+            visco_setup(g, vfx, vfy);
+            if (step_solve(s, comp ? VDO_KIND_VY : VDO_KIND_VX, s->time)) return 1;
+        }
+        pass_visco_copy(s);
+    }
+
+    pass_k14(s, dt);
+    /* FEM solve, fluid_2d.cc:380 */
+    if (step_solve(s, VDO_KIND_FEM, s->time)) return 1;
+    pass_k15(s, dt);
 
     /* fluid_2d.cc:399-401 */
     set_viscous_boundaries(s);
     return 0;
+}
+
+/* ---- the passes one at a time (pinning tests) ---- */
+
+int vdo_pass(vdo_sim *s, int pass, double dt) {
+    switch (pass) {
+    case VDO_PASS_K1: pass_k1(s, dt); return 0;
+    case VDO_PASS_K2: pass_k2(s); return 0;
+    case VDO_PASS_K3: pass_k3(s); return 0;
+    case VDO_PASS_K4: pass_k4(s, dt); return 0;
+    case VDO_PASS_K5: pass_k5(s); return 0;
+    case VDO_PASS_K6: pass_k6(s, dt); return 0;
+    case VDO_PASS_K7: pass_k7(s); return 0;
+    case VDO_PASS_K8: pass_k8(s, dt); return 0;
+    case VDO_PASS_INVISCID_BC: set_inviscid_boundaries(s, dt); return 0;
+    case VDO_PASS_VISCOUS_BC: set_viscous_boundaries(s); return 0;
+    case VDO_PASS_VISCO_RHS_U: if (!s->vx) return -1; pass_visco_rhs(s, 0, dt); return 0;
+    case VDO_PASS_VISCO_RHS_V: if (!s->vy) return -1; pass_visco_rhs(s, 1, dt); return 0;
+    case VDO_PASS_VISCO_COPY: if (!s->vx) return -1; pass_visco_copy(s); return 0;
+    case VDO_PASS_K14: pass_k14(s, dt); return 0;
+    case VDO_PASS_K15: pass_k15(s, dt); return 0;
+    case VDO_PASS_GAS_LIQUID_FIELDS: gas_liquid_fields(s); return 0;
+    case VDO_PASS_GAS_TAU_BC: gas_apply_tau_bc(s); return 0;
+    case VDO_PASS_GAS_HX: gas_calculate_hx(s); return 0;
+    case VDO_PASS_GAS_INIT: gas_init_fields(s); return 0;
+    default: return -1;
+    }
+}
+
+int vdo_initial_guess(vdo_sim *s, int which, double t) {
+    vdo_mg *g = which == VDO_KIND_MAC ? s->mac : which == VDO_KIND_FEM ? s->fem : which == VDO_KIND_VX ? s->vx : s->vy;
+    double *z = which == VDO_KIND_MAC ? s->zmac : which == VDO_KIND_FEM ? s->zfem : which == VDO_KIND_VX ? s->zvx : s->zvy;
+    if (!g) return -1;
+    mg_initial_guess(g, z, t);
+    return 0;
+}
+
+int vdo_solve_v_cycle(vdo_sim *s, int which, double t) {
+    if (which != VDO_KIND_MAC && which != VDO_KIND_FEM) return -1;
+    if (step_solve(s, which, t)) return -1;
+    return vdo_last_cycles(s, which);
+}
+
+int vdo_visco_setup_and_solve(vdo_sim *s, int which, double vfx, double vfy, double t) {
+    if ((which != VDO_KIND_VX && which != VDO_KIND_VY) || !s->vx) return -1;
+    visco_setup(which == VDO_KIND_VX ? s->vx : s->vy, vfx, vfy);
+    if (step_solve(s, which, t)) return -1;
+    return vdo_last_cycles(s, which);
 }

@@ -58,7 +58,33 @@ enum {
     VDO_GP, VDO_GTAU, VDO_GP_OLD, VDO_GTAU_OLD,     /* me, ml, me, ml */
     VDO_H, VDO_HX, VDO_HT, VDO_FBD, VDO_PG, VDO_PG_OLD, /* m, me, m, m, me, me */
     VDO_Z_MAC, VDO_Z_FEM, VDO_Z_VX, VDO_Z_VY,       /* mn, me*ne, mn, mn */
-    VDO_NARRAYS
+    VDO_NARRAYS,
+    /* oracle only: the step's scratch fields (padded), for pinning the passes */
+    VDO_UX = 64, VDO_UY, VDO_VXS, VDO_VYS, VDO_UR, VDO_VR, VDO_UU, VDO_VU, VDO_C2, VDO_C3
+};
+
+/* The passes of one step (fluid_2d.cc:93-405) in their order, for pinning
+ * each against facts of its own; vdo_pass runs one on the current arrays. */
+enum {
+    VDO_PASS_K1 = 1,         /* limited slopes + normal face extrapolation, fluid_2d.cc:105-132 */
+    VDO_PASS_K2,             /* Godunov choice (tangential form), :136-150 */
+    VDO_PASS_K3,             /* transverse terms c0..c3, :153-160 */
+    VDO_PASS_K4,             /* force predictor, :164-201 */
+    VDO_PASS_K5,             /* Godunov choice, :205-215 */
+    VDO_PASS_K6,             /* MAC right-hand side and s_bc, :218-232 */
+    VDO_PASS_K7,             /* MAC correction, :236-248 */
+    VDO_PASS_K8,             /* advective update, :253-261 */
+    VDO_PASS_INVISCID_BC,    /* far-field ghosts, fluid_2d_sio.cc:256-290 */
+    VDO_PASS_VISCOUS_BC,     /* left/bottom ghosts, fluid_2d_sio.cc:294-343 */
+    VDO_PASS_VISCO_RHS_U,    /* fluid_2d.cc:272-299 */
+    VDO_PASS_VISCO_RHS_V,    /* fluid_2d.cc:305-324 */
+    VDO_PASS_VISCO_COPY,     /* fluid_2d.cc:330-334 */
+    VDO_PASS_K14,            /* FEM right-hand side, fluid_2d.cc:338-377 */
+    VDO_PASS_K15,            /* velocity and pressure update, fluid_2d.cc:383-396 */
+    VDO_PASS_GAS_LIQUID_FIELDS, /* gas_layer_solve.cc:229-249 */
+    VDO_PASS_GAS_TAU_BC,     /* gas_layer.cc:25-30 */
+    VDO_PASS_GAS_HX,         /* gas_layer_solve.cc:58-62 */
+    VDO_PASS_GAS_INIT        /* gas_layer_solve.cc:32-53 */
 };
 
 /* Multigrid operator kinds. */
@@ -85,6 +111,17 @@ long vdo_array_len(const vdo_sim *s, int id);
 int vdo_get_array(const vdo_sim *s, int id, double *out, long len);
 int vdo_set_array(vdo_sim *s, int id, const double *in, long len);
 int vdo_set_time(vdo_sim *s, double t);
+/* One pass of the step on the current arrays (VDO_PASS_*); nonzero when the
+ * pass does not apply (viscous passes without implicit viscosity). */
+int vdo_pass(vdo_sim *s, int pass, double dt);
+/* mgs_common::initial_guess(t) (mgs_common.cc:17-50) on solver `which`'s z. */
+int vdo_initial_guess(vdo_sim *s, int which, double t);
+/* mgs_mac/mgs_fem::solve_v_cycle(t) (mgs_mac.hh:54-59, mgs_fem.hh:61-66) on the
+ * shared src: time-extrapolated guess, then V-cycles.  Returns the V-cycles
+ * or -1.  which = VDO_KIND_MAC or VDO_KIND_FEM. */
+int vdo_solve_v_cycle(vdo_sim *s, int which, double t);
+/* visco_impl::setup_and_solve(vfx, vfy, t) (visco_impl.cc:21-39). */
+int vdo_visco_setup_and_solve(vdo_sim *s, int which, double vfx, double vfy, double t);
 /* V-cycles spent by the last solve of each kind, and Newton steps of the
  * last gas update. */
 int vdo_last_cycles(const vdo_sim *s, int kind);

@@ -19,6 +19,12 @@ ORACLE_SO = os.path.join(ORACLE_DIR, "libvdo.so")
 (U, V, P, UL, VL, UD, VD, C0, C1, SRC, GP, GTAU, GP_OLD, GTAU_OLD, H, HX, HT, FBD, PG, PG_OLD,
  Z_MAC, Z_FEM, Z_VX, Z_VY) = range(24)
 KIND_MAC, KIND_FEM, KIND_VX, KIND_VY = 0, 1, 2, 3
+# oracle-only scratch fields of the step (oracle/vdo.h)
+UX, UY, VXS, VYS, UR, VR, UU, VU, C2, C3 = range(64, 74)
+# passes of the step (oracle/vdo.h, VDO_PASS_*)
+(PASS_K1, PASS_K2, PASS_K3, PASS_K4, PASS_K5, PASS_K6, PASS_K7, PASS_K8, PASS_INVISCID_BC, PASS_VISCOUS_BC,
+ PASS_VISCO_RHS_U, PASS_VISCO_RHS_V, PASS_VISCO_COPY, PASS_K14, PASS_K15, PASS_GAS_LIQUID_FIELDS,
+ PASS_GAS_TAU_BC, PASS_GAS_HX, PASS_GAS_INIT) = range(1, 20)
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _lib = None
@@ -60,6 +66,14 @@ def lib():
         L.vdo_set_time.argtypes = [vp, ctypes.c_double]
         L.vdo_last_cycles.argtypes = [vp, ctypes.c_int]
         L.vdo_last_newton.argtypes = [vp]
+        L.vdo_pass.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+        L.vdo_pass.restype = ctypes.c_int
+        L.vdo_initial_guess.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+        L.vdo_initial_guess.restype = ctypes.c_int
+        L.vdo_solve_v_cycle.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+        L.vdo_solve_v_cycle.restype = ctypes.c_int
+        L.vdo_visco_setup_and_solve.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        L.vdo_visco_setup_and_solve.restype = ctypes.c_int
         L.vdo_mg_create.restype = vp
         L.vdo_mg_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                     ctypes.c_double, ctypes.c_int]
@@ -153,6 +167,27 @@ class OracleSim:
 
     def last_cycles(self, kind):
         return lib().vdo_last_cycles(self.h, kind)
+
+    def run_pass(self, pass_id, dt=0.0):
+        """One pass of the step (PASS_*) on the current arrays."""
+        if lib().vdo_pass(self.h, pass_id, dt) != 0:
+            raise ValueError(f"pass {pass_id} does not apply")
+
+    def initial_guess(self, which, t):
+        assert lib().vdo_initial_guess(self.h, which, t) == 0
+
+    def solve_v_cycle(self, which, t):
+        """mgs_mac/mgs_fem::solve_v_cycle(t) on the shared src; V-cycles."""
+        c = lib().vdo_solve_v_cycle(self.h, which, t)
+        if c < 0:
+            raise RuntimeError("oracle solve failed")
+        return c
+
+    def visco_setup_and_solve(self, which, vfx, vfy, t):
+        c = lib().vdo_visco_setup_and_solve(self.h, which, vfx, vfy, t)
+        if c < 0:
+            raise RuntimeError("oracle viscous solve failed")
+        return c
 
     def last_newton(self):
         return lib().vdo_last_newton(self.h)

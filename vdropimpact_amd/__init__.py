@@ -104,6 +104,8 @@ ABI = {
     "vdi_mg_apply": (_i, [_vp, _dp, _dp, _l]),
     "vdi_mg_solve": (_i, [_vp, _d, _i, _ip]),
     "vdi_mg_vcycles": (_i, [_vp, _i]),
+    "vdi_mg_solve_fixed": (_i, [_vp, _d, _i, _ip, _ip]),
+    "vdi_mg_device_arrays": (_i, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
     "vdi_mg_residual": (_i, [_vp, _dp, _dp]),
     "vdi_mg_synchronize": (_i, [_vp]),
     "vdi_mg_cycle_bytes": (_i, [_vp, _dp]),
@@ -120,6 +122,7 @@ ABI = {
     "vdi_batch_step": (_i, [_vp]),
     "vdi_batch_autotune": (_i, [_vp, _i]),
     "vdi_batch_step_times": (_i, [_vp, _dp, _dp, _i]),
+    "vdi_batch_member_spans": (_i, [_vp, _dp, _i]),
     "vdi_batch_counters": (_i, [_vp, ctypes.POINTER(_l), ctypes.POINTER(_l)]),
     "vdi_batch_tiles": (_i, [_vp, _ip, _i]),
     "vdi_guess_attach": (_i, [_vp, _i, ctypes.c_ulonglong]),
@@ -469,6 +472,13 @@ class Batch:
         _check(self.lib.vdi_batch_step_times(self.h, ctypes.byref(g), _ptr(busy), self.size), "step_times")
         return g.value, busy
 
+    def member_spans(self) -> np.ndarray:
+        """Per-member step time (ms) of the last batched step: first work item
+        start to last work item end, from the device clock."""
+        out = np.zeros(self.size)
+        _check(self.lib.vdi_batch_member_spans(self.h, _ptr(out), self.size), "member_spans")
+        return out
+
     def counters(self):
         a, b = ctypes.c_long(), ctypes.c_long()
         _check(self.lib.vdi_batch_counters(self.h, ctypes.byref(a), ctypes.byref(b)), "counters")
@@ -555,6 +565,20 @@ class MGSystem:
 
     def vcycles(self, k: int):
         _check(self.lib.vdi_mg_vcycles(self.h, k), "vdi_mg_vcycles")
+
+    def device_arrays(self):
+        """(z, b) device addresses of the level-0 arrays (as ctypes void pointers)."""
+        z, b = _vp(), _vp()
+        _check(self.lib.vdi_mg_device_arrays(self.h, ctypes.byref(z), ctypes.byref(b)), "vdi_mg_device_arrays")
+        return z, b
+
+    def solve_fixed(self, rel_tol: float, budget: int):
+        """Fixed V-cycle budget without host reads in between (a captured
+        step's form); returns (done flag, V-cycles run)."""
+        d, c = ctypes.c_int(0), ctypes.c_int(0)
+        _check(self.lib.vdi_mg_solve_fixed(self.h, rel_tol, budget, ctypes.byref(d), ctypes.byref(c)),
+               "vdi_mg_solve_fixed")
+        return d.value, c.value
 
     def residual(self):
         r, b = ctypes.c_double(), ctypes.c_double()

@@ -72,10 +72,14 @@ struct vdi_batch {
     hipStream_t st = nullptr;
     // tile shape per [kind][prec][which][level]; which 0 down, 1 up, 2 mixed level-0 up
     std::vector<int> tiles[4][2][3];
-    DevBuf blob, coef_dev, rb_dev, busy_dev;
-    double *coef_host = nullptr;               // pinned, 8 per simulation
-    vdi_ctx::Readback *rb = nullptr;           // pinned, one per simulation
-    unsigned long long *busy_host = nullptr;   // pinned, one per simulation
+    DevBuf blob, coef_dev, busy_dev;
+    // pinned, mapped, coherent host memory the captured step's kernels read
+    // or write directly (the graph holds kernel nodes only): extrapolation
+    // coefficients (8 per simulation), readbacks (one per simulation) and the
+    // per-simulation clocks (3 per simulation, see busy_end)
+    double *coef_host = nullptr, *coef_map = nullptr;
+    vdi_ctx::Readback *rb = nullptr, *rb_map = nullptr;
+    unsigned long long *busy_host = nullptr, *busy_map = nullptr;
     std::vector<StepRec> recs_host;
     BLaunch step_l[SK_COUNT];
     size_t recs_off = 0;
@@ -255,7 +259,7 @@ static int build_plan(vdi_batch *b) {
     for (int s = 0; s < b->S; s++) {
         vdi_ctx *c = b->sims[s];
         StepRec a = step_record(c, c->P.dt_reg);
-        vdi_ctx::Readback *rbd = b->rb_dev.as<vdi_ctx::Readback>() + s;
+        vdi_ctx::Readback *rbd = b->rb_map + s;
         a.rb_st = rbd->st;
         a.rb_gas = rbd->gas;
         b->recs_host.push_back(a);
@@ -395,9 +399,13 @@ static int launch_solve(vdi_batch *b, int k, int budget) {
     return launch_vec(b, P.solve_end);
 }
 
-__global__ void k_zero_u64(unsigned long long *p, int n) {
+__global__ void k_busy_reset(unsigned long long *p, int n) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) p[k] = 0;
+    if (k < n) {
+        p[3 * k] = 0;
+        p[3 * k + 1] = ~0ull;
+        p[3 * k + 2] = 0;
+    }
 }
 
 static int capture(vdi_batch *b, vdi_batch::GraphEntry &e) {
@@ -405,9 +413,9 @@ static int capture(vdi_batch *b, vdi_batch::GraphEntry &e) {
     VDI_HIP(hipStreamBeginCapture(b->st, hipStreamCaptureModeRelaxed));
     int rc = VDI_OK;
     do {
-        if (hipMemcpyAsync(b->coef_dev.p, b->coef_host, 8 * sizeof(double) * b->S, hipMemcpyHostToDevice, b->st) !=
-            hipSuccess) { rc = VDI_ERR_HIP; break; }
-        hipLaunchKernelGGL(k_zero_u64, dim3(cdiv(b->S, 256)), dim3(256), 0, b->st, b->busy_dev.as<unsigned long long>(), b->S);
+        if ((rc = enqueue_fetch(b->coef_map, b->coef_dev.as<double>(), 8 * b->S, b->st)) != VDI_OK) break;
+        hipLaunchKernelGGL(k_busy_reset, dim3(cdiv(b->S, 256)), dim3(256), 0, b->st,
+                           b->busy_dev.as<unsigned long long>(), b->S);
         if ((rc = launch_step(b, SK_SNAPSHOT)) != VDI_OK) break;
         if ((rc = launch_step(b, SK_GAS)) != VDI_OK) break;
         if ((rc = launch_step(b, SK_GODUNOV)) != VDI_OK) break;
@@ -427,10 +435,9 @@ static int capture(vdi_batch *b, vdi_batch::GraphEntry &e) {
         if ((rc = launch_step(b, SK_UPDATE)) != VDI_OK) break;
         if ((rc = launch_step(b, SK_VISCOUS_BC)) != VDI_OK) break;
         if ((rc = launch_step(b, SK_GATHER)) != VDI_OK) break;
-        if (hipMemcpyAsync(b->rb, b->rb_dev.p, sizeof(vdi_ctx::Readback) * b->S, hipMemcpyDeviceToHost, b->st) !=
-            hipSuccess) { rc = VDI_ERR_HIP; break; }
-        if (hipMemcpyAsync(b->busy_host, b->busy_dev.p, sizeof(unsigned long long) * b->S, hipMemcpyDeviceToHost,
-                           b->st) != hipSuccess) { rc = VDI_ERR_HIP; break; }
+        // the gather wrote the readbacks into mapped host memory; the clocks follow
+        if ((rc = enqueue_fetch(reinterpret_cast<const double *>(b->busy_dev.p),
+                                reinterpret_cast<double *>(b->busy_map), 3 * b->S, b->st)) != VDI_OK) break;
     } while (0);
     hipError_t ec = hipStreamEndCapture(b->st, &graph);
     if (rc != VDI_OK) return rc;
@@ -574,11 +581,10 @@ int vdi_batch_create(const vdi_params *params, int count, int form, vdi_batch **
             }
     }
     VDI_TRY(b->coef_dev.alloc(8 * sizeof(double) * count));
-    VDI_TRY(b->rb_dev.alloc(sizeof(vdi_ctx::Readback) * count));
-    VDI_TRY(b->busy_dev.alloc(sizeof(unsigned long long) * count));
-    VDI_HIP(hipHostMalloc((void **)&b->coef_host, 8 * sizeof(double) * count, hipHostMallocDefault));
-    VDI_HIP(hipHostMalloc((void **)&b->rb, sizeof(vdi_ctx::Readback) * count, hipHostMallocDefault));
-    VDI_HIP(hipHostMalloc((void **)&b->busy_host, sizeof(unsigned long long) * count, hipHostMallocDefault));
+    VDI_TRY(b->busy_dev.alloc(3 * sizeof(unsigned long long) * count));
+    VDI_TRY(mapped_host_alloc((void **)&b->coef_host, (void **)&b->coef_map, 8 * sizeof(double) * count));
+    VDI_TRY(mapped_host_alloc((void **)&b->rb, (void **)&b->rb_map, sizeof(vdi_ctx::Readback) * count));
+    VDI_TRY(mapped_host_alloc((void **)&b->busy_host, (void **)&b->busy_map, 3 * sizeof(unsigned long long) * count));
     VDI_HIP(hipEventCreate(&b->ev0));
     VDI_HIP(hipEventCreate(&b->ev1));
     int dev = 0, khz = 0;
@@ -622,7 +628,16 @@ int vdi_batch_step_times(vdi_batch *b, double *graph_ms, double *busy_ms, int n)
     if (!b || !graph_ms || (busy_ms && n < b->S)) return VDI_ERR_ARG;
     *graph_ms = b->last_ms;
     if (busy_ms)
-        for (int s = 0; s < b->S; s++) busy_ms[s] = (double)b->busy_host[s] / b->ticks_per_ms;
+        for (int s = 0; s < b->S; s++) busy_ms[s] = (double)b->busy_host[3 * s] / b->ticks_per_ms;
+    return VDI_OK;
+}
+
+int vdi_batch_member_spans(vdi_batch *b, double *span_ms, int n) {
+    if (!b || !span_ms || n < b->S) return VDI_ERR_ARG;
+    for (int s = 0; s < b->S; s++) {
+        const unsigned long long t0 = b->busy_host[3 * s + 1], t1 = b->busy_host[3 * s + 2];
+        span_ms[s] = (t0 == ~0ull || t1 < t0) ? 0.0 : (double)(t1 - t0) / b->ticks_per_ms;
+    }
     return VDI_OK;
 }
 

@@ -715,7 +715,10 @@ template <class T, bool FEM>
 __global__ void __launch_bounds__(kLvThreads) k_down(LvArgs<T> a) {
     extern __shared__ double lds_d[];
     __shared__ double sh[10];
-    if (a.ctl.state->done) return;
+    // A launch that finishes the start-of-solve norms runs fin_norm in every
+    // workgroup, also in those dispatched after workgroup 0 has raised the
+    // flag: each must zero its own tile when the right-hand side is zero.
+    if (a.ctl.state->done && !a.fin_begin) return;
     stamp_begin(a.ts);
     down_body<T, FEM>(a, xcd_tile(blockIdx.x, a.nb), reinterpret_cast<T *>(lds_d), sh);
     stamp_end(a.ts);
@@ -1477,8 +1480,11 @@ static int default_tile(int m, int n) {
     return 1;
 }
 
-int MG::init(int kind_, int m, int n, double cx, double cy, int x_sym, double *z0, double *b0, hipStream_t stream) {
+int MG::init(int kind_, int m, int n, double cx, double cy, int x_sym, double *z0, double *b0, hipStream_t stream,
+             Arena *arena) {
     VDI_TRY(mgk::set_attributes());
+    arena_ = arena;
+    if (arena && (!z0 || !b0)) return VDI_ERR_ARG;
     kind = kind_;
     fem = kind == VDI_FEM;
     st = stream;
@@ -1521,12 +1527,12 @@ int MG::init(int kind_, int m, int n, double cx, double cy, int x_sym, double *z
     for (int l = 1; l < nl; l++) coarse += (size_t)lm[l] * ln[l];
     const long N0 = (long)m * n;
     // FP64: zalt on level 0; z, b, zalt on every coarser level
-    VDI_TRY(store_.alloc((N0 + 3 * coarse) * sizeof(double)));
+    VDI_TRY(store_.alloc((N0 + 3 * coarse) * sizeof(double), arena));
     // FP32 (mixed form): level 0 zalt and two r32 buffers; z, b, zalt below
-    VDI_TRY(store_f_.alloc((3 * (size_t)N0 + 3 * coarse) * sizeof(float)));
+    if (!arena) VDI_TRY(store_f_.alloc((3 * (size_t)N0 + 3 * coarse) * sizeof(float)));
     if (!z0) { VDI_TRY(own_z_.alloc(N0 * sizeof(double))); z0 = own_z_.as<double>(); }
     if (!b0) { VDI_TRY(own_b_.alloc(N0 * sizeof(double))); b0 = own_b_.as<double>(); }
-    VDI_TRY(zprev_.alloc(N0 * sizeof(double)));
+    if (!arena) VDI_TRY(zprev_.alloc(N0 * sizeof(double)));
     L.assign(nl, Level{});
     LF.assign(nl, LevelF{});
     double *p = store_.as<double>();
@@ -1540,27 +1546,30 @@ int MG::init(int kind_, int m, int n, double cx, double cy, int x_sym, double *z
             L[l].b = b0;
             L[l].zalt = p; p += Nl;
             LF[l].z = nullptr;
-            LF[l].zalt = q; q += Nl;
-            r32_[0] = q; q += Nl;
-            r32_[1] = q; q += Nl;
+            if (q) {
+                LF[l].zalt = q; q += Nl;
+                r32_[0] = q; q += Nl;
+                r32_[1] = q; q += Nl;
+            }
             LF[l].b = r32_[0];
         } else {
             L[l].z = p; p += Nl;
             L[l].b = p; p += Nl;
             L[l].zalt = p; p += Nl;
-            LF[l].z = q; q += Nl;
-            LF[l].b = q; q += Nl;
-            LF[l].zalt = q; q += Nl;
+            if (q) {
+                LF[l].z = q; q += Nl;
+                LF[l].b = q; q += Nl;
+                LF[l].zalt = q; q += Nl;
+            }
         }
     }
     {
         const long Nd = (long)lm[nl - 1] * ln[nl - 1];
-        if (Nd <= kDenseMax) {
-            VDI_TRY(inv64_.alloc(Nd * Nd * sizeof(double)));
-            VDI_TRY(inv32_.alloc(Nd * Nd * sizeof(float)));
-        }
+        inv_.reset();
+        inv_n_ = Nd <= kDenseMax ? Nd : 0;
+        if (arena && Nd > kDensePoints) return VDI_ERR_ARG;
     }
-    set_coeffs(cx, cy);
+    VDI_TRY(set_coeffs(cx, cy));
     for (int pr = 0; pr < 2; pr++)
         for (int w = 0; w < 2; w++) {
             tile[pr][w].assign(nl, 0);
@@ -1570,9 +1579,10 @@ int MG::init(int kind_, int m, int n, double cx, double cy, int x_sym, double *z
     // partial sums: one per tile of the smallest shape, two per begin block
     const TileShape &small = kTileShapes[1];
     long maxblk = std::max<long>(2048, (long)cdiv(lm[0], small.tx) * cdiv(ln[0], small.ty));
-    VDI_TRY(partial_.alloc(2 * maxblk * sizeof(double)));
-    VDI_TRY(state_.alloc(sizeof(MGState)));
-    if (!host_state_) VDI_HIP(hipHostMalloc((void **)&host_state_, sizeof(MGState), hipHostMallocDefault));
+    VDI_TRY(partial_.alloc(2 * maxblk * sizeof(double), arena));
+    VDI_TRY(state_.alloc(sizeof(MGState), arena));
+    if (!host_state_ && !arena)
+        VDI_HIP(hipHostMalloc((void **)&host_state_, sizeof(MGState), hipHostMallocDefault));
     coarse_lds_ = (kCoarseThreads + kDenseReg + 9 * kMaxCoarse) * sizeof(double);
     for (int l = lc; l < nl; l++) coarse_lds_ += 2 * sizeof(double) * (size_t)(lm[l] + 2) * (ln[l] + 2);
     return VDI_OK;
@@ -1594,7 +1604,7 @@ MG::~MG() {
     if (host_state_) (void)hipHostFree(host_state_);
 }
 
-void MG::set_coeffs(double cx, double cy) {
+void MG::level_coeffs(double cx, double cy) {
     for (int l = 0; l < nl; l++) {
         L[l].cx = cx;
         L[l].cy = cy;
@@ -1609,14 +1619,34 @@ void MG::set_coeffs(double cx, double cy) {
         cx *= 0.25;
         cy *= 0.25;
     }
-    (void)dense_inverse();
+}
+
+int MG::set_coeffs(double cx, double cy) {
+    level_coeffs(cx, cy);
+    return dense_inverse();
+}
+
+void MG::adopt_coeffs(double cx, double cy, const std::shared_ptr<CoarseInverse> &inv) {
+    level_coeffs(cx, cy);
+    if (kind == VDI_VX || kind == VDI_VY) {
+        vfx = cx;
+        vfy = cy;
+    }
+    inv_ = inv;
 }
 
 // Dense inverse of the coarsest level: columns A e_k from the level operator
 // applied to unit vectors, Gauss-Jordan elimination with partial pivoting on
 // the host, stored column-major (inv[k N + i] = (A^-1)_ik) in FP64 and FP32.
 int MG::dense_inverse() {
-    if (!inv64_.p) return VDI_OK;
+    if (inv_n_ == 0) return VDI_OK;
+    if (!inv_ || inv_.use_count() > 1) {
+        // a captured step still holds the current buffer: write a new one
+        auto fresh = std::make_shared<CoarseInverse>();
+        VDI_TRY(fresh->d64.alloc((size_t)inv_n_ * inv_n_ * sizeof(double), arena_));
+        VDI_TRY(fresh->d32.alloc((size_t)inv_n_ * inv_n_ * sizeof(float), arena_));
+        inv_ = fresh;
+    }
     const int l = nl - 1, M = lm[l], Nn = ln[l], N = M * Nn, W = 2 * N;
     std::vector<double> a((size_t)N * W, 0.0), e(N, 0.0);
     for (int k = 0; k < N; k++) {
@@ -1648,8 +1678,8 @@ int MG::dense_inverse() {
             t[(size_t)k * N + i] = a[(size_t)i * W + N + k];
             tf[(size_t)k * N + i] = (float)t[(size_t)k * N + i];
         }
-    VDI_HIP(hipMemcpy(inv64_.p, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
-    VDI_HIP(hipMemcpy(inv32_.p, tf.data(), tf.size() * sizeof(float), hipMemcpyHostToDevice));
+    VDI_HIP(hipMemcpy(inv_->d64.p, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+    VDI_HIP(hipMemcpy(inv_->d32.p, tf.data(), tf.size() * sizeof(float), hipMemcpyHostToDevice));
     return VDI_OK;
 }
 
@@ -1747,7 +1777,7 @@ CoarseArgs<T> MG::coarse_args(const std::vector<LevelT<T>> &H, double tol2) {
     d.gz = reinterpret_cast<double *>(H[lc].z);
     d.gb = reinterpret_cast<double *>(H[lc].b);
     d.top = lc == 0;
-    d.inv_dense = sizeof(T) == 8 ? inv64_.p : inv32_.p;
+    d.inv_dense = !inv_ ? nullptr : sizeof(T) == 8 ? inv_->d64.p : inv_->d32.p;
     a.cop = cc;
     a.fop = fe;
     a.fem = fem;
@@ -1976,7 +2006,23 @@ int MG::begin(double tol) {
     return mgk::norm(norm_args(1, tol * tol), st);
 }
 
+int MG::clear_solve_state() {
+    hipLaunchKernelGGL(k_zero, dim3(cdiv(sizeof(MGState) / sizeof(double), 256)), dim3(256), 0, st,
+                       state_.as<double>(), (long)(sizeof(MGState) / sizeof(double)));
+    VDI_LAUNCHED();
+    const long np = (long)(partial_.bytes / sizeof(double));
+    hipLaunchKernelGGL(k_zero, dim3(cdiv(np, 256)), dim3(256), 0, st, partial_.as<double>(), np);
+    VDI_LAUNCHED();
+    return VDI_OK;
+}
+
 int MG::read_state(MGState *out) {
+    if (!host_state_) {
+        // arena mode: no pinned buffer, a pageable copy
+        VDI_HIP(hipMemcpyAsync(out, state_.p, sizeof(MGState), hipMemcpyDeviceToHost, st));
+        VDI_HIP(hipStreamSynchronize(st));
+        return VDI_OK;
+    }
     VDI_HIP(hipMemcpyAsync(host_state_, state_.p, sizeof(MGState), hipMemcpyDeviceToHost, st));
     VDI_HIP(hipStreamSynchronize(st));
     *out = *host_state_;
@@ -2119,6 +2165,27 @@ int MG::set_tile(int prec, int which, int level, int shape) {
 int MG::set_schedule(int n1, int n2, int n1s, int n2s) {
     if (n1 < 0 || n2 < 0 || n1 + n2 == 0 || n1 > kMaxNu || n2 > kMaxNu) return VDI_ERR_ARG;
     if (n1s < 0 || n2s < 0 || n1s + n2s == 0 || n1s > kMaxNu || n2s > kMaxNu) return VDI_ERR_ARG;
+    // applied to the settings only when every tile fits: a rejected schedule
+    // leaves sweeps and tile shapes as they were
+    const int keep[4] = {nu1, nu2, nu1s, nu2s};
+    std::vector<int> keep_tile[2][2];
+    for (int pr = 0; pr < 2; pr++)
+        for (int w = 0; w < 2; w++) keep_tile[pr][w] = tile[pr][w];
+    const int keep_mx = tile_mx;
+    const int rc = apply_schedule(n1, n2, n1s, n2s);
+    if (rc != VDI_OK) {
+        nu1 = keep[0];
+        nu2 = keep[1];
+        nu1s = keep[2];
+        nu2s = keep[3];
+        for (int pr = 0; pr < 2; pr++)
+            for (int w = 0; w < 2; w++) tile[pr][w] = keep_tile[pr][w];
+        tile_mx = keep_mx;
+    }
+    return rc;
+}
+
+int MG::apply_schedule(int n1, int n2, int n1s, int n2s) {
     nu1 = n1;
     nu2 = n2;
     nu1s = n1s;

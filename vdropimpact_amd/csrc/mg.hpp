@@ -19,9 +19,18 @@
 // one launch advances the same stage of many independent systems.
 #pragma once
 
+#include <memory>
+
 #include "vdi_internal.hpp"
 
 namespace vdi {
+
+// Dense inverse of a coarsest level (FP64 and FP32 copies).  Shared: a
+// captured step keeps the inverse its coarse kernels were recorded with, and
+// a coefficient change while one is held writes a fresh buffer instead.
+struct CoarseInverse {
+    DevBuf d64, d32;
+};
 
 constexpr int kMaxCoarse = 12;  // levels handled by the single-workgroup kernel
 constexpr int kMaxNu = 4;       // largest nu1/nu2 the tiled kernels accept
@@ -212,10 +221,18 @@ class MG {
     KernelTimer *timer = nullptr;  // optional launch timing
 
     // Sets up the hierarchy.  z0/b0 are the level-0 solution and RHS buffers
-    // (owned by the caller); when null the MG allocates its own.
+    // (owned by the caller); when null the MG allocates its own.  With an
+    // arena every device buffer is a piece of it and nothing is allocated
+    // (FP64 form only: no FP32 hierarchy, no time-extrapolation history, no
+    // pinned readback; coarsest dense level of at most kDensePoints).
     int init(int kind, int m, int n, double cx, double cy, int x_sym, double *z0, double *b0,
-             hipStream_t stream);
-    void set_coeffs(double cx, double cy);
+             hipStream_t stream, Arena *arena = nullptr);
+    // Sets the level couplings and recomputes the dense inverse; returns its status.
+    int set_coeffs(double cx, double cy);
+    // The coarse inverse in use, and re-adopting one recorded earlier for the
+    // couplings (cx, cy) (no recomputation; the level couplings follow).
+    std::shared_ptr<CoarseInverse> coarse_inverse() const { return inv_; }
+    void adopt_coeffs(double cx, double cy, const std::shared_ptr<CoarseInverse> &inv);
     int unknowns() const { return lm[0] * ln[0]; }
     double *z() const { return L[0].z; }
     double *b() const { return L[0].b; }
@@ -303,8 +320,11 @@ class MG {
     int set_coarse_points(int pts);
     // Recomputes the coarsest level's dense inverse (after a coefficient change).
     int dense_inverse();
+    // Zeroes the solve state and the partial sums (fill kernels on st).
+    int clear_solve_state();
 
   private:
+    int apply_schedule(int nu1, int nu2, int nu1s, int nu2s);
     template <class T> int launch_down(const std::vector<LevelT<T>> &H, int l, int zero_z, int fin = 0,
                                        double tol2 = 0, int fin_begin = 0);
     template <class T> int launch_up(const std::vector<LevelT<T>> &H, int l, int norm, double tol2);
@@ -318,7 +338,10 @@ class MG {
     DevBuf partial_;
     DevBuf state_;
     DevBuf own_z_, own_b_;
-    DevBuf inv64_, inv32_;  // dense inverse of the coarsest level
+    Arena *arena_ = nullptr;              // set: buffers come from it (see init)
+    std::shared_ptr<CoarseInverse> inv_;  // dense inverse of the coarsest level (null: none)
+    long inv_n_ = 0;                       // its unknowns
+    void level_coeffs(double cx, double cy);
     MGState *host_state_ = nullptr;  // pinned
     size_t coarse_lds_ = 0;
 

@@ -853,6 +853,23 @@ __global__ void __launch_bounds__(kGasThreads) k_gas(StepRec a, Stamp *ts) {
 }
 __global__ void __launch_bounds__(256) k_snapshot(StepRec a, int restore) { snapshot_body(a, blockIdx.x, restore); }
 __global__ void __launch_bounds__(64) k_gather_status(StepRec a) { gather_body(a); }
+__global__ void __launch_bounds__(256) k_fetch(const double *__restrict__ src, double *__restrict__ dst, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) dst[k] = src[k];
+}
+
+int mapped_host_alloc(void **host, void **dev, size_t bytes) {
+    VDI_HIP(hipHostMalloc(host, bytes, hipHostMallocMapped | hipHostMallocCoherent));
+    memset(*host, 0, bytes);
+    VDI_HIP(hipHostGetDevicePointer(dev, *host, 0));
+    return VDI_OK;
+}
+int enqueue_fetch(const double *src, double *dst, int n, hipStream_t st) {
+    if (n < 1) return VDI_ERR_ARG;
+    hipLaunchKernelGGL(k_fetch, dim3(cdiv(n, 256)), dim3(256), 0, st, src, dst, n);
+    VDI_LAUNCHED();
+    return VDI_OK;
+}
 
 __global__ void k_copy(const double *__restrict__ a, double *__restrict__ b, long n) {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) b[i] = a[i];
@@ -1007,19 +1024,20 @@ StepRec step_record(vdi_ctx *c, double dt) {
     a.np = (long)g.ml * (g.n + 4);
     a.ng = (long)(c->gasbuf.bytes / sizeof(double));
     for (int k = 0; k < 4; k++) a.mgst[k] = solver_used(c, k) ? step_solver(c, k)->state() : nullptr;
-    vdi_ctx::Readback *rbd = c->rb_dev.as<vdi_ctx::Readback>();
+    vdi_ctx::Readback *rbd = c->rb_map;
     a.rb_st = rbd->st;
     a.rb_gas = rbd->gas;
     return a;
 }
 
 // visco_impl.cc:21-39: re-set the hierarchy when the coupling changed.
-void visco_setup(MG &g, double vfx, double vfy) {
+int visco_setup(MG &g, double vfx, double vfy) {
     if (std::fabs(vfx - g.vfx) > 1e2 * std::numeric_limits<double>::epsilon() * g.vfx) {
         g.vfx = vfx;
         g.vfy = vfy;
-        g.set_coeffs(vfx, vfy);
+        return g.set_coeffs(vfx, vfy);
     }
+    return VDI_OK;
 }
 
 }  // namespace vdi
@@ -1112,9 +1130,8 @@ int vdi_create(const vdi_params *p, vdi_ctx **out) {
     }
     VDI_TRY(c->snap.alloc((3 * NP + ng) * sizeof(double)));
     VDI_TRY(c->coef_dev.alloc(8 * sizeof(double)));
-    VDI_TRY(c->rb_dev.alloc(sizeof(vdi_ctx::Readback)));
-    VDI_HIP(hipHostMalloc((void **)&c->coef_host, 8 * sizeof(double), hipHostMallocDefault));
-    VDI_HIP(hipHostMalloc((void **)&c->rb, sizeof(vdi_ctx::Readback), hipHostMallocDefault));
+    VDI_TRY(mapped_host_alloc((void **)&c->coef_host, (void **)&c->coef_map, 8 * sizeof(double)));
+    VDI_TRY(mapped_host_alloc((void **)&c->rb, (void **)&c->rb_map, sizeof(vdi_ctx::Readback)));
     c->gas_lds = stepk::gas_lds(g);
     static size_t gas_attr = 0;
     if (c->gas_lds > 64 * 1024 && c->gas_lds > gas_attr) {
@@ -1141,8 +1158,25 @@ int vdi_set_solver(vdi_ctx *c, int nu1, int nu2, double rel_tol, int max_cycles)
     VDI_HIP(hipStreamSynchronize(c->st));
     c->drop_graphs();
     MG *gs[4] = {&c->mac, &c->fem, &c->vx, &c->vy};
-    for (MG *g : gs)
-        if (g->nl && !keep) VDI_TRY(g->set_schedule(nu1, nu2));
+    if (!keep) {
+        // all four or none: a schedule one solver rejects restores the others
+        int old[4][4];
+        for (int k = 0; k < 4; k++) {
+            old[k][0] = gs[k]->nu1;
+            old[k][1] = gs[k]->nu2;
+            old[k][2] = gs[k]->nu1s;
+            old[k][3] = gs[k]->nu2s;
+        }
+        for (int k = 0; k < 4; k++) {
+            if (!gs[k]->nl) continue;
+            const int rc = gs[k]->set_schedule(nu1, nu2);
+            if (rc != VDI_OK) {
+                for (int q = 0; q < k; q++)
+                    if (gs[q]->nl) (void)gs[q]->set_schedule(old[q][0], old[q][1], old[q][2], old[q][3]);
+                return rc;
+            }
+        }
+    }
     c->rel_tol = rel_tol;
     c->max_cycles = max_cycles;
     return VDI_OK;
@@ -1262,8 +1296,15 @@ extern "C" int vdi_mg_solve_v_cycle(vdi_ctx *c, int which, double t, int *vcycle
 
 extern "C" int vdi_visco_setup_and_solve(vdi_ctx *c, int which, double vfx, double vfy, double t, int *vcycles) {
     if (!c || !c->P.implicit_visc || (which != VDI_VX && which != VDI_VY)) return VDI_ERR_ARG;
+    if (!(vfx >= 0) || !(vfy >= 0)) return VDI_ERR_ARG;
     MG &g = which == VDI_VX ? c->vx : c->vy;
-    visco_setup(g, vfx, vfy);
+    VDI_TRY(visco_setup(g, vfx, vfy));
+    // Both reference solvers read the shared src (visco_impl.cc:13); inside
+    // the step the v solve has a right-hand side buffer of its own (so that
+    // the two solves can run side by side), which this call fills from src.
+    if (which == VDI_VY)
+        VDI_HIP(hipMemcpyAsync(c->F.src2, c->F.src, (size_t)c->g.m * c->g.n * sizeof(double),
+                               hipMemcpyDeviceToDevice, c->st));
     return solve_mg(c, g, t, vcycles);
 }
 
@@ -1350,8 +1391,8 @@ static int enqueue_body(vdi_ctx *c, const StepRec &a, double t0, SolveMode mode,
     if (c->P.implicit_visc) {
         hipLaunchKernelGGL(k_visco_rhs, gc, dim3(256), 0, c->st, a, 0);
         VDI_LAUNCHED();
-        visco_setup(c->vx, a.vfx, a.vfy);
-        visco_setup(c->vy, a.vfx, a.vfy);
+        VDI_TRY(visco_setup(c->vx, a.vfx, a.vfy));
+        VDI_TRY(visco_setup(c->vy, a.vfx, a.vfy));
         if (mode == SOLVE_FIXED && c->par_visco) {
             // The u and v solves are independent (separate right-hand sides
             // and solutions): in a captured step the v branch (its right-hand
@@ -1413,12 +1454,23 @@ static int capture_step(vdi_ctx *c, double dt, vdi_ctx::GraphEntry &e) {
     const Geo &g = c->g;
     hipGraph_t graph;
     e.slot_a = c->timer.next;
+    // The viscous couplings of this time step (a new coarse inverse when they
+    // change) are set up before capture, so that nothing inside the capture
+    // allocates or copies; the graph keeps the inverses it was recorded with.
+    if (c->P.implicit_visc) {
+        VDI_TRY(visco_setup(c->vx, a.vfx, a.vfy));
+        VDI_TRY(visco_setup(c->vy, a.vfx, a.vfy));
+        e.vfx = c->vx.vfx;
+        e.vfy = c->vx.vfy;
+        e.vinv[0] = c->vx.coarse_inverse();
+        e.vinv[1] = c->vy.coarse_inverse();
+    }
     VDI_HIP(hipStreamBeginCapture(c->st, hipStreamCaptureModeRelaxed));
     int rc = VDI_OK;
     do {
-        // extrapolation coefficients of this step, written by the host
-        if (hipMemcpyAsync(c->coef_dev.p, c->coef_host, 8 * sizeof(double), hipMemcpyHostToDevice, c->st) !=
-            hipSuccess) { rc = VDI_ERR_HIP; break; }
+        // extrapolation coefficients of this step, written by the host into
+        // mapped memory before each replay
+        if ((rc = enqueue_fetch(c->coef_map, c->coef_dev.as<double>(), 8, c->st)) != VDI_OK) break;
         // the slots this graph will use are cleared at its start; their
         // count is known only after capture, so clear the rest of the pool
         if (c->timer.cls >= 0 && (rc = c->timer.reset(e.slot_a, c->timer.cap, c->st)) != VDI_OK) break;
@@ -1429,12 +1481,8 @@ static int capture_step(vdi_ctx *c, double dt, vdi_ctx::GraphEntry &e) {
             if ((rc = enqueue_gas(c, a)) != VDI_OK) break;
             if ((rc = enqueue_body(c, a, c->time, SOLVE_FIXED)) != VDI_OK) break;
         }
+        // the solve states and the gas status, written into mapped host memory
         hipLaunchKernelGGL(k_gather_status, dim3(1), dim3(64), 0, c->st, a);
-        if (hipMemcpyAsync(c->rb, c->rb_dev.p, sizeof(vdi_ctx::Readback), hipMemcpyDeviceToHost, c->st) !=
-            hipSuccess) {
-            rc = VDI_ERR_HIP;
-            break;
-        }
     } while (0);
     hipError_t ec = hipStreamEndCapture(c->st, &graph);
     if (rc != VDI_OK) return rc;
@@ -1477,6 +1525,15 @@ static int step_graph(vdi_ctx *c, double dt) {
     for (auto &x : c->graphs)
         if (x.dt == dt && x.form == c->form && std::equal(x.budget, x.budget + 4, c->budget)) e = &x;
     if (!e) {
+        if ((int)c->graphs.size() >= vdi_ctx::kMaxGraphs) {
+            // bounded cache: destroy the least recently replayed graph
+            auto lru = std::min_element(c->graphs.begin(), c->graphs.end(),
+                                        [](const vdi_ctx::GraphEntry &x, const vdi_ctx::GraphEntry &y) {
+                                            return x.used < y.used;
+                                        });
+            (void)hipGraphExecDestroy(lru->exec);
+            c->graphs.erase(lru);
+        }
         vdi_ctx::GraphEntry ne{};
         std::copy(c->budget, c->budget + 4, ne.budget);
         ne.dt = dt;
@@ -1484,7 +1541,14 @@ static int step_graph(vdi_ctx *c, double dt) {
         VDI_TRY(capture_step(c, dt, ne));
         c->graphs.push_back(ne);
         e = &c->graphs.back();
+    } else if (c->P.implicit_visc && (c->vx.vfx != e->vfx || c->vy.vfx != e->vfx)) {
+        // another time step was set up since this graph was recorded: the
+        // host-side couplings and inverses go back to the graph's (a redo on
+        // the direct path then finds them consistent)
+        c->vx.adopt_coeffs(e->vfx, e->vfy, e->vinv[0]);
+        c->vy.adopt_coeffs(e->vfx, e->vfy, e->vinv[1]);
     }
+    e->used = ++c->graph_clock;
     VDI_HIP(hipGraphLaunch(e->exec, c->st));
     VDI_HIP(hipStreamSynchronize(c->st));
     if (c->timer.cls >= 0) VDI_TRY(c->timer.harvest(e->slot_a, e->slot_b, c->st));
@@ -1868,6 +1932,23 @@ int vdi_mg_solve(vdi_mg *h, double rel_tol, int max_cycles, int *vcycles) {
 int vdi_mg_vcycles(vdi_mg *h, int k) {
     if (!h || k < 0) return VDI_ERR_ARG;
     return h->g.vcycles(k);
+}
+
+int vdi_mg_device_arrays(vdi_mg *h, double **z_dev, double **b_dev) {
+    if (!h || !z_dev || !b_dev) return VDI_ERR_ARG;
+    *z_dev = h->g.z();
+    *b_dev = h->g.b();
+    return VDI_OK;
+}
+
+int vdi_mg_solve_fixed(vdi_mg *h, double rel_tol, int budget, int *done, int *vcycles) {
+    if (!h || !done || !vcycles || !(rel_tol > 0) || budget < 0) return VDI_ERR_ARG;
+    VDI_TRY(h->g.enqueue_fixed(rel_tol, budget));
+    MGState st;
+    VDI_TRY(h->g.read_state(&st));
+    *done = st.done;
+    *vcycles = st.cycles;
+    return VDI_OK;
 }
 
 int vdi_mg_residual(vdi_mg *h, double *rnorm, double *bnorm) {

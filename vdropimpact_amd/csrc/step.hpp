@@ -153,7 +153,16 @@ struct vdi_ctx {
         int form;
         hipGraphExec_t exec;
         int slot_a, slot_b;  // timer slots owned by the graph
+        // couplings and coarse inverses of the two viscous solves the graph's
+        // kernels were recorded with (re-adopted before a replay)
+        double vfx, vfy;
+        std::shared_ptr<vdi::CoarseInverse> vinv[2];
+        long used;  // replay clock of the last launch (least recently used is evicted)
     };
+    // Cached graphs (distinct budget tuples, time steps and forms); the least
+    // recently used one is destroyed when a new one would exceed this.
+    static constexpr int kMaxGraphs = 8;
+    long graph_clock = 0;
     struct Readback {
         vdi::MGState st[4];
         int gas[3];
@@ -181,9 +190,14 @@ struct vdi_ctx {
         budget[k] = b;
     }
     std::vector<GraphEntry> graphs;
-    vdi::DevBuf snap, coef_dev, rb_dev;
-    double *coef_host = nullptr;  // pinned, 2 extrapolation coefficients per solver
-    Readback *rb = nullptr;       // pinned
+    vdi::DevBuf snap, coef_dev;
+    // Host memory the captured graph's kernels read and write directly
+    // (pinned, mapped, coherent): the extrapolation coefficients of the next
+    // step (2 per solver), fetched into coef_dev by the graph's first kernel,
+    // and the step's readback, written by its last kernel.  A captured step
+    // thereby holds kernel nodes only.
+    double *coef_host = nullptr, *coef_map = nullptr;
+    Readback *rb = nullptr, *rb_map = nullptr;
     long graph_steps = 0, redo_steps = 0;
     vdi::LearnedGuess *guess = nullptr;  // optional learned MAC initial guess
     void drop_graphs() {
@@ -222,5 +236,10 @@ double end_time(double t0, double dt);
 // Restores the snapshot and runs the step on the direct path (a redo).
 int redo_step(vdi_ctx *c, double dt, double t0);
 // visco_impl.cc:26-30
-void visco_setup(MG &g, double vfx, double vfy);
+int visco_setup(MG &g, double vfx, double vfy);
+// Pinned host memory that kernels read and write directly (mapped, coherent):
+// *host for the host, *dev for kernel arguments.
+int mapped_host_alloc(void **host, void **dev, size_t bytes);
+// dst[0..n) = src[0..n) of doubles (or 8-byte words), one capturable kernel.
+int enqueue_fetch(const double *src, double *dst, int n, hipStream_t st);
 }  // namespace vdi

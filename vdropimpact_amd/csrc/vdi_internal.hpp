@@ -35,22 +35,44 @@ void set_error(const char *what, hipError_t e, const char *file, int line);
 
 inline unsigned cdiv(long a, long b) { return (unsigned)((a + b - 1) / b); }
 
-// Device buffer owned by the library.
+// Device memory handed out from a region the caller owns (no allocation):
+// 256-byte aligned pieces, in order, until the region is used up.
+struct Arena {
+    char *base = nullptr;
+    size_t cap = 0, used = 0;
+};
+
+// Device buffer: owned by the library (alloc), or a piece of an Arena
+// (borrowed, never freed here).
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    bool owned = false;
     int alloc(size_t nbytes) {
         release();
         bytes = nbytes;
         if (nbytes == 0) return VDI_OK;
         VDI_HIP(hipMalloc(&p, nbytes));
+        owned = true;
         VDI_HIP(hipMemset(p, 0, nbytes));
         return VDI_OK;
     }
+    // A piece of the arena when one is given, otherwise an allocation.
+    int alloc(size_t nbytes, Arena *arena) {
+        if (!arena) return alloc(nbytes);
+        release();
+        const size_t off = (arena->used + 255) & ~(size_t)255;
+        if (off + nbytes > arena->cap) return VDI_ERR_ARG;
+        p = arena->base + off;
+        bytes = nbytes;
+        arena->used = off + nbytes;
+        return VDI_OK;
+    }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && owned) (void)hipFree(p);
         p = nullptr;
         bytes = 0;
+        owned = false;
     }
     template <class T> T *as() const { return static_cast<T *>(p); }
     ~DevBuf() { release(); }
@@ -111,12 +133,20 @@ __device__ __forceinline__ void stamp_end(Stamp *ts) {
     __syncthreads();
     if (threadIdx.x == 0 && threadIdx.y == 0) atomicMax(&ts->t1, (unsigned long long)wall_clock64());
 }
-// Workgroup busy time of one work item of a batched launch, per simulation.
+// Per-simulation clocks of a batched step, three counters per simulation:
+// busy[3 s] the workgroup busy ticks summed over its work items, busy[3 s + 1]
+// the start of its first work item and busy[3 s + 2] the end of its last one
+// (its step's span inside the batched step).
 __device__ __forceinline__ unsigned long long busy_begin(const unsigned long long *busy) {
     return busy ? (unsigned long long)wall_clock64() : 0ull;
 }
 __device__ __forceinline__ void busy_end(unsigned long long *busy, int sim, unsigned long long t0) {
-    if (busy && threadIdx.x == 0 && threadIdx.y == 0) atomicAdd(&busy[sim], (unsigned long long)wall_clock64() - t0);
+    if (busy && threadIdx.x == 0 && threadIdx.y == 0) {
+        const unsigned long long t1 = (unsigned long long)wall_clock64();
+        atomicAdd(&busy[3 * sim], t1 - t0);
+        atomicMin(&busy[3 * sim + 1], t0);
+        atomicMax(&busy[3 * sim + 2], t1);
+    }
 }
 #endif
 
