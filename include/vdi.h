@@ -220,6 +220,21 @@ int vdi_mg_synchronize(vdi_mg *g);
 /* Algorithmic bytes of one V-cycle in the current form (measurement). */
 int vdi_mg_cycle_bytes(vdi_mg *g, double *bytes);
 
+/* Size of the device scratch array of vdi_mg_solve_noalloc, per GPU. */
+#define VDI_NOALLOC_SCRATCH_MB 24
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): the same solve
+ * as vdi_mg_create(kind, m, n, cx, cy, x_sym) + vdi_mg_solve(rel_tol,
+ * max_cycles) with the default schedule and tiles, on caller-owned device
+ * arrays b_dev (read) and z_dev (initial guess in, solution out), on the null
+ * stream, returning when done.  Every other buffer lives in one static device
+ * array of VDI_NOALLOC_SCRATCH_MB MiB per GPU (enough for 1025 x 513
+ * unknowns); larger systems, and ones whose coarsening stops above the dense
+ * coarsest size, return VDI_ERR_ARG.  FP64 form only.  Calls on one GPU share
+ * that array: calls that overlap in time race. */
+int vdi_mg_solve_noalloc(int kind, int m, int n, double cx, double cy, int x_sym, const double *b_dev,
+                         double *z_dev, double rel_tol, int max_cycles, int *vcycles);
+
 /* ---- tile shapes of the multigrid level kernels ----
  * No counterpart in the reference (TGMG's sweeps are not tiled).  Shapes are
  * indexes into a fixed candidate list (vdi_tile_shape gives tx, ty).

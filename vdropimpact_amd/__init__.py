@@ -106,6 +106,7 @@ ABI = {
     "vdi_mg_vcycles": (_i, [_vp, _i]),
     "vdi_mg_solve_fixed": (_i, [_vp, _d, _i, _ip, _ip]),
     "vdi_mg_device_arrays": (_i, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
+    "vdi_mg_solve_noalloc": (_i, [_i, _i, _i, _d, _d, _i, _vp, _vp, _d, _i, _ip]),
     "vdi_mg_residual": (_i, [_vp, _dp, _dp]),
     "vdi_mg_synchronize": (_i, [_vp]),
     "vdi_mg_cycle_bytes": (_i, [_vp, _dp]),

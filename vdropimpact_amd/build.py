@@ -16,7 +16,7 @@ CSRC = os.path.join(PKG, "csrc")
 LIBDIR = os.path.join(PKG, "_lib")
 LIB = os.path.join(LIBDIR, "libvdi_hip.so")
 
-SOURCES = ["mg.hip", "step.hip", "batch.hip", "nn.hip"]
+SOURCES = ["mg.hip", "step.hip", "batch.hip", "nn.hip", "noalloc.hip"]
 HEADERS = ["vdi_internal.hpp", "mg.hpp", "step.hpp", "nn.hpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
