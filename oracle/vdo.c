@@ -577,6 +577,10 @@ struct vdo_sim {
     double *zmac, *zfem, *zvx, *zvy; /* solver solutions (mgs_common.hh:38) */
     double rel_tol;
     int max_cycles, last_newton;
+    /* the reference's quirks Q1 (tau written 2 cells off) and Q13 (in-place
+     * height sweep); 1 = as in the reference (the default).  Tests switch
+     * them off to check the mirror symmetry they break. */
+    int q1_tau_shift, q13_inplace;
 };
 
 #define AT(a, i, j) ((a)[((size_t)(j) + 2) * ml + (size_t)((i) + 2)])
@@ -625,7 +629,14 @@ vdo_sim *vdo_create(const vdo_params *p) {
         s->zvy = (double *)xcalloc(s->mn, sizeof(double));
     }
     s->rel_tol = 1e-10; s->max_cycles = 200;
+    s->q1_tau_shift = s->q13_inplace = 1;
     return s;
+}
+
+int vdo_set_quirks(vdo_sim *s, int q1_tau_shift, int q13_inplace) {
+    s->q1_tau_shift = q1_tau_shift != 0;
+    s->q13_inplace = q13_inplace != 0;
+    return 0;
 }
 
 void vdo_destroy(vdo_sim *s) {
@@ -751,7 +762,8 @@ static void gas_liquid_fields(vdo_sim *s) {
     } else { p[0] = pg[0]; p[1] = pg[1]; }
     for (int i = 2; i < m - 1; i++) p[i] = pg[i] + fac * (h[i + 1] - h[i] - h[i - 1] + h[i - 2]) - fac2;
     p[m - 1] = pg[m - 1]; p[m] = pg[m];
-    for (int i = 0; i < m; i++) s->gtau[i] = -0.5 * s->gxsp * s->P.mug * (p[i + 1] - p[i]) * h[i];
+    const int off = s->q1_tau_shift ? 0 : 2;
+    for (int i = 0; i < m; i++) s->gtau[i + off] = -0.5 * s->gxsp * s->P.mug * (p[i + 1] - p[i]) * h[i];
 }
 
 /* gas_layer.cc:25-30 */
@@ -793,6 +805,20 @@ static inline double eno2(double p0, double p1, double p2, double p3) {
 static void gas_update_height(vdo_sim *s, double dt) {
     const int m = s->m, ml = s->ml;
     const double *u = s->F[F_U], *v = s->F[F_V];
+    if (!s->q13_inplace) {
+        /* every cell from the old heights, then all updated */
+        for (int i = 0; i < m; i++) {
+            double uc = 0.25 * s->gxsp * (AT(u, i, 0) + AT(u, i, -1));
+            double vc = 0.5 * (AT(v, i, 0) + AT(v, i, -1));
+            double dhdx = uc > 0 ? eno2(gas_h_ref(s, i + 1), s->h[i], gas_h_ref(s, i - 1), gas_h_ref(s, i - 2))
+                                 : -eno2(gas_h_ref(s, i - 1), s->h[i], gas_h_ref(s, i + 1), gas_h_ref(s, i + 2));
+            s->ht[i] = -uc * dhdx + vc - s->Vframe;
+            s->fbd[i] += dt * vc;
+        }
+        for (int i = 0; i < m; i++) s->h[i] += dt * s->ht[i];
+        gas_calculate_hx(s);
+        return;
+    }
     for (int i = 0; i < m; i++) {
         double uc = 0.25 * s->gxsp * (AT(u, i, 0) + AT(u, i, -1));
         double vc = 0.5 * (AT(v, i, 0) + AT(v, i, -1));

@@ -111,6 +111,11 @@ long vdo_array_len(const vdo_sim *s, int id);
 int vdo_get_array(const vdo_sim *s, int id, double *out, long len);
 int vdo_set_array(vdo_sim *s, int id, const double *in, long len);
 int vdo_set_time(vdo_sim *s, double t);
+/* Test switches for two reference quirks that break mirror symmetry: Q1
+ * (the gas shear of cell i written at tau[i] and read at tau[i+2]) and Q13
+ * (the height update sweeping in place).  1 = as in the reference (default);
+ * 0 = shear written at tau[i+2], heights updated from the old values. */
+int vdo_set_quirks(vdo_sim *s, int q1_tau_shift, int q13_inplace);
 /* One pass of the step on the current arrays (VDO_PASS_*); nonzero when the
  * pass does not apply (viscous passes without implicit viscosity). */
 int vdo_pass(vdo_sim *s, int pass, double dt);

@@ -66,6 +66,7 @@ def lib():
         L.vdo_set_time.argtypes = [vp, ctypes.c_double]
         L.vdo_last_cycles.argtypes = [vp, ctypes.c_int]
         L.vdo_last_newton.argtypes = [vp]
+        L.vdo_set_quirks.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.vdo_pass.argtypes = [vp, ctypes.c_int, ctypes.c_double]
         L.vdo_pass.restype = ctypes.c_int
         L.vdo_initial_guess.argtypes = [vp, ctypes.c_int, ctypes.c_double]
@@ -167,6 +168,10 @@ class OracleSim:
 
     def last_cycles(self, kind):
         return lib().vdo_last_cycles(self.h, kind)
+
+    def set_quirks(self, q1_tau_shift=1, q13_inplace=1):
+        """Test switches of the reference quirks Q1 and Q13 (1 = as in the reference)."""
+        lib().vdo_set_quirks(self.h, q1_tau_shift, q13_inplace)
 
     def run_pass(self, pass_id, dt=0.0):
         """One pass of the step (PASS_*) on the current arrays."""
