@@ -13,7 +13,7 @@ import numpy as np
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(REPO, "oracle")
-ORACLE_SO = os.path.join(ORACLE_DIR, "libvdo.so")
+ORACLE_SO = os.environ.get("VDO_ORACLE_SO") or os.path.join(ORACLE_DIR, "libvdo.so")
 
 # array ids (oracle/vdo.h, same numbering as include/vdi.h)
 (U, V, P, UL, VL, UD, VD, C0, C1, SRC, GP, GTAU, GP_OLD, GTAU_OLD, H, HX, HT, FBD, PG, PG_OLD,
@@ -31,6 +31,8 @@ _lib = None
 
 
 def build(force: bool = False) -> str:
+    if os.environ.get("VDO_ORACLE_SO"):
+        return ORACLE_SO          # a prebuilt variant (mutation tests)
     if force or not os.path.exists(ORACLE_SO):
         subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
     return ORACLE_SO
