@@ -156,6 +156,17 @@ int vdi_mg_solve_v_cycle(vdi_ctx *ctx, int which, double t, int *vcycles);
 /* visco_impl::setup_and_solve(double vfx, double vfy, double t)
  * (visco_impl.cc:21-39).  which = VDI_VX or VDI_VY. */
 int vdi_visco_setup_and_solve(vdi_ctx *ctx, int which, double vfx, double vfy, double t, int *vcycles);
+/* The advection passes alone, on the context's current arrays, reps times
+ * (the average launch time in *ms, HIP events).  VDI_PASS_GODUNOV: limited
+ * slopes, half-step face predictor, upwind choices and the MAC right-hand
+ * side (fluid_2d.cc:105-232, gas pressure from VDI_GP/VDI_GP_OLD);
+ * reads u, v, p, writes ul, vl, ud, vd, c0, c1 and src.
+ * VDI_PASS_CORRECT_ADVECT: MAC correction of the faces by VDI_Z_MAC and the
+ * advective update (fluid_2d.cc:236-261); writes c0, c1 (the corrected faces
+ * are not stored).  No reference counterpart as separate calls. */
+#define VDI_PASS_GODUNOV 0
+#define VDI_PASS_CORRECT_ADVECT 1
+int vdi_run_pass(vdi_ctx *ctx, int which, double dt, int reps, double *ms);
 /* Host copies of state arrays (ids above); len must equal vdi_array_len. */
 long vdi_array_len(vdi_ctx *ctx, int id);
 int vdi_get_array(vdi_ctx *ctx, int id, double *host, long len);

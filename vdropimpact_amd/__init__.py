@@ -31,6 +31,8 @@ OK, ERR_MG_NOCONV, ERR_NEWTON_NOCONV, ERR_TRIDIAG, ERR_HIP, ERR_ARG = range(6)
 MAC, FEM, VX, VY = 0, 1, 2, 3
 # arithmetic forms
 FORM_FP64, FORM_MIXED = 0, 1
+# passes for Fluid2D.run_pass
+PASS_GODUNOV, PASS_CORRECT_ADVECT = 0, 1
 # kernel classes for Fluid2D.time_kernels
 KCLASS_MG_DOWN0, KCLASS_MG_UP0, KCLASS_MG_MID, KCLASS_MG_COARSE, KCLASS_GAS, KCLASS_SIMPSON = range(6)
 KCLASS_NAMES = {0: "k_cc_down/k_fem_down (level 0)", 1: "k_cc_up/k_fem_up (level 0)", 2: "level kernels 1..lc-1",
@@ -78,6 +80,7 @@ ABI = {
     "vdi_gas_layer_update": (_i, [_vp, _d, _d]),
     "vdi_mg_solve_v_cycle": (_i, [_vp, _i, _d, _ip]),
     "vdi_visco_setup_and_solve": (_i, [_vp, _i, _d, _d, _d, _ip]),
+    "vdi_run_pass": (_i, [_vp, _i, _d, _i, _dp]),
     "vdi_array_len": (_l, [_vp, _i]),
     "vdi_get_array": (_i, [_vp, _i, _dp, _l]),
     "vdi_set_array": (_i, [_vp, _i, _dp, _l]),
@@ -275,6 +278,13 @@ class Fluid2D:
 
     def synchronize(self):
         _check(self.lib.vdi_synchronize(self.h), "synchronize")
+
+    def run_pass(self, which: int, dt: float, reps: int = 1) -> float:
+        """The advection passes alone (PASS_GODUNOV, PASS_CORRECT_ADVECT);
+        returns the average launch time in ms."""
+        ms = ctypes.c_double(0)
+        _check(self.lib.vdi_run_pass(self.h, which, dt, reps, ctypes.byref(ms)), "vdi_run_pass")
+        return ms.value
 
     @property
     def time(self) -> float:

@@ -1590,6 +1590,32 @@ int vdi_step_forward(vdi_ctx *c, double dt) {
     return rc;
 }
 
+int vdi_run_pass(vdi_ctx *c, int which, double dt, int reps, double *ms) {
+    if (!c || !(dt > 0) || reps < 1 || (which != VDI_PASS_GODUNOV && which != VDI_PASS_CORRECT_ADVECT))
+        return VDI_ERR_ARG;
+    const StepRec a = step_record(c, dt);
+    const Geo &g = c->g;
+    hipEvent_t e0, e1;
+    VDI_HIP(hipEventCreate(&e0));
+    VDI_HIP(hipEventCreate(&e1));
+    VDI_HIP(hipEventRecord(e0, c->st));
+    for (int r = 0; r < reps; r++) {
+        if (which == VDI_PASS_GODUNOV)
+            hipLaunchKernelGGL(k_godunov, dim3(cdiv(g.m, kGX), cdiv(g.n, kGY)), dim3(256), 0, c->st, a);
+        else
+            hipLaunchKernelGGL(k_correct_advect, cells_grid(g), dim3(256), 0, c->st, a);
+    }
+    VDI_LAUNCHED();
+    VDI_HIP(hipEventRecord(e1, c->st));
+    VDI_HIP(hipEventSynchronize(e1));
+    float t = 0;
+    VDI_HIP(hipEventElapsedTime(&t, e0, e1));
+    if (ms) *ms = t / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return VDI_OK;
+}
+
 static double *array_ptr(vdi_ctx *c, int id, long *len) {
     long NP = (long)c->g.ml * (c->g.n + 4), mn = (long)c->g.m * c->g.n, nn = (long)c->g.me * c->g.ne;
     *len = 0;

@@ -84,6 +84,27 @@ def velocity(m: int, n: int, seed: int = SEED):
     return u, v, ul, vd
 
 
+def advection_state(m: int, n: int, seed: int = SEED):
+    """Padded (n+4) x (m+4) arrays for the advection passes alone: u, v at the
+    cell centres (ghost cells included) from the 64-mode field, the pressure
+    at the nodes (kept in the padded layout, node (i, j) at (i dx, j dy))
+    from the 64 modes of seed + 1, and a smooth MAC solution z (m x n) from
+    those of seed + 2; domain [0, 1] x [0, n/m]."""
+    cu, cv = fourier_modes(seed)
+    cp, _ = fourier_modes(seed + 1)
+    cz, _ = fourier_modes(seed + 2)
+    dx = 1.0 / m
+    i = np.arange(-2, m + 2)
+    j = np.arange(-2, n + 2)
+    xc, yc = (i + 0.5) * dx, (j + 0.5) * dx
+    xn, yn = i * dx, j * dx
+    u = _field(cu, xc[None, :], yc[:, None])
+    v = _field(cv, xc[None, :], yc[:, None])
+    p = _field(cp, xn[None, :], yn[:, None])
+    z = 1e-3 * _field(cz, xc[None, 2:m + 2], yc[2:n + 2, None])
+    return u, v, p, z
+
+
 def mac_rhs(m: int, n: int, seed: int = SEED) -> np.ndarray:
     _u, _v, ul, vd = velocity(m, n, seed)
     xsp = ysp = float(m)
@@ -155,5 +176,5 @@ def batch_members(count: int = 64, seed: int = SEED, grids=None):
     return cfgs, prms
 
 
-__all__ = ["SplitMix64", "fourier_modes", "batch_members", "BATCH_GRIDS", "velocity", "mac_rhs", "fem_rhs", "helmholtz_rhs", "operator_case",
+__all__ = ["SplitMix64", "fourier_modes", "advection_state", "batch_members", "BATCH_GRIDS", "velocity", "mac_rhs", "fem_rhs", "helmholtz_rhs", "operator_case",
            "SEED", "VISCO_COUPLING"]
