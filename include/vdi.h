@@ -156,6 +156,13 @@ int vdi_mg_solve_v_cycle(vdi_ctx *ctx, int which, double t, int *vcycles);
 /* visco_impl::setup_and_solve(double vfx, double vfy, double t)
  * (visco_impl.cc:21-39).  which = VDI_VX or VDI_VY. */
 int vdi_visco_setup_and_solve(vdi_ctx *ctx, int which, double vfx, double vfy, double t, int *vcycles);
+/* fluid_2d::tm and init_tracers (fluid_2d.cc:495-505): tracer positions
+ * (x0, y0, x1, y1, ...); ntrace = 0 removes them.  Each vdi_step_forward then
+ * moves them first, by the bilinear velocity at the step's start
+ * (fluid_2d::update_tracers, fluid_2d.cc:507-525).  vdi_get_tracers reads
+ * them back (ntrace must match). */
+int vdi_set_tracers(vdi_ctx *ctx, const double *pos, int ntrace);
+int vdi_get_tracers(vdi_ctx *ctx, double *pos, int ntrace);
 /* The advection passes alone, on the context's current arrays, reps times
  * (the average launch time in *ms, HIP events).  VDI_PASS_GODUNOV: limited
  * slopes, half-step face predictor, upwind choices and the MAC right-hand

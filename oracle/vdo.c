@@ -581,6 +581,8 @@ struct vdo_sim {
      * height sweep); 1 = as in the reference (the default).  Tests switch
      * them off to check the mirror symmetry they break. */
     int q1_tau_shift, q13_inplace;
+    double *tm;  /* tracer positions (fluid_2d::tm), 2 ntrace */
+    int ntrace;
 };
 
 #define AT(a, i, j) ((a)[((size_t)(j) + 2) * ml + (size_t)((i) + 2)])
@@ -633,6 +635,46 @@ vdo_sim *vdo_create(const vdo_params *p) {
     return s;
 }
 
+int vdo_set_tracers(vdo_sim *s, const double *pos, int ntrace) {
+    free(s->tm);
+    s->tm = NULL;
+    s->ntrace = ntrace > 0 ? ntrace : 0;
+    if (s->ntrace) {
+        s->tm = (double *)xcalloc(2 * (size_t)ntrace, sizeof(double));
+        memcpy(s->tm, pos, 2 * (size_t)ntrace * sizeof(double));
+    }
+    return 0;
+}
+
+int vdo_get_tracers(const vdo_sim *s, double *pos, int ntrace) {
+    if (ntrace != s->ntrace) return -1;
+    if (ntrace) memcpy(pos, s->tm, 2 * (size_t)ntrace * sizeof(double));
+    return 0;
+}
+
+/* fluid_2d.cc:507-525: bilinear velocity at each tracer, from the cell whose
+ * centre is at or below-left of it (C truncation toward zero, then the index
+ * clamped to [-1, m-1] x [-1, n-1]) */
+static void update_tracers(vdo_sim *s, double dt) {
+    const int m = s->m, n = s->n, ml = s->ml;
+    const double *u = s->F[F_U], *v = s->F[F_V];
+    for (int k = 0; k < s->ntrace; k++) {
+        double *tp = s->tm + 2 * k;
+        double x = (tp[0] - s->P.ax) * s->xsp - 0.5, y = (tp[1] - s->P.ay) * s->ysp - 0.5;
+        int i = (int)x, j = (int)y;
+        if (i < -1) i = -1;
+        if (i > m - 1) i = m - 1;
+        if (j < -1) j = -1;
+        if (j > n - 1) j = n - 1;
+        x -= i;
+        y -= j;
+        tp[0] += dt * ((1 - y) * (AT(u, i, j) * (1 - x) + AT(u, i + 1, j) * x) +
+                       y * (AT(u, i, j + 1) * (1 - x) + AT(u, i + 1, j + 1) * x));
+        tp[1] += dt * ((1 - y) * (AT(v, i, j) * (1 - x) + AT(v, i + 1, j) * x) +
+                       y * (AT(v, i, j + 1) * (1 - x) + AT(v, i + 1, j + 1) * x));
+    }
+}
+
 int vdo_set_quirks(vdo_sim *s, int q1_tau_shift, int q13_inplace) {
     s->q1_tau_shift = q1_tau_shift != 0;
     s->q13_inplace = q13_inplace != 0;
@@ -648,6 +690,7 @@ void vdo_destroy(vdo_sim *s) {
     free(s->ta); free(s->tb); free(s->tc); free(s->td);
     vdo_mg_destroy(s->mac); vdo_mg_destroy(s->fem); vdo_mg_destroy(s->vx); vdo_mg_destroy(s->vy);
     free(s->zmac); free(s->zfem); free(s->zvx); free(s->zvy);
+    free(s->tm);
     free(s);
 }
 
@@ -1255,6 +1298,8 @@ static void visco_setup(vdo_mg *g, double vfx, double vfy) {
 int vdo_step_forward(vdo_sim *s, double dt) {
     int rc;
 
+    /* fluid_2d.cc:98 */
+    update_tracers(s, dt);
     /* fluid_2d.cc:99-102 */
     if ((rc = vdo_gas_update(s, s->time, dt))) return rc == 2 ? 2 : 3;
     s->time += 0.5 * dt;
@@ -1321,6 +1366,7 @@ int vdo_pass(vdo_sim *s, int pass, double dt) {
     case VDO_PASS_GAS_TAU_BC: gas_apply_tau_bc(s); return 0;
     case VDO_PASS_GAS_HX: gas_calculate_hx(s); return 0;
     case VDO_PASS_GAS_INIT: gas_init_fields(s); return 0;
+    case VDO_PASS_TRACERS: update_tracers(s, dt); return 0;
     default: return -1;
     }
 }

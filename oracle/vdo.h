@@ -84,7 +84,8 @@ enum {
     VDO_PASS_GAS_LIQUID_FIELDS, /* gas_layer_solve.cc:229-249 */
     VDO_PASS_GAS_TAU_BC,     /* gas_layer.cc:25-30 */
     VDO_PASS_GAS_HX,         /* gas_layer_solve.cc:58-62 */
-    VDO_PASS_GAS_INIT        /* gas_layer_solve.cc:32-53 */
+    VDO_PASS_GAS_INIT,       /* gas_layer_solve.cc:32-53 */
+    VDO_PASS_TRACERS         /* fluid_2d.cc:507-525 */
 };
 
 /* Multigrid operator kinds. */
@@ -111,6 +112,10 @@ long vdo_array_len(const vdo_sim *s, int id);
 int vdo_get_array(const vdo_sim *s, int id, double *out, long len);
 int vdo_set_array(vdo_sim *s, int id, const double *in, long len);
 int vdo_set_time(vdo_sim *s, double t);
+/* Tracer positions (x0, y0, x1, y1, ...), moved at the start of each step
+ * (fluid_2d.cc:98, :507-525). */
+int vdo_set_tracers(vdo_sim *s, const double *pos, int ntrace);
+int vdo_get_tracers(const vdo_sim *s, double *pos, int ntrace);
 /* Test switches for two reference quirks that break mirror symmetry: Q1
  * (the gas shear of cell i written at tau[i] and read at tau[i+2]) and Q13
  * (the height update sweeping in place).  1 = as in the reference (default);

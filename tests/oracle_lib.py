@@ -24,7 +24,7 @@ UX, UY, VXS, VYS, UR, VR, UU, VU, C2, C3 = range(64, 74)
 # passes of the step (oracle/vdo.h, VDO_PASS_*)
 (PASS_K1, PASS_K2, PASS_K3, PASS_K4, PASS_K5, PASS_K6, PASS_K7, PASS_K8, PASS_INVISCID_BC, PASS_VISCOUS_BC,
  PASS_VISCO_RHS_U, PASS_VISCO_RHS_V, PASS_VISCO_COPY, PASS_K14, PASS_K15, PASS_GAS_LIQUID_FIELDS,
- PASS_GAS_TAU_BC, PASS_GAS_HX, PASS_GAS_INIT) = range(1, 20)
+ PASS_GAS_TAU_BC, PASS_GAS_HX, PASS_GAS_INIT, PASS_TRACERS) = range(1, 21)
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _lib = None
@@ -69,6 +69,8 @@ def lib():
         L.vdo_last_cycles.argtypes = [vp, ctypes.c_int]
         L.vdo_last_newton.argtypes = [vp]
         L.vdo_set_quirks.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.vdo_set_tracers.argtypes = [vp, _dp, ctypes.c_int]
+        L.vdo_get_tracers.argtypes = [vp, _dp, ctypes.c_int]
         L.vdo_pass.argtypes = [vp, ctypes.c_int, ctypes.c_double]
         L.vdo_pass.restype = ctypes.c_int
         L.vdo_initial_guess.argtypes = [vp, ctypes.c_int, ctypes.c_double]
@@ -170,6 +172,22 @@ class OracleSim:
 
     def last_cycles(self, kind):
         return lib().vdo_last_cycles(self.h, kind)
+
+    def set_tracers(self, pos):
+        a = np.ascontiguousarray(pos, np.float64).ravel()
+        self.ntrace = a.size // 2
+        lib().vdo_set_tracers(self.h, ptr(a), self.ntrace)
+
+    def tracers(self):
+        nt = getattr(self, "ntrace", 0)
+        out = np.empty(2 * nt, np.float64)
+        assert lib().vdo_get_tracers(self.h, ptr(out), nt) == 0
+        return out.reshape(nt, 2)
+
+    def run_pass_tracers(self, dt):
+        """The tracer update alone (the first thing a step does)."""
+        if lib().vdo_pass(self.h, PASS_TRACERS, dt) != 0:
+            raise ValueError("tracer pass")
 
     def set_quirks(self, q1_tau_shift=1, q13_inplace=1):
         """Test switches of the reference quirks Q1 and Q13 (1 = as in the reference)."""

@@ -189,3 +189,27 @@ def test_gpu_rejected_schedule_leaves_settings(vdi):
         a.set_solver(nu1=5, nu2=2)
     assert [a.schedule(k) for k in (vdi.MAC, vdi.FEM, vdi.VX, vdi.VY)] == before
     a.close()
+
+
+def test_gpu_tracers_match_oracle(vdi, oracle):
+    """fluid_2d::update_tracers (fluid_2d.cc:507-525) on the GPU, moved at the
+    start of every step (direct and captured), against the oracle's tracers
+    over ten steps; tracers include ones outside the cell centres."""
+    say("tracers 64x32")
+    cfg = PhysicalConfig(m=64, n=32)
+    p, gpu, orc = tp.make_pair(vdi, oracle, cfg, tol=1e-13)
+    rng = np.random.default_rng(17)
+    pos = np.column_stack([rng.uniform(p.ax - 0.02 * (p.bx - p.ax), p.bx, 512),
+                           rng.uniform(p.ay, p.by * 1.01, 512)])
+    gpu.set_tracers(pos)
+    orc.set_tracers(pos)
+    for k in range(10):
+        gpu.step_forward(p.dt_reg)
+        orc.step_forward(p.dt_reg)
+        tp.compare(gpu, orc, f"step {k}")
+        d = gpu.tracers() - orc.tracers()
+        moved = np.abs(orc.tracers() - pos).max()
+        assert np.abs(d).max() <= 1e-9 * moved + 1e-18, (k, np.abs(d).max(), moved)
+    assert gpu.counters()[0] > 0
+    gpu.close()
+    orc.close()

@@ -42,6 +42,8 @@ PLANTS = [
     ("liquid p x_sym edge", "p[0] = pg[0] + 2 * fac * (h[1] - h[0]) - fac2;", "p[0] = pg[0] + fac * (h[1] - h[0]) - fac2;"),
     ("extrapolation coefficient", "double a = 1. / (g->t0 - g->t_prev), bb = (t - g->t_prev) * a, c = (t - g->t0) * a;",
      "double a = 1. / (g->t0 - g->t_prev), bb = (t - g->t_prev) * a, c = (t - g->t_prev) * a;"),
+    ("tracer weights swapped", "tp[0] += dt * ((1 - y) * (AT(u, i, j) * (1 - x) + AT(u, i + 1, j) * x) +",
+     "tp[0] += dt * ((1 - y) * (AT(u, i, j) * x + AT(u, i + 1, j) * (1 - x)) +"),
     ("hx x_sym edge", "s->hx[0] = s->P.x_sym ? 0 : s->hx[1];", "s->hx[0] = s->P.x_sym ? s->hx[1] : s->hx[1];"),
 ]
 

@@ -906,3 +906,38 @@ def test_symmetric_half_domain_equals_full_width_run(oracle):
     d = _half_and_full(oracle, 32, 16, 6, 1, 1)
     for k, err in d.items():
         assert err < 1e-4, (k, d)
+
+
+# ====================================================================== tracers
+
+def test_tracers_move_with_the_bilinear_velocity(oracle):
+    """fluid_2d.cc:507-525: each tracer moves by dt times the velocity
+    interpolated bilinearly between the four cell centres around it.  Exact
+    for a field linear in x and y everywhere (also for tracers outside the
+    cell centres, which extrapolate from the edge cells); for a field with a
+    kink along a column of cell centres, exact only when the cell to the
+    lower left is the one used."""
+    p = mkparams(16, 12, bx=1.0, by=0.9)
+    G = Grid(p)
+    rng = np.random.default_rng(13)
+    pos = np.column_stack([rng.uniform(-0.05, 1.05, 200), rng.uniform(-0.05, 0.95, 200)])
+    dt = 0.01
+    s = sim(oracle, p)
+    s.set(oracle.U, 0.3 + 0.7 * G.XC - 0.4 * G.YC)
+    s.set(oracle.V, -0.2 + 0.5 * G.XC + 0.25 * G.YC)
+    s.set_tracers(pos)
+    s.run_pass_tracers(dt)
+    t = s.tracers()
+    x, y = pos[:, 0], pos[:, 1]
+    np.testing.assert_allclose(t[:, 0], x + dt * (0.3 + 0.7 * x - 0.4 * y), rtol=1e-14, atol=1e-15)
+    np.testing.assert_allclose(t[:, 1], y + dt * (-0.2 + 0.5 * x + 0.25 * y), rtol=1e-14, atol=1e-15)
+    # kink at the centre of column 7: |x - x_7| is linear between neighbouring
+    # centres, so tracers strictly inside the domain move exactly
+    xk = G.at(G.XC, 7, 0)
+    s2 = sim(oracle, p)
+    s2.set(oracle.U, np.abs(G.XC - xk))
+    inside = np.column_stack([rng.uniform(G.dx, 1 - G.dx, 300), rng.uniform(G.dy, 0.9 - G.dy, 300)])
+    s2.set_tracers(inside)
+    s2.run_pass_tracers(dt)
+    np.testing.assert_allclose(s2.tracers()[:, 0], inside[:, 0] + dt * np.abs(inside[:, 0] - xk), rtol=1e-14,
+                               atol=1e-15)

@@ -81,6 +81,8 @@ ABI = {
     "vdi_mg_solve_v_cycle": (_i, [_vp, _i, _d, _ip]),
     "vdi_visco_setup_and_solve": (_i, [_vp, _i, _d, _d, _d, _ip]),
     "vdi_run_pass": (_i, [_vp, _i, _d, _i, _dp]),
+    "vdi_set_tracers": (_i, [_vp, _dp, _i]),
+    "vdi_get_tracers": (_i, [_vp, _dp, _i]),
     "vdi_array_len": (_l, [_vp, _i]),
     "vdi_get_array": (_i, [_vp, _i, _dp, _l]),
     "vdi_set_array": (_i, [_vp, _i, _dp, _l]),
@@ -278,6 +280,18 @@ class Fluid2D:
 
     def synchronize(self):
         _check(self.lib.vdi_synchronize(self.h), "synchronize")
+
+    def set_tracers(self, pos):
+        """Tracer positions, shape (ntrace, 2) or flat (x0, y0, x1, ...)."""
+        a = np.ascontiguousarray(pos, np.float64).ravel()
+        self.ntrace = a.size // 2
+        _check(self.lib.vdi_set_tracers(self.h, _ptr(a), self.ntrace), "vdi_set_tracers")
+
+    def tracers(self) -> np.ndarray:
+        nt = getattr(self, "ntrace", 0)
+        out = np.empty(2 * nt, np.float64)
+        _check(self.lib.vdi_get_tracers(self.h, _ptr(out), nt), "vdi_get_tracers")
+        return out.reshape(nt, 2)
 
     def run_pass(self, which: int, dt: float, reps: int = 1) -> float:
         """The advection passes alone (PASS_GODUNOV, PASS_CORRECT_ADVECT);

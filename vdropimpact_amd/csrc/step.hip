@@ -853,6 +853,30 @@ __global__ void __launch_bounds__(kGasThreads) k_gas(StepRec a, Stamp *ts) {
 }
 __global__ void __launch_bounds__(256) k_snapshot(StepRec a, int restore) { snapshot_body(a, blockIdx.x, restore); }
 __global__ void __launch_bounds__(64) k_gather_status(StepRec a) { gather_body(a); }
+
+// fluid_2d::update_tracers (fluid_2d.cc:507-525): each tracer moves by dt times
+// the bilinear interpolation of u, v at its position, from the cell whose
+// centre lies at or below-left of it (indices truncated toward zero, then
+// clamped to [-1, m-1] x [-1, n-1], so tracers near the edges extrapolate
+// into the ghost cells).  The arithmetic is left uncontracted.
+__global__ void __launch_bounds__(256) k_tracers(StepRec a, double *tm, int nt) {
+#pragma clang fp contract(off)
+    const Geo &g = a.g;
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= nt) return;
+    double x = (tm[2 * k] - g.ax) * g.xsp - 0.5, y = (tm[2 * k + 1] - g.ay) * g.ysp - 0.5;
+    int i = (int)x, j = (int)y;
+    if (i < -1) i = -1;
+    if (i > g.m - 1) i = g.m - 1;
+    if (j < -1) j = -1;
+    if (j > g.n - 1) j = g.n - 1;
+    x -= i;
+    y -= j;
+    const size_t c = ((size_t)j + 2) * (size_t)g.ml + (size_t)(i + 2), r = c + 1, t = c + g.ml, tr = t + 1;
+    const double *u = a.F.u, *v = a.F.v, dt = a.dt;
+    tm[2 * k] += dt * ((1 - y) * (u[c] * (1 - x) + u[r] * x) + y * (u[t] * (1 - x) + u[tr] * x));
+    tm[2 * k + 1] += dt * ((1 - y) * (v[c] * (1 - x) + v[r] * x) + y * (v[t] * (1 - x) + v[tr] * x));
+}
 __global__ void __launch_bounds__(256) k_fetch(const double *__restrict__ src, double *__restrict__ dst, int n) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) dst[k] = src[k];
@@ -1585,9 +1609,38 @@ extern "C" {
 
 int vdi_step_forward(vdi_ctx *c, double dt) {
     if (!c || !(dt > 0)) return VDI_ERR_ARG;
+    // fluid_2d.cc:98: the tracers move first, with the velocity at the start
+    // of the step (outside the captured step, so a redo does not move them
+    // twice)
+    if (c->ntrace > 0) {
+        hipLaunchKernelGGL(k_tracers, dim3(cdiv(c->ntrace, 256)), dim3(256), 0, c->st, step_record(c, dt),
+                           c->tracers.as<double>(), c->ntrace);
+        VDI_LAUNCHED();
+    }
     int rc = graph_ready(c) ? step_graph(c, dt) : step_direct(c, dt, SOLVE_DIRECT);
     if (rc == VDI_OK) c->steps++;
     return rc;
+}
+
+int vdi_set_tracers(vdi_ctx *c, const double *pos, int ntrace) {
+    if (!c || ntrace < 0 || (ntrace > 0 && !pos)) return VDI_ERR_ARG;
+    VDI_HIP(hipStreamSynchronize(c->st));
+    if (ntrace == 0) {
+        c->tracers.release();
+        c->ntrace = 0;
+        return VDI_OK;
+    }
+    VDI_TRY(c->tracers.alloc(2 * sizeof(double) * (size_t)ntrace));
+    VDI_HIP(hipMemcpy(c->tracers.p, pos, 2 * sizeof(double) * (size_t)ntrace, hipMemcpyHostToDevice));
+    c->ntrace = ntrace;
+    return VDI_OK;
+}
+
+int vdi_get_tracers(vdi_ctx *c, double *pos, int ntrace) {
+    if (!c || ntrace != c->ntrace || (ntrace > 0 && !pos)) return VDI_ERR_ARG;
+    VDI_HIP(hipStreamSynchronize(c->st));
+    if (ntrace > 0) VDI_HIP(hipMemcpy(pos, c->tracers.p, 2 * sizeof(double) * (size_t)ntrace, hipMemcpyDeviceToHost));
+    return VDI_OK;
 }
 
 int vdi_run_pass(vdi_ctx *c, int which, double dt, int reps, double *ms) {

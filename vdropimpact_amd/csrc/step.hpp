@@ -200,6 +200,8 @@ struct vdi_ctx {
     Readback *rb = nullptr, *rb_map = nullptr;
     long graph_steps = 0, redo_steps = 0;
     vdi::LearnedGuess *guess = nullptr;  // optional learned MAC initial guess
+    vdi::DevBuf tracers;                  // 2 ntrace positions (fluid_2d::tm)
+    int ntrace = 0;
     void drop_graphs() {
         for (auto &e : graphs) (void)hipGraphExecDestroy(e.exec);
         graphs.clear();
