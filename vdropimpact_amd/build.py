@@ -2,7 +2,9 @@
 
 Usage: python -m vdropimpact_amd.build   (or call build()).
 Compiles every HIP source under csrc/ for gfx950 with hipcc and links one
-shared library exporting the C ABI declared in include/vdi.h.
+shared library exporting the C ABI declared in include/vdi.h, then builds the
+C++ host layer above it (host/: fluid_test, cfg_dump, tt_analysis in
+_lib/bin).
 """
 from __future__ import annotations
 
@@ -65,6 +67,8 @@ def build(verbose: bool = False) -> str:
         print(" ".join(link), flush=True)
     subprocess.run(link, check=True)
     os.replace(tmp, LIB)
+    # the C++ host layer (configuration reader, frame loop, files, tools)
+    subprocess.run(["make", "-s", "-B", "-C", os.path.join(PKG, "host")], check=True)
     return LIB
 
 
