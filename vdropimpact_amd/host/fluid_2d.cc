@@ -16,6 +16,8 @@ fluid_2d::fluid_2d(const char *infile)
       xsp(1. / dx), ysp(1. / dy), xxsp(xsp * xsp), yysp(ysp * ysp),
       dt_reg(implicit_visc ? dx * tmult : dx * dx * rho * muinv * tmult * 0.25), vix(true), viy(false),
       buf(m > 123 ? m + 5 : 128) {
+    tracer_draws.resize(2 * (size_t)ntrace);
+    for (int &r : tracer_draws) r = rand();
     vdi_params P;
     P.m = m;
     P.n = n;
@@ -103,9 +105,16 @@ void fluid_2d::init_fields() {
 
 void fluid_2d::init_tracers() {
     tm.assign(2 * (size_t)ntrace, 0.0);
+    // the first call uses the draws taken at construction, later calls draw anew
+    std::vector<int> r;
+    r.swap(tracer_draws);
+    if (r.size() != tm.size()) {
+        r.resize(tm.size());
+        for (int &v : r) v = rand();
+    }
     for (size_t k = 0; k < tm.size(); k += 2) {
-        tm[k] = ax + (bx - ax) / RAND_MAX * static_cast<double>(rand());
-        tm[k + 1] = ay + (by - ay) / RAND_MAX * static_cast<double>(rand());
+        tm[k] = ax + (bx - ax) / RAND_MAX * static_cast<double>(r[k]);
+        tm[k + 1] = ay + (by - ay) / RAND_MAX * static_cast<double>(r[k + 1]);
     }
     gpu_check(vdi_set_tracers(ctx, tm.data(), ntrace), "vdi_set_tracers");
 }

@@ -63,6 +63,10 @@ class fluid_2d : public fileinfo {
     visco_impl vix, viy;
     std::vector<float> buf;
     std::vector<double> hu, hv, hp;   // host copies of u, v, p for a frame
+    // rand() draws for init_tracers, taken before the GPU runtime starts (it
+    // calls rand() itself), so the positions are the ones the reference's
+    // process draws from the C library's default seed
+    std::vector<int> tracer_draws;
     bool restart_exists(int sn) const;
     void fetch_fields();
     double vorticity(int i, int j) const;
