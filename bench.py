@@ -45,45 +45,72 @@ def field_errors(vdi, a, b):
     return {name: rel(a.get(aid), b.get(aid)) for name, aid in fields.items()}
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(m, n, seconds, warmup, schedules):
     """Times the oracle on the host (test infrastructure used as the checker
     and as the reported CPU baseline), with the GPU run's V(nu1, nu2) per
-    solver."""
+    solver: a bounded sample with all the threads this process may use
+    (OMP_NUM_THREADS, threads bound close), then one with a single thread."""
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
     sys.path.insert(0, os.path.join(REPO, "tests"))
     import oracle_lib
     from vdropimpact_amd.params import drop_impact_params
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
     p = drop_impact_params(m, n)
-    s = oracle_lib.OracleSim(p, rel_tol=1e-10)
-    for k, nu in enumerate(schedules):
-        s.set_schedule(k, *nu)
-    s.init_fields()
-    for _ in range(warmup):
-        s.step_forward(p.dt_reg)
-    k = 0
-    t0 = time.perf_counter()
-    while True:
-        s.step_forward(p.dt_reg)
-        k += 1
-        el = time.perf_counter() - t0
-        if el >= seconds and k >= 2:
-            break
-    s.close()
-    return {"value": k / el, "unit": "steps/s", "cores": threads, "kind": "port",
-            "sample": f"{k} steps of the {m}x{n} workload after {warmup} warm-up steps, {el:.1f} s, "
-                      f"OMP_NUM_THREADS={threads}, V(nu1,nu2) per solver {schedules}"}
+
+    def sample(nthreads, secs):
+        used = oracle_lib.lib().vdo_set_threads(nthreads)
+        s = oracle_lib.OracleSim(p, rel_tol=1e-10)
+        for k, nu in enumerate(schedules):
+            s.set_schedule(k, *nu)
+        s.init_fields()
+        for _ in range(warmup):
+            s.step_forward(p.dt_reg)
+        k = 0
+        t0 = time.perf_counter()
+        while True:
+            s.step_forward(p.dt_reg)
+            k += 1
+            el = time.perf_counter() - t0
+            if el >= secs and k >= 2:
+                break
+        s.close()
+        return k / el, k, el, used
+
+    rate, k, el, used = sample(threads, seconds)
+    rate1, k1, el1, _ = sample(1, seconds / 2)
+    return {"value": rate, "unit": "steps/s", "cores": used, "kind": "port",
+            "steps_per_s_1_thread": rate1,
+            "host": {"nproc": os.cpu_count(), "cpu_model": cpu_model(),
+                     "OMP_NUM_THREADS": used, "OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND")},
+            "sample": f"steps {warmup + 1}..{warmup + k} of the {m}x{n} workload ({k} steps, {el:.1f} s) with "
+                      f"{used} threads bound close, and steps {warmup + 1}..{warmup + k1} ({el1:.1f} s) with 1 "
+                      f"thread; every solve to 1e-10 with V(nu1,nu2) per solver {schedules}"}
 
 
 def operator_level(vdi, reps=3):
     """Operator-level solves (projection and Helmholtz systems alone): the
-    64-mode Fourier inputs, every solve from zero to relative residual 1e-10.
-    FP64 at 1024x1024 and 2048x1024, mixed FP32/FP64 at 2048x1024 and
-    4096x2048.  GB/s = V-cycles x algorithmic bytes per V-cycle / time."""
+    64-mode Fourier inputs, every solve from zero to relative residual 1e-10,
+    FP64 at 1024x1024, 2048x1024 and 4096x2048, mixed FP32/FP64 at 2048x1024
+    and 4096x2048.  GB/s = V-cycles x algorithmic bytes per V-cycle / time.
+    Then the advection passes alone on the same field at 1024x1024 and
+    2048x1024 (GB/s = algorithmic bytes per launch / launch time)."""
     import numpy as np
     from vdropimpact_amd import workloads
+    from vdropimpact_amd.params import VdiParams
     out = []
-    cases = [(form, m, n) for form, grids in ((vdi.FORM_FP64, [(1024, 1024), (2048, 1024)]),
+    cases = [(form, m, n) for form, grids in ((vdi.FORM_FP64, [(1024, 1024), (2048, 1024), (4096, 2048)]),
                                               (vdi.FORM_MIXED, [(2048, 1024), (4096, 2048)])) for m, n in grids]
     names = {0: "mac", 1: "fem", 2: "helmholtz"}
     for form, m, n in cases:
@@ -107,6 +134,24 @@ def operator_level(vdi, reps=3):
             out.append({"system": names[kind], "grid": [m, n], "form": "fp64" if form == 0 else "mixed",
                         "vcycles": cyc, "ms_per_solve": 1e3 * t, "GB_per_s": gbs, "frac_of_8TBps": gbs / PEAK_HBM})
             g.close()
+    for m, n in ((1024, 1024), (2048, 1024)):
+        p = VdiParams(m=m, n=n, x_sym=1, implicit_visc=1, ax=0.0, bx=1.0, ay=0.0, by=n / m, rho=1.0, mu=1e-3,
+                      dt_reg=0.2 / m, alpha=1 / 1.4, mug=1.8e-5, Pamb=1e5, h0=5e-6, R=1e-3, V=0.5, sigma=0.02)
+        sim = vdi.Fluid2D(p)
+        u, v, pn, z = workloads.advection_state(m, n)
+        for aid, arr in ((vdi.U, u), (vdi.V, v), (vdi.P, pn)):
+            sim.set(aid, arr)
+        sim.set(vdi.Z_MAC, z.ravel())
+        # reads u, v, p; writes ul, vl, ud, vd, c0, c1, src: 10 words a cell
+        # reads z, ul, vl, ud, vd, u, v, c0, c1; writes c0, c1: 11 words a cell
+        for which, name, words in ((vdi.PASS_GODUNOV, "godunov K1-K6", 10),
+                                   (vdi.PASS_CORRECT_ADVECT, "correct+advect K7-K8", 11)):
+            sim.run_pass(which, p.dt_reg, 3)
+            ms = sim.run_pass(which, p.dt_reg, 50)
+            gbs = words * 8.0 * m * n / (ms * 1e-3) / 1e9
+            out.append({"system": name, "grid": [m, n], "form": "fp64", "ms_per_launch": ms, "GB_per_s": gbs,
+                        "frac_of_8TBps": gbs / PEAK_HBM})
+        sim.close()
     return out
 
 
@@ -148,63 +193,80 @@ def run_plain(vdi, args, p):
         sim.set_concurrency(False)
     sim.init_fields()
     schedules = [list(sim.schedule(k)) for k in (vdi.MAC, vdi.FEM, vdi.VX, vdi.VY)]
-    for _ in range(args.warmup):
-        sim.step_forward(p.dt_reg)
+    nstep = 0
+
+    def steps(k):
+        nonlocal nstep
+        for _ in range(k):
+            sim.step_forward(p.dt_reg)
+        nstep += k
+
+    steps(args.warmup)
     if args.autotune > 0:
         sim.autotune(args.autotune)
-        for _ in range(3):
-            sim.step_forward(p.dt_reg)
+        steps(3)
     log(f"plain: {args.warmup} warm-up steps, tiles tuned, schedules {schedules}")
-    # Which kernel class dominates the step: every launch of one class is
-    # timed by the kernels' own wall-clock stamps over a few steps.
+    # Which kernel function dominates the step: every launch of one function
+    # (all levels) is timed by the kernels' own wall-clock stamps over a few
+    # steps; the level classes are surveyed too.
     class_ms, class_n = {}, {}
-    for kc in vdi.KCLASS_NAMES:
+    for kc in list(vdi.KFUNC_NAMES) + [vdi.KCLASS_MG_DOWN0, vdi.KCLASS_MG_UP0, vdi.KCLASS_MG_MID]:
         sim.time_kernels(kc, 8192)
-        for _ in range(3):
-            sim.step_forward(p.dt_reg)
+        steps(3)
         n_, ms, _b = sim.kernel_times()
         class_ms[kc] = ms / 3.0
         class_n[kc] = n_ / 3.0
-    dominant = max(class_ms, key=class_ms.get)
-    log("kernel classes, ms/step:", {vdi.KCLASS_NAMES[k]: round(v, 4) for k, v in class_ms.items()})
-    log("kernel classes, launches/step:", {vdi.KCLASS_NAMES[k]: round(v, 2) for k, v in class_n.items()})
-    # Timed region (the captured-step path) with the dominant class stamped.
+    funcs = {k: v for k, v in class_ms.items() if k in vdi.KFUNC_NAMES}
+    dominant = max(funcs, key=funcs.get)
+    names = {**vdi.KCLASS_NAMES, **vdi.KFUNC_NAMES}
+    log("kernel functions / classes, ms/step:", {names[k]: round(v, 4) for k, v in class_ms.items()})
+    log("kernel functions / classes, launches/step:", {names[k]: round(v, 2) for k, v in class_n.items()})
+    # Timed region with the dominant function stamped (a region of its own:
+    # the stamps' atomics slow the step).
     sim.time_kernels(dominant, 16384)
-    for _ in range(3):
-        sim.step_forward(p.dt_reg)  # step graphs captured with the stamps
+    steps(3)   # step graphs captured with the stamps
     sim.reset_stats()
     el = time_steps(sim, p.dt_reg, args.steps)
+    nstep += args.steps
     launches, kms, kbytes = sim.kernel_times()
+    sim.time_kernels(-1)
+    steps(3)
+    # The clean timed region: `value`.
+    sim.reset_stats()
+    g0, r0 = sim.counters()
+    b0 = sim.budgets()
+    first = nstep + 1
+    el_clean = time_steps(sim, p.dt_reg, args.steps)
+    nstep += args.steps
+    g1, r1 = sim.counters()
     st = sim.stats()
     cyc = {name: st.total_cycles[k] / max(1, st.solves[k]) for k, name in enumerate(["mac", "fem", "vx", "vy"])}
-    sim.time_kernels(-1)
-    for _ in range(3):
-        sim.step_forward(p.dt_reg)
-    g0, r0 = sim.counters()
-    el_clean = time_steps(sim, p.dt_reg, args.steps)
-    g1, r1 = sim.counters()
-    height_passes = sim.stats().last_height_passes
+    height_passes = st.last_height_passes
     budgets = sim.budgets()
-    log(f"plain: {args.steps} steps in {el:.3f} s (stamped), {el_clean:.3f} s (no stamps)")
+    log(f"plain: {args.steps} steps in {el:.3f} s (stamped), {el_clean:.3f} s (no stamps), "
+        f"simulation steps {first}..{nstep}, V-cycles per solve {cyc}")
     achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
     sim.close()
-    traffic = measured_traffic(vdi.KCLASS_NAMES[dominant])
+    traffic = measured_traffic(names[dominant])
     return {
         # the stamps (two atomics per workgroup of the stamped class) slow the
         # step; the value is the same timed region without them
         "value": args.steps / el_clean, "ms_per_step": 1e3 * el_clean / args.steps,
         "steps_per_s_stamped": args.steps / el,
+        "timed_simulation_steps": [first, nstep],
         "avg_vcycles": cyc,
         "captured_steps": g1 - g0, "redone_steps": r1 - r0,
         "height_passes_last_step": height_passes,
-        "budgets_mac_vx_vy_fem": budgets,
+        "budgets_mac_vx_vy_fem": {"before": b0, "after": budgets},
         "schedules": schedules,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM, "traffic": traffic,
-                     "kernel": vdi.KCLASS_NAMES[dominant], "launches": launches,
+                     "kernel": names[dominant], "launches": launches,
                      "avg_launch_us": 1e3 * kms / max(1, launches),
                      "alg_bytes_per_launch": kbytes / max(1, launches),
-                     "class_ms_per_step": {vdi.KCLASS_NAMES[k]: v for k, v in class_ms.items()}},
+                     "duration": "first workgroup start to last workgroup end (device wall clock), "
+                                 "launches that returned at once on a converged flag left out",
+                     "ms_per_step": {names[k]: v for k, v in class_ms.items()}},
     }
 
 
@@ -277,23 +339,25 @@ def run_batch(vdi, args):
     b.autotune(2)
     for _ in range(3):
         b.step()
-    busy, graph = [], []
+    spans, graph = [], []
     t0 = time.perf_counter()
     for _ in range(args.batch_steps):
         b.step()
-        g, bz = b.step_times()
+        g, _busy = b.step_times()
         graph.append(g)
-        busy.append(bz)
+        spans.append(b.member_spans())
     el = time.perf_counter() - t0
-    busy = np.array(busy)
+    spans = np.array(spans)
     _nb, redo = b.counters()
     sizes = [f"{p.m}x{p.n}" for p in prms]
     out = {"simulations": len(prms), "steps_each": args.batch_steps, "form": "mixed",
            "aggregate_steps_per_s": len(prms) * args.batch_steps / el,
            "batched_step_ms_p50": float(np.percentile(graph, 50)),
            "batched_step_ms_p99": float(np.percentile(graph, 99)),
-           "member_busy_ms_p50": float(np.percentile(busy, 50)),
-           "member_busy_ms_p99": float(np.percentile(busy, 99)),
+           # per simulation and step: from the start of its first work item to
+           # the end of its last one in the batched step (device clock)
+           "sim_step_ms_p50": float(np.percentile(spans, 50)),
+           "sim_step_ms_p99": float(np.percentile(spans, 99)),
            "grids": {s: sizes.count(s) for s in sorted(set(sizes))},
            "members_redone": redo}
     log(f"batch: {out['aggregate_steps_per_s']:.0f} member-steps/s")
@@ -321,6 +385,9 @@ def main():
     ap.add_argument("--pairs", type=int, default=64, help="training pairs recorded for the learned guess")
     ap.add_argument("--epochs", type=int, default=12)
     ap.add_argument("--batch-sims", type=int, default=64)
+    # the shortest member (256x128 at 0.89 m/s) closes its gas film a little
+    # after 200 steps; past that point its gas-layer Newton solve fails, as
+    # the reference's would, so the batched run stays inside every window
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many doubles (counter calibration) and exit")
@@ -355,11 +422,12 @@ def main():
         "config": {"workload": f"single drop {args.m}x{args.n}, 32 cSt, V=0.5 m/s, R=1 mm, h_min=5 um, "
                                "x_sym, implicit viscosity, all solves to rel. residual 1e-10",
                    "grid": [args.m, args.n], "form": "fp64", "rel_tol": 1e-10,
+                   "timed_simulation_steps": plain["timed_simulation_steps"],
                    "avg_vcycles": plain["avg_vcycles"],
                    "captured_steps": plain["captured_steps"], "redone_steps": plain["redone_steps"],
                    "gas_height_passes_last_step": plain["height_passes_last_step"],
                    "vcycle_schedule_mac_fem_vx_vy": plain["schedules"],
-                   "vcycle_budgets_mac_vx_vy_fem_after_run": plain["budgets_mac_vx_vy_fem"],
+                   "vcycle_budgets_mac_vx_vy_fem": plain["budgets_mac_vx_vy_fem"],
                    "concurrent_branches": not args.serial_visco,
                    "steps_per_s_with_kernel_stamps": plain["steps_per_s_stamped"]},
         "roofline": plain["roofline"],

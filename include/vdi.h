@@ -89,6 +89,12 @@ typedef struct vdi_stats {
 #define VDI_KCLASS_MG_COARSE 3  /* single-workgroup coarse-level V-cycle */
 #define VDI_KCLASS_GAS 4        /* gas-layer update */
 #define VDI_KCLASS_SIMPSON 5    /* far-field boundary integrals */
+/* The same launches grouped by kernel function (every level), as a profiler
+ * names them: */
+#define VDI_KCLASS_CC_DOWN 6    /* k_down<double, false>: cell-centred smooth + restrict */
+#define VDI_KCLASS_CC_UP 7      /* k_up<double, false>: cell-centred prolong + smooth */
+#define VDI_KCLASS_FEM_DOWN 8   /* k_down<double, true>: nodal smooth + restrict */
+#define VDI_KCLASS_FEM_UP 9     /* k_up<double, true>: nodal prolong + smooth */
 
 typedef struct vdi_ctx vdi_ctx;
 typedef struct vdi_mg vdi_mg;

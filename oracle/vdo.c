@@ -8,6 +8,9 @@
 #include "vdo.h"
 
 #include <float.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -673,6 +676,16 @@ static void update_tracers(vdo_sim *s, double dt) {
         tp[1] += dt * ((1 - y) * (AT(v, i, j) * (1 - x) + AT(v, i + 1, j) * x) +
                        y * (AT(v, i, j + 1) * (1 - x) + AT(v, i + 1, j + 1) * x));
     }
+}
+
+int vdo_set_threads(int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+    return omp_get_max_threads();
+#else
+    (void)nthreads;
+    return 1;
+#endif
 }
 
 int vdo_set_quirks(vdo_sim *s, int q1_tau_shift, int q13_inplace) {

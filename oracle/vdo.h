@@ -112,6 +112,9 @@ long vdo_array_len(const vdo_sim *s, int id);
 int vdo_get_array(const vdo_sim *s, int id, double *out, long len);
 int vdo_set_array(vdo_sim *s, int id, const double *in, long len);
 int vdo_set_time(vdo_sim *s, double t);
+/* OpenMP threads of the oracle's loops (0 keeps the current count);
+ * returns the count in use. */
+int vdo_set_threads(int nthreads);
 /* Tracer positions (x0, y0, x1, y1, ...), moved at the start of each step
  * (fluid_2d.cc:98, :507-525). */
 int vdo_set_tracers(vdo_sim *s, const double *pos, int ntrace);

@@ -69,6 +69,8 @@ def lib():
         L.vdo_last_cycles.argtypes = [vp, ctypes.c_int]
         L.vdo_last_newton.argtypes = [vp]
         L.vdo_set_quirks.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.vdo_set_threads.argtypes = [ctypes.c_int]
+        L.vdo_set_threads.restype = ctypes.c_int
         L.vdo_set_tracers.argtypes = [vp, _dp, ctypes.c_int]
         L.vdo_get_tracers.argtypes = [vp, _dp, ctypes.c_int]
         L.vdo_pass.argtypes = [vp, ctypes.c_int, ctypes.c_double]

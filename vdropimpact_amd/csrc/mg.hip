@@ -1896,7 +1896,9 @@ int MG::launch_down(const std::vector<LevelT<T>> &H, int l, int zero_z, int fin,
     a.fin = fin;
     a.fin_begin = fin_begin;
     a.tol2 = tol2;
-    if (timer) a.ts = timer->slot(l == 0 ? VDI_KCLASS_MG_DOWN0 : VDI_KCLASS_MG_MID, down_bytes(prec, l, zero_z));
+    if (timer)
+        a.ts = timer->slot(l == 0 ? VDI_KCLASS_MG_DOWN0 : VDI_KCLASS_MG_MID, down_bytes(prec, l, zero_z),
+                           prec ? -1 : (fem ? VDI_KCLASS_FEM_DOWN : VDI_KCLASS_CC_DOWN));
     return mgk::down<T>(a, fem, down_lds(prec, l), st);
 }
 
@@ -1904,7 +1906,9 @@ template <class T>
 int MG::launch_up(const std::vector<LevelT<T>> &H, int l, int norm, double tol2) {
     const int prec = sizeof(T) == 4;
     LvArgs<T> a = up_args<T>(H, l, norm, tol2);
-    if (timer) a.ts = timer->slot(l == 0 ? VDI_KCLASS_MG_UP0 : VDI_KCLASS_MG_MID, up_bytes(prec, l));
+    if (timer)
+        a.ts = timer->slot(l == 0 ? VDI_KCLASS_MG_UP0 : VDI_KCLASS_MG_MID, up_bytes(prec, l),
+                           prec ? -1 : (fem ? VDI_KCLASS_FEM_UP : VDI_KCLASS_CC_UP));
     return mgk::up<T>(a, fem, up_lds(prec, l), st);
 }
 

@@ -166,8 +166,10 @@ struct KernelTimer {
     std::vector<double> slot_bytes;
     std::vector<float> dur_ms;
     std::vector<double> bytes;
-    Stamp *slot(int c, double alg_bytes) {
-        if (c != cls || next >= cap) return nullptr;
+    // A launch belongs to a level class c and (FP64 level kernels) to the
+    // class of its kernel function c2 (-1: none).
+    Stamp *slot(int c, double alg_bytes, int c2 = -1) {
+        if ((c != cls && c2 != cls) || next >= cap) return nullptr;
         slot_bytes[next] = alg_bytes;
         return stamps.as<Stamp>() + next++;
     }
