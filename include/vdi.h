@@ -286,8 +286,18 @@ int vdi_batch_size(vdi_batch *b);
 /* Borrowed handle of member k (for vdi_get_array etc.; do not destroy). */
 int vdi_batch_sim(vdi_batch *b, int k, vdi_ctx **out);
 int vdi_batch_init_fields(vdi_batch *b);
-/* Advances every member by its own dt_reg. */
+/* Advances every member by its own dt_reg.  A member whose step fails (the
+ * gas layer's "Too many Newton steps", where the reference's run of that
+ * simulation would stop) does not advance; the others do, and the call
+ * returns the first such status.  Until an ended member is restarted, later
+ * calls return VDI_ERR_ARG. */
 int vdi_batch_step(vdi_batch *b);
+/* Status of each member's last step (n >= size). */
+int vdi_batch_member_status(vdi_batch *b, int *status, int n);
+/* Starts member k over as a new simulation with the same parameters: initial
+ * fields and gas layer, time 0, empty time-extrapolation histories and zero
+ * iterates, as a newly created context (a batch used as slots of a sweep). */
+int vdi_batch_restart_member(vdi_batch *b, int k);
 int vdi_batch_autotune(vdi_batch *b, int reps);
 /* Last batched step: graph time (ms) and each member's workgroup busy time
  * (ms, summed over its work items; n >= size). */

@@ -133,6 +133,8 @@ ABI = {
     "vdi_batch_autotune": (_i, [_vp, _i]),
     "vdi_batch_step_times": (_i, [_vp, _dp, _dp, _i]),
     "vdi_batch_member_spans": (_i, [_vp, _dp, _i]),
+    "vdi_batch_member_status": (_i, [_vp, _ip, _i]),
+    "vdi_batch_restart_member": (_i, [_vp, _i]),
     "vdi_batch_counters": (_i, [_vp, ctypes.POINTER(_l), ctypes.POINTER(_l)]),
     "vdi_batch_tiles": (_i, [_vp, _ip, _i]),
     "vdi_guess_attach": (_i, [_vp, _i, ctypes.c_ulonglong]),
@@ -507,6 +509,16 @@ class Batch:
         out = np.zeros(self.size)
         _check(self.lib.vdi_batch_member_spans(self.h, _ptr(out), self.size), "member_spans")
         return out
+
+    def member_status(self) -> np.ndarray:
+        """Status of each member's last step (OK, or why its run ended)."""
+        out = (ctypes.c_int * self.size)()
+        _check(self.lib.vdi_batch_member_status(self.h, out, self.size), "member_status")
+        return np.array(out[:], dtype=int)
+
+    def restart(self, k: int):
+        """Starts member k over as a new simulation with the same parameters."""
+        _check(self.lib.vdi_batch_restart_member(self.h, k), "vdi_batch_restart_member")
 
     def counters(self):
         a, b = ctypes.c_long(), ctypes.c_long()

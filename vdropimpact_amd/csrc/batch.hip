@@ -90,7 +90,13 @@ struct vdi_batch {
     };
     std::vector<GraphEntry> graphs;
     int budget[4] = {0, 0, 0, 0};
-    long batch_steps = 0, redo_sims = 0;
+    long batch_steps = 0, redo_sims = 0, restarts = 0;
+    // status of each member's last step (VDI_OK, or why its run ended)
+    std::vector<int> status;
+    // once every member had its first steps, later steps are always batched
+    // (a restarted member's first solves take (1, 0) coefficients, which is
+    // what mgs_common::initial_guess does while its history fills)
+    bool primed = false;
     double last_ms = 0;
     double ticks_per_ms = 1e5;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -449,11 +455,13 @@ static int capture(vdi_batch *b, vdi_batch::GraphEntry &e) {
 }
 
 static bool batch_ready(vdi_batch *b) {
+    if (b->primed) return true;
     for (vdi_ctx *c : b->sims)
         for (int k = 0; k < 4; k++) {
             if (!solver_used(c, k)) continue;
             if (step_solver(c, k)->scount < 2 || c->budget[k] < 1) return false;
         }
+    b->primed = true;
     return true;
 }
 
@@ -499,12 +507,19 @@ static int batch_step_graph(vdi_batch *b) {
     if (hipEventElapsedTime(&ms, b->ev0, b->ev1) == hipSuccess) b->last_ms = ms;
     (void)hipGetLastError();
     b->batch_steps++;
+    int first_err = VDI_OK;
     for (int s = 0; s < b->S; s++) {
         vdi_ctx *c = b->sims[s];
         const vdi_ctx::Readback &r = b->rb[s];
         c->last_newton = r.gas[1];
         c->last_height_passes = r.gas[2];
-        if (r.gas[0] != VDI_OK) return r.gas[0];
+        b->status[s] = r.gas[0];
+        if (r.gas[0] != VDI_OK) {
+            // this member's run ends here (the reference stops with the gas
+            // layer's error); the others' steps stand
+            if (first_err == VDI_OK) first_err = r.gas[0];
+            continue;
+        }
         bool ok = true;
         for (int k = 0; k < 4; k++) {
             if (!solver_used(c, k)) continue;
@@ -526,9 +541,12 @@ static int batch_step_graph(vdi_batch *b) {
             continue;
         }
         b->redo_sims++;
-        VDI_TRY(redo_step(c, c->P.dt_reg, t0[s]));
+        const int rc = redo_step(c, c->P.dt_reg, t0[s]);
+        b->status[s] = rc;
+        if (rc == VDI_ERR_HIP) return rc;
+        if (rc != VDI_OK && first_err == VDI_OK) first_err = rc;
     }
-    return VDI_OK;
+    return first_err;
 }
 
 }  // namespace
@@ -543,6 +561,7 @@ int vdi_batch_create(const vdi_params *params, int count, int form, vdi_batch **
         if (params[s].x_sym != params[0].x_sym || params[s].implicit_visc != params[0].implicit_visc) return VDI_ERR_ARG;
     std::unique_ptr<vdi_batch> b(new vdi_batch());
     b->S = count;
+    b->status.assign(count, VDI_OK);
     b->form = form;
     b->implicit = params[0].implicit_visc;
     VDI_HIP(hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking));
@@ -618,9 +637,39 @@ int vdi_batch_init_fields(vdi_batch *b) {
 
 int vdi_batch_step(vdi_batch *b) {
     if (!b) return VDI_ERR_ARG;
+    for (int s = 0; s < b->S; s++)
+        if (b->status[s] != VDI_OK) return VDI_ERR_ARG;  // an ended member must be restarted first
     if (batch_ready(b)) return batch_step_graph(b);
     // the first steps (time history of the initial guesses) run one by one
-    for (vdi_ctx *c : b->sims) VDI_TRY(vdi_step_forward(c, c->P.dt_reg));
+    int first_err = VDI_OK;
+    for (int s = 0; s < b->S; s++) {
+        const int rc = vdi_step_forward(b->sims[s], b->sims[s]->P.dt_reg);
+        b->status[s] = rc;
+        if (rc == VDI_ERR_HIP) return rc;
+        if (rc != VDI_OK && first_err == VDI_OK) first_err = rc;
+    }
+    return first_err;
+}
+
+int vdi_batch_member_status(vdi_batch *b, int *status, int n) {
+    if (!b || !status || n < b->S) return VDI_ERR_ARG;
+    std::copy(b->status.begin(), b->status.end(), status);
+    return VDI_OK;
+}
+
+int vdi_batch_restart_member(vdi_batch *b, int k) {
+    if (!b || k < 0 || k >= b->S) return VDI_ERR_ARG;
+    VDI_HIP(hipStreamSynchronize(b->st));
+    vdi_ctx *c = b->sims[k];
+    for (int w = 0; w < 4; w++) {
+        if (!solver_used(c, w)) continue;
+        MG &g = *step_solver(c, w);
+        VDI_TRY(g.reset_history());
+        VDI_HIP(hipStreamSynchronize(g.st));
+    }
+    VDI_TRY(vdi_init_fields(c));
+    b->status[k] = VDI_OK;
+    b->restarts++;
     return VDI_OK;
 }
 

@@ -2020,6 +2020,20 @@ int MG::clear_solve_state() {
     return VDI_OK;
 }
 
+int MG::reset_history() {
+    scount = 0;
+    t0 = t_prev = 0;
+    last_cycles = 0;
+    const long N = unknowns();
+    hipLaunchKernelGGL(k_zero, dim3(cdiv(N, 256)), dim3(256), 0, st, L[0].z, N);
+    VDI_LAUNCHED();
+    if (zprev_.p) {
+        hipLaunchKernelGGL(k_zero, dim3(cdiv(N, 256)), dim3(256), 0, st, zprev_.as<double>(), N);
+        VDI_LAUNCHED();
+    }
+    return clear_solve_state();
+}
+
 int MG::read_state(MGState *out) {
     if (!host_state_) {
         // arena mode: no pinned buffer, a pageable copy

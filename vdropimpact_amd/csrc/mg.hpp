@@ -322,6 +322,9 @@ class MG {
     int dense_inverse();
     // Zeroes the solve state and the partial sums (fill kernels on st).
     int clear_solve_state();
+    // A fresh solver for a new simulation on the same grid and couplings:
+    // empty time history, z = z_prev = 0 (as a newly constructed one).
+    int reset_history();
 
   private:
     int apply_schedule(int nu1, int nu2, int nu1s, int nu2s);
