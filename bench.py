@@ -156,7 +156,7 @@ def operator_level(vdi, reps=3):
 
 
 def measured_traffic(kernel_class):
-    """HBM bytes per launch of the dominant kernel class from the committed
+    """HBM bytes per launch of the dominant kernel function (or class) from the committed
     rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE per dispatch, each
     scaled by the factor the same counter reads for a plain copy kernel of
     known size), or None when no summary for this class is present."""
@@ -166,7 +166,7 @@ def measured_traffic(kernel_class):
             d = json.load(f)
     except (OSError, ValueError):
         return None
-    k = d.get("classes", {}).get(kernel_class)
+    k = d.get("kernels", {}).get(kernel_class) or d.get("classes", {}).get(kernel_class)
     return None if k is None else k.get("hbm_bytes_per_launch")
 
 
