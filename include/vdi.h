@@ -86,7 +86,7 @@ typedef struct vdi_stats {
 #define VDI_KCLASS_MG_DOWN0 0   /* level-0 smooth+restrict of every solve */
 #define VDI_KCLASS_MG_UP0 1     /* level-0 prolong+smooth+residual of every solve */
 #define VDI_KCLASS_MG_MID 2     /* down/up kernels of levels 1..lc-1 */
-#define VDI_KCLASS_MG_COARSE 3  /* single-workgroup coarse-level V-cycle */
+#define VDI_KCLASS_MG_COARSE 3  /* single-workgroup exact solve of the coarsest level */
 #define VDI_KCLASS_GAS 4        /* gas-layer update */
 #define VDI_KCLASS_SIMPSON 5    /* far-field boundary integrals */
 /* The same launches grouped by kernel function (every level), as a profiler
@@ -253,8 +253,7 @@ int vdi_mg_cycle_bytes(vdi_mg *g, double *bytes);
  * arrays b_dev (read) and z_dev (initial guess in, solution out), on the null
  * stream, returning when done.  Every other buffer lives in one static device
  * array of VDI_NOALLOC_SCRATCH_MB MiB per GPU (enough for 1025 x 513
- * unknowns); larger systems, and ones whose coarsening stops above the dense
- * coarsest size, return VDI_ERR_ARG.  FP64 form only.  Calls on one GPU share
+ * unknowns); larger systems return VDI_ERR_ARG.  FP64 form only.  Calls on one GPU share
  * that array: calls that overlap in time race. */
 int vdi_mg_solve_noalloc(int kind, int m, int n, double cx, double cy, int x_sym, const double *b_dev,
                          double *z_dev, double rel_tol, int max_cycles, int *vcycles);

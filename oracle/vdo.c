@@ -17,11 +17,12 @@
 #include <string.h>
 
 #define VDO_MAXLEV 24
-/* Coarsening stops at the first level with at most VDO_DENSE_POINTS
- * unknowns; that level is solved exactly by a dense inverse when it has at
- * most VDO_DENSE_MAX unknowns (coarsening can stop earlier, on odd sizes). */
-#define VDO_DENSE_POINTS 256
-#define VDO_DENSE_MAX 1024
+/* Coarsening stops at the first level with at most VDO_EXACT_POINTS
+ * unknowns (or earlier, on odd sizes).  That level is solved exactly by
+ * separation of variables when its shorter side (without the nodal
+ * Dirichlet row) has at most VDO_SEP_MAXT unknowns, else by ncs sweeps. */
+#define VDO_EXACT_POINTS 3072
+#define VDO_SEP_MAXT 64
 
 /* ------------------------------------------------------------------------ */
 /* Multigrid                                                                 */
@@ -55,7 +56,13 @@ struct vdo_mg {
     double t0, t_prev;
     double vfx, vfy; /* visco: coefficients currently set up (visco_impl.cc) */
     int last_cycles;
-    double *inv;     /* dense inverse of the coarsest level (row-major), or NULL */
+    /* exact solve of the coarsest level: z = (V (x) I) (S + lambda_k Q)^-1 (V^T (x) I) b,
+     * with unknown (p, t) at column p, row t + yoff (tdir 0) or column t, row p + yoff (tdir 1) */
+    int sep, tdir, sT, sP, yoff;
+    double *V;       /* sT x sT, V[t * sT + k] */
+    double *lam;     /* sT generalized eigenvalues */
+    double *Sd, *Qd; /* diagonals of S and Q along p (sP each) */
+    double Se, Qe;   /* their constant off-diagonals */
 };
 
 static double bc_diag_factor(int bc) { return bc == BC_DF ? 2.0 : (bc == BC_DC ? 1.0 : 0.0); }
@@ -83,7 +90,7 @@ static void mg_build_levels(vdo_mg *g, int m, int n, double cx, double cy) {
         L->r = (double *)xcalloc(N, sizeof(double));
         l++;
         if (l >= VDO_MAXLEV) break;
-        if (N <= VDO_DENSE_POINTS) break;
+        if (N <= VDO_EXACT_POINTS) break;
         if (mc % 2 || nc % 2 || mc / 2 < 2 || nc / 2 < 2) break;
         mc /= 2; nc /= 2; cx *= 0.25; cy *= 0.25;
     }
@@ -92,50 +99,190 @@ static void mg_build_levels(vdo_mg *g, int m, int n, double cx, double cy) {
 
 static void level_apply(const vdo_mg *g, const vdo_level *L, const double *z, double *out);
 
-/* Dense inverse of the coarsest level's operator: columns A e_k by applying
- * the level operator to unit vectors, then Gauss-Jordan elimination with
- * partial pivoting. */
-static void mg_dense_inverse(vdo_mg *g) {
-    const vdo_level *L = &g->L[g->nl - 1];
-    const int N = L->m * L->n;
-    free(g->inv);
-    g->inv = NULL;
-    if (N > VDO_DENSE_MAX) return;
-    const int W = 2 * N;
-    double *a = (double *)xcalloc((size_t)N * W, sizeof(double)); /* [A | I], row-major */
-    double *e = (double *)xcalloc(N, sizeof(double)), *col = (double *)xcalloc(N, sizeof(double));
-    for (int k = 0; k < N; k++) {
-        e[k] = 1;
-        level_apply(g, L, e, col);
-        e[k] = 0;
-        for (int i = 0; i < N; i++) a[(size_t)i * W + k] = col[i];
-        a[(size_t)k * W + N + k] = 1;
-    }
-    for (int c = 0; c < N; c++) {
-        int piv = c;
-        for (int i = c + 1; i < N; i++)
-            if (fabs(a[(size_t)i * W + c]) > fabs(a[(size_t)piv * W + c])) piv = i;
-        if (piv != c)
-            for (int k = 0; k < W; k++) {
-                double t = a[(size_t)c * W + k];
-                a[(size_t)c * W + k] = a[(size_t)piv * W + k];
-                a[(size_t)piv * W + k] = t;
+/* ---- exact solve of the coarsest level by separation of variables ----
+ * Each level operator is a sum of two Kronecker products of 1-D symmetric
+ * tridiagonal matrices, A = R (x) S + Pm (x) Q, R positive definite, the
+ * first factor acting along the transformed direction t and the second along
+ * the kept one p.  The generalized eigenvectors Pm v = lambda R v with
+ * V^T R V = I split A z = b into (S + lambda_k Q) zh_k = (V^T b)_k, one
+ * tridiagonal system per mode, and z = V zh.  Cell-centred
+ * A = sigma I + s (cx Dx (x) I + cy I (x) Dy) (second differences with the
+ * boundary types' diagonal factors); nodal Q1 on the rows above the
+ * Dirichlet row: A = alpha Kx (x) My + beta Mx (x) Ky with 1-D stiffness
+ * K = (-1, 2, -1) and mass M = (1, 4, 1), halved at the natural ends. */
+
+static inline double fem_acc(const vdo_mg *g, const vdo_level *L, int i, int j);
+
+/* Cyclic Jacobi rotations on the symmetric n x n matrix a (row-major,
+ * overwritten); eigenvalues w, eigenvectors in the columns of q. */
+static void sep_jacobi(int n, double *a, double *w, double *q) {
+    for (int i = 0; i < n * n; i++) q[i] = 0;
+    for (int i = 0; i < n; i++) q[i * n + i] = 1;
+    double tot = 0;
+    for (int i = 0; i < n * n; i++) tot += a[i] * a[i];
+    for (int sweep = 0; sweep < 64; sweep++) {
+        double off = 0;
+        for (int i = 0; i < n; i++)
+            for (int j = i + 1; j < n; j++) off += a[i * n + j] * a[i * n + j];
+        if (off <= 1e-30 * tot) break;
+        for (int p = 0; p < n; p++)
+            for (int r = p + 1; r < n; r++) {
+                double apr = a[p * n + r];
+                if (apr == 0) continue;
+                double th = (a[r * n + r] - a[p * n + p]) / (2 * apr);
+                double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1));
+                double c = 1 / sqrt(t * t + 1), sn = t * c;
+                for (int k = 0; k < n; k++) {
+                    double x = a[k * n + p], y = a[k * n + r];
+                    a[k * n + p] = c * x - sn * y;
+                    a[k * n + r] = sn * x + c * y;
+                }
+                for (int k = 0; k < n; k++) {
+                    double x = a[p * n + k], y = a[r * n + k];
+                    a[p * n + k] = c * x - sn * y;
+                    a[r * n + k] = sn * x + c * y;
+                }
+                a[p * n + r] = a[r * n + p] = 0;
+                for (int k = 0; k < n; k++) {
+                    double x = q[k * n + p], y = q[k * n + r];
+                    q[k * n + p] = c * x - sn * y;
+                    q[k * n + r] = sn * x + c * y;
+                }
             }
-        const double d = 1.0 / a[(size_t)c * W + c];
-        for (int k = 0; k < W; k++) a[(size_t)c * W + k] *= d;
-        for (int i = 0; i < N; i++) {
-            if (i == c) continue;
-            const double f = a[(size_t)i * W + c];
-            if (f == 0) continue;
-            for (int k = 0; k < W; k++) a[(size_t)i * W + k] -= f * a[(size_t)c * W + k];
-        }
     }
-    g->inv = (double *)xcalloc((size_t)N * N, sizeof(double));
-    for (int i = 0; i < N; i++)
-        for (int k = 0; k < N; k++) g->inv[(size_t)i * N + k] = a[(size_t)i * W + N + k];
-    free(a);
-    free(e);
-    free(col);
+    for (int i = 0; i < n; i++) w[i] = a[i * n + i];
+}
+
+/* 1-D line: diagonal d[0..n), off-diagonal e. */
+static void cc_line(int n, double c, double sigma, int lo, int hi, double *d, double *e) {
+    for (int i = 0; i < n; i++)
+        d[i] = sigma - c * ((i > 0 ? 1.0 : bc_diag_factor(lo)) + (i < n - 1 ? 1.0 : bc_diag_factor(hi)));
+    *e = c;
+}
+
+static void mg_sep_free(vdo_mg *g) {
+    free(g->V); free(g->lam); free(g->Sd); free(g->Qd);
+    g->V = g->lam = g->Sd = g->Qd = NULL;
+    g->sep = 0;
+}
+
+static void mg_sep_setup(vdo_mg *g) {
+    mg_sep_free(g);
+    const vdo_level *L = &g->L[g->nl - 1];
+    const int M = L->m, Nn = L->n, rows = g->fem ? Nn - 1 : Nn;
+    const int tdir = rows <= M ? 0 : 1, T = tdir == 0 ? rows : M, P = tdir == 0 ? M : rows;
+    if (T > VDO_SEP_MAXT || rows < 1) return;
+    double *Rd = xcalloc(T, sizeof(double)), *Pd = xcalloc(T, sizeof(double)), Re = 0, Pe = 0;
+    g->Sd = xcalloc(P, sizeof(double));
+    g->Qd = xcalloc(P, sizeof(double));
+    if (!g->fem) {
+        const int lo_t = tdir == 0 ? g->bc[2] : g->bc[0], hi_t = tdir == 0 ? g->bc[3] : g->bc[1];
+        const int lo_p = tdir == 0 ? g->bc[0] : g->bc[2], hi_p = tdir == 0 ? g->bc[1] : g->bc[3];
+        const double ct = tdir == 0 ? L->cy : L->cx, cp = tdir == 0 ? L->cx : L->cy;
+        cc_line(T, g->s * ct, g->sigma, lo_t, hi_t, Pd, &Pe);
+        cc_line(P, g->s * cp, 0.0, lo_p, hi_p, g->Sd, &g->Se);
+        for (int i = 0; i < T; i++) Rd[i] = 1;
+        for (int i = 0; i < P; i++) g->Qd[i] = 1;
+        Re = 0; g->Qe = 0;
+    } else {
+        /* scales from the edge couplings: fex = -4 alpha + 2 beta, fey = 2 alpha - 4 beta */
+        const double beta = -(g->fex + 2 * g->fey) / 6, alpha = (2 * beta - g->fex) / 4;
+        /* x line (all M nodes, natural ends) and y line (rows 1..Nn-1, natural top) */
+        #define KX(i) ((i) == 0 || (i) == M - 1 ? 1.0 : 2.0)
+        #define MX(i) ((i) == 0 || (i) == M - 1 ? 2.0 : 4.0)
+        #define KY(q) ((q) == Nn - 2 ? 1.0 : 2.0)
+        #define MY(q) ((q) == Nn - 2 ? 2.0 : 4.0)
+        if (tdir == 0) { /* A = My (x) alpha Kx + Ky (x) beta Mx */
+            for (int q = 0; q < T; q++) { Rd[q] = MY(q); Pd[q] = KY(q); }
+            Re = 1; Pe = -1;
+            for (int i = 0; i < P; i++) { g->Sd[i] = alpha * KX(i); g->Qd[i] = beta * MX(i); }
+            g->Se = -alpha; g->Qe = beta;
+        } else {         /* A = Mx (x) beta Ky + Kx (x) alpha My */
+            for (int i = 0; i < T; i++) { Rd[i] = MX(i); Pd[i] = KX(i); }
+            Re = 1; Pe = -1;
+            for (int q = 0; q < P; q++) { g->Sd[q] = beta * KY(q); g->Qd[q] = alpha * MY(q); }
+            g->Se = -beta; g->Qe = alpha;
+        }
+        #undef KX
+        #undef MX
+        #undef KY
+        #undef MY
+    }
+    /* R = L L^T (bidiagonal L: diagonal l, subdiagonal gs), C = L^-1 Pm L^-T */
+    double *l = xcalloc(T, sizeof(double)), *gs = xcalloc(T, sizeof(double));
+    for (int i = 0; i < T; i++) {
+        if (i > 0) gs[i] = Re / l[i - 1];
+        l[i] = sqrt(Rd[i] - gs[i] * gs[i]);
+    }
+    double *C = xcalloc((size_t)T * T, sizeof(double)), *X = xcalloc((size_t)T * T, sizeof(double));
+    for (int j = 0; j < T; j++) /* X = L^-1 Pm, column j */
+        for (int i = 0; i < T; i++) {
+            double v = i == j ? Pd[i] : (abs(i - j) == 1 ? Pe : 0.0);
+            if (i > 0) v -= gs[i] * X[(i - 1) * T + j];
+            X[i * T + j] = v / l[i];
+        }
+    for (int j = 0; j < T; j++) /* C^T = L^-1 X^T, so row j of C solves L c = (row j of X) */
+        for (int i = 0; i < T; i++) {
+            double v = X[j * T + i];
+            if (i > 0) v -= gs[i] * C[j * T + i - 1];
+            C[j * T + i] = v / l[i];
+        }
+    for (int i = 0; i < T; i++)
+        for (int j = i + 1; j < T; j++) C[i * T + j] = C[j * T + i] = 0.5 * (C[i * T + j] + C[j * T + i]);
+    double *W = xcalloc((size_t)T * T, sizeof(double));
+    g->lam = xcalloc(T, sizeof(double));
+    sep_jacobi(T, C, g->lam, W);
+    g->V = xcalloc((size_t)T * T, sizeof(double));
+    for (int k = 0; k < T; k++) /* V = L^-T W */
+        for (int i = T - 1; i >= 0; i--) {
+            double v = W[i * T + k];
+            if (i + 1 < T) v -= gs[i + 1] * g->V[(i + 1) * T + k];
+            g->V[i * T + k] = v / l[i];
+        }
+    free(Rd); free(Pd); free(l); free(gs); free(C); free(X); free(W);
+    g->sep = 1; g->tdir = tdir; g->sT = T; g->sP = P; g->yoff = g->fem ? 1 : 0;
+}
+
+/* z = A^-1 b on the coarsest level (sep set). */
+static void mg_sep_solve(vdo_mg *g, const vdo_level *L, const double *b, double *z) {
+    const int M = L->m, N = M * L->n, T = g->sT, P = g->sP;
+    double *rhs = xcalloc(N, sizeof(double));
+    memcpy(rhs, b, sizeof(double) * N);
+    memset(z, 0, sizeof(double) * N);
+    if (g->fem) { /* Dirichlet row, then its coupling moved to the right-hand side */
+        double *az = xcalloc(N, sizeof(double));
+        for (int i = 0; i < M; i++) z[i] = b[i] / fem_acc(g, L, i, 0);
+        level_apply(g, L, z, az);
+        for (int k = M; k < N; k++) rhs[k] -= az[k];
+        free(az);
+    }
+    double *h = xcalloc((size_t)T * P, sizeof(double));
+    #define IDX(p, t) (g->tdir == 0 ? (p) + (size_t)M * ((t) + g->yoff) : (t) + (size_t)M * ((p) + g->yoff))
+    for (int k = 0; k < T; k++)
+        for (int p = 0; p < P; p++) {
+            double v = 0;
+            for (int t = 0; t < T; t++) v += g->V[t * T + k] * rhs[IDX(p, t)];
+            h[(size_t)k * P + p] = v;
+        }
+    double *cpv = xcalloc(P, sizeof(double));
+    for (int k = 0; k < T; k++) { /* Thomas on S + lambda_k Q */
+        double *hk = h + (size_t)k * P, e = g->Se + g->lam[k] * g->Qe, y = 0;
+        for (int p = 0; p < P; p++) {
+            double den = g->Sd[p] + g->lam[k] * g->Qd[p] - (p > 0 ? e * cpv[p - 1] : 0.0);
+            y = (hk[p] - (p > 0 ? e * y : 0.0)) / den;
+            cpv[p] = e / den;
+            hk[p] = y;
+        }
+        for (int p = P - 2; p >= 0; p--) hk[p] -= cpv[p] * hk[p + 1];
+    }
+    for (int t = 0; t < T; t++)
+        for (int p = 0; p < P; p++) {
+            double v = 0;
+            for (int k = 0; k < T; k++) v += g->V[t * T + k] * h[(size_t)k * P + p];
+            z[IDX(p, t)] = v;
+        }
+    #undef IDX
+    free(h); free(cpv); free(rhs);
 }
 
 vdo_mg *vdo_mg_create(int kind, int m, int n, double cx, double cy, int x_sym) {
@@ -165,7 +312,7 @@ vdo_mg *vdo_mg_create(int kind, int m, int n, double cx, double cy, int x_sym) {
         g->fc = -1. / 6. * (dxdy + dydx);
     }
     mg_build_levels(g, m, n, cx, cy);
-    mg_dense_inverse(g);
+    mg_sep_setup(g);
     g->z_prev = (double *)xcalloc((size_t)m * n, sizeof(double));
     return g;
 }
@@ -177,7 +324,7 @@ void vdo_mg_destroy(vdo_mg *g) {
         free(g->L[l].r);
     }
     free(g->z_prev);
-    free(g->inv);
+    mg_sep_free(g);
     free(g);
 }
 
@@ -191,7 +338,7 @@ int vdo_mg_levels(const vdo_mg *g) { return g->nl; }
 /* Re-scale the cell-centred couplings (visco re-setup, visco_impl.cc:26-30). */
 static void mg_set_coeffs(vdo_mg *g, double cx, double cy) {
     for (int l = 0; l < g->nl; l++) { g->L[l].cx = cx; g->L[l].cy = cy; cx *= 0.25; cy *= 0.25; }
-    mg_dense_inverse(g);
+    mg_sep_setup(g);
 }
 
 /* ---- cell-centred operator ---- */
@@ -351,14 +498,8 @@ static void level_prolong(const vdo_mg *g, vdo_level *F, double *zf, const vdo_l
 static void mg_vcycle_level(vdo_mg *g, int l, const double *b, double *z) {
     vdo_level *L = &g->L[l];
     if (l == g->nl - 1) {
-        if (g->inv) {
-            /* exact solve z = A^-1 b on the coarsest level */
-            const int N = L->m * L->n;
-            for (int i = 0; i < N; i++) {
-                double v = 0;
-                for (int k = 0; k < N; k++) v += g->inv[(size_t)i * N + k] * b[k];
-                z[i] = v;
-            }
+        if (g->sep) {
+            mg_sep_solve(g, L, b, z);
         } else {
             for (int k = 0; k < g->ncs; k++) level_sweep(g, L, b, z);
         }

@@ -17,6 +17,8 @@
  * Pinning status of each part (what checks it in tests/):
  *   operators (MAC, FEM, visco)  closed-form separable eigenvectors + dense solves
  *   multigrid solve              residual of the pinned operator; closed-form solutions
+ *   exact coarsest-level solve   numpy dense solve of the assembled level operator,
+ *                                both transform directions (test_oracle_pins.py)
  *   tridiagonal solve            tri_test closed form x(x-1)/2; scipy dgtsv
  *   mono_diff limiter            exact for linear data, 0 at extrema, odd symmetry,
  *                                fourth-order accuracy on smooth data
@@ -25,7 +27,11 @@
  *   ENO2 height update           exact for quadratic h; upwind side by sign of u
  *   Newton gas layer             PDE residual written from the lubrication equation,
  *                                Jacobian checked by finite differences
- *   whole coupled step           parity unpinned: the reference cannot be compiled
+ *   step passes K1-K15, BCs,     exactness on linear / bilinear / uniform fields,
+ *   liquid fields, extrapolation discrete identities, Taylor-Green convergence,
+ *                                mirror symmetry of the whole step
+ *                                (test_oracle_step_pins.py; planted mistakes in
+ *                                test_oracle_mutations.py)
  *                                here (TGMG and LAPACK are absent), so only the
  *                                components above are pinned.
  */

@@ -75,19 +75,28 @@ def test_mg_solves_closed_form(oracle, kind, m, n, cx, cy):
     assert np.linalg.norm(sol - z) / np.linalg.norm(z) < 1e-10
 
 
+@pytest.mark.parametrize("x_sym", [1, 0])
+@pytest.mark.parametrize("wide", [True, False])
 @pytest.mark.parametrize("kind", [cf.MAC, cf.FEM, cf.VX, cf.VY])
-def test_mg_matches_dense_solve(oracle, kind):
-    m, n = (17, 9) if kind == cf.FEM else (16, 8)
+def test_mg_matches_dense_solve(oracle, kind, wide, x_sym):
+    """A level of at most 3072 unknowns is the coarsest one and is solved
+    exactly (separation of variables, transformed along the shorter side:
+    rows for the wide grid, columns for the tall one): one V-cycle from zero
+    gives numpy's dense solve of the assembled operator."""
+    m, n = (16, 8) if wide else (10, 22)
+    if kind == cf.FEM:
+        m, n = m + 1, n + 1
     cx, cy = (0.1, 0.07) if kind == cf.FEM else ((100.0, 150.0) if kind == cf.MAC else (0.4, 0.3))
-    g = oracle.OracleMG(kind, m, n, cx, cy, 1)
+    g = oracle.OracleMG(kind, m, n, cx, cy, x_sym)
+    assert g.levels == 1
     N = m * n
     A = np.stack([g.apply(np.eye(N)[c]) for c in range(N)], axis=1)
     rng = np.random.default_rng(7)
     b = rng.standard_normal(N)
     z_dense = np.linalg.solve(A, b)
-    z, cycles = g.solve(b, tol=1e-14)
-    assert cycles >= 1
-    np.testing.assert_allclose(z, z_dense, rtol=0, atol=1e-10 * np.abs(z_dense).max())
+    z, cycles = g.solve(b, tol=1e-12)
+    assert cycles == 1
+    np.testing.assert_allclose(z, z_dense, rtol=0, atol=1e-11 * np.abs(z_dense).max())
 
 
 def test_mg_convergence_rate(oracle):

@@ -571,14 +571,12 @@ int vdi_batch_create(const vdi_params *params, int count, int form, vdi_batch **
         b->sims.push_back(c);
         c->use_graphs = 0;  // the batch captures the steps
         VDI_TRY(vdi_set_form(c, form));
-        // the mixed form needs at least one tiled level: small grids hand
-        // over to the single-workgroup kernel later
+        // the mixed form needs at least one tiled level: a grid of at most
+        // kExactPoints cells is solved exactly at once and cannot take it
         if (form == VDI_FORM_MIXED)
             for (int k = 0; k < 4; k++) {
                 if (!solver_used(c, k)) continue;
-                MG &g = *step_solver(c, k);
-                for (int pts = kCoarsePoints; g.lc < 1 && pts >= 8; pts /= 2) (void)g.set_coarse_points(pts);
-                if (g.lc < 1) return VDI_ERR_ARG;
+                if (step_solver(c, k)->lc < 1) return VDI_ERR_ARG;
             }
     }
     for (int k = 0; k < 4; k++) {

@@ -1,15 +1,15 @@
 // mg.hip -- multigrid kernels and driver (see mg.hpp for the scheme).
 //
-// Kernel structure of one V-cycle on an L-level hierarchy whose levels
-// lc..L-1 hold at most kCoarsePoints unknowns:
+// Kernel structure of one V-cycle on an L-level hierarchy whose coarsest
+// level lc = L-1 holds at most kExactPoints unknowns:
 //   down(l), l = 0..lc-1 : nu1 smoothing sweeps of level l, residual,
 //                          restriction into level l+1   (one kernel per level)
-//   coarse               : a complete V-cycle on levels lc..L-1 inside one
-//                          workgroup's LDS
+//   coarse               : the exact solve of level lc (separation of
+//                          variables) inside one workgroup's LDS
 //   up(l), l = lc-1..0   : prolongation + correction, nu2 sweeps; at level 0
 //                          also per-workgroup partials of ||b - A z||^2
 //                          (level 0 takes V(nu1, nu2), every restricted level
-//                          V(nu1s, nu2s), in these kernels and in coarse)
+//                          V(nu1s, nu2s))
 //   norm                 : one workgroup sums the partials in a fixed order
 //                          and raises the device-side convergence flag.
 // Every kernel returns at once when the flag is set, so a solve enqueues the
@@ -30,6 +30,8 @@
 // XCD one contiguous band of tiles, whose shared halos then hit that XCD's
 // own L2.
 #include "mg.hpp"
+
+#include "coarse_sep.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -800,11 +802,11 @@ __global__ void __launch_bounds__(kLvThreads) k_mx_up0_b(const MxArgs *A, const 
 }
 
 // ------------------------------------------------------------------------
-// Coarse levels: level k is kept in LDS with a one-point zero border, as an
-// (m + 2) x (n + 2) tile for z and one for b, so the sweeps, residuals and
-// restrictions are the tiled kernels' own code on a tile covering the whole
-// level (a level gives the same bits whichever kernel smooths it), and no
-// LDS read sits behind a boundary test.  Threads are laid out 32 x 16.
+// The coarsest level on one workgroup.  The level sits in LDS as an
+// (m + 2) x (n + 2) tile with a zero border, for z and for b, so that its
+// residual (and, where the level is not separable, its sweeps) is the tiled
+// kernels' own code on a tile covering the whole level, with no boundary
+// test in front of an LDS read.  Threads are laid out 32 x 16.
 __device__ __forceinline__ Tile coarse_tile(int m, int n) {
     Tile t;
     t.H = 1;
@@ -818,29 +820,182 @@ __device__ __forceinline__ Tile coarse_tile(int m, int n) {
 }
 
 template <class T>
-__device__ __forceinline__ LevelT<T> coarse_level(const CoarseDesc &d, int k) {
+__device__ __forceinline__ LevelT<T> coarse_level(const CoarseDesc &d) {
     LevelT<T> L{};
-    L.m = d.m[k];
-    L.n = d.n[k];
-    L.cx = T(d.cx[k]);
-    L.cy = T(d.cy[k]);
-    L.inv_int = T(d.inv[k]);
+    L.m = d.m;
+    L.n = d.n;
+    L.cx = T(d.cx);
+    L.cy = T(d.cy);
+    L.inv_int = T(d.inv);
     return L;
 }
 
-// LDS of the coarse kernel after the levels: kCoarseThreads partial
-// products, kDenseReg right-hand-side values, then the reciprocal tables of
-// the levels (9 each).
-__device__ __forceinline__ int coarse_tab_off(const CoarseDesc &d) { return d.red_off + kCoarseThreads + kDenseReg; }
+// Exact solve of the level by separation of variables (the factors of
+// MG::coarse_factors, staged in the double workspace ws): bh = (V^T (x) I) b,
+// one Thomas solve per mode along the kept direction, z = (V (x) I) zh.
+// Unknown (p, t) sits at tile column p, row t + yoff (tdir 0) or column t,
+// row p + yoff (tdir 1); yoff = 1 for the nodal system, whose row 0 is the
+// Dirichlet row z = b / a_cc: its coupling to row 1 goes to the right-hand
+// side, kept in z's row 1 until the solve overwrites it.
+template <class T, bool FEM>
+__device__ __forceinline__ void coarse_exact(const CoarseArgs<T> &a, T *z, const T *b, double *ws) {
+    const CoarseDesc &d = a.d;
+    const int tid = threadIdx.x + kCoarseBX * threadIdx.y, Tn = kCoarseThreads;
+    const int RX = d.m + 2, Tt = d.T, P = d.P, PP = P + 1;
+    double *hat = ws;
+    const double *V = hat + (size_t)Tt * PP, *iv = V + (size_t)Tt * Tt, *cp = iv + (size_t)Tt * PP;
+    if (FEM) {
+        for (int i = tid; i < d.m; i += Tn) z[RX + i + 1] = b[RX + i + 1] / fem_acc<T>(a.fop, d.m, d.n, i, 0);
+        __syncthreads();
+        for (int i = tid; i < d.m; i += Tn) {
+            const int l = 2 * RX + i + 1;
+            const T wi = (i == 0 || i == d.m - 1) ? T(0.5) : T(1);
+            z[l] = b[l] - (T(a.fop.fc) * (z[l - RX - 1] + z[l - RX + 1]) + wi * T(a.fop.fey) * z[l - RX]);
+        }
+        __syncthreads();
+    }
+    const int yoff = FEM ? 1 : 0;
+    // tile index of (p, t = 0) and the step per t
+    const int tstep = d.tdir == 0 ? RX : 1;
+    // Both transforms are blocked kSepModes x kSepCols per thread: modes
+    // (rows) k0.., columns p, p + PG, ...; a value read from LDS serves
+    // kSepModes or kSepCols products.
+    const int KG = (Tt + kSepModes - 1) / kSepModes, PG = (P + kSepCols - 1) / kSepCols;
+    for (int o = tid; o < KG * PG; o += Tn) {
+        const int kg = o / PG, pg = o - kg * PG, k0 = kg * kSepModes;
+        int l[kSepCols];
+#pragma unroll
+        for (int j = 0; j < kSepCols; j++) {
+            const int p = min(pg + j * PG, P - 1);
+            l[j] = d.tdir == 0 ? (yoff + 1) * RX + p + 1 : (p + yoff + 1) * RX + 1;
+        }
+        double acc[kSepModes][kSepCols];
+#pragma unroll
+        for (int u = 0; u < kSepModes; u++)
+#pragma unroll
+            for (int j = 0; j < kSepCols; j++) acc[u][j] = 0;
+#pragma unroll 4
+        for (int t = 0; t < Tt; t++) {
+            double bv[kSepCols], vv[kSepModes];
+#pragma unroll
+            for (int j = 0; j < kSepCols; j++) {
+                const bool row1 = FEM && l[j] < 3 * RX;  // the corrected right-hand side of row 1
+                bv[j] = double(row1 ? z[l[j]] : b[l[j]]);
+                l[j] += tstep;
+            }
+#pragma unroll
+            for (int u = 0; u < kSepModes; u++) vv[u] = V[t * Tt + min(k0 + u, Tt - 1)];
+#pragma unroll
+            for (int u = 0; u < kSepModes; u++)
+#pragma unroll
+                for (int j = 0; j < kSepCols; j++) acc[u][j] = fma(vv[u], bv[j], acc[u][j]);
+        }
+#pragma unroll
+        for (int u = 0; u < kSepModes; u++)
+#pragma unroll
+            for (int j = 0; j < kSepCols; j++) {
+                const int p = pg + j * PG;
+                if (k0 + u < Tt && p < P) hat[(k0 + u) * PP + p] = acc[u][j];
+            }
+    }
+    __syncthreads();
+    // one Thomas solve per mode, y_p = r_p iv_p - cp_p y_(p-1) (cp_p =
+    // e iv_p), so that each step is one dependent multiply-add; the next four
+    // values are loaded while the current four are eliminated
+    for (int k = tid; k < Tt; k += Tn) {
+        double *h = hat + k * PP;
+        const double *v = iv + k * PP, *c = cp + k * PP;
+        double y = 0;
+        const int P4 = P & ~3;
+        double hn[4], wn[4], cn[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            hn[u] = h[min(u, P - 1)];
+            wn[u] = v[min(u, P - 1)];
+            cn[u] = c[min(u, P - 1)];
+        }
+        for (int p = 0; p < P4; p += 4) {
+            double rc[4], cc[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                rc[u] = hn[u] * wn[u];
+                cc[u] = cn[u];
+                hn[u] = h[min(p + 4 + u, P - 1)];
+                wn[u] = v[min(p + 4 + u, P - 1)];
+                cn[u] = c[min(p + 4 + u, P - 1)];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                y = fma(-cc[u], y, rc[u]);
+                h[p + u] = y;
+            }
+        }
+        for (int p = P4; p < P; p++) h[p] = y = fma(-c[p], y, h[p] * v[p]);
+        // back substitution x_p = y_p - cp_p x_(p+1) from p = P - 2 down, four at a time
+        int p = P - 2;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            hn[u] = h[max(p - u, 0)];
+            wn[u] = c[max(p - u, 0)];
+        }
+        for (; p >= 3; p -= 4) {
+            double hc[4], wc[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                hc[u] = hn[u];
+                wc[u] = wn[u];
+                hn[u] = h[max(p - 4 - u, 0)];
+                wn[u] = c[max(p - 4 - u, 0)];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                y = fma(-wc[u], y, hc[u]);
+                h[p - u] = y;
+            }
+        }
+        for (; p >= 0; p--) h[p] = y = fma(-c[p], y, h[p]);
+    }
+    __syncthreads();
+    // the back transform, blocked the same way (rows t0.., columns p, p + PG, ...)
+    for (int o = tid; o < KG * PG; o += Tn) {
+        const int tg = o / PG, pg = o - tg * PG, t0 = tg * kSepModes;
+        double acc[kSepModes][kSepCols];
+#pragma unroll
+        for (int u = 0; u < kSepModes; u++)
+#pragma unroll
+            for (int j = 0; j < kSepCols; j++) acc[u][j] = 0;
+#pragma unroll 4
+        for (int k = 0; k < Tt; k++) {
+            double hv[kSepCols], vv[kSepModes];
+#pragma unroll
+            for (int j = 0; j < kSepCols; j++) hv[j] = hat[k * PP + min(pg + j * PG, P - 1)];
+#pragma unroll
+            for (int u = 0; u < kSepModes; u++) vv[u] = V[min(t0 + u, Tt - 1) * Tt + k];
+#pragma unroll
+            for (int u = 0; u < kSepModes; u++)
+#pragma unroll
+                for (int j = 0; j < kSepCols; j++) acc[u][j] = fma(vv[u], hv[j], acc[u][j]);
+        }
+#pragma unroll
+        for (int u = 0; u < kSepModes; u++)
+#pragma unroll
+            for (int j = 0; j < kSepCols; j++) {
+                const int t = t0 + u, p = pg + j * PG;
+                if (t >= Tt || p >= P) continue;
+                const int l = d.tdir == 0 ? (t + yoff + 1) * RX + p + 1 : (p + yoff + 1) * RX + t + 1;
+                z[l] = T(acc[u][j]);
+            }
+    }
+    __syncthreads();
+}
 
-// count sweeps of coarse level k
+// ncs smoothing sweeps of a level that is not separable (coarsening stopped
+// early on odd sizes with a long shorter side).
 template <class T>
-__device__ __forceinline__ void coarse_sweeps(bool fem, const CoarseDesc &d, const CCOp &cop, const FemOp &fop, int k,
-                                              T *lds, int count) {
-    const LevelT<T> L = coarse_level<T>(d, k);
+__device__ __forceinline__ void coarse_sweeps(bool fem, const CoarseDesc &d, const CCOp &cop, const FemOp &fop,
+                                              T *z, const T *b, const T *itab, int count) {
+    const LevelT<T> L = coarse_level<T>(d);
     const Tile t = coarse_tile(L.m, L.n);
-    T *z = lds + d.off[k], *b = z + t.RX * t.RY;
-    const T *itab = lds + coarse_tab_off(d) + 9 * k;
     for (int c = 0; c < count; c++) {
         if (fem) {
             for (int q = 0; q < 4; q++) {
@@ -862,13 +1017,16 @@ __device__ __forceinline__ void coarse_body_t(const CoarseArgs<T> &a, T *lds, do
     const int tid = threadIdx.x + kCoarseBX * threadIdx.y, Tn = kCoarseThreads;
     const T *gb = reinterpret_cast<const T *>(d.gb);
     T *gz = reinterpret_cast<T *>(d.gz);
-    const int ld = d.nl - 1, Nd = d.m[ld] * d.n[ld];
-    // load the first level into the padded tiles; the first round of loads
-    // is issued before every level is cleared (the borders stay zero) and the
-    // reciprocal tables are written, so that its round trip overlaps them
+    const int m = d.m, N = m * d.n, RX = m + 2, NT = RX * (d.n + 2);
+    T *z = lds, *b = z + NT, *itab = b + NT;
+    double *ws = reinterpret_cast<double *>(reinterpret_cast<char *>(lds) + d.ws);
+    // the sweeps start from the incoming iterate on a top level, from zero
+    // below it; the exact solve needs no start
+    const bool load_z = !d.sep && d.top;
+    // load the level into the padded tiles; the first round of loads is
+    // issued before the tiles are cleared (the borders stay zero) and the
+    // factors are staged, so that its round trip overlaps them
     {
-        const int m = d.m[0], N = m * d.n[0], RX = m + 2;
-        T *z = lds + d.off[0], *b = z + RX * (d.n[0] + 2);
         Walk w(tid, m, Tn);
         T zv[kLoadUnroll], bv[kLoadUnroll];
         int li[kLoadUnroll];
@@ -878,7 +1036,7 @@ __device__ __forceinline__ void coarse_body_t(const CoarseArgs<T> &a, T *lds, do
                 const int k = min(w.q, N - 1);
                 li[u] = w.q < N ? (w.P + 1) * RX + w.r + 1 : -1;
                 bv[u] = gb[k];
-                zv[u] = d.top ? gz[k] : T(0);
+                zv[u] = load_z ? gz[k] : T(0);
                 w.next(Tn);
             }
         };
@@ -890,111 +1048,50 @@ __device__ __forceinline__ void coarse_body_t(const CoarseArgs<T> &a, T *lds, do
                     z[li[u]] = zv[u];
                 }
         };
+        // V, the Thomas factors and the off-diagonals, in their LDS layout:
+        // kSepLoads values per thread and round, issued with the level's
+        const int nf = d.sep ? d.T * d.T + 2 * d.T * (d.P + 1) + d.T : 0;
+        double *dst = ws + d.T * (d.P + 1);
+        double fv[kSepLoads];
+        int fbase = tid;
+        auto fac_round = [&]() {
+#pragma unroll
+            for (int u = 0; u < kSepLoads; u++) fv[u] = d.fac[min(fbase + u * Tn, nf - 1)];
+        };
+        auto fac_store = [&]() {
+#pragma unroll
+            for (int u = 0; u < kSepLoads; u++)
+                if (fbase + u * Tn < nf) dst[fbase + u * Tn] = fv[u];
+            fbase += kSepLoads * Tn;
+        };
         load_round();
-        for (int k = tid; k < d.red_off; k += Tn) lds[k] = T(0);
-        if (!FEM && tid < 9 * d.nl)
-            cc_inv_table<T>(coarse_level<T>(d, tid / 9), a.cop, lds + coarse_tab_off(d) + 9 * (tid / 9), tid % 9);
+        if (fbase < nf) fac_round();
+        for (int k = tid; k < 2 * NT; k += Tn) lds[k] = T(0);
+        if (!d.sep && !FEM && tid < 9) cc_inv_table<T>(coarse_level<T>(d), a.cop, itab, tid);
         __syncthreads();
         store_round();
+        if (fbase < nf) fac_store();
+        while (fbase < nf) {
+            fac_round();
+            fac_store();
+        }
         while (w.q < N) {
             load_round();
             store_round();
         }
     }
     __syncthreads();
-    for (int l = 0; l + 1 < d.nl; l++) {
-        coarse_sweeps<T>(FEM, d, a.cop, a.fop, l, lds, d.top && l == 0 ? a.nu1 : a.nu1s);
-        const LevelT<T> F = coarse_level<T>(d, l);
-        const Tile t = coarse_tile(F.m, F.n);
-        const int Mc = d.m[l + 1], Nc = d.n[l + 1], RC = Mc + 2;
-        const T *z = lds + d.off[l], *b = z + t.RX * t.RY;
-        T *bc = lds + d.off[l + 1] + RC * (Nc + 2);
-        for (int J = threadIdx.y; J < Nc; J += kCoarseBY)
-            for (int I = threadIdx.x; I < Mc; I += kCoarseBX)
-                bc[(J + 1) * RC + I + 1] = restrict_at<T, FEM>(F, a.cop, a.fop, z, b, t, I, J);
-        __syncthreads();
-    }
-    {
-        const int Md = d.m[ld], RD = Md + 2;
-        T *z = lds + d.off[ld];
-        const T *b = z + RD * (d.n[ld] + 2);
-        if (d.inv_dense && Nd <= kDenseReg) {
-            // exact solve: thread t takes column block p = t / Nd of row
-            // i = t % Nd and loads its entries of the inverse kDenseRegPer at
-            // a time, all in flight together (unused slots load a valid entry
-            // and are zeroed); the right-hand side is gathered into a plain
-            // vector meanwhile; then the blocks' partial products are summed
-            // in order by row
-            const int dP = Tn / Nd, K = (Nd + dP - 1) / dP, di = tid % Nd, dp = tid / Nd;
-            const int dk0 = min(Nd - 1, dp * K), dk1 = dp < dP ? min(Nd, dp * K + K) : dk0;
-            const T *inv = reinterpret_cast<const T *>(d.inv_dense);
-            T *red = lds + d.red_off, *bp = red + kCoarseThreads;
-            T v = 0;
-            for (int k0 = dk0; k0 < dk0 + K; k0 += kDenseRegPer) {
-                T pre[kDenseRegPer];
-#pragma unroll
-                for (int u = 0; u < kDenseRegPer; u++) {
-                    const T e = inv[(size_t)min(k0 + u, Nd - 1) * Nd + di];
-                    pre[u] = k0 + u < dk1 ? e : T(0);
-                }
-                if (k0 == dk0) {
-                    if (tid < Nd) bp[tid] = b[(tid / Md + 1) * RD + tid % Md + 1];
-                    __syncthreads();
-                }
-#pragma unroll
-                for (int u = 0; u < kDenseRegPer; u++) v += pre[u] * bp[min(k0 + u, Nd - 1)];
-            }
-            if (dp < dP) red[dp * Nd + di] = v;
-            __syncthreads();
-            if (tid < Nd) {
-                T v = 0;
-                for (int p = 0; p < dP; p++) v += red[p * Nd + tid];
-                z[(tid / Md + 1) * RD + tid % Md + 1] = v;
-            }
-            __syncthreads();
-        } else if (d.inv_dense) {
-            // exact solve with the inverse read from memory (stored column by
-            // column, coalesced over the rows)
-            const T *inv = reinterpret_cast<const T *>(d.inv_dense);
-            for (int i = tid; i < Nd; i += Tn) {
-                T v = 0;
-                for (int k = 0; k < Nd; k++) v += inv[(size_t)k * Nd + i] * b[(k / Md + 1) * RD + k % Md + 1];
-                z[(i / Md + 1) * RD + i % Md + 1] = v;
-            }
-            __syncthreads();
-        } else {
-            coarse_sweeps<T>(FEM, d, a.cop, a.fop, ld, lds, a.ncs);
-        }
-    }
-    for (int l = d.nl - 2; l >= 0; l--) {
-        const int m = d.m[l], n = d.n[l], RX = m + 2, Mc = d.m[l + 1], Nc = d.n[l + 1];
-        T *z = lds + d.off[l];
-        // coarse correction from element (0, 0) of the padded coarse level
-        const T *ec = lds + d.off[l + 1] + (Mc + 2) + 1;
-        for (int j = threadIdx.y; j < n; j += kCoarseBY)
-            for (int i = threadIdx.x; i < m; i += kCoarseBX) {
-                const int k = (j + 1) * RX + i + 1;
-                if (FEM) {
-                    const T e = fem_prolong_all<T>(ec, Mc, Nc, i, j, Mc + 2);
-                    if (j > 0) z[k] += e;
-                } else {
-                    z[k] += cc_prolong_at<T>(a.cop, Mc, Nc, ec, i, j, Mc + 2);
-                }
-            }
-        __syncthreads();
-        coarse_sweeps<T>(FEM, d, a.cop, a.fop, l, lds, d.top && l == 0 ? a.nu2 : a.nu2s);
-    }
-    const LevelT<T> F = coarse_level<T>(d, 0);
-    const Tile t = coarse_tile(F.m, F.n);
-    const T *z = lds + d.off[0], *b = z + t.RX * t.RY;
+    if (d.sep) coarse_exact<T, FEM>(a, z, b, ws);
+    else coarse_sweeps<T>(FEM, d, a.cop, a.fop, z, b, itab, a.ncs);
+    const LevelT<T> F = coarse_level<T>(d);
     double rr = 0;
     for (int j = threadIdx.y; j < F.n; j += kCoarseBY)
         for (int i = threadIdx.x; i < F.m; i += kCoarseBX) {
-            const int l0 = (j + 1) * t.RX + i + 1;
+            const int l0 = (j + 1) * RX + i + 1;
             gz[i + (size_t)F.m * j] = z[l0];
             if (d.top) {
-                const double r = FEM ? fem_resid_t<T>(a.fop, F.m, F.n, z, b, t.RX, l0, i, j)
-                                     : cc_resid_t<T>(F, a.cop, z, b, t.RX, l0, i, j);
+                const double r = FEM ? fem_resid_t<T>(a.fop, F.m, F.n, z, b, RX, l0, i, j)
+                                     : cc_resid_t<T>(F, a.cop, z, b, RX, l0, i, j);
                 rr += r * r;
             }
         }
@@ -1254,7 +1351,7 @@ int set_attributes() {
     VDI_HIP(hipGetDevice(&dev));
     VDI_HIP(hipDeviceGetAttribute(&g_max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     VDI_HIP(hipDeviceGetAttribute(&g_cus, hipDeviceAttributeMultiprocessorCount, dev));
-    if (g_max_lds < 2 * (int)sizeof(double) * kCoarsePoints * 2) return VDI_ERR_ARG;
+    if (g_max_lds < 65536) return VDI_ERR_ARG;
     const void *ks[] = {
         (const void *)k_down<double, false>, (const void *)k_down<double, true>, (const void *)k_down<float, false>,
         (const void *)k_down<float, true>, (const void *)k_up<double, false>, (const void *)k_up<double, true>,
@@ -1514,15 +1611,13 @@ int MG::init(int kind_, int m, int n, double cx, double cy, int x_sym, double *z
         lm.push_back(fem ? mc + 1 : mc);
         ln.push_back(fem ? nc + 1 : nc);
         if ((int)lm.size() >= 24) break;
-        if ((long)lm.back() * ln.back() <= kDensePoints) break;
+        if ((long)lm.back() * ln.back() <= kExactPoints) break;
         if (mc % 2 || nc % 2 || mc / 2 < 2 || nc / 2 < 2) break;
         mc /= 2;
         nc /= 2;
     }
     nl = (int)lm.size();
     lc = nl - 1;
-    while (lc > 0 && (long)lm[lc - 1] * ln[lc - 1] <= coarse_pts) lc--;
-    if (nl - lc > kMaxCoarse) return VDI_ERR_ARG;
     size_t coarse = 0;
     for (int l = 1; l < nl; l++) coarse += (size_t)lm[l] * ln[l];
     const long N0 = (long)m * n;
@@ -1564,10 +1659,24 @@ int MG::init(int kind_, int m, int n, double cx, double cy, int x_sym, double *z
         }
     }
     {
-        const long Nd = (long)lm[nl - 1] * ln[nl - 1];
-        inv_.reset();
-        inv_n_ = Nd <= kDenseMax ? Nd : 0;
-        if (arena && Nd > kDensePoints) return VDI_ERR_ARG;
+        // the coarsest level: solved exactly when its shorter side (without
+        // the nodal Dirichlet row) has at most kSepMaxT unknowns
+        const int Mc = lm[lc], Nc = ln[lc], rows = fem ? Nc - 1 : Nc;
+        fac_.reset();
+        sep_tdir_ = rows <= Mc ? 0 : 1;
+        sep_T_ = sep_tdir_ == 0 ? rows : Mc;
+        sep_P_ = sep_tdir_ == 0 ? Mc : rows;
+        sep_t0_ = fem ? 1 : 0;
+        if (sep_T_ > kSepMaxT || rows < 1) sep_T_ = sep_P_ = 0;
+        // LDS: z and b tiles and the reciprocal table, then the double workspace
+        for (int pr = 0; pr < 2; pr++) {
+            const size_t elem = pr ? sizeof(float) : sizeof(double);
+            size_t bytes = (2 * (size_t)(Mc + 2) * (Nc + 2) + kInvTab) * elem;
+            bytes = (bytes + 7) & ~(size_t)7;
+            if (sep_T_ > 0) bytes += (sep::factor_doubles(sep_T_, sep_P_) + (size_t)sep_T_ * (sep_P_ + 1)) * sizeof(double);
+            coarse_lds_[pr] = bytes;
+        }
+        if (coarse_lds_[0] > (size_t)mgk::max_lds()) return VDI_ERR_ARG;
     }
     VDI_TRY(set_coeffs(cx, cy));
     for (int pr = 0; pr < 2; pr++)
@@ -1583,20 +1692,6 @@ int MG::init(int kind_, int m, int n, double cx, double cy, int x_sym, double *z
     VDI_TRY(state_.alloc(sizeof(MGState), arena));
     if (!host_state_ && !arena)
         VDI_HIP(hipHostMalloc((void **)&host_state_, sizeof(MGState), hipHostMallocDefault));
-    coarse_lds_ = (kCoarseThreads + kDenseReg + 9 * kMaxCoarse) * sizeof(double);
-    for (int l = lc; l < nl; l++) coarse_lds_ += 2 * sizeof(double) * (size_t)(lm[l] + 2) * (ln[l] + 2);
-    return VDI_OK;
-}
-
-int MG::set_coarse_points(int pts) {
-    if (pts < 1 || pts > kCoarsePoints) return VDI_ERR_ARG;
-    int l2 = nl - 1;
-    while (l2 > 0 && (long)lm[l2 - 1] * ln[l2 - 1] <= pts) l2--;
-    if (nl - l2 > kMaxCoarse) return VDI_ERR_ARG;
-    coarse_pts = pts;
-    lc = l2;
-    coarse_lds_ = (kCoarseThreads + kDenseReg + 9 * kMaxCoarse) * sizeof(double);
-    for (int l = lc; l < nl; l++) coarse_lds_ += 2 * sizeof(double) * (size_t)(lm[l] + 2) * (ln[l] + 2);
     return VDI_OK;
 }
 
@@ -1623,63 +1718,64 @@ void MG::level_coeffs(double cx, double cy) {
 
 int MG::set_coeffs(double cx, double cy) {
     level_coeffs(cx, cy);
-    return dense_inverse();
+    return coarse_factors();
 }
 
-void MG::adopt_coeffs(double cx, double cy, const std::shared_ptr<CoarseInverse> &inv) {
+void MG::adopt_coeffs(double cx, double cy, const std::shared_ptr<CoarseFactors> &fac) {
     level_coeffs(cx, cy);
     if (kind == VDI_VX || kind == VDI_VY) {
         vfx = cx;
         vfy = cy;
     }
-    inv_ = inv;
+    fac_ = fac;
 }
 
-// Dense inverse of the coarsest level: columns A e_k from the level operator
-// applied to unit vectors, Gauss-Jordan elimination with partial pivoting on
-// the host, stored column-major (inv[k N + i] = (A^-1)_ik) in FP64 and FP32.
-int MG::dense_inverse() {
-    if (inv_n_ == 0) return VDI_OK;
-    if (!inv_ || inv_.use_count() > 1) {
+// Factors of the coarsest level's exact solve (coarse_sep.hpp), from the
+// level's couplings and boundary types.  Checked once on the host: the solve
+// must return a known solution of the level operator (Az_plain) to 1e-9.
+int MG::coarse_factors() {
+    if (sep_T_ == 0) return VDI_OK;
+    const int l = lc, M = lm[l], Nn = ln[l];
+    sep::Tri Pt, Rt, Sp, Qp;
+    if (fem) {
+        if (!sep::fem_pencil(fe.fm, fe.fex, fe.fey, fe.fc, M, Nn, sep_tdir_, Pt, Rt, Sp, Qp)) return VDI_ERR_ARG;
+    } else {
+        sep::cc_pencil(cc.sigma, cc.s, cc.bcl, cc.bcr, cc.bcb, cc.bct, M, Nn, L[l].cx, L[l].cy, sep_tdir_, Pt, Rt, Sp,
+                       Qp);
+    }
+    std::vector<double> f;
+    if (!sep::factor(Pt, Rt, Sp, Qp, f)) return VDI_ERR_ARG;
+    {
+        const int T = sep_T_, P = sep_P_, y0 = sep_t0_;
+        std::vector<double> zt((size_t)M * Nn, 0.0), bfull((size_t)M * Nn), bs((size_t)T * P), zs;
+        for (int j = y0; j < Nn; j++)
+            for (int i = 0; i < M; i++) zt[i + (size_t)M * j] = std::cos(0.37 * i + 1.3 * j) + 0.01 * ((i * 7 + j * 3) % 11);
+        for (int j = 0; j < Nn; j++)
+            for (int i = 0; i < M; i++)
+                bfull[i + (size_t)M * j] = Az_plain<double>(fem, cc, fe, M, Nn, L[l].cx, L[l].cy, zt.data(), i, j);
+        for (int t = 0; t < T; t++)
+            for (int p = 0; p < P; p++) {
+                const int i = sep_tdir_ == 0 ? p : t, j = (sep_tdir_ == 0 ? t : p) + y0;
+                bs[p + (size_t)P * t] = bfull[i + (size_t)M * j];
+            }
+        sep::solve(T, P, f, bs, zs);
+        double e2 = 0, n2 = 0;
+        for (int t = 0; t < T; t++)
+            for (int p = 0; p < P; p++) {
+                const int i = sep_tdir_ == 0 ? p : t, j = (sep_tdir_ == 0 ? t : p) + y0;
+                const double d = zs[p + (size_t)P * t] - zt[i + (size_t)M * j];
+                e2 += d * d;
+                n2 += zt[i + (size_t)M * j] * zt[i + (size_t)M * j];
+            }
+        if (!(e2 <= 1e-18 * n2)) return VDI_ERR_ARG;
+    }
+    if (!fac_ || fac_.use_count() > 1) {
         // a captured step still holds the current buffer: write a new one
-        auto fresh = std::make_shared<CoarseInverse>();
-        VDI_TRY(fresh->d64.alloc((size_t)inv_n_ * inv_n_ * sizeof(double), arena_));
-        VDI_TRY(fresh->d32.alloc((size_t)inv_n_ * inv_n_ * sizeof(float), arena_));
-        inv_ = fresh;
+        auto fresh = std::make_shared<CoarseFactors>();
+        VDI_TRY(fresh->f.alloc(f.size() * sizeof(double), arena_));
+        fac_ = fresh;
     }
-    const int l = nl - 1, M = lm[l], Nn = ln[l], N = M * Nn, W = 2 * N;
-    std::vector<double> a((size_t)N * W, 0.0), e(N, 0.0);
-    for (int k = 0; k < N; k++) {
-        e[k] = 1;
-        for (int q = 0; q < N; q++)
-            a[(size_t)q * W + k] = Az_plain<double>(fem, cc, fe, M, Nn, L[l].cx, L[l].cy, e.data(), q % M, q / M);
-        e[k] = 0;
-        a[(size_t)k * W + N + k] = 1;
-    }
-    for (int c = 0; c < N; c++) {
-        int piv = c;
-        for (int i = c + 1; i < N; i++)
-            if (std::fabs(a[(size_t)i * W + c]) > std::fabs(a[(size_t)piv * W + c])) piv = i;
-        if (piv != c)
-            for (int k = 0; k < W; k++) std::swap(a[(size_t)c * W + k], a[(size_t)piv * W + k]);
-        const double d = 1.0 / a[(size_t)c * W + c];
-        for (int k = 0; k < W; k++) a[(size_t)c * W + k] *= d;
-        for (int i = 0; i < N; i++) {
-            if (i == c) continue;
-            const double f = a[(size_t)i * W + c];
-            if (f == 0) continue;
-            for (int k = 0; k < W; k++) a[(size_t)i * W + k] -= f * a[(size_t)c * W + k];
-        }
-    }
-    std::vector<double> t((size_t)N * N);
-    std::vector<float> tf((size_t)N * N);
-    for (int i = 0; i < N; i++)
-        for (int k = 0; k < N; k++) {
-            t[(size_t)k * N + i] = a[(size_t)i * W + N + k];
-            tf[(size_t)k * N + i] = (float)t[(size_t)k * N + i];
-        }
-    VDI_HIP(hipMemcpy(inv_->d64.p, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
-    VDI_HIP(hipMemcpy(inv_->d32.p, tf.data(), tf.size() * sizeof(float), hipMemcpyHostToDevice));
+    VDI_HIP(hipMemcpy(fac_->f.p, f.data(), f.size() * sizeof(double), hipMemcpyHostToDevice));
     return VDI_OK;
 }
 
@@ -1690,11 +1786,6 @@ MGCtl MG::ctl() const {
     return c;
 }
 
-int MG::coarse_points() const {
-    int s = 0;
-    for (int l = lc; l < nl; l++) s += lm[l] * ln[l];
-    return s;
-}
 
 size_t MG::down_lds(int prec, int l) const {
     const TileShape &t = kTileShapes[tile[prec][0][l]];
@@ -1761,23 +1852,23 @@ template <class T>
 CoarseArgs<T> MG::coarse_args(const std::vector<LevelT<T>> &H, double tol2) {
     CoarseArgs<T> a{};
     CoarseDesc &d = a.d;
-    d.nl = nl - lc;
-    int off = 0;
-    for (int k = 0; k < d.nl; k++) {
-        const LevelT<T> &v = H[lc + k];
-        d.m[k] = v.m;
-        d.n[k] = v.n;
-        d.cx[k] = (double)v.cx;
-        d.cy[k] = (double)v.cy;
-        d.inv[k] = (double)v.inv_int;
-        d.off[k] = off;
-        off += 2 * (v.m + 2) * (v.n + 2);
-    }
-    d.red_off = off;
-    d.gz = reinterpret_cast<double *>(H[lc].z);
-    d.gb = reinterpret_cast<double *>(H[lc].b);
+    const LevelT<T> &v = H[lc];
+    d.m = v.m;
+    d.n = v.n;
+    d.cx = (double)v.cx;
+    d.cy = (double)v.cy;
+    d.inv = (double)v.inv_int;
+    d.gz = reinterpret_cast<double *>(v.z);
+    d.gb = reinterpret_cast<double *>(v.b);
     d.top = lc == 0;
-    d.inv_dense = !inv_ ? nullptr : sizeof(T) == 8 ? inv_->d64.p : inv_->d32.p;
+    d.sep = fac_ ? 1 : 0;
+    d.tdir = sep_tdir_;
+    d.T = sep_T_;
+    d.P = sep_P_;
+    d.t0 = sep_t0_;
+    const size_t tiles = (2 * (size_t)(v.m + 2) * (v.n + 2) + kInvTab) * sizeof(T);
+    d.ws = (int)((tiles + 7) & ~(size_t)7);
+    d.fac = fac_ ? fac_->f.as<double>() : nullptr;
     a.cop = cc;
     a.fop = fe;
     a.fem = fem;
@@ -1878,9 +1969,12 @@ double MG::up_bytes(int prec, int l) const {
     const double e = prec ? 4.0 : 8.0;
     return e * (3.0 * lm[l] * ln[l] + 1.0 * lm[l + 1] * ln[l + 1]);
 }
+// reads b (and z on a top level without the exact solve), writes z; the
+// exact solve also reads its FP64 factors
 double MG::coarse_bytes(int prec) const {
     const double e = prec ? 4.0 : 8.0;
-    return e * (lc == 0 ? 3.0 : 2.0) * lm[lc] * ln[lc];
+    const double f = sep_T_ > 0 ? 8.0 * (double)sep::factor_doubles(sep_T_, sep_P_) : 0.0;
+    return e * (lc == 0 && sep_T_ == 0 ? 3.0 : 2.0) * lm[lc] * ln[lc] + f;
 }
 // reads e (zalt), r32, coarse e, z64, b64; writes z64, r32
 double MG::mx_bytes() const {
@@ -2263,8 +2357,6 @@ int MG::autotune(int reps) {
     KernelTimer *tsave = timer;
     timer = nullptr;
     int rc = VDI_OK;
-    // tile every level that any hand-over candidate below leaves tiled
-    (void)set_coarse_points(128);
     for (int prec = 0; prec < 2 && rc == VDI_OK; prec++) {
         for (int which = 0; which < 3 && rc == VDI_OK; which++) {
             for (int l = 0; l < lc && rc == VDI_OK; l++) {
@@ -2293,23 +2385,6 @@ int MG::autotune(int reps) {
                 if (best >= 0) (void)set_tile(prec, which, l, best);
             }
         }
-    }
-    // Where the single-workgroup coarse solve takes over: whole V-cycles
-    // (tol2 < 0: never converge) for each candidate size.
-    if (rc == VDI_OK) {
-        const int cands[] = {kCoarsePoints, 1024, 512, 128};
-        int best = coarse_pts;
-        float best_ms = -1;
-        for (int pts : cands) {
-            if (set_coarse_points(pts) != VDI_OK) continue;
-            hipLaunchKernelGGL(k_reset_state, dim3(1), dim3(1), 0, st, state_.as<MGState>());
-            float ms = 0;
-            auto launch = [&]() -> int { return launch_cycle(-1.0); };
-            rc = time_launches(st, reps, launch, &ms);
-            if (rc != VDI_OK) break;
-            if (best_ms < 0 || ms < best_ms) { best = pts; best_ms = ms; }
-        }
-        (void)set_coarse_points(best);
     }
     timer = tsave;
     VDI_HIP(hipMemcpyAsync(L[0].z, save.p, N0 * sizeof(double), hipMemcpyDeviceToDevice, st));

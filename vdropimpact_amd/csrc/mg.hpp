@@ -11,7 +11,11 @@
 //   nodal FEM: four-colour Gauss-Seidel, full-weighting restriction (the
 //     transpose of bilinear prolongation) so that the coarse Q1 stiffness is
 //     the Galerkin one, Dirichlet row kept at zero correction;
-//   coarsest level: a fixed number of smoothing sweeps.
+//   coarsest level (the first with at most kExactPoints unknowns): solved
+//     exactly by separation of variables -- every operator here is a sum of
+//     two Kronecker products of 1-D tridiagonal matrices, so a generalized
+//     eigenbasis of the shorter direction turns it into independent
+//     tridiagonal systems along the longer one (see MG::coarse_factors).
 //
 // Every kernel takes one argument record.  A direct launch passes it by
 // value; a batched launch passes an array of records (one per simulation)
@@ -25,14 +29,15 @@
 
 namespace vdi {
 
-// Dense inverse of a coarsest level (FP64 and FP32 copies).  Shared: a
-// captured step keeps the inverse its coarse kernels were recorded with, and
-// a coefficient change while one is held writes a fresh buffer instead.
-struct CoarseInverse {
-    DevBuf d64, d32;
+// Factors of the coarsest level's exact solve (FP64): the eigenbasis V of the
+// transform direction and the Thomas factors of every mode's tridiagonal
+// system.  Shared: a captured step keeps the factors its coarse kernels were
+// recorded with, and a coefficient change while they are held writes a fresh
+// buffer instead.
+struct CoarseFactors {
+    DevBuf f;
 };
 
-constexpr int kMaxCoarse = 12;  // levels handled by the single-workgroup kernel
 constexpr int kMaxNu = 4;       // largest nu1/nu2 the tiled kernels accept
 // Level-kernel workgroups: 32 x 16 = 512 threads (eight waves).  With the
 // 64 x 32 tiles of the finest level there is about one tile per CU, and 512
@@ -43,19 +48,19 @@ constexpr int kBX = 32, kBY = 16;
 constexpr int kLvThreads = kBX * kBY;
 constexpr int kCoarseBX = 32, kCoarseBY = 16;  // coarse-kernel threads, 2-D
 constexpr int kCoarseThreads = kCoarseBX * kCoarseBY;
-constexpr int kCoarsePoints = 3072;   // largest level the single-workgroup kernel takes
-// Coarsening stops at the first level with at most kDensePoints unknowns,
-// which is solved exactly with a dense inverse when it has at most kDenseMax
-// unknowns (coarsening can also stop earlier, on odd sizes; then the coarsest
-// level gets ncs smoothing sweeps instead).
-constexpr int kDensePoints = 256;
-constexpr int kDenseMax = 1024;
-// The exact solve of a coarsest level of at most kDenseReg unknowns splits
-// the rows into column blocks, one block per thread, and loads a block's
-// entries kDenseRegPer at a time, all in flight together; a larger level
-// reads the inverse entry by entry.
-constexpr int kDenseReg = 256;
-constexpr int kDenseRegPer = 32;
+// Coarsening stops at the first level with at most kExactPoints unknowns
+// (or earlier, on odd sizes); the single-workgroup kernel solves that level,
+// exactly when its shorter direction has at most kSepMaxT unknowns, else by
+// ncs smoothing sweeps.
+constexpr int kExactPoints = 3072;
+constexpr int kSepMaxT = 64;
+// The exact solve's transforms take a block of kSepModes modes (rows) by
+// kSepCols columns per thread; its factors are staged kSepLoads values per
+// thread and round (one round for the 64 x 32 levels of the 1024 x 512
+// workload).
+constexpr int kSepModes = 2;
+constexpr int kSepCols = 2;
+constexpr int kSepLoads = 12;
 
 // Device-side solve state: convergence flag (0 running, 1 converged,
 // 2 NaN, 3 zero right-hand side), V-cycles run, ||b-Az||^2 and ||b||^2.
@@ -117,17 +122,21 @@ struct MxArgs {
     Stamp *ts;
 };
 
-// Levels lc..L-1 for the single-workgroup kernel (LDS offsets in elements;
-// z at off, b at off + m*n).
+// The coarsest level for the single-workgroup kernel.  LDS holds the level
+// as two (m + 2) x (n + 2) tiles with a zero border (z, then b, in the
+// kernel's precision), then a double workspace at byte offset ws: the
+// transformed right-hand side (T rows of P + 1), V (T x T), the two Thomas
+// factors (T rows of P + 1 each) and the T off-diagonals.
 struct CoarseDesc {
-    int nl, top;
-    int m[kMaxCoarse], n[kMaxCoarse], off[kMaxCoarse];
-    int red_off;  // LDS after the levels: kCoarseThreads partial products, then kDenseReg values
-    double cx[kMaxCoarse], cy[kMaxCoarse], inv[kMaxCoarse];
+    int top, m, n;
+    double cx, cy, inv;
     double *gz, *gb;
-    const void *inv_dense;  // column-major inverse of the coarsest level, or null
+    int sep;       // 1: exact separable solve; 0: ncs smoothing sweeps
+    int tdir;      // transform direction: 0 along y (rows), 1 along x (columns)
+    int T, P, t0;  // transform unknowns, tridiagonal length, first transformed row/column
+    int ws;        // byte offset of the double workspace
+    const double *fac;  // V ([t][k], T x T), inv and cp ([k][p], T x P each), off-diagonals (T)
 };
-
 template <class T>
 struct CoarseArgs {
     CoarseDesc d;
@@ -196,7 +205,7 @@ class MG {
     int kind = VDI_MAC;
     bool fem = false;
     int nl = 0;   // levels
-    int lc = 0;   // first level of the single-workgroup kernel
+    int lc = 0;   // the coarsest level (nl - 1), solved by the single-workgroup kernel
     std::vector<Level> L;    // FP64 hierarchy
     std::vector<LevelF> LF;  // FP32 hierarchy of the mixed form (level 0: e32, r32)
     std::vector<int> lm, ln;
@@ -204,7 +213,6 @@ class MG {
     FemOp fe{};
     int nu1 = 2, nu2 = 2, ncs = 8;
     int nu1s = 2, nu2s = 2;  // V(nu1s, nu2s) on levels 1 .. nl - 2
-    int coarse_pts = kCoarsePoints;  // levels this small go to the single-workgroup kernel
     int form = VDI_FORM_FP64;
     hipStream_t st = nullptr;
     // tile shape index (into kTileShapes) per level: [precision][down/up][level]
@@ -224,15 +232,15 @@ class MG {
     // (owned by the caller); when null the MG allocates its own.  With an
     // arena every device buffer is a piece of it and nothing is allocated
     // (FP64 form only: no FP32 hierarchy, no time-extrapolation history, no
-    // pinned readback; coarsest dense level of at most kDensePoints).
+    // pinned readback).
     int init(int kind, int m, int n, double cx, double cy, int x_sym, double *z0, double *b0,
              hipStream_t stream, Arena *arena = nullptr);
-    // Sets the level couplings and recomputes the dense inverse; returns its status.
+    // Sets the level couplings and recomputes the coarse factors; returns their status.
     int set_coeffs(double cx, double cy);
-    // The coarse inverse in use, and re-adopting one recorded earlier for the
+    // The coarse factors in use, and re-adopting ones recorded earlier for the
     // couplings (cx, cy) (no recomputation; the level couplings follow).
-    std::shared_ptr<CoarseInverse> coarse_inverse() const { return inv_; }
-    void adopt_coeffs(double cx, double cy, const std::shared_ptr<CoarseInverse> &inv);
+    std::shared_ptr<CoarseFactors> coarse_factors_in_use() const { return fac_; }
+    void adopt_coeffs(double cx, double cy, const std::shared_ptr<CoarseFactors> &fac);
     int unknowns() const { return lm[0] * ln[0]; }
     double *z() const { return L[0].z; }
     double *b() const { return L[0].b; }
@@ -313,13 +321,10 @@ class MG {
     size_t down_lds(int prec, int l) const;
     size_t up_lds(int prec, int l) const;
     size_t mx_lds() const;
-    size_t coarse_lds(int prec) const { return coarse_lds_ / sizeof(double) * (prec ? sizeof(float) : sizeof(double)); }
-    int coarse_points() const;
-    // Moves the hand-over to the single-workgroup kernel (pts <= kCoarsePoints);
-    // the cycle computes the same values either way.
-    int set_coarse_points(int pts);
-    // Recomputes the coarsest level's dense inverse (after a coefficient change).
-    int dense_inverse();
+    size_t coarse_lds(int prec) const { return coarse_lds_[prec ? 1 : 0]; }
+    // Recomputes the factors of the coarsest level's exact solve (after a
+    // coefficient change).
+    int coarse_factors();
     // Zeroes the solve state and the partial sums (fill kernels on st).
     int clear_solve_state();
     // A fresh solver for a new simulation on the same grid and couplings:
@@ -342,11 +347,11 @@ class MG {
     DevBuf state_;
     DevBuf own_z_, own_b_;
     Arena *arena_ = nullptr;              // set: buffers come from it (see init)
-    std::shared_ptr<CoarseInverse> inv_;  // dense inverse of the coarsest level (null: none)
-    long inv_n_ = 0;                       // its unknowns
+    std::shared_ptr<CoarseFactors> fac_;  // exact solve of the coarsest level (null: sweeps)
+    int sep_tdir_ = 0, sep_T_ = 0, sep_P_ = 0, sep_t0_ = 0;
     void level_coeffs(double cx, double cy);
     MGState *host_state_ = nullptr;  // pinned
-    size_t coarse_lds_ = 0;
+    size_t coarse_lds_[2] = {0, 0};
 
   public:
     ~MG();

@@ -5,10 +5,10 @@
 // capture, tight serving loops, harnesses that forbid allocation).  Every work
 // buffer the solve needs beyond b and z (coarse-level iterates and right-hand
 // sides, the pre-smoothed level-0 iterate, per-workgroup partial sums, the
-// solve state and the coarsest level's dense inverse) is carved out of one
-// file-scope device array, `scratch`, of kScratchBytes.  Each GPU holds its
-// own copy of that array for the life of the process, and every call on a GPU
-// shares it: calls that overlap on one GPU race.
+// solve state and the factors of the coarsest level's exact solve) is carved
+// out of one file-scope device array, `scratch`, of kScratchBytes.  Each GPU
+// holds its own copy of that array for the life of the process, and every
+// call on a GPU shares it: calls that overlap on one GPU race.
 #include <mutex>
 
 #include "mg.hpp"
@@ -57,9 +57,9 @@ extern "C" int vdi_mg_solve_noalloc(int kind, int m, int n, double cx, double cy
     arena.cap = kScratchBytes;
     const hipStream_t st = nullptr;
     MG g;
-    // The hierarchy is laid out in the arena (init also writes the coarsest
-    // level's dense inverse there); a size that does not fit, or whose
-    // coarsest level is not a dense-solve level, is rejected here.
+    // The hierarchy is laid out in the arena (init also writes the factors of
+    // the coarsest level's exact solve there); a size that does not fit is
+    // rejected here.
     VDI_TRY(g.init(kind, m, n, cx, cy, x_sym, z_dev, const_cast<double *>(b_dev), st, &arena));
     // The array keeps the previous call's contents: the solve state and the
     // partial sums start from zero (fill kernels, no memset).  Every other

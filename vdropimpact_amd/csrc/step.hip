@@ -1486,8 +1486,8 @@ static int capture_step(vdi_ctx *c, double dt, vdi_ctx::GraphEntry &e) {
         VDI_TRY(visco_setup(c->vy, a.vfx, a.vfy));
         e.vfx = c->vx.vfx;
         e.vfy = c->vx.vfy;
-        e.vinv[0] = c->vx.coarse_inverse();
-        e.vinv[1] = c->vy.coarse_inverse();
+        e.vinv[0] = c->vx.coarse_factors_in_use();
+        e.vinv[1] = c->vy.coarse_factors_in_use();
     }
     VDI_HIP(hipStreamBeginCapture(c->st, hipStreamCaptureModeRelaxed));
     int rc = VDI_OK;

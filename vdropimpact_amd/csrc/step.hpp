@@ -156,7 +156,7 @@ struct vdi_ctx {
         // couplings and coarse inverses of the two viscous solves the graph's
         // kernels were recorded with (re-adopted before a replay)
         double vfx, vfy;
-        std::shared_ptr<vdi::CoarseInverse> vinv[2];
+        std::shared_ptr<vdi::CoarseFactors> vinv[2];
         long used;  // replay clock of the last launch (least recently used is evicted)
     };
     // Cached graphs (distinct budget tuples, time steps and forms); the least
