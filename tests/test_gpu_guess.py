@@ -63,7 +63,11 @@ def test_gpu_learned_guess_trains_and_keeps_the_solution(vdi):
     # the fallback rule ran at every solve, and the solution is the same to
     # the solve tolerance whichever guess the solve started from
     assert st["accepted"] + st["rejected"] == K
-    assert ca <= cb + 1.0
+    # The network does not gain here (its guesses have the smaller initial
+    # residual but leave low-frequency error that costs cycles: measured
+    # 3.900 against 3.725 MAC cycles per step).  The bound fails a network
+    # that costs half a cycle per step or more.
+    assert ca <= cb + 0.5
     for aid in (vdi.U, vdi.V, vdi.P, vdi.PG, vdi.H):
         e = rel(a.get(aid), b.get(aid))
         assert e < 1e-8, (aid, e)
