@@ -40,8 +40,12 @@
 
 namespace vdi {
 
-const TileShape kTileShapes[kNumTileShapes] = {{64, 32}, {32, 8},  {32, 16}, {64, 8},  {64, 16},
-                                               {32, 32}, {128, 16}, {128, 32}, {64, 64}};
+// The last three fit the nodal grids (2^k + 1 nodes a side), whose extra
+// node adds a column and a row of nearly empty tiles to the power-of-two
+// shapes: at 1025 x 513, 56 x 40 gives 247 tiles (one per CU) where 64 x 32
+// gives 289 and 64 x 64 gives 153.
+const TileShape kTileShapes[kNumTileShapes] = {{64, 32}, {32, 8},   {32, 16}, {64, 8},  {64, 16}, {32, 32},
+                                               {128, 16}, {128, 32}, {64, 64}, {56, 40}, {72, 36}, {48, 24}};
 
 // Position b of n work items -> item index such that the items of one XCD
 // (b mod 8) are contiguous.

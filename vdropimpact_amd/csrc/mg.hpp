@@ -82,7 +82,7 @@ struct TileShape {
 };
 
 // Tile shapes the autotuner chooses from.
-constexpr int kNumTileShapes = 9;
+constexpr int kNumTileShapes = 12;
 extern const TileShape kTileShapes[kNumTileShapes];
 
 // Down / up kernel of one level.
