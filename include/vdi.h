@@ -95,6 +95,7 @@ typedef struct vdi_stats {
 #define VDI_KCLASS_CC_UP 7      /* k_up<double, false>: cell-centred prolong + smooth */
 #define VDI_KCLASS_FEM_DOWN 8   /* k_down<double, true>: nodal smooth + restrict */
 #define VDI_KCLASS_FEM_UP 9     /* k_up<double, true>: nodal prolong + smooth */
+#define VDI_KCLASS_XSOLVE 10    /* k_xsolve<double>: the three phases of a larger coarsest level's exact solve */
 
 typedef struct vdi_ctx vdi_ctx;
 typedef struct vdi_mg vdi_mg;

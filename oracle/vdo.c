@@ -23,6 +23,10 @@
  * Dirichlet row) has at most VDO_SEP_MAXT unknowns, else by ncs sweeps. */
 #define VDO_EXACT_POINTS 3072
 #define VDO_SEP_MAXT 64
+/* A restricted level (not level 0) of at most VDO_EXACT_BIG unknowns, at most
+ * 256 a side and at most VDO_SEP_MAXT unknown rows stops the coarsening too
+ * (the GPU solves it with several workgroups). */
+#define VDO_EXACT_BIG 9000
 
 /* ------------------------------------------------------------------------ */
 /* Multigrid                                                                 */
@@ -90,6 +94,13 @@ static void mg_build_levels(vdo_mg *g, int m, int n, double cx, double cy) {
         L->r = (double *)xcalloc(N, sizeof(double));
         l++;
         if (l >= VDO_MAXLEV) break;
+        {
+            /* a restricted level of at most VDO_EXACT_BIG unknowns (at most
+             * 256 a side, at most VDO_SEP_MAXT unknown rows) is the coarsest */
+            const int rows = g->fem ? L->n - 1 : L->n;
+            if (l >= 2 && N <= VDO_EXACT_BIG && (L->m > L->n ? L->m : L->n) <= 256 && rows >= 1 && rows <= VDO_SEP_MAXT)
+                break;
+        }
         if (N <= VDO_EXACT_POINTS) break;
         if (mc % 2 || nc % 2 || mc / 2 < 2 || nc / 2 < 2) break;
         mc /= 2; nc /= 2; cx *= 0.25; cy *= 0.25;

@@ -20,9 +20,9 @@ DRIVER = r'''
 #include "coarse_sep.hpp"
 using namespace vdi::sep;
 int main() {
-    int fem, tdir, m, n, bc[4];
+    int full, fem, tdir, m, n, bc[4];
     double a[6];
-    if (scanf("%d %d %d %d", &fem, &tdir, &m, &n) != 4) return 2;
+    if (scanf("%d %d %d %d %d", &full, &fem, &tdir, &m, &n) != 5) return 2;
     for (int k = 0; k < 4; k++) if (scanf("%d", &bc[k]) != 1) return 2;
     for (int k = 0; k < 6; k++) if (scanf("%lf", &a[k]) != 1) return 2;
     Tri Pt, Rt, Sp, Qp;
@@ -33,10 +33,11 @@ int main() {
     }
     const int T = (int)Pt.d.size(), P = (int)Sp.d.size();
     std::vector<double> f;
-    if (!factor(Pt, Rt, Sp, Qp, f)) return 4;
+    if (!(full ? factor2(Pt, Rt, Sp, Qp, f) : factor(Pt, Rt, Sp, Qp, f))) return 4;
     std::vector<double> b((size_t)T * P), z;
     for (auto &v : b) if (scanf("%lf", &v) != 1) return 2;
-    solve(T, P, f, b, z);
+    if (full) solve2(T, P, f, b, z);
+    else solve(T, P, f, b, z);
     printf("%d %d\n", T, P);
     for (int i = 0; i < T * T; i++) printf("%.17g\n", f[i]);
     for (double v : z) printf("%.17g\n", v);
@@ -58,8 +59,8 @@ def driver(tmp_path_factory):
     return str(exe)
 
 
-def run(driver, fem, tdir, m, n, bc, a, b):
-    inp = f"{fem} {tdir} {m} {n}\n" + " ".join(map(str, bc)) + "\n" + " ".join(repr(float(v)) for v in a) + "\n"
+def run(driver, fem, tdir, m, n, bc, a, b, full=0):
+    inp = f"{full} {fem} {tdir} {m} {n}\n" + " ".join(map(str, bc)) + "\n" + " ".join(repr(float(v)) for v in a) + "\n"
     inp += "\n".join(repr(float(v)) for v in b) + "\n"
     r = subprocess.run([driver], input=inp, capture_output=True, text=True)
     assert r.returncode == 0, (r.returncode, r.stderr)
@@ -84,8 +85,10 @@ CASES = [
 
 @pytest.mark.parametrize("x_sym", [1, 0])
 @pytest.mark.parametrize("kind,m,n,cx,cy", CASES)
-@pytest.mark.parametrize("tdir", [0, 1])
-def test_separable_solve_matches_dense_solve(oracle, driver, kind, m, n, cx, cy, x_sym, tdir):
+@pytest.mark.parametrize("tdir,full", [(0, 0), (1, 0), (0, 1)])
+def test_separable_solve_matches_dense_solve(oracle, driver, kind, m, n, cx, cy, x_sym, tdir, full):
+    """full = 1: both directions diagonalised (the multi-workgroup solve of
+    larger coarsest levels), rows transformed along y as the kernels do."""
     g = oracle.OracleMG(kind, m, n, cx, cy, x_sym)
     N = m * n
     A = np.stack([g.apply(np.eye(N)[c]) for c in range(N)], axis=1)
@@ -113,7 +116,7 @@ def test_separable_solve_matches_dense_solve(oracle, driver, kind, m, n, cx, cy,
     bb = b.reshape(n, m)[1:] if fem else b.reshape(n, m)
     # the driver indexes unknowns p + P t: t is the row for tdir 0, the column for tdir 1
     bt = bb.ravel() if tdir == 0 else bb.T.ravel()
-    T, P, V, zs = run(driver, int(fem), tdir, m, n, bcs, a, bt)
+    T, P, V, zs = run(driver, int(fem), tdir, m, n, bcs, a, bt, full)
     assert (T, P) == ((rows, m) if tdir == 0 else (m, rows))
     zz = zs.reshape(T, P) if tdir == 0 else zs.reshape(T, P).T
     z = np.concatenate([np.zeros(m), zz.ravel()]) if fem else zz.ravel()

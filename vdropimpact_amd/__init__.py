@@ -35,12 +35,13 @@ FORM_FP64, FORM_MIXED = 0, 1
 PASS_GODUNOV, PASS_CORRECT_ADVECT = 0, 1
 # kernel classes for Fluid2D.time_kernels
 KCLASS_MG_DOWN0, KCLASS_MG_UP0, KCLASS_MG_MID, KCLASS_MG_COARSE, KCLASS_GAS, KCLASS_SIMPSON = range(6)
-KCLASS_CC_DOWN, KCLASS_CC_UP, KCLASS_FEM_DOWN, KCLASS_FEM_UP = range(6, 10)
+KCLASS_CC_DOWN, KCLASS_CC_UP, KCLASS_FEM_DOWN, KCLASS_FEM_UP, KCLASS_XSOLVE = range(6, 11)
 KCLASS_NAMES = {0: "k_cc_down/k_fem_down (level 0)", 1: "k_cc_up/k_fem_up (level 0)", 2: "level kernels 1..lc-1",
                 3: "k_coarse", 4: "k_gas", 5: "k_simpson"}
 # the same launches by kernel function, as rocprofv3 names them
 KFUNC_NAMES = {6: "vdi::k_down<double, false>", 7: "vdi::k_up<double, false>", 8: "vdi::k_down<double, true>",
-               9: "vdi::k_up<double, true>", 3: "vdi::k_coarse<double>", 4: "vdi::k_gas", 5: "vdi::k_simpson"}
+               9: "vdi::k_up<double, true>", 3: "vdi::k_coarse<double>", 10: "vdi::k_xsolve<double>",
+               4: "vdi::k_gas", 5: "vdi::k_simpson"}
 # array ids
 (U, V, P, UL, VL, UD, VD, C0, C1, SRC, GP, GTAU, GP_OLD, GTAU_OLD, H, HX, HT, FBD, PG, PG_OLD,
  Z_MAC, Z_FEM, Z_VX, Z_VY) = range(24)
@@ -641,7 +642,7 @@ class MGSystem:
         _check(self.lib.vdi_mg_autotune(self.h, reps), "vdi_mg_autotune")
 
 
-NUM_TILE_SHAPES = 12
+NUM_TILE_SHAPES = 14
 
 __all__ = [
     "Fluid2D", "MGSystem", "Batch", "UNet", "tile_shape", "NUM_TILE_SHAPES", "VdiParams", "VdiStats", "VdiError", "PhysicalConfig",

@@ -24,6 +24,8 @@
 #include <limits>
 #include <memory>
 
+#include <type_traits>
+
 #include "step.hpp"
 
 namespace vdi {
@@ -56,9 +58,10 @@ struct SolvePlan {
     BLaunch nbegin, ncycle;            // one-workgroup norm reductions (records only)
     std::vector<BLaunch> down, up;     // FP64 path, per level
     BLaunch coarse;
+    BLaunch xs[3];                     // multi-workgroup exact solves (members with a larger coarsest level)
     BLaunch down0[2], mx[2];           // mixed path, level 0, by parity
     std::vector<BLaunch> downf, upf;   // mixed path, levels 1..
-    BLaunch coarsef;
+    BLaunch coarsef, xsf[3];
 };
 
 }  // namespace
@@ -194,24 +197,47 @@ static BLaunch coarse_launch(vdi_batch *b, Blob &B, int k, int prec, double tol2
         std::vector<CoarseArgs<double>> recs;
         for (int s = 0; s < b->S; s++) {
             MG &g = *step_solver(b->sims[s], k);
+            if (g.big_exact()) continue;
             work.push_back(WorkItem{(int)recs.size(), 0, s});
             recs.push_back(g.coarse_args<double>(g.L, tol2));
             lds = std::max(lds, g.coarse_lds(0));
         }
         BLaunch L = add_launch(B, recs, work, lds, nullptr);
-        L.grid = b->S;
+        L.grid = std::max(1, L.nw);
         return L;
     }
     std::vector<CoarseArgs<float>> recs;
     for (int s = 0; s < b->S; s++) {
         MG &g = *step_solver(b->sims[s], k);
+        if (g.big_exact()) continue;
         work.push_back(WorkItem{(int)recs.size(), 0, s});
         recs.push_back(g.coarse_args<float>(g.LF, tol2));
         lds = std::max(lds, g.coarse_lds(1));
     }
     BLaunch L = add_launch(B, recs, work, lds, nullptr);
-    L.grid = b->S;
+    L.grid = std::max(1, L.nw);
     return L;
+}
+
+// Phase ph of the multi-workgroup exact solve of the members whose coarsest
+// level takes it.
+template <class T>
+static BLaunch x_launch_t(vdi_batch *b, Blob &B, int k, int ph) {
+    std::vector<XArgs<T>> recs;
+    std::vector<WorkItem> work;
+    size_t lds = 0;
+    for (int s = 0; s < b->S; s++) {
+        MG &g = *step_solver(b->sims[s], k);
+        if (!g.big_exact()) continue;
+        XArgs<T> a;
+        if constexpr (std::is_same<T, double>::value) a = g.x_args<double>(g.L, ph);
+        else a = g.x_args<float>(g.LF, ph);
+        const int nb = ph == 2 ? (a.P + kXColsV - 1) / kXColsV : a.nbr * a.nbc;
+        for (int t = 0; t < nb; t++) work.push_back(WorkItem{(int)recs.size(), t, s});
+        recs.push_back(a);
+        lds = std::max(lds, g.x_lds(ph));
+    }
+    return add_launch(B, recs, work, lds, mgk::xsolve_kernel(sizeof(T) == 4, true));
 }
 
 static BLaunch vec_launch(vdi_batch *b, Blob &B, int k, int op, const double *coef_base) {
@@ -311,6 +337,7 @@ static int build_plan(vdi_batch *b) {
                 P.up.push_back(level_launch(b, B, k, 0, 1, l, 0, tol2));
             }
             P.coarse = coarse_launch(b, B, k, 0, tol2);
+            for (int ph = 1; ph <= 3; ph++) P.xs[ph - 1] = x_launch_t<double>(b, B, k, ph);
             P.ncycle = norm_launch(b, B, k, 0, tol2);
         } else {
             for (int par = 0; par < 2; par++) {
@@ -324,6 +351,7 @@ static int build_plan(vdi_batch *b) {
                 P.upf.push_back(level_launch(b, B, k, 1, 1, l, 0, tol2));
             }
             P.coarsef = coarse_launch(b, B, k, 1, tol2);
+            for (int ph = 1; ph <= 3; ph++) P.xsf[ph - 1] = x_launch_t<float>(b, B, k, ph);
             P.ncycle = norm_launch(b, B, k, 0, tol2);
         }
     }
@@ -361,7 +389,20 @@ static int launch_level(vdi_batch *b, const BLaunch &L, int prec, int which, boo
     return mgk::up_b<float>(recs_at<LvArgs<float>>(b, L), w, L.nw, fem, L.lds, L.grid, aux_of(b), b->st);
 }
 
+static int launch_xsolve(vdi_batch *b, const BLaunch *X, int prec) {
+    for (int ph = 0; ph < 3; ph++) {
+        const BLaunch &L = X[ph];
+        if (L.nw == 0) continue;
+        if (prec == 0)
+            VDI_TRY(mgk::xsolve_b<double>(recs_at<XArgs<double>>(b, L), work_at(b, L), L.nw, L.lds, L.grid, aux_of(b), b->st));
+        else
+            VDI_TRY(mgk::xsolve_b<float>(recs_at<XArgs<float>>(b, L), work_at(b, L), L.nw, L.lds, L.grid, aux_of(b), b->st));
+    }
+    return VDI_OK;
+}
+
 static int launch_coarse(vdi_batch *b, const BLaunch &L, int prec) {
+    if (L.nw == 0) return VDI_OK;
     if (prec == 0)
         return mgk::coarse_b<double>(recs_at<CoarseArgs<double>>(b, L), work_at(b, L), L.nw, L.lds, L.grid, aux_of(b), b->st);
     return mgk::coarse_b<float>(recs_at<CoarseArgs<float>>(b, L), work_at(b, L), L.nw, L.lds, L.grid, aux_of(b), b->st);
@@ -382,6 +423,7 @@ static int launch_cycle(vdi_batch *b, int k, int c) {
     if (!P.mixed) {
         for (int l = 0; l < P.maxlc; l++) VDI_TRY(launch_level(b, P.down[l], 0, 0, P.fem));
         VDI_TRY(launch_coarse(b, P.coarse, 0));
+        VDI_TRY(launch_xsolve(b, P.xs, 0));
         for (int l = P.maxlc - 1; l >= 0; l--) VDI_TRY(launch_level(b, P.up[l], 0, 1, P.fem));
         return mgk::norm_b(recs_at<NormArgs>(b, P.ncycle), P.ncycle.nw, b->st);
     }
@@ -389,6 +431,7 @@ static int launch_cycle(vdi_batch *b, int k, int c) {
     VDI_TRY(launch_level(b, P.down0[par], 1, 0, P.fem));
     for (int l = 1; l < P.maxlc; l++) VDI_TRY(launch_level(b, P.downf[l], 1, 0, P.fem));
     VDI_TRY(launch_coarse(b, P.coarsef, 1));
+    VDI_TRY(launch_xsolve(b, P.xsf, 1));
     for (int l = P.maxlc - 1; l >= 1; l--) VDI_TRY(launch_level(b, P.upf[l], 1, 1, P.fem));
     VDI_TRY(launch_level(b, P.mx[par], 1, 2, P.fem));
     return mgk::norm_b(recs_at<NormArgs>(b, P.ncycle), P.ncycle.nw, b->st);

@@ -12,9 +12,14 @@
 // The eigenproblem is reduced to a standard one through the Cholesky factor
 // R_t = L L^T (C = L^-1 P_t L^-T, V = L^-T W) and solved by cyclic Jacobi
 // rotations; each mode's system is factored once for the Thomas algorithm
-// (no pivoting: every mode matrix here is definite).
+// (no pivoting: every mode matrix here is definite).  A larger level is
+// diagonalised in both directions instead (factor2): with Sp U = Qp U
+// diag(mu), U^T Qp U = I as well, (V (x) U)^T A (V (x) U) = diag(lam_k +
+// mu_j), and the solve is four dense transforms and a scaling, which several
+// workgroups share.
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -78,49 +83,122 @@ inline bool jacobi(int n, std::vector<double> &a, std::vector<double> &w, std::v
 // mode k's Thomas factors); then the T off-diagonals.
 inline size_t factor_doubles(int T, int P) { return (size_t)T * T + 2 * (size_t)T * (P + 1) + T; }
 
-// Factors of A = Rt (x) Sp + Pt (x) Qp (Rt of size T, Sp and Qp of size P).
-// Returns false if the Cholesky factor, the eigen-decomposition or a Thomas
-// pivot fails.
-inline bool factor(const Tri &Pt, const Tri &Rt, const Tri &Sp, const Tri &Qp, std::vector<double> &out) {
-    const int T = (int)Pt.d.size(), P = (int)Sp.d.size(), PP = P + 1;
-    // Cholesky of the tridiagonal Rt: L has diagonal l[i] and subdiagonal g[i] (row i, column i - 1)
+// Generalized eigenbasis of the pencil (A, B), B positive definite: A V =
+// B V diag(lam), V^T B V = I; V row-major, V[i * n + k] = component i of
+// eigenvector k.  Returns false if the Cholesky factor or Jacobi fails.
+inline bool geig(const Tri &A, const Tri &B, std::vector<double> &lam, std::vector<double> &V) {
+    const int T = (int)A.d.size();
     std::vector<double> l(T), g(T, 0.0);
     for (int i = 0; i < T; i++) {
-        if (i > 0) g[i] = Rt.e / l[i - 1];
-        const double v = Rt.d[i] - g[i] * g[i];
+        if (i > 0) g[i] = B.e / l[i - 1];
+        const double v = B.d[i] - g[i] * g[i];
         if (!(v > 0)) return false;
         l[i] = std::sqrt(v);
     }
-    // C = L^-1 Pt L^-T: X = L^-1 Pt (column by column), then C = (L^-1 X^T)^T
     std::vector<double> x((size_t)T * T, 0.0), c((size_t)T * T, 0.0);
     for (int col = 0; col < T; col++)
         for (int i = 0; i < T; i++) {
-            double v = (i == col ? Pt.d[i] : (i == col - 1 || i == col + 1 ? Pt.e : 0.0));
+            double v = (i == col ? A.d[i] : (i == col - 1 || i == col + 1 ? A.e : 0.0));
             if (i > 0) v -= g[i] * x[(size_t)(i - 1) * T + col];
             x[(size_t)i * T + col] = v / l[i];
         }
-    for (int row = 0; row < T; row++)  // row of C = L^-1 applied to row `row` of X (a column of X^T)
+    for (int row = 0; row < T; row++)
         for (int i = 0; i < T; i++) {
             double v = x[(size_t)row * T + i];
             if (i > 0) v -= g[i] * c[(size_t)row * T + i - 1];
             c[(size_t)row * T + i] = v / l[i];
         }
-    for (int i = 0; i < T; i++)  // symmetrise the rounding
+    for (int i = 0; i < T; i++)
         for (int j = i + 1; j < T; j++) {
             const double v = 0.5 * (c[(size_t)i * T + j] + c[(size_t)j * T + i]);
             c[(size_t)i * T + j] = c[(size_t)j * T + i] = v;
         }
-    std::vector<double> lam, wv;
+    std::vector<double> wv;
     if (!jacobi(T, c, lam, wv)) return false;
-    out.assign(factor_doubles(T, P), 0.0);
-    double *V = out.data(), *iv = V + (size_t)T * T, *cp = iv + (size_t)T * PP, *sub = cp + (size_t)T * PP;
-    // V = L^-T W: back substitution with L^T, column k of W at a time
+    V.assign((size_t)T * T, 0.0);
     for (int k = 0; k < T; k++)
         for (int i = T - 1; i >= 0; i--) {
             double v = wv[(size_t)i * T + k];
             if (i + 1 < T) v -= g[i + 1] * V[(size_t)(i + 1) * T + k];
             V[(size_t)i * T + k] = v / l[i];
         }
+    return true;
+}
+
+// Layout of the factors of the fully diagonalised solve (both directions
+// transformed): V (T x T), U (P x P) and its transpose, then 1 / (lam_k +
+// mu_j) as T rows of P.
+inline size_t factor2_doubles(int T, int P) { return (size_t)T * T + 2 * (size_t)P * P + (size_t)T * P; }
+
+// Factors of A = Rt (x) Sp + Pt (x) Qp diagonalised in both directions:
+// Pt V = Rt V diag(lam), Sp U = Qp U diag(mu) (Qp positive definite), so that
+// (V (x) U)^T A (V (x) U) = diag(lam_k + mu_j).
+inline bool factor2(const Tri &Pt, const Tri &Rt, const Tri &Sp, const Tri &Qp, std::vector<double> &out) {
+    const int T = (int)Pt.d.size(), P = (int)Sp.d.size();
+    std::vector<double> lam, V, mu, U;
+    if (!geig(Pt, Rt, lam, V) || !geig(Sp, Qp, mu, U)) return false;
+    out.assign(factor2_doubles(T, P), 0.0);
+    double *o = out.data();
+    std::copy(V.begin(), V.end(), o);
+    o += (size_t)T * T;
+    std::copy(U.begin(), U.end(), o);
+    o += (size_t)P * P;
+    for (int i = 0; i < P; i++)
+        for (int j = 0; j < P; j++) o[(size_t)j * P + i] = U[(size_t)i * P + j];
+    o += (size_t)P * P;
+    for (int k = 0; k < T; k++)
+        for (int j = 0; j < P; j++) {
+            const double d = lam[k] + mu[j];
+            if (!(std::fabs(d) > 0)) return false;
+            o[(size_t)k * P + j] = 1 / d;
+        }
+    return true;
+}
+
+// The fully diagonalised solve with the factors of factor2, in the order of
+// the kernels: rows times U, columns through V^T, the diagonal and V, rows
+// times U^T.  b and z indexed [p + P * t].
+inline void solve2(int T, int P, const std::vector<double> &f, const std::vector<double> &b, std::vector<double> &z) {
+    const double *V = f.data(), *U = V + (size_t)T * T, *Ut = U + (size_t)P * P, *D = Ut + (size_t)P * P;
+    std::vector<double> h((size_t)T * P, 0.0), g((size_t)T * P, 0.0);
+    for (int t = 0; t < T; t++)
+        for (int j = 0; j < P; j++) {
+            double acc = 0;
+            for (int p = 0; p < P; p++) acc += b[p + (size_t)P * t] * U[(size_t)p * P + j];
+            h[j + (size_t)P * t] = acc;
+        }
+    for (int j = 0; j < P; j++) {
+        std::vector<double> hk(T);
+        for (int k = 0; k < T; k++) {
+            double acc = 0;
+            for (int t = 0; t < T; t++) acc += V[(size_t)t * T + k] * h[j + (size_t)P * t];
+            hk[k] = acc * D[(size_t)k * P + j];
+        }
+        for (int t = 0; t < T; t++) {
+            double acc = 0;
+            for (int k = 0; k < T; k++) acc += V[(size_t)t * T + k] * hk[k];
+            g[j + (size_t)P * t] = acc;
+        }
+    }
+    z.assign((size_t)T * P, 0.0);
+    for (int t = 0; t < T; t++)
+        for (int p = 0; p < P; p++) {
+            double acc = 0;
+            for (int j = 0; j < P; j++) acc += g[j + (size_t)P * t] * Ut[(size_t)j * P + p];
+            z[p + (size_t)P * t] = acc;
+        }
+}
+
+// Factors of A = Rt (x) Sp + Pt (x) Qp (Rt of size T, Sp and Qp of size P).
+// Returns false if the Cholesky factor, the eigen-decomposition or a Thomas
+// pivot fails.
+inline bool factor(const Tri &Pt, const Tri &Rt, const Tri &Sp, const Tri &Qp, std::vector<double> &out) {
+    const int T = (int)Pt.d.size(), P = (int)Sp.d.size(), PP = P + 1;
+    std::vector<double> lam, Vm;
+    if (!geig(Pt, Rt, lam, Vm)) return false;
+    out.assign(factor_doubles(T, P), 0.0);
+    double *V = out.data(), *iv = V + (size_t)T * T, *cp = iv + (size_t)T * PP, *sub = cp + (size_t)T * PP;
+    std::copy(Vm.begin(), Vm.end(), V);
     for (int k = 0; k < T; k++) {
         const double e = Sp.e + lam[k] * Qp.e;
         sub[k] = e;

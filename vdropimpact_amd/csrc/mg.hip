@@ -40,12 +40,13 @@
 
 namespace vdi {
 
-// The last three fit the nodal grids (2^k + 1 nodes a side), whose extra
+// The last five fit the nodal grids (2^k + 1 nodes a side), whose extra
 // node adds a column and a row of nearly empty tiles to the power-of-two
 // shapes: at 1025 x 513, 56 x 40 gives 247 tiles (one per CU) where 64 x 32
-// gives 289 and 64 x 64 gives 153.
-const TileShape kTileShapes[kNumTileShapes] = {{64, 32}, {32, 8},   {32, 16}, {64, 8},  {64, 16}, {32, 32},
-                                               {128, 16}, {128, 32}, {64, 64}, {56, 40}, {72, 36}, {48, 24}};
+// gives 289 and 64 x 64 gives 153; at 513 x 257, 28 x 20 gives 247.
+const TileShape kTileShapes[kNumTileShapes] = {{64, 32},  {32, 8},   {32, 16}, {64, 8},  {64, 16},
+                                               {32, 32},  {128, 16}, {128, 32}, {64, 64}, {56, 40},
+                                               {72, 36},  {48, 24},  {28, 20},  {36, 16}};
 
 // Position b of n work items -> item index such that the items of one XCD
 // (b mod 8) are contiguous.
@@ -1141,6 +1142,143 @@ __global__ void __launch_bounds__(kCoarseThreads) k_coarse_b(const CoarseArgs<T>
 }
 
 // ------------------------------------------------------------------------
+// Multi-workgroup exact solve of a larger coarsest level (both directions
+// diagonalised, MG::coarse_factors).  Phases 1 and 3: out = in M over rows
+// (M = U, then U^T), a workgroup owning kXRows x kXCols outputs with its
+// input rows and M's columns staged in LDS; phase 2: kXColsV columns of w
+// through V^T, the diagonal 1 / (lam + mu) and V.  Arithmetic in FP64.
+// ------------------------------------------------------------------------
+
+// Right-hand side of unknown row t (level row t + yoff) at column p: the
+// nodal row 1 loses its coupling to the Dirichlet row z0 = b0 / a_cc.
+template <class T>
+__device__ __forceinline__ double x_rhs(const XArgs<T> &a, int t, int p) {
+    const int y = t + a.yoff, m = a.m;
+    double v = double(a.b[(size_t)y * m + p]);
+    if (a.yoff && y == 1) {
+        auto z0 = [&](int q) {
+            return (q < 0 || q >= m) ? 0.0 : double(a.b[q]) / fem_acc<double>(a.fop, m, a.n, q, 0);
+        };
+        const double wi = (p == 0 || p == m - 1) ? 0.5 : 1.0;
+        v -= a.fop.fc * (z0(p - 1) + z0(p + 1)) + wi * a.fop.fey * z0(p);
+    }
+    return v;
+}
+
+// Staging of two arrays into LDS in shared rounds: each thread loads up to
+// kXStage values of the first and kXStage2 of the second before it stores any
+// of them, so that a round costs one memory round trip.
+constexpr int kXStage = 16, kXStage2 = 4;
+template <class F1, class F2>
+__device__ __forceinline__ void x_stage2(double *d1, int n1, F1 f1, double *d2, int n2, F2 f2) {
+    for (int b1 = threadIdx.x, b2 = threadIdx.x; b1 < n1 || b2 < n2;
+         b1 += kXStage * kXThreads, b2 += kXStage2 * kXThreads) {
+        double v1[kXStage], v2[kXStage2];
+#pragma unroll
+        for (int u = 0; u < kXStage; u++) v1[u] = f1(min(b1 + u * kXThreads, n1 - 1));
+#pragma unroll
+        for (int u = 0; u < kXStage2; u++) v2[u] = f2(min(b2 + u * kXThreads, n2 - 1));
+#pragma unroll
+        for (int u = 0; u < kXStage; u++)
+            if (b1 + u * kXThreads < n1) d1[b1 + u * kXThreads] = v1[u];
+#pragma unroll
+        for (int u = 0; u < kXStage2; u++)
+            if (b2 + u * kXThreads < n2) d2[b2 + u * kXThreads] = v2[u];
+    }
+}
+
+// sum_q x[q * sx] y[q * sy], q < n, with four independent partial sums
+__device__ __forceinline__ double x_dot(const double *x, int sx, const double *y, int sy, int n) {
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    int q = 0;
+    for (; q + 4 <= n; q += 4) {
+        s0 = fma(x[q * sx], y[q * sy], s0);
+        s1 = fma(x[(q + 1) * sx], y[(q + 1) * sy], s1);
+        s2 = fma(x[(q + 2) * sx], y[(q + 2) * sy], s2);
+        s3 = fma(x[(q + 3) * sx], y[(q + 3) * sy], s3);
+    }
+    for (; q < n; q++) s0 = fma(x[q * sx], y[q * sy], s0);
+    return (s0 + s1) + (s2 + s3);
+}
+
+template <class T>
+__device__ void xsolve_body(const XArgs<T> &a, int blk, double *lds) {
+    const int tid = threadIdx.x, P = a.P, Tn = a.Tn;
+    const double *V = a.fac, *U = V + (size_t)Tn * Tn, *Ut = U + (size_t)P * P, *D = Ut + (size_t)P * P;
+    if (a.phase != 2) {
+        const int rb = blk / a.nbc, cb = blk - rb * a.nbc, r0 = rb * kXRows, c0 = cb * kXCols;
+        double *in = lds, *mx = lds + kXRows * P;
+        const double *M = a.phase == 1 ? U : Ut;
+        x_stage2(
+            mx, P * kXCols,
+            [&](int i) {
+                const int q = i >> 5, c = i & (kXCols - 1);
+                return M[(size_t)q * P + min(c0 + c, P - 1)];
+            },
+            in, kXRows * P,
+            [&](int i) {
+                const int r = i / P, p = i - r * P, t = min(r0 + r, Tn - 1);
+                return a.phase == 1 ? x_rhs<T>(a, t, p) : double(a.w[(size_t)t * P + p]);
+            });
+        __syncthreads();
+        const int r = tid / kXCols, c = tid - r * kXCols, t = r0 + r, j = c0 + c;
+        const double acc = x_dot(in + r * P, 1, mx + c, kXCols, P);
+        if (t < Tn && j < P) {
+            if (a.phase == 1) a.w[(size_t)t * P + j] = T(acc);
+            else a.z[(size_t)(t + a.yoff) * a.m + j] = T(acc);
+        }
+        // the nodal Dirichlet row z0 = b0 / a_cc (row-block 0 of phase 1)
+        if (a.phase == 1 && a.yoff && rb == 0 && tid < kXCols && c0 + tid < P)
+            a.z[c0 + tid] = T(double(a.b[c0 + tid]) / fem_acc<double>(a.fop, a.m, a.n, c0 + tid, 0));
+        return;
+    }
+    const int j0 = blk * kXColsV;
+    double *Vl = lds, *wl = Vl + Tn * Tn, *hl = wl + Tn * kXColsV;
+    x_stage2(
+        Vl, Tn * Tn, [&](int i) { return V[i]; }, wl, Tn * kXColsV,
+        [&](int i) {
+            const int t = i / kXColsV, c = i - t * kXColsV;
+            return double(a.w[(size_t)t * P + min(j0 + c, P - 1)]);
+        });
+    __syncthreads();
+    for (int i = tid; i < Tn * kXColsV; i += kXThreads) {
+        const int k = i / kXColsV, c = i - k * kXColsV;
+        hl[i] = x_dot(Vl + k, Tn, wl + c, kXColsV, Tn) * D[(size_t)k * P + min(j0 + c, P - 1)];
+    }
+    __syncthreads();
+    for (int i = tid; i < Tn * kXColsV; i += kXThreads) {
+        const int t = i / kXColsV, c = i - t * kXColsV;
+        const double acc = x_dot(Vl + t * Tn, 1, hl + c, kXColsV, Tn);
+        if (j0 + c < P) a.w[(size_t)t * P + j0 + c] = T(acc);
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(kXThreads) k_xsolve(XArgs<T> a) {
+    extern __shared__ double lds_d[];
+    if (a.ctl.state->done) return;
+    stamp_begin(a.ts);
+    xsolve_body<T>(a, blockIdx.x, lds_d);
+    stamp_end(a.ts);
+}
+
+template <class T>
+__global__ void __launch_bounds__(kXThreads) k_xsolve_b(const XArgs<T> *A, const WorkItem *W, int nw, BatchAux aux) {
+    extern __shared__ double lds_d[];
+    stamp_begin(aux.ts);
+    for (int p = blockIdx.x; p < nw; p += gridDim.x) {
+        const WorkItem w = W[p];
+        const XArgs<T> &a = A[w.rec];  // read in place: a copy would go to scratch
+        if (a.ctl.state->done) continue;
+        const unsigned long long t0 = busy_begin(aux.busy);
+        xsolve_body<T>(a, w.tile, lds_d);
+        __syncthreads();
+        busy_end(aux.busy, w.sim, t0);
+    }
+    stamp_end(aux.ts);
+}
+
+// ------------------------------------------------------------------------
 // Start of a solve, element-wise operations, operator application
 // ------------------------------------------------------------------------
 
@@ -1364,7 +1502,9 @@ int set_attributes() {
         (const void *)k_up_b<double, false>, (const void *)k_up_b<double, true>, (const void *)k_up_b<float, false>,
         (const void *)k_up_b<float, true>, (const void *)k_coarse<double>, (const void *)k_coarse<float>,
         (const void *)k_coarse_b<double>, (const void *)k_coarse_b<float>, (const void *)k_mx_up0<false>,
-        (const void *)k_mx_up0<true>, (const void *)k_mx_up0_b<false>, (const void *)k_mx_up0_b<true>};
+        (const void *)k_mx_up0<true>, (const void *)k_mx_up0_b<false>, (const void *)k_mx_up0_b<true>,
+        (const void *)k_xsolve<double>, (const void *)k_xsolve<float>, (const void *)k_xsolve_b<double>,
+        (const void *)k_xsolve_b<float>};
     // dynamic LDS may use what the kernel's own static LDS leaves
     int dyn = g_max_lds;
     for (const void *k : ks) {
@@ -1468,6 +1608,22 @@ template <class T> int up_b(const LvArgs<T> *A, const WorkItem *w, int nw, bool 
     VDI_LAUNCHED();
     return VDI_OK;
 }
+template <class T> int xsolve(const XArgs<T> &a, size_t lds, hipStream_t st) {
+    const int nb = a.phase == 2 ? (a.P + kXColsV - 1) / kXColsV : a.nbr * a.nbc;
+    hipLaunchKernelGGL(k_xsolve<T>, dim3(nb), dim3(kXThreads), lds, st, a);
+    VDI_LAUNCHED();
+    return VDI_OK;
+}
+template <class T> int xsolve_b(const XArgs<T> *A, const WorkItem *w, int nw, size_t lds, int grid, BatchAux aux,
+                                hipStream_t st) {
+    hipLaunchKernelGGL(k_xsolve_b<T>, dim3(grid), dim3(kXThreads), lds, st, A, w, nw, aux);
+    VDI_LAUNCHED();
+    return VDI_OK;
+}
+const void *xsolve_kernel(int prec, bool b) {
+    if (prec == 0) return b ? (const void *)k_xsolve_b<double> : (const void *)k_xsolve<double>;
+    return b ? (const void *)k_xsolve_b<float> : (const void *)k_xsolve<float>;
+}
 template <class T> int coarse_b(const CoarseArgs<T> *A, const WorkItem *w, int nw, size_t lds, int grid,
                                 BatchAux aux, hipStream_t st) {
     hipLaunchKernelGGL(k_coarse_b<T>, dim3(grid), dim3(kCoarseBX, kCoarseBY), lds, st, A, w, nw, aux);
@@ -1502,6 +1658,10 @@ template int down_b<float>(const LvArgs<float> *, const WorkItem *, int, bool, s
 template int up_b<double>(const LvArgs<double> *, const WorkItem *, int, bool, size_t, int, BatchAux, hipStream_t);
 template int up_b<float>(const LvArgs<float> *, const WorkItem *, int, bool, size_t, int, BatchAux, hipStream_t);
 template int coarse_b<double>(const CoarseArgs<double> *, const WorkItem *, int, size_t, int, BatchAux, hipStream_t);
+template int xsolve<double>(const XArgs<double> &, size_t, hipStream_t);
+template int xsolve<float>(const XArgs<float> &, size_t, hipStream_t);
+template int xsolve_b<double>(const XArgs<double> *, const WorkItem *, int, size_t, int, BatchAux, hipStream_t);
+template int xsolve_b<float>(const XArgs<float> *, const WorkItem *, int, size_t, int, BatchAux, hipStream_t);
 template int coarse_b<float>(const CoarseArgs<float> *, const WorkItem *, int, size_t, int, BatchAux, hipStream_t);
 
 }  // namespace mgk
@@ -1615,6 +1775,14 @@ int MG::init(int kind_, int m, int n, double cx, double cy, int x_sym, double *z
         lm.push_back(fem ? mc + 1 : mc);
         ln.push_back(fem ? nc + 1 : nc);
         if ((int)lm.size() >= 24) break;
+        {
+            // a restricted level small enough for the multi-workgroup exact
+            // solve is the coarsest one
+            const int l = (int)lm.size() - 1, rows = fem ? ln.back() - 1 : ln.back();
+            if (l >= 1 && (long)lm.back() * ln.back() <= kExactBig && std::max(lm.back(), ln.back()) <= kExactBigSide &&
+                rows >= 1 && rows <= kSepMaxT)
+                break;
+        }
         if ((long)lm.back() * ln.back() <= kExactPoints) break;
         if (mc % 2 || nc % 2 || mc / 2 < 2 || nc / 2 < 2) break;
         mc /= 2;
@@ -1667,20 +1835,23 @@ int MG::init(int kind_, int m, int n, double cx, double cy, int x_sym, double *z
         // the nodal Dirichlet row) has at most kSepMaxT unknowns
         const int Mc = lm[lc], Nc = ln[lc], rows = fem ? Nc - 1 : Nc;
         fac_.reset();
-        sep_tdir_ = rows <= Mc ? 0 : 1;
+        big_ = lc >= 1 && (long)Mc * Nc <= kExactBig && std::max(Mc, Nc) <= kExactBigSide && rows >= 1 &&
+               rows <= kSepMaxT;
+        sep_tdir_ = big_ || rows <= Mc ? 0 : 1;
         sep_T_ = sep_tdir_ == 0 ? rows : Mc;
         sep_P_ = sep_tdir_ == 0 ? Mc : rows;
         sep_t0_ = fem ? 1 : 0;
-        if (sep_T_ > kSepMaxT || rows < 1) sep_T_ = sep_P_ = 0;
+        if (!big_ && (sep_T_ > kSepMaxT || rows < 1)) sep_T_ = sep_P_ = 0;
         // LDS: z and b tiles and the reciprocal table, then the double workspace
         for (int pr = 0; pr < 2; pr++) {
             const size_t elem = pr ? sizeof(float) : sizeof(double);
             size_t bytes = (2 * (size_t)(Mc + 2) * (Nc + 2) + kInvTab) * elem;
             bytes = (bytes + 7) & ~(size_t)7;
             if (sep_T_ > 0) bytes += (sep::factor_doubles(sep_T_, sep_P_) + (size_t)sep_T_ * (sep_P_ + 1)) * sizeof(double);
-            coarse_lds_[pr] = bytes;
+            coarse_lds_[pr] = big_ ? 0 : bytes;
         }
         if (coarse_lds_[0] > (size_t)mgk::max_lds()) return VDI_ERR_ARG;
+        if (big_ && std::max(x_lds(1), x_lds(2)) > (size_t)mgk::max_lds()) return VDI_ERR_ARG;
     }
     VDI_TRY(set_coeffs(cx, cy));
     for (int pr = 0; pr < 2; pr++)
@@ -1741,6 +1912,10 @@ int MG::coarse_factors() {
     if (sep_T_ == 0) return VDI_OK;
     const int l = lc, M = lm[l], Nn = ln[l];
     sep::Tri Pt, Rt, Sp, Qp;
+    auto solve_host = [&](const std::vector<double> &f, const std::vector<double> &b, std::vector<double> &z) {
+        if (big_) sep::solve2(sep_T_, sep_P_, f, b, z);
+        else sep::solve(sep_T_, sep_P_, f, b, z);
+    };
     if (fem) {
         if (!sep::fem_pencil(fe.fm, fe.fex, fe.fey, fe.fc, M, Nn, sep_tdir_, Pt, Rt, Sp, Qp)) return VDI_ERR_ARG;
     } else {
@@ -1748,7 +1923,7 @@ int MG::coarse_factors() {
                        Qp);
     }
     std::vector<double> f;
-    if (!sep::factor(Pt, Rt, Sp, Qp, f)) return VDI_ERR_ARG;
+    if (!(big_ ? sep::factor2(Pt, Rt, Sp, Qp, f) : sep::factor(Pt, Rt, Sp, Qp, f))) return VDI_ERR_ARG;
     {
         const int T = sep_T_, P = sep_P_, y0 = sep_t0_;
         std::vector<double> zt((size_t)M * Nn, 0.0), bfull((size_t)M * Nn), bs((size_t)T * P), zs;
@@ -1762,7 +1937,7 @@ int MG::coarse_factors() {
                 const int i = sep_tdir_ == 0 ? p : t, j = (sep_tdir_ == 0 ? t : p) + y0;
                 bs[p + (size_t)P * t] = bfull[i + (size_t)M * j];
             }
-        sep::solve(T, P, f, bs, zs);
+        solve_host(f, bs, zs);
         double e2 = 0, n2 = 0;
         for (int t = 0; t < T; t++)
             for (int p = 0; p < P; p++) {
@@ -1977,8 +2152,39 @@ double MG::up_bytes(int prec, int l) const {
 // exact solve also reads its FP64 factors
 double MG::coarse_bytes(int prec) const {
     const double e = prec ? 4.0 : 8.0;
+    if (big_) {
+        // three launches: b -> w, w -> w, w -> z, and the factors once each
+        return 3.0 * (2.0 * e * lm[lc] * ln[lc]) + 8.0 * (double)sep::factor2_doubles(sep_T_, sep_P_);
+    }
     const double f = sep_T_ > 0 ? 8.0 * (double)sep::factor_doubles(sep_T_, sep_P_) : 0.0;
     return e * (lc == 0 && sep_T_ == 0 ? 3.0 : 2.0) * lm[lc] * ln[lc] + f;
+}
+
+template <class T>
+XArgs<T> MG::x_args(const std::vector<LevelT<T>> &H, int phase) {
+    XArgs<T> a{};
+    const LevelT<T> &v = H[lc];
+    a.phase = phase;
+    a.m = v.m;
+    a.n = v.n;
+    a.yoff = sep_t0_;
+    a.Tn = sep_T_;
+    a.P = sep_P_;
+    a.nbr = (sep_T_ + kXRows - 1) / kXRows;
+    a.nbc = (sep_P_ + kXCols - 1) / kXCols;
+    a.b = v.b;
+    a.w = v.zalt;
+    a.z = v.z;
+    a.fac = fac_ ? fac_->f.as<double>() : nullptr;
+    a.fop = fe;
+    a.ctl = ctl();
+    a.ts = nullptr;
+    return a;
+}
+
+size_t MG::x_lds(int phase) const {
+    if (phase == 2) return ((size_t)sep_T_ * sep_T_ + 2 * (size_t)sep_T_ * kXColsV) * sizeof(double);
+    return ((size_t)kXRows * sep_P_ + (size_t)sep_P_ * kXCols) * sizeof(double);
 }
 // reads e (zalt), r32, coarse e, z64, b64; writes z64, r32
 double MG::mx_bytes() const {
@@ -2013,6 +2219,14 @@ int MG::launch_up(const std::vector<LevelT<T>> &H, int l, int norm, double tol2)
 template <class T>
 int MG::launch_coarse(const std::vector<LevelT<T>> &H, double tol2) {
     const int prec = sizeof(T) == 4;
+    if (big_) {
+        for (int ph = 1; ph <= 3; ph++) {
+            XArgs<T> x = x_args<T>(H, ph);
+            if (timer) x.ts = timer->slot(prec ? VDI_KCLASS_MG_COARSE : VDI_KCLASS_XSOLVE, coarse_bytes(prec) / 3.0);
+            VDI_TRY(mgk::xsolve<T>(x, x_lds(ph), st));
+        }
+        return VDI_OK;
+    }
     CoarseArgs<T> a = coarse_args<T>(H, tol2);
     if (timer) a.ts = timer->slot(VDI_KCLASS_MG_COARSE, coarse_bytes(prec));
     return mgk::coarse<T>(a, coarse_lds(prec), st);
@@ -2404,6 +2618,8 @@ template LvArgs<float> MG::down_args<float>(const std::vector<LevelF> &, int, in
 template LvArgs<double> MG::up_args<double>(const std::vector<Level> &, int, int, double);
 template LvArgs<float> MG::up_args<float>(const std::vector<LevelF> &, int, int, double);
 template CoarseArgs<double> MG::coarse_args<double>(const std::vector<Level> &, double);
+template XArgs<double> MG::x_args<double>(const std::vector<Level> &, int);
+template XArgs<float> MG::x_args<float>(const std::vector<LevelF> &, int);
 template CoarseArgs<float> MG::coarse_args<float>(const std::vector<LevelF> &, double);
 
 }  // namespace vdi

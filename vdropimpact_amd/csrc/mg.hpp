@@ -54,6 +54,16 @@ constexpr int kCoarseThreads = kCoarseBX * kCoarseBY;
 // ncs smoothing sweeps.
 constexpr int kExactPoints = 3072;
 constexpr int kSepMaxT = 64;
+// A restricted level (not level 0) of at most kExactBig unknowns and at most
+// kExactBigSide a side is the coarsest one already, and is solved exactly by
+// three multi-workgroup launches (both directions diagonalised): rows times
+// U, columns through V^T, 1 / (lam + mu) and V, rows times U^T.
+constexpr int kExactBig = 9000;
+constexpr int kExactBigSide = 256;
+constexpr int kXRows = 8, kXCols = 32;  // row-transform block (256 threads, one output each)
+static_assert(kXCols == 32, "the row transform's staging indexes columns by shift and mask");
+constexpr int kXColsV = 4;               // column-transform block: columns per workgroup
+constexpr int kXThreads = 256;
 // The exact solve's transforms take a block of kSepModes modes (rows) by
 // kSepCols columns per thread; its factors are staged kSepLoads values per
 // thread and round (one round for the 64 x 32 levels of the 1024 x 512
@@ -82,7 +92,7 @@ struct TileShape {
 };
 
 // Tile shapes the autotuner chooses from.
-constexpr int kNumTileShapes = 12;
+constexpr int kNumTileShapes = 14;
 extern const TileShape kTileShapes[kNumTileShapes];
 
 // Down / up kernel of one level.
@@ -137,6 +147,22 @@ struct CoarseDesc {
     int ws;        // byte offset of the double workspace
     const double *fac;  // V ([t][k], T x T), inv and cp ([k][p], T x P each), off-diagonals (T)
 };
+// One phase of the multi-workgroup exact solve of the coarsest level
+// (phase 1: w = B U over rows; 2: each column of w through V^T, the
+// diagonal and V; 3: z = w U^T over rows).  w is the level's zalt array
+// ([t * P + j], T x P), b and z the level's own; unknown row t is level row
+// t + yoff (yoff = 1 for the nodal system's Dirichlet row 0).
+template <class T>
+struct XArgs {
+    int phase, m, n, yoff, Tn, P, nbr, nbc;
+    const T *b;
+    T *w, *z;
+    const double *fac;  // V (Tn x Tn), U (P x P), U^T, 1 / (lam + mu) (Tn x P)
+    FemOp fop;
+    MGCtl ctl;
+    Stamp *ts;
+};
+
 template <class T>
 struct CoarseArgs {
     CoarseDesc d;
@@ -337,6 +363,11 @@ class MG {
                                        double tol2 = 0, int fin_begin = 0);
     template <class T> int launch_up(const std::vector<LevelT<T>> &H, int l, int norm, double tol2);
     template <class T> int launch_coarse(const std::vector<LevelT<T>> &H, double tol2);
+  public:
+    template <class T> XArgs<T> x_args(const std::vector<LevelT<T>> &H, int phase);
+    size_t x_lds(int phase) const;
+    bool big_exact() const { return big_; }
+  private:
     int launch_vec(const VecArgs &a);
     DevBuf store_;  // coarse z/b
     DevBuf store_f_;
@@ -349,6 +380,7 @@ class MG {
     Arena *arena_ = nullptr;              // set: buffers come from it (see init)
     std::shared_ptr<CoarseFactors> fac_;  // exact solve of the coarsest level (null: sweeps)
     int sep_tdir_ = 0, sep_T_ = 0, sep_P_ = 0, sep_t0_ = 0;
+    bool big_ = false;  // the multi-workgroup exact solve (kExactBig)
     void level_coeffs(double cx, double cy);
     MGState *host_state_ = nullptr;  // pinned
     size_t coarse_lds_[2] = {0, 0};
@@ -363,6 +395,7 @@ namespace mgk {
 template <class T> int down(const LvArgs<T> &a, bool fem, size_t lds, hipStream_t st);
 template <class T> int up(const LvArgs<T> &a, bool fem, size_t lds, hipStream_t st);
 template <class T> int coarse(const CoarseArgs<T> &a, size_t lds, hipStream_t st);
+template <class T> int xsolve(const XArgs<T> &a, size_t lds, hipStream_t st);
 int mx(const MxArgs &a, bool fem, size_t lds, hipStream_t st);
 int begin(const BeginArgs &a, hipStream_t st);
 int vec(const VecArgs &a, hipStream_t st);
@@ -374,6 +407,9 @@ template <class T> int down_b(const LvArgs<T> *args, const WorkItem *w, int nw, 
                               BatchAux aux, hipStream_t st);
 template <class T> int up_b(const LvArgs<T> *args, const WorkItem *w, int nw, bool fem, size_t lds, int grid,
                             BatchAux aux, hipStream_t st);
+template <class T> int xsolve_b(const XArgs<T> *args, const WorkItem *w, int nw, size_t lds, int grid, BatchAux aux,
+                                hipStream_t st);
+const void *xsolve_kernel(int prec, bool batched);
 template <class T> int coarse_b(const CoarseArgs<T> *args, const WorkItem *w, int nw, size_t lds, int grid,
                                 BatchAux aux, hipStream_t st);
 int mx_b(const MxArgs *args, const WorkItem *w, int nw, bool fem, size_t lds, int grid, BatchAux aux, hipStream_t st);

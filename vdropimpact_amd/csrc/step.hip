@@ -1770,7 +1770,7 @@ int vdi_synchronize(vdi_ctx *c) {
 }
 
 int vdi_time_kernels(vdi_ctx *c, int kclass, int max_launches) {
-    if (!c || max_launches < 0 || kclass < -1 || kclass > VDI_KCLASS_FEM_UP) return VDI_ERR_ARG;
+    if (!c || max_launches < 0 || kclass < -1 || kclass > VDI_KCLASS_XSOLVE) return VDI_ERR_ARG;
     VDI_HIP(hipStreamSynchronize(c->st));
     c->drop_graphs();  // captured graphs own slots of the previous pool
     VDI_TRY(c->timer.setup(kclass, max_launches));
