@@ -642,7 +642,7 @@ class MGSystem:
         _check(self.lib.vdi_mg_autotune(self.h, reps), "vdi_mg_autotune")
 
 
-NUM_TILE_SHAPES = 14
+NUM_TILE_SHAPES = 18
 
 __all__ = [
     "Fluid2D", "MGSystem", "Batch", "UNet", "tile_shape", "NUM_TILE_SHAPES", "VdiParams", "VdiStats", "VdiError", "PhysicalConfig",

@@ -634,8 +634,13 @@ int vdi_batch_create(const vdi_params *params, int count, int form, vdi_batch **
                 if (!solver_used(c, k)) continue;
                 MG &g = *step_solver(c, k);
                 if (l >= g.lc) continue;
+                // batched launches walk tiles: a streaming shape of a
+                // member's own tuning falls back to the default shape
                 for (int pr = 0; pr < 2; pr++)
-                    for (int w = 0; w < 2; w++) b->tiles[k][pr][w][l] = g.tile[pr][w][l];
+                    for (int w = 0; w < 2; w++) {
+                        const int sh = g.tile[pr][w][l];
+                        b->tiles[k][pr][w][l] = kTileShapes[sh].stream ? 0 : sh;
+                    }
                 b->tiles[k][1][2][0] = g.tile_mx;
                 break;
             }
@@ -771,6 +776,7 @@ int vdi_batch_autotune(vdi_batch *b, int reps) {
                     int best = -1;
                     float best_ms = 0;
                     for (int shape = 0; shape < kNumTileShapes; shape++) {
+                        if (kTileShapes[shape].stream) continue;  // batched launches walk tiles
                         // the shape must fit every simulation that has the level
                         bool fits = true;
                         for (vdi_ctx *c : b->sims) {

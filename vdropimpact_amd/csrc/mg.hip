@@ -44,9 +44,11 @@ namespace vdi {
 // node adds a column and a row of nearly empty tiles to the power-of-two
 // shapes: at 1025 x 513, 56 x 40 gives 247 tiles (one per CU) where 64 x 32
 // gives 289 and 64 x 64 gives 153; at 513 x 257, 28 x 20 gives 247.
-const TileShape kTileShapes[kNumTileShapes] = {{64, 32},  {32, 8},   {32, 16}, {64, 8},  {64, 16},
-                                               {32, 32},  {128, 16}, {128, 32}, {64, 64}, {56, 40},
-                                               {72, 36},  {48, 24},  {28, 20},  {36, 16}};
+// The last four are streaming launches (segments of 16 .. 128 rows).
+const TileShape kTileShapes[kNumTileShapes] = {{64, 32, 0},  {32, 8, 0},   {32, 16, 0},  {64, 8, 0},  {64, 16, 0},
+                                               {32, 32, 0},  {128, 16, 0}, {128, 32, 0}, {64, 64, 0}, {56, 40, 0},
+                                               {72, 36, 0},  {48, 24, 0},  {28, 20, 0},  {36, 16, 0}, {128, 16, 1},
+                                               {128, 32, 1}, {128, 64, 1}, {128, 128, 1}};
 
 // Position b of n work items -> item index such that the items of one XCD
 // (b mod 8) are contiguous.
@@ -189,6 +191,17 @@ __device__ __forceinline__ Tile make_tile(int bx, int by, int H, int TX, int TY)
     return t;
 }
 
+// Bilinear weights of a cell-centred prolongation: the coarse value q00 of
+// the fine point's own coarse cell, q10 / q01 / q11 of its neighbours across
+// x / y / both (read at boundary-clamped coarse points), fx and fy the ghost
+// factors of a neighbour outside the coarse level (1 inside).
+template <class T>
+__device__ __forceinline__ T cc_prolong_combine(T fx, T fy, T q00, T q10, T q01, T q11) {
+#pragma clang fp contract(off)
+    const T e00 = q00, e10 = fx * q10, e01 = fy * q01, e11 = fx * fy * q11;
+    return fma(T(0.0625), e11, fma(T(0.5625), e00, T(0.1875) * (e10 + e01)));
+}
+
 // Bilinear cell-centred prolongation of the coarse correction at fine (i,j).
 template <class T>
 __device__ __forceinline__ T cc_prolong_at(const CCOp &op, int Mc, int Nc, const T *e, int i, int j,
@@ -202,9 +215,7 @@ __device__ __forceinline__ T cc_prolong_at(const CCOp &op, int Mc, int Nc, const
     else if (I2 >= Mc) { fx = T(bc_ghost_factor(op.bcr)); I2 = I; }
     if (J2 < 0) { fy = T(bc_ghost_factor(op.bcb)); J2 = J; }
     else if (J2 >= Nc) { fy = T(bc_ghost_factor(op.bct)); J2 = J; }
-    T e00 = e[I + pitch * J], e10 = fx * e[I2 + pitch * J];
-    T e01 = fy * e[I + pitch * J2], e11 = fx * fy * e[I2 + pitch * J2];
-    return fma(T(0.0625), e11, fma(T(0.5625), e00, T(0.1875) * (e10 + e01)));
+    return cc_prolong_combine<T>(fx, fy, e[I + pitch * J], e[I2 + pitch * J], e[I + pitch * J2], e[I2 + pitch * J2]);
 }
 
 // Nodal bilinear prolongation at fine node (i,j), j >= 1.
@@ -805,6 +816,12 @@ __global__ void __launch_bounds__(kLvThreads) k_mx_up0_b(const MxArgs *A, const 
     }
     stamp_end(aux.ts);
 }
+
+}  // namespace vdi
+
+#include "mg_stream.hpp"
+
+namespace vdi {
 
 // ------------------------------------------------------------------------
 // The coarsest level on one workgroup.  The level sits in LDS as an
@@ -1549,14 +1566,29 @@ const void *mx_kernel(bool fem, bool b) {
                : (b ? (const void *)k_mx_up0_b<false> : (const void *)k_mx_up0<false>);
 }
 
+template <class T, template <class, int> class K>
+static void launch_stream(const LvArgs<T> &a, hipStream_t st) {
+    switch (a.nu) {
+        case 0: hipLaunchKernelGGL((K<T, 0>::fn), dim3(a.nb), dim3(kSThreads), 0, st, a); break;
+        case 1: hipLaunchKernelGGL((K<T, 1>::fn), dim3(a.nb), dim3(kSThreads), 0, st, a); break;
+        case 2: hipLaunchKernelGGL((K<T, 2>::fn), dim3(a.nb), dim3(kSThreads), 0, st, a); break;
+        case 3: hipLaunchKernelGGL((K<T, 3>::fn), dim3(a.nb), dim3(kSThreads), 0, st, a); break;
+        default: hipLaunchKernelGGL((K<T, 4>::fn), dim3(a.nb), dim3(kSThreads), 0, st, a); break;
+    }
+}
+template <class T, int NU> struct SDown { static constexpr auto fn = k_sdown<T, NU>; };
+template <class T, int NU> struct SUp { static constexpr auto fn = k_sup<T, NU>; };
+
 template <class T> int down(const LvArgs<T> &a, bool fem, size_t lds, hipStream_t st) {
-    if (fem) hipLaunchKernelGGL((k_down<T, true>), dim3(a.nb), dim3(kBX, kBY), lds, st, a);
+    if (a.items) launch_stream<T, SDown>(a, st);
+    else if (fem) hipLaunchKernelGGL((k_down<T, true>), dim3(a.nb), dim3(kBX, kBY), lds, st, a);
     else hipLaunchKernelGGL((k_down<T, false>), dim3(a.nb), dim3(kBX, kBY), lds, st, a);
     VDI_LAUNCHED();
     return VDI_OK;
 }
 template <class T> int up(const LvArgs<T> &a, bool fem, size_t lds, hipStream_t st) {
-    if (fem) hipLaunchKernelGGL((k_up<T, true>), dim3(a.nb), dim3(kBX, kBY), lds, st, a);
+    if (a.items) launch_stream<T, SUp>(a, st);
+    else if (fem) hipLaunchKernelGGL((k_up<T, true>), dim3(a.nb), dim3(kBX, kBY), lds, st, a);
     else hipLaunchKernelGGL((k_up<T, false>), dim3(a.nb), dim3(kBX, kBY), lds, st, a);
     VDI_LAUNCHED();
     return VDI_OK;
@@ -1968,16 +2000,29 @@ MGCtl MG::ctl() const {
 
 size_t MG::down_lds(int prec, int l) const {
     const TileShape &t = kTileShapes[tile[prec][0][l]];
+    if (t.stream) return 0;
     return tile_lds_bytes(halo_down(fem, l == 0 ? nu1 : nu1s), prec ? sizeof(float) : sizeof(double), t.tx, t.ty);
 }
 size_t MG::up_lds(int prec, int l) const {
     const TileShape &t = kTileShapes[tile[prec][1][l]];
+    if (t.stream) return 0;
     return tile_lds_bytes(halo_up(fem, l == 0 ? nu2 : nu2s, l == 0), prec ? sizeof(float) : sizeof(double), t.tx, t.ty);
 }
 size_t MG::mx_lds() const {
     const TileShape &t = kTileShapes[tile_mx];
     const int H = halo_up(fem, nu2, 1);
     return (size_t)(t.tx + 2 * H) * (t.ty + 2 * H) * (sizeof(double) + 2 * sizeof(float)) + kInvTab * sizeof(float);
+}
+
+// Streaming launch of a level with nu sweeps (a.nu), segments of a.ty rows:
+// strips of stream_wout columns, four wavefront items per workgroup.
+static int stream_wout(int nu) { return kSW - 2 * ((2 * nu + 2) & ~1); }
+
+template <class T>
+static void stream_geometry(LvArgs<T> &a) {
+    a.nbx = cdiv(a.F.m, stream_wout(a.nu));
+    a.items = a.nbx * cdiv(a.F.n, a.ty);
+    a.nb = cdiv(a.items, kSWaves);
 }
 
 template <class T>
@@ -1993,7 +2038,9 @@ LvArgs<T> MG::down_args(const std::vector<LevelT<T>> &H, int l, int zero_z) {
     const TileShape &t = kTileShapes[tile[prec][0][l]];
     a.tx = t.tx;
     a.ty = t.ty;
-    if (fem) {
+    if (t.stream) {
+        stream_geometry(a);
+    } else if (fem) {
         a.nbx = cdiv(a.C.m, t.tx / 2);
         a.nb = a.nbx * cdiv(a.C.n, t.ty / 2);
     } else {
@@ -2019,8 +2066,12 @@ LvArgs<T> MG::up_args(const std::vector<LevelT<T>> &H, int l, int norm, double t
     const TileShape &t = kTileShapes[tile[prec][1][l]];
     a.tx = t.tx;
     a.ty = t.ty;
-    a.nbx = cdiv(a.F.m, t.tx);
-    a.nb = a.nbx * cdiv(a.F.n, t.ty);
+    if (t.stream) {
+        stream_geometry(a);
+    } else {
+        a.nbx = cdiv(a.F.m, t.tx);
+        a.nb = a.nbx * cdiv(a.F.n, t.ty);
+    }
     a.ctl = ctl();
     a.tol2 = tol2;
     a.ts = nullptr;
@@ -2115,11 +2166,17 @@ NormArgs MG::norm_args(int begin, double tol2) {
     a.begin = begin;
     a.tol2 = tol2;
     if (begin) a.nb = begin_args(tol2).nb;
-    else {
-        const TileShape &t = kTileShapes[mixed_active() ? tile_mx : tile[0][1][0]];
-        a.nb = (int)(cdiv(lm[0], t.tx) * cdiv(ln[0], t.ty));
-    }
+    else a.nb = (int)up0_partials();
     return a;
+}
+
+// Partial sums of ||b - A z||^2 the level-0 up launch stores (one per
+// workgroup).
+long MG::up0_partials() const {
+    if (lc <= 0) return 0;
+    const TileShape &t = kTileShapes[mixed_active() ? tile_mx : tile[0][1][0]];
+    if (t.stream) return cdiv((long)cdiv(lm[0], stream_wout(nu2)) * cdiv(ln[0], t.ty), (long)kSWaves);
+    return (long)cdiv(lm[0], t.tx) * cdiv(ln[0], t.ty);
 }
 
 VecArgs MG::vec_args(int op, const double *coef) {
@@ -2240,8 +2297,7 @@ int MG::launch_cycle(double tol2) { return launch_cycle(tol2, 0, 1); }
 
 bool MG::can_fold_norm() const {
     if (lc <= 0) return false;
-    const TileShape &t = kTileShapes[mixed_active() ? tile_mx : tile[0][1][0]];
-    return (long)cdiv(lm[0], t.tx) * cdiv(ln[0], t.ty) <= kFinMax;
+    return up0_partials() <= kFinMax;
 }
 
 int MG::begin_folded(double tol) {
@@ -2477,6 +2533,13 @@ int MG::apply(const double *z, double *out) {
 
 int MG::set_tile(int prec, int which, int level, int shape) {
     if (prec < 0 || prec > 1 || which < 0 || which > 2 || shape < 0 || shape >= kNumTileShapes) return VDI_ERR_ARG;
+    // streaming launches: cell-centred down and up launches only, on levels
+    // whose arrays a buffer resource's 32-bit byte range covers
+    if (kTileShapes[shape].stream) {
+        if (fem || which == 2) return VDI_ERR_ARG;
+        if (level >= 0 && level < lc && (long)lm[level] * ln[level] * (prec ? 4 : 8) >= (1L << 31) - (1L << 20))
+            return VDI_ERR_ARG;
+    }
     if (which == 2) {
         if (level != 0) return VDI_ERR_ARG;
         const int old = tile_mx;

@@ -86,13 +86,17 @@ struct MGCtl {
     double *partial;  // one (or two) doubles per workgroup
 };
 
-// Output tile of a level kernel (cells or nodes); both even.
+// Output tile of a level kernel (cells or nodes); both even.  stream = 1:
+// a streaming launch (cell-centred levels only) -- one wavefront per strip
+// of tx = 128 columns and ty rows (mg_stream.hpp); its output strips are
+// narrower by the halo.
 struct TileShape {
     int tx, ty;
+    int stream;
 };
 
 // Tile shapes the autotuner chooses from.
-constexpr int kNumTileShapes = 14;
+constexpr int kNumTileShapes = 18;
 extern const TileShape kTileShapes[kNumTileShapes];
 
 // Down / up kernel of one level.
@@ -113,6 +117,10 @@ struct LvArgs {
     int fin;
     // 1: those partials are k_begin's (||b - A z||^2, ||b||^2) pairs
     int fin_begin;
+    // streaming launch (mg_stream.hpp): wavefront items (strips x segments);
+    // nbx is then the number of strips and nb the number of workgroups.
+    // 0: a tiled launch.
+    int items;
 };
 constexpr int kFinMax = 1024;
 
@@ -304,6 +312,7 @@ class MG {
     int begin_folded(double tol);
     // Whether a run of cycles can fold each cycle's norm into the next one.
     bool can_fold_norm() const;
+    long up0_partials() const;
     // Mixed form in use (needs at least one tiled level).
     bool mixed_active() const;
     int mixed_finish(int executed);

@@ -1,0 +1,482 @@
+// mg_stream.hpp -- streaming level kernels for large cell-centred levels.
+//
+// Included by mg.hip after the tiled level kernels, whose operator helpers
+// (cc_diag, cc_prolong_at) and norm bookkeeping (fin_load, fin_norm,
+// store_partial) these kernels share, so that they compute the tiled
+// kernels' bits: the same red-black Gauss-Seidel sweeps, residual,
+// restriction and prolongation as down_body / up_body, with every rounding
+// in the same order.
+//
+// Layout.  One wavefront owns a strip of kSW = 128 columns -- lane k holds
+// the even column c0 + 2k and the odd column c0 + 2k + 1 -- and a segment of
+// S rows, and walks up the rows of its strip.  The rows it is working on sit
+// in a rolling window of registers: z and b of both columns for the last
+// 2P + 4 rows and kSAhead rows loaded ahead.  A point's vertical neighbours
+// are in the lane's own registers; of its two horizontal neighbours one is
+// the lane's other column and the other comes from the next lane with one
+// DPP whole-wave shift (wave_shr:1 / wave_shl:1).  No LDS, no barriers.
+//
+// Schedule.  When row y has been loaded, pass p (p = 1..P, colour (p-1)&1)
+// runs on row y - 2p.  A point of row r in pass p then reads row r+1 after
+// pass p-1 (run one step earlier) and not yet pass p+1, and row r-1 after
+// pass p-1 and before pass p+1 -- exactly the values a global sweep gives --
+// and the P updates of one step are independent of each other.  The
+// residual, the stores and (down) the restriction run on row y - 2P - 2.
+// Each pass shrinks the region whose values equal the global sweep's by one
+// row and column at each end, so a strip carries a halo of P + 2 columns
+// (even, so that the restriction's 2 x 2 blocks lie within one lane) and a
+// segment a halo of P + 1 rows; the halos are recomputed by the neighbouring
+// wavefronts.  Points outside the level are zero and never updated, as in
+// the tiled kernels.
+#pragma once
+
+#include <utility>
+
+namespace vdi {
+
+constexpr int kSW = 128;       // columns of a strip (two per lane)
+constexpr int kSWaves = 4;     // independent wavefronts per workgroup
+constexpr int kSThreads = 64 * kSWaves;
+constexpr int kSAhead = 4;     // rows of z and b loaded ahead of the step's row
+constexpr int kSQAhead = 2;    // fine rows by which a coarse row (up launches) is loaded ahead
+
+// Lane k receives lane k-1's value (lane 0: zero) / lane k+1's (lane 63: zero).
+__device__ __forceinline__ int lane_from_left_i(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, 0x138 /* wave_shr:1 */, 0xf, 0xf, true);
+}
+__device__ __forceinline__ int lane_from_right_i(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, 0x130 /* wave_shl:1 */, 0xf, 0xf, true);
+}
+__device__ __forceinline__ float lane_from_left(float v) { return __int_as_float(lane_from_left_i(__float_as_int(v))); }
+__device__ __forceinline__ float lane_from_right(float v) { return __int_as_float(lane_from_right_i(__float_as_int(v))); }
+__device__ __forceinline__ double lane_from_left(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = lane_from_left_i((int)b), hi = lane_from_left_i((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double lane_from_right(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = lane_from_right_i((int)b), hi = lane_from_right_i((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Rolling window slot of the row k rows below the step's row, in a step
+// whose position in the unrolled window is U.
+template <int W>
+__device__ __forceinline__ constexpr int sslot(int U, int k) {
+    return ((U - k) % W + W) % W;
+}
+
+// Per-wavefront state of a streaming launch (level F, lane columns xa, xb).
+template <class T>
+struct SLane {
+    int xa, xb;          // the lane's even and odd column (may lie outside the level)
+    int ca, cb;          // the same clamped into the level (unconditional loads)
+    bool ina, inb;       // inside the level
+    bool outa, outb;     // inside the strip's output columns
+    // reciprocal diagonals and diagonals (residual) of the lane's columns
+    // for an interior (0), the first (1) and the last (2) row; scalars, so
+    // that a per-row choice stays a register select
+    T ia0, ia1, ia2, ib0, ib1, ib2;
+    T da0, da1, da2, db0, db1, db2;
+    // up launches: coarse column of both columns, the coarse columns across
+    // x of the even / odd one (clamped) and their ghost factors
+    int ci, cia, cib;
+    T fxa, fxb;
+};
+
+template <class T>
+__device__ __forceinline__ void slane_init(const LevelT<T> &F, const LevelT<T> &C, const CCOp &op, int x0, int wout,
+                                           int hh, SLane<T> &s) {
+    const int lane = threadIdx.x & 63;
+    s.xa = x0 - hh + 2 * lane;
+    s.xb = s.xa + 1;
+    s.ina = s.xa >= 0 && s.xa < F.m;
+    s.inb = s.xb >= 0 && s.xb < F.m;
+    s.outa = s.ina && s.xa >= x0 && s.xa < x0 + wout;
+    s.outb = s.inb && s.xb >= x0 && s.xb < x0 + wout;
+    s.ca = min(max(s.xa, 0), F.m - 1);
+    s.cb = min(max(s.xb, 0), F.m - 1);
+    // a row of each class (cc_inv_table's representatives); the interior
+    // point's reciprocal is the level's stored one
+    const int j0 = min(1, F.n - 1), j2 = F.n - 1;
+    s.da0 = cc_diag<T>(op, F.m, F.n, F.cx, F.cy, s.ca, j0);
+    s.da1 = cc_diag<T>(op, F.m, F.n, F.cx, F.cy, s.ca, 0);
+    s.da2 = cc_diag<T>(op, F.m, F.n, F.cx, F.cy, s.ca, j2);
+    s.db0 = cc_diag<T>(op, F.m, F.n, F.cx, F.cy, s.cb, j0);
+    s.db1 = cc_diag<T>(op, F.m, F.n, F.cx, F.cy, s.cb, 0);
+    s.db2 = cc_diag<T>(op, F.m, F.n, F.cx, F.cy, s.cb, j2);
+    s.ia0 = (s.ca > 0 && s.ca < F.m - 1) ? F.inv_int : T(1) / s.da0;
+    s.ib0 = (s.cb > 0 && s.cb < F.m - 1) ? F.inv_int : T(1) / s.db0;
+    s.ia1 = T(1) / s.da1;
+    s.ia2 = T(1) / s.da2;
+    s.ib1 = T(1) / s.db1;
+    s.ib2 = T(1) / s.db2;
+    // cc_prolong_at's coarse columns: for an even fine column 2I the
+    // neighbour across x is I - 1, for the odd one I + 1, each replaced by I
+    // with the side's ghost factor outside the coarse level
+    s.ci = min(s.ca >> 1, max(C.m - 1, 0));
+    s.cia = s.ci - 1;
+    s.cib = s.ci + 1;
+    s.fxa = T(1);
+    s.fxb = T(1);
+    if (s.cia < 0) { s.fxa = T(bc_ghost_factor(op.bcl)); s.cia = s.ci; }
+    if (s.cib >= C.m) { s.fxb = T(bc_ghost_factor(op.bcr)); s.cib = s.ci; }
+}
+
+// Row class: 0 interior, 1 first, 2 last row.  INT: the row is known to be
+// an interior one.
+template <bool INT, class T>
+__device__ __forceinline__ T by_row_class(T v0, T v1, T v2, int r, int n) {
+    if constexpr (INT) return v0;
+    else return r == 0 ? v1 : (r == n - 1 ? v2 : v0);
+}
+
+// Buffer access with range checking: a load at an offset outside the
+// buffer returns 0 and a store there is dropped (MUBUF out-of-range
+// behaviour), so points outside the level or the strip need no branch --
+// the level's zero padding and the output guards are offsets.  Every array
+// a streaming launch touches is below 2 GiB (MG::set_tile checks).
+using SRsrc = __amdgpu_buffer_rsrc_t;
+constexpr int kSOut = (int)0x80000000u;  // an offset outside every buffer
+
+__device__ __forceinline__ SRsrc srsrc(const void *p, long bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
+}
+typedef int sint2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double sload(double, SRsrc r, int off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ float sload(float, SRsrc r, int off) {
+    return __int_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ __forceinline__ void sstore(SRsrc r, int off, double v) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(sint2, v), r, off, 0, 0);
+}
+__device__ __forceinline__ void sstore(SRsrc r, int off, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_int(v), r, off, 0, 0);
+}
+
+// Buffers of one launch: the level's z source (z for down, zalt for up; a
+// zero-length buffer when the level starts from zero), b, the output (zalt
+// for down, z for up) and the coarse level's array (b for down, z for up).
+struct SBufs {
+    SRsrc zsrc, b, dst, coarse;
+};
+
+// The rolling window: z and b of both columns per row slot and, for up
+// launches, a ring of W / 2 coarse rows (the three coarse values a lane's
+// prolongation reads from each: its own coarse column and the neighbours
+// across x of its even and odd column).  A coarse row serves four fine rows
+// and is loaded once.
+template <class T, int W>
+struct SWin {
+    T za[W], zb[W], ba[W], bb[W];
+    T qc[W / 2], qa[W / 2], qb[W / 2];
+};
+
+// Coarse rows of cc_prolong_at for fine row y (inside the level): J and the
+// neighbour across y, J2, with its ghost factor.
+template <class T>
+__device__ __forceinline__ void sprolong_rows(const LevelT<T> &C, const CCOp &op, int y, int &J, int &J2, T &fy) {
+    J = y >> 1;
+    J2 = (y & 1) ? J + 1 : J - 1;
+    fy = T(1);
+    if (J2 < 0) { fy = T(bc_ghost_factor(op.bcb)); J2 = J; }
+    else if (J2 >= C.n) { fy = T(bc_ghost_factor(op.bct)); J2 = J; }
+}
+
+// Stage 1 of a row, kSAhead steps before it is needed: the loads of z and
+// b of row y into slot K, each returning zero for a point outside the level
+// or the segment's rows.  No value is used here, so the loads stay in
+// flight.
+template <class T, int W, int K>
+__device__ __forceinline__ void srow_issue(const LevelT<T> &F, const SBufs &bf, const SLane<T> &s, int y, int ylast,
+                                           SWin<T, W> &w) {
+    const bool row = y >= 0 && y < F.n && y <= ylast;
+    const int base = F.m * y;
+    const int oa = (row && s.ina) ? (int)sizeof(T) * (base + s.xa) : kSOut;
+    const int ob = (row && s.inb) ? (int)sizeof(T) * (base + s.xb) : kSOut;
+    w.za[K] = sload(T(), bf.zsrc, oa);
+    w.zb[K] = sload(T(), bf.zsrc, ob);
+    w.ba[K] = sload(T(), bf.b, oa);
+    w.bb[K] = sload(T(), bf.b, ob);
+}
+
+// Up launches: the three coarse values of coarse row J (clamped into the
+// coarse level) into ring slot K.
+template <class T, int W, int K>
+__device__ __forceinline__ void sq_issue(const LevelT<T> &C, const SBufs &bf, const SLane<T> &s, int J,
+                                         SWin<T, W> &w) {
+    const int r = C.m * min(max(J, 0), C.n - 1);
+    const bool in = s.ina || s.inb;
+    w.qc[K] = sload(T(), bf.coarse, in ? (int)sizeof(T) * (r + s.ci) : kSOut);
+    w.qa[K] = sload(T(), bf.coarse, in ? (int)sizeof(T) * (r + s.cia) : kSOut);
+    w.qb[K] = sload(T(), bf.coarse, in ? (int)sizeof(T) * (r + s.cib) : kSOut);
+}
+
+// Stage 2 (up launches), when row y reaches the step front: add the
+// prolongated correction from ring slots KJ (coarse row J = y >> 1) and KJ2
+// (J2 = J -+ 1; cc_prolong_at's ghost rule when J2 lies outside the coarse
+// level), and zero what lies outside the level or the segment's rows
+// (tile_load's z += cc_prolong_at, zero outside).
+template <class T, int W, int K, int KJ, int KJ2>
+__device__ __forceinline__ void srow_fix(const LevelT<T> &F, const LevelT<T> &C, const CCOp &op, const SLane<T> &s,
+                                         int y, int ylast, SWin<T, W> &w) {
+    const bool row = y >= 0 && y < F.n && y <= ylast;
+    int J, J2;
+    T fy;
+    sprolong_rows<T>(C, op, min(max(y, 0), F.n - 1), J, J2, fy);
+    const bool same = J2 == J;  // J2 outside the coarse level: row J with the ghost factor
+    const T c1 = same ? w.qc[KJ] : w.qc[KJ2], a1 = same ? w.qa[KJ] : w.qa[KJ2], b1 = same ? w.qb[KJ] : w.qb[KJ2];
+    const T va = w.za[K] + cc_prolong_combine<T>(s.fxa, fy, w.qc[KJ], w.qa[KJ], c1, a1);
+    const T vb = w.zb[K] + cc_prolong_combine<T>(s.fxb, fy, w.qc[KJ], w.qb[KJ], c1, b1);
+    w.za[K] = (row && s.ina) ? va : T(0);
+    w.zb[K] = (row && s.inb) ? vb : T(0);
+    // keep the sum here: sunk to the row's first use, it would keep coarse
+    // loads in flight (and their registers live) for longer
+    asm volatile("" : "+v"(w.za[K]), "+v"(w.zb[K]));
+}
+
+// Off-diagonal part of the cell-centred operator at the lane's even (A) or
+// odd (B) column of the row in slot c, rows below / above in slots d / u:
+// cc_off_t's rounding.
+template <class T>
+__device__ __forceinline__ T soff_a(T cx, T cy, const T *za, const T *zb, int c, int d, int u) {
+#pragma clang fp contract(off)
+    return fma(cx, lane_from_left(zb[c]) + zb[c], cy * (za[d] + za[u]));
+}
+template <class T>
+__device__ __forceinline__ T soff_b(T cx, T cy, const T *za, const T *zb, int c, int d, int u) {
+#pragma clang fp contract(off)
+    return fma(cx, za[c] + lane_from_right(za[c]), cy * (zb[d] + zb[u]));
+}
+
+// Pass p of one step (step position U in the unrolled window, step row y):
+// row y - LAG p, points of colour (p-1)&1, inside the level and inside the
+// region still exact after p passes (a register select, not a branch).  Y is
+// the parity of the step's row, known at compile time (segments start on
+// even rows, W is even), so the column a pass updates is too: a runtime
+// choice of za or zb would merge the two stores into one through a selected
+// pointer and keep the window out of registers.
+//   LAG = 2: a point of row r in pass p reads row r+1 after pass p-1 (run
+// one step earlier) and row r-1 after pass p-1 and before pass p+1, and the
+// P updates of a step are independent.  LAG = 1: row r+1's pass p-1 and row
+// r-1's pass p+1 fall in the same step, before and after pass p since the
+// passes of a step run in order; the window is P rows shorter, the updates
+// of a step a chain.
+template <bool INT, class T, int W, int U, int Y, int LAG, int p>
+__device__ __forceinline__ void spass(const LevelT<T> &F, const CCOp &op, const SLane<T> &s, int y, int ylo, int yhi,
+                                      SWin<T, W> &w) {
+#pragma clang fp contract(off)
+    constexpr int c = sslot<W>(U, LAG * p), d = sslot<W>(U, LAG * p + 1), u = sslot<W>(U, LAG * p - 1);
+    const int r = y - LAG * p;
+    const bool ok = r >= max(ylo + p, 0) && r <= min(yhi - p, F.n - 1);
+    const T sgn = T(op.s);
+    if constexpr (((Y - LAG * p) & 1) == ((p - 1) & 1)) {
+        const T rhs = fma(-sgn, soff_a<T>(F.cx, F.cy, w.za, w.zb, c, d, u), w.ba[c]);
+        const T v = rhs * by_row_class<INT, T>(s.ia0, s.ia1, s.ia2, r, F.n);
+        w.za[c] = (ok && s.ina) ? v : w.za[c];
+    } else {
+        const T rhs = fma(-sgn, soff_b<T>(F.cx, F.cy, w.za, w.zb, c, d, u), w.bb[c]);
+        const T v = rhs * by_row_class<INT, T>(s.ib0, s.ib1, s.ib2, r, F.n);
+        w.zb[c] = (ok && s.inb) ? v : w.zb[c];
+    }
+}
+
+// The P colour passes of one step (p = 1..P as template constants, so that
+// every window slot is a compile-time register).
+template <bool INT, class T, int W, int U, int Y, int LAG, int... Q>
+__device__ __forceinline__ void spasses(const LevelT<T> &F, const CCOp &op, const SLane<T> &s, int y, int ylo,
+                                        int yhi, SWin<T, W> &w, std::integer_sequence<int, Q...>) {
+    (spass<INT, T, W, U, Y, LAG, Q + 1>(F, op, s, y, ylo, yhi, w), ...);
+}
+
+// Residual b - A z at both columns of the row in slot c (cc_resid_t's
+// rounding).
+template <bool INT, class T, int W>
+__device__ __forceinline__ void sresid(const LevelT<T> &F, const CCOp &op, const SLane<T> &s, int r,
+                                       const SWin<T, W> &w, int c, int d, int u, T &ra, T &rb) {
+#pragma clang fp contract(off)
+    const T sgn = T(op.s);
+    ra = w.ba[c] - fma(sgn, soff_a<T>(F.cx, F.cy, w.za, w.zb, c, d, u),
+                       by_row_class<INT, T>(s.da0, s.da1, s.da2, r, F.n) * w.za[c]);
+    rb = w.bb[c] - fma(sgn, soff_b<T>(F.cx, F.cy, w.za, w.zb, c, d, u),
+                       by_row_class<INT, T>(s.db0, s.db1, s.db2, r, F.n) * w.zb[c]);
+}
+
+// The arithmetic of one step: the passes and the residual of row q in slot
+// c (rows below / above in d / u).
+template <bool INT, class T, int P, int LAG, int W, int U, int Y>
+__device__ __forceinline__ void scompute(const LevelT<T> &F, const CCOp &op, const SLane<T> &s, int y, int ylo,
+                                         int yhi, int q, int c, int d, int u, SWin<T, W> &w, T &ra, T &rb) {
+    spasses<INT, T, W, U, Y, LAG>(F, op, s, y, ylo, yhi, w, std::make_integer_sequence<int, P>{});
+    sresid<INT, T, W>(F, op, s, q, w, c, d, u, ra, rb);
+}
+
+// One step (row y, window position U, row parity Y), without branches
+// around memory operations:
+// issue the loads of row y + kSAhead, finish row y (up), run the passes,
+// then store z of row q = y - LAG P - 2 (zalt for down, z for up; dropped
+// outside the segment's output rows and the strip's output columns) and add
+// its residual to the restriction (down, stored after the odd row of each
+// pair) or to ||r||^2 (up with the norm).
+template <class T, int P, bool UP, int LAG, int W, int U, int Y>
+__device__ __forceinline__ void sstep(const LvArgs<T> &a, const SLane<T> &s, const SBufs &bf, int y, int ylo, int yhi,
+                                      int y0, int y1, int ck, int tb, SWin<T, W> &w, T &acc, double &rr) {
+#pragma clang fp contract(off)
+    const LevelT<T> &F = a.F, &C = a.C;
+    // a step is a scheduling region of its own: the loads stay kSAhead steps
+    // ahead (hoisting a window's loads to its top would keep every value of
+    // the window live at once)
+    __builtin_amdgcn_sched_barrier(0);
+    srow_issue<T, W, sslot<W>(U, -kSAhead)>(F, bf, s, y + kSAhead, yhi, w);
+    if constexpr (UP) {
+        // with the window's first row ys = 2k + 1, step t = tb + U finishes a
+        // row whose coarse rows are k + (t >> 1) and the next one (J and J2
+        // in the order of the row's parity); an even step loads coarse row
+        // k + (t + kSQAhead) / 2 + 1, first needed kSQAhead + 1 steps later
+        constexpr int R = W / 2, A = U >> 1;
+        if constexpr (!(U & 1))
+            sq_issue<T, W, ((U + kSQAhead) / 2 + 1) % R>(C, bf, s, ck + (tb + U + kSQAhead) / 2 + 1, w);
+        if constexpr (!(U & 1)) srow_fix<T, W, sslot<W>(U, 0), A % R, (A + 1) % R>(F, C, a.cop, s, y, yhi, w);
+        else srow_fix<T, W, sslot<W>(U, 0), (A + 1) % R, A % R>(F, C, a.cop, s, y, yhi, w);
+    }
+    // the passes and the residual: rows y - LAG P - 2 .. y - LAG; away from
+    // the first and last row of the level every point has the interior
+    // row coefficients (a branch without memory operations)
+    constexpr int c = sslot<W>(U, LAG * P + 2), d = sslot<W>(U, LAG * P + 3), u = sslot<W>(U, LAG * P + 1);
+    const int q = y - LAG * P - 2;
+    T ra, rb;
+    if (q > 0 && y - LAG < F.n - 1) scompute<true, T, P, LAG, W, U, Y>(F, a.cop, s, y, ylo, yhi, q, c, d, u, w, ra, rb);
+    else scompute<false, T, P, LAG, W, U, Y>(F, a.cop, s, y, ylo, yhi, q, c, d, u, w, ra, rb);
+    const bool qrow = q >= y0 && q < y1;
+    const int base = F.m * q;
+    const bool va = qrow && s.outa, vb = qrow && s.outb;
+    sstore(bf.dst, va ? (int)sizeof(T) * (base + s.xa) : kSOut, w.za[c]);
+    sstore(bf.dst, vb ? (int)sizeof(T) * (base + s.xb) : kSOut, w.zb[c]);
+    if constexpr (UP) {
+        const double da = va ? (double)ra : 0.0, db = vb ? (double)rb : 0.0;
+        rr += da * da;
+        rr += db * db;
+    } else if constexpr (!(Y & 1)) {
+        acc = T(0);
+        acc += ra;
+        acc += rb;
+    } else {
+        acc += ra;
+        acc += rb;
+        acc *= T(0.25);
+        sstore(bf.coarse, va ? (int)sizeof(T) * ((s.xa >> 1) + C.m * (q >> 1)) : kSOut, acc);
+    }
+}
+
+// W consecutive steps (U = 0..W-1), so that every window slot is a
+// compile-time register; the first row ys has parity YS.
+template <class T, int P, bool UP, int LAG, int W, int YS, int... U>
+__device__ __forceinline__ void sstep_window(const LvArgs<T> &a, const SLane<T> &s, const SBufs &bf, int ys, int tb,
+                                             int ylo, int yhi, int y0, int y1, int ck, SWin<T, W> &w, T &acc,
+                                             double &rr, std::integer_sequence<int, U...>) {
+    (sstep<T, P, UP, LAG, W, U, (YS + U) & 1>(a, s, bf, ys + tb + U, ylo, yhi, y0, y1, ck, tb, w, acc, rr), ...);
+}
+
+// Item (strip, segment) of wavefront w of workgroup b.
+// (readfirstlane: the item is the same for the whole wavefront, so every row
+// test below is scalar)
+__device__ __forceinline__ int stream_item(int nb) {
+    return __builtin_amdgcn_readfirstlane(kSWaves * xcd_tile(blockIdx.x, nb) + (int)(threadIdx.x >> 6));
+}
+
+// The rows loaded before the first step: z and b of rows ys..ys+kSAhead-1,
+// and (up) coarse rows ck .. ck + kSQAhead / 2.
+template <class T, int W, bool UP, int... K>
+__device__ __forceinline__ void sprologue(const LevelT<T> &F, const LevelT<T> &C, const SBufs &bf, const SLane<T> &s,
+                                          int ys, int ylast, int ck, SWin<T, W> &w, std::integer_sequence<int, K...>) {
+    (srow_issue<T, W, K>(F, bf, s, ys + K, ylast, w), ...);
+    if constexpr (UP) ((K <= kSQAhead / 2 ? sq_issue<T, W, K>(C, bf, s, ck + K, w) : void()), ...);
+}
+
+template <class T, int NU, bool UP>
+__device__ __forceinline__ void stream_body(const LvArgs<T> &a, int item, double &rr) {
+    constexpr int P = 2 * NU;
+    constexpr int HH = (P + 2) & ~1;   // columns of halo (even)
+    constexpr int HV = P + 1;          // rows of halo
+    constexpr int WOUT = kSW - 2 * HH;
+    // pass lag: down launches 2 (independent updates per step); up launches
+    // 1, whose window also holds the coarse values in flight
+    constexpr int LAG = UP ? 1 : 2;
+    constexpr int W = LAG * P + 4 + kSAhead;  // even: a window's rows keep their parities
+    static_assert(W % 2 == 0 && (P + 1) % 2 == 1, "even window, odd first row");
+    static_assert(kSQAhead % 2 == 0 && W / 2 >= kSQAhead / 2 + 3, "the coarse ring holds the rows in use");
+    const LevelT<T> &F = a.F, &C = a.C;
+    const int S = a.ty;
+    const int sx = item % a.nbx, sy = item / a.nbx;
+    const int x0 = sx * WOUT, y0 = sy * S, y1 = min(y0 + S, F.n);
+    SLane<T> s;
+    slane_init<T>(F, C, a.cop, x0, WOUT, HH, s);
+    const long nf = (long)F.m * F.n * sizeof(T), nc = (long)C.m * C.n * sizeof(T);
+    SBufs bf;
+    bf.zsrc = srsrc(UP ? F.zalt : F.z, (!UP && a.flag) ? 0 : nf);  // zero start: every z load is out of range
+    bf.b = srsrc(F.b, nf);
+    bf.dst = srsrc(UP ? F.z : F.zalt, nf);
+    bf.coarse = srsrc(UP ? C.z : C.b, nc);
+    SWin<T, W> w = {};
+    // rows ys..ylast are loaded (those inside the level); the region exact
+    // after pass p is [ys + p, ylast - p] clamped to the level
+    const int ys = y0 - HV, ylast = y0 + S + HV - 1;
+    const int ck = (ys - 1) >> 1;  // ys is odd: coarse row of the window's first row pair
+    sprologue<T, W, UP>(F, C, bf, s, ys, ylast, ck, w, std::make_integer_sequence<int, kSAhead>{});
+    // whole windows until the last output row has had its residual; the
+    // steps past it load nothing, update nothing and store nothing
+    const int nsteps = (y1 - 1) + LAG * P + 2 - ys + 1;
+    T acc = T(0);
+    for (int tb = 0; tb < nsteps; tb += W)
+        sstep_window<T, P, UP, LAG, W, HV & 1>(a, s, bf, ys, tb, ys, ylast, y0, y1, ck, w, acc, rr,
+                                          std::make_integer_sequence<int, W>{});
+}
+
+// Zero right-hand side found by the folded start-of-solve norm: z = 0 on the
+// wavefront's output region.
+template <class T, int NU>
+__device__ __forceinline__ void stream_zero(const LvArgs<T> &a, int item) {
+    constexpr int P = 2 * NU;
+    constexpr int WOUT = kSW - 2 * ((P + 2) & ~1);
+    const int sx = item % a.nbx, sy = item / a.nbx;
+    const int x0 = sx * WOUT, y0 = sy * a.ty;
+    const int lane = threadIdx.x & 63;
+    for (int y = y0; y < min(y0 + a.ty, a.F.n); y++)
+        for (int x = x0 + lane; x < min(x0 + WOUT, a.F.m); x += 64) a.F.z[x + (size_t)a.F.m * y] = T(0);
+}
+
+template <class T, int NU>
+__global__ void __launch_bounds__(kSThreads) k_sdown(LvArgs<T> a) {
+    __shared__ double sh[10];
+    if (a.ctl.state->done && !a.fin_begin) return;
+    stamp_begin(a.ts);
+    const int item = stream_item(a.nb);
+    double pf[2 * kFinRegs];
+    if (a.fin) {
+        fin_load(a.ctl, a.fin, a.fin_begin, pf);
+        const int nd = fin_norm(a.ctl, a.fin, a.fin_begin, a.tol2, pf, sh);
+        if (nd == 3 && a.fin_begin && sizeof(T) == sizeof(double) && item < a.items) stream_zero<T, NU>(a, item);
+        if (nd) return;
+    }
+    double rr = 0;
+    if (item < a.items) stream_body<T, NU, false>(a, item, rr);
+    stamp_end(a.ts);
+}
+
+template <class T, int NU>
+__global__ void __launch_bounds__(kSThreads) k_sup(LvArgs<T> a) {
+    __shared__ double sh[8];
+    if (a.ctl.state->done) return;
+    stamp_begin(a.ts);
+    const int bid = xcd_tile(blockIdx.x, a.nb);
+    const int item = stream_item(a.nb);
+    double rr = 0;
+    if (item < a.items) stream_body<T, NU, true>(a, item, rr);
+    if (a.flag) store_partial(a.ctl, rr, bid, sh);
+    stamp_end(a.ts);
+}
+
+}  // namespace vdi
