@@ -136,12 +136,23 @@ __device__ __forceinline__ T cc_off_t(T cx, T cy, const T *s, int l, int RX) {
 
 // 9-point Q1 stencil with missing neighbours zero: edge columns halve the
 // vertical coupling (hey), the top row halves the horizontal one (hex).
+// The rounding is fixed in the source (fem_off_v), like the cell-centred
+// formulas', so that the tiled and the streaming kernels agree bit for bit:
+// no fused multiply-adds, every product and sum rounded in the order of the
+// oracle's fem_off (oracle/vdo.c, built without contraction), which it then
+// matches bit for bit for the same iterate.
+template <class T>
+__device__ __forceinline__ T fem_off_v(const FemOp &op, T wi, T wj, T dl, T d, T dr, T l, T r, T ul, T u, T ur) {
+#pragma clang fp contract(off)
+    const T diag = ((dl + dr) + ul) + ur;
+    return (T(op.fc) * diag + (wi * T(op.fey)) * (d + u)) + (wj * T(op.fex)) * (l + r);
+}
 template <class T>
 __device__ __forceinline__ T fem_off_t(const FemOp &op, int M, int N, const T *s, int l, int RX, int i, int j) {
     const T wi = (i == 0 || i == M - 1) ? T(0.5) : T(1);
     const T wj = (j == N - 1) ? T(0.5) : T(1);
-    return T(op.fc) * (s[l - RX - 1] + s[l - RX + 1] + s[l + RX - 1] + s[l + RX + 1]) +
-           wi * T(op.fey) * (s[l - RX] + s[l + RX]) + wj * T(op.fex) * (s[l - 1] + s[l + 1]);
+    return fem_off_v<T>(op, wi, wj, s[l - RX - 1], s[l - RX], s[l - RX + 1], s[l - 1], s[l + 1], s[l + RX - 1],
+                        s[l + RX], s[l + RX + 1]);
 }
 
 // Boundary class of a cell-centred point: xc + 3 yc with xc, yc = 0 inside,
@@ -228,6 +239,19 @@ __device__ __forceinline__ T fem_prolong_at(const T *ez, int Mc, int i, int j) {
     return T(0.25) * (q[0] + q[1] + q[Mc] + q[Mc + 1]);
 }
 
+// Weights of the nodal bilinear prolongation at a fine node of column /
+// row parity io / jo from the coarse values q00 (own), q10 (next column),
+// q01 (next row), q11.  Power-of-two weights by parity: (1,0,0,0),
+// (1/2,1/2,0,0), (1/2,0,1/2,0) or (1/4,1/4,1/4,1/4).  Scaling by a power of
+// two is exact, so the weighted sum is bit for bit the parity formula (up to
+// the sign of a zero) whatever the compiler fuses.
+template <class T>
+__device__ __forceinline__ T fem_prolong_combine(int io, int jo, T q00, T q10, T q01, T q11) {
+    const T wx = io ? T(0.5) : T(1), wy = jo ? T(0.5) : T(1);
+    const T w10 = io ? wx * wy : T(0), w01 = jo ? wx * wy : T(0), w11 = (io && jo) ? T(0.25) : T(0);
+    return wx * wy * q00 + w10 * q10 + w01 * q01 + w11 * q11;
+}
+
 // Nodal bilinear prolongation at fine node (i,j) with every load made
 // unconditionally (the neighbours a parity does not use are clamped into the
 // level and ignored), so that a round of tile loads issues all its loads
@@ -238,13 +262,7 @@ __device__ __forceinline__ T fem_prolong_all(const T *ez, int Mc, int Nc, int i,
     const int I = i >> 1, J = j >> 1, I1 = min(I + 1, Mc - 1), J1 = min(J + 1, Nc - 1);
     const T q00 = ez[I + pitch * J], q10 = ez[I1 + pitch * J];
     const T q01 = ez[I + pitch * J1], q11 = ez[I1 + pitch * J1];
-    // Power-of-two weights by parity: (1,0,0,0), (1/2,1/2,0,0), (1/2,0,1/2,0)
-    // or (1/4,1/4,1/4,1/4).  Scaling by a power of two is exact, so the
-    // weighted sum is bit for bit the parity formula (up to the sign of a
-    // zero); written this way no load sits behind a branch.
-    const T wx = (i & 1) ? T(0.5) : T(1), wy = (j & 1) ? T(0.5) : T(1);
-    const T w10 = (i & 1) ? wx * wy : T(0), w01 = (j & 1) ? wx * wy : T(0), w11 = ((i & 1) && (j & 1)) ? T(0.25) : T(0);
-    return wx * wy * q00 + w10 * q10 + w01 * q01 + w11 * q11;
+    return fem_prolong_combine<T>(i & 1, j & 1, q00, q10, q01, q11);
 }
 
 // q = a / b, r = a % b for 0 <= a < 2^22, 1 <= b < 2^16, from the FP32
@@ -396,6 +414,7 @@ __device__ __forceinline__ T cc_resid_t(const LevelT<T> &L, const CCOp &op, cons
 template <class T>
 __device__ __forceinline__ T fem_resid_t(const FemOp &op, int M, int N, const T *sz, const T *sb, int RX, int l,
                                          int gx, int gy) {
+#pragma clang fp contract(off)
     const T acc = fem_acc<T>(op, M, N, gx, gy);
     if (gy == 0) return sb[l] - acc * sz[l];
     return sb[l] - (acc * sz[l] + fem_off_t<T>(op, M, N, sz, l, RX, gx, gy));
@@ -406,6 +425,7 @@ __device__ __forceinline__ T fem_resid_t(const FemOp &op, int M, int N, const T 
 template <class T>
 __device__ __forceinline__ T fem_resid_nd(const FemOp &op, int M, int N, const T *sz, const T *sb, int RX, int l,
                                           int gx, int gy) {
+#pragma clang fp contract(off)
     const T acc = fem_acc<T>(op, M, N, gx, gy);
     return sb[l] - (acc * sz[l] + fem_off_t<T>(op, M, N, sz, l, RX, gx, gy));
 }
@@ -1578,16 +1598,20 @@ static void launch_stream(const LvArgs<T> &a, hipStream_t st) {
 }
 template <class T, int NU> struct SDown { static constexpr auto fn = k_sdown<T, NU>; };
 template <class T, int NU> struct SUp { static constexpr auto fn = k_sup<T, NU>; };
+template <class T, int NU> struct SNDown { static constexpr auto fn = k_sndown<T, NU>; };
+template <class T, int NU> struct SNUp { static constexpr auto fn = k_snup<T, NU>; };
 
 template <class T> int down(const LvArgs<T> &a, bool fem, size_t lds, hipStream_t st) {
-    if (a.items) launch_stream<T, SDown>(a, st);
+    if (a.items && fem) launch_stream<T, SNDown>(a, st);
+    else if (a.items) launch_stream<T, SDown>(a, st);
     else if (fem) hipLaunchKernelGGL((k_down<T, true>), dim3(a.nb), dim3(kBX, kBY), lds, st, a);
     else hipLaunchKernelGGL((k_down<T, false>), dim3(a.nb), dim3(kBX, kBY), lds, st, a);
     VDI_LAUNCHED();
     return VDI_OK;
 }
 template <class T> int up(const LvArgs<T> &a, bool fem, size_t lds, hipStream_t st) {
-    if (a.items) launch_stream<T, SUp>(a, st);
+    if (a.items && fem) launch_stream<T, SNUp>(a, st);
+    else if (a.items) launch_stream<T, SUp>(a, st);
     else if (fem) hipLaunchKernelGGL((k_up<T, true>), dim3(a.nb), dim3(kBX, kBY), lds, st, a);
     else hipLaunchKernelGGL((k_up<T, false>), dim3(a.nb), dim3(kBX, kBY), lds, st, a);
     VDI_LAUNCHED();
@@ -2015,12 +2039,14 @@ size_t MG::mx_lds() const {
 }
 
 // Streaming launch of a level with nu sweeps (a.nu), segments of a.ty rows:
-// strips of stream_wout columns, four wavefront items per workgroup.
-static int stream_wout(int nu) { return kSW - 2 * ((2 * nu + 2) & ~1); }
+// strips of stream_wout columns (128 less the halo on both sides: 2 nu + 1
+// rounded up to even for the cell-centred levels, 4 nu + 2 for the nodal
+// ones), four wavefront items per workgroup.
+static int stream_wout(bool fem, int nu) { return kSW - 2 * (fem ? 4 * nu + 2 : (2 * nu + 2) & ~1); }
 
 template <class T>
-static void stream_geometry(LvArgs<T> &a) {
-    a.nbx = cdiv(a.F.m, stream_wout(a.nu));
+static void stream_geometry(LvArgs<T> &a, bool fem) {
+    a.nbx = cdiv(a.F.m, stream_wout(fem, a.nu));
     a.items = a.nbx * cdiv(a.F.n, a.ty);
     a.nb = cdiv(a.items, kSWaves);
 }
@@ -2039,7 +2065,7 @@ LvArgs<T> MG::down_args(const std::vector<LevelT<T>> &H, int l, int zero_z) {
     a.tx = t.tx;
     a.ty = t.ty;
     if (t.stream) {
-        stream_geometry(a);
+        stream_geometry(a, fem);
     } else if (fem) {
         a.nbx = cdiv(a.C.m, t.tx / 2);
         a.nb = a.nbx * cdiv(a.C.n, t.ty / 2);
@@ -2067,7 +2093,7 @@ LvArgs<T> MG::up_args(const std::vector<LevelT<T>> &H, int l, int norm, double t
     a.tx = t.tx;
     a.ty = t.ty;
     if (t.stream) {
-        stream_geometry(a);
+        stream_geometry(a, fem);
     } else {
         a.nbx = cdiv(a.F.m, t.tx);
         a.nb = a.nbx * cdiv(a.F.n, t.ty);
@@ -2175,7 +2201,7 @@ NormArgs MG::norm_args(int begin, double tol2) {
 long MG::up0_partials() const {
     if (lc <= 0) return 0;
     const TileShape &t = kTileShapes[mixed_active() ? tile_mx : tile[0][1][0]];
-    if (t.stream) return cdiv((long)cdiv(lm[0], stream_wout(nu2)) * cdiv(ln[0], t.ty), (long)kSWaves);
+    if (t.stream) return cdiv((long)cdiv(lm[0], stream_wout(fem, nu2)) * cdiv(ln[0], t.ty), (long)kSWaves);
     return (long)cdiv(lm[0], t.tx) * cdiv(ln[0], t.ty);
 }
 
@@ -2533,10 +2559,10 @@ int MG::apply(const double *z, double *out) {
 
 int MG::set_tile(int prec, int which, int level, int shape) {
     if (prec < 0 || prec > 1 || which < 0 || which > 2 || shape < 0 || shape >= kNumTileShapes) return VDI_ERR_ARG;
-    // streaming launches: cell-centred down and up launches only, on levels
-    // whose arrays a buffer resource's 32-bit byte range covers
+    // streaming launches: down and up launches, on levels whose arrays a
+    // buffer resource's 32-bit byte range covers
     if (kTileShapes[shape].stream) {
-        if (fem || which == 2) return VDI_ERR_ARG;
+        if (which == 2) return VDI_ERR_ARG;
         if (level >= 0 && level < lc && (long)lm[level] * ln[level] * (prec ? 4 : 8) >= (1L << 31) - (1L << 20))
             return VDI_ERR_ARG;
     }

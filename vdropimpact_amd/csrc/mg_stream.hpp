@@ -190,8 +190,8 @@ __device__ __forceinline__ void sprolong_rows(const LevelT<T> &C, const CCOp &op
 // b of row y into slot K, each returning zero for a point outside the level
 // or the segment's rows.  No value is used here, so the loads stay in
 // flight.
-template <class T, int W, int K>
-__device__ __forceinline__ void srow_issue(const LevelT<T> &F, const SBufs &bf, const SLane<T> &s, int y, int ylast,
+template <class T, int W, int K, class LN>
+__device__ __forceinline__ void srow_issue(const LevelT<T> &F, const SBufs &bf, const LN &s, int y, int ylast,
                                            SWin<T, W> &w) {
     const bool row = y >= 0 && y < F.n && y <= ylast;
     const int base = F.m * y;
@@ -436,16 +436,14 @@ __device__ __forceinline__ void stream_body(const LvArgs<T> &a, int item, double
 }
 
 // Zero right-hand side found by the folded start-of-solve norm: z = 0 on the
-// wavefront's output region.
-template <class T, int NU>
-__device__ __forceinline__ void stream_zero(const LvArgs<T> &a, int item) {
-    constexpr int P = 2 * NU;
-    constexpr int WOUT = kSW - 2 * ((P + 2) & ~1);
+// wavefront's output region (strips of wout columns).
+template <class T>
+__device__ __forceinline__ void stream_zero(const LvArgs<T> &a, int item, int wout) {
     const int sx = item % a.nbx, sy = item / a.nbx;
-    const int x0 = sx * WOUT, y0 = sy * a.ty;
+    const int x0 = sx * wout, y0 = sy * a.ty;
     const int lane = threadIdx.x & 63;
     for (int y = y0; y < min(y0 + a.ty, a.F.n); y++)
-        for (int x = x0 + lane; x < min(x0 + WOUT, a.F.m); x += 64) a.F.z[x + (size_t)a.F.m * y] = T(0);
+        for (int x = x0 + lane; x < min(x0 + wout, a.F.m); x += 64) a.F.z[x + (size_t)a.F.m * y] = T(0);
 }
 
 template <class T, int NU>
@@ -458,7 +456,7 @@ __global__ void __launch_bounds__(kSThreads) k_sdown(LvArgs<T> a) {
     if (a.fin) {
         fin_load(a.ctl, a.fin, a.fin_begin, pf);
         const int nd = fin_norm(a.ctl, a.fin, a.fin_begin, a.tol2, pf, sh);
-        if (nd == 3 && a.fin_begin && sizeof(T) == sizeof(double) && item < a.items) stream_zero<T, NU>(a, item);
+        if (nd == 3 && a.fin_begin && sizeof(T) == sizeof(double) && item < a.items) stream_zero<T>(a, item, kSW - 2 * ((2 * NU + 2) & ~1));
         if (nd) return;
     }
     double rr = 0;
@@ -475,6 +473,294 @@ __global__ void __launch_bounds__(kSThreads) k_sup(LvArgs<T> a) {
     const int item = stream_item(a.nb);
     double rr = 0;
     if (item < a.items) stream_body<T, NU, true>(a, item, rr);
+    if (a.flag) store_partial(a.ctl, rr, bid, sh);
+    stamp_end(a.ts);
+}
+
+// ------------------------------------------------------------------------
+// Nodal (FEM) levels
+// ------------------------------------------------------------------------
+// The same scheme for the four-colour Gauss-Seidel of the 9-point Q1
+// operator (fem_tile_sweep).  Pass p has colour (p-1)&3: columns of parity
+// (p-1)&1 -- the lane's even (A) or odd (B) column -- on rows of parity
+// ((p-1)>>1)&1; on a row of the other parity it updates nothing.  It runs one
+// row behind the previous pass (LAG 1); a point of row r then reads rows r-1
+// and r+1 between the passes a global sweep would give it, as for the
+// cell-centred levels.  An update reads the other column's three rows, one
+// of them from the neighbouring lane.  Row 0 is Dirichlet and never updated.
+// The halo is 4 nu + 2 rows and columns: each pass shrinks the exact region
+// by one, the residual by one more, and the full-weighting restriction
+// reads residuals one node beyond the output.
+
+template <class T>
+struct SNLane {
+    int xa, xb;            // the lane's even and odd column (may lie outside the level)
+    bool ina, inb, outa, outb;
+    T wia, wib;            // edge-column factor of the vertical coupling (fem_off_t's wi)
+    T ia0, ia2, ib0, ib2;  // fem_inv_acc for an interior row (0) and the last row (2)
+    T aa0, aa1, ab0, ab1;  // fem_acc for an interior row (0) and the first or last row (1)
+    T wla, wrb;            // restriction weight of the column left of A / of B (zero outside)
+    int ci, ci1;           // up launches: coarse columns I = xa >> 1 and min(I + 1, Mc - 1)
+};
+
+template <class T>
+__device__ __forceinline__ void snlane_init(const LevelT<T> &F, const LevelT<T> &C, int x0, int wout, int hh,
+                                            const FemOp &op, SNLane<T> &s) {
+    const int lane = threadIdx.x & 63;
+    s.xa = x0 - hh + 2 * lane;
+    s.xb = s.xa + 1;
+    s.ina = s.xa >= 0 && s.xa < F.m;
+    s.inb = s.xb >= 0 && s.xb < F.m;
+    s.outa = s.ina && s.xa >= x0 && s.xa < x0 + wout;
+    s.outb = s.inb && s.xb >= x0 && s.xb < x0 + wout;
+    const int ca = min(max(s.xa, 0), F.m - 1), cb = min(max(s.xb, 0), F.m - 1);
+    s.wia = (ca == 0 || ca == F.m - 1) ? T(0.5) : T(1);
+    s.wib = (cb == 0 || cb == F.m - 1) ? T(0.5) : T(1);
+    const int j0 = min(1, F.n - 1), j2 = F.n - 1;
+    s.ia0 = fem_inv_acc<T>(F, ca, j0);
+    s.ia2 = fem_inv_acc<T>(F, ca, j2);
+    s.ib0 = fem_inv_acc<T>(F, cb, j0);
+    s.ib2 = fem_inv_acc<T>(F, cb, j2);
+    s.aa0 = fem_acc<T>(op, F.m, F.n, ca, j0);
+    s.aa1 = fem_acc<T>(op, F.m, F.n, ca, 0);
+    s.ab0 = fem_acc<T>(op, F.m, F.n, cb, j0);
+    s.ab1 = fem_acc<T>(op, F.m, F.n, cb, 0);
+    s.wla = (s.xa - 1 < 0 || s.xa - 1 >= F.m) ? T(0) : T(0.5);
+    s.wrb = (s.xb >= F.m) ? T(0) : T(0.5);
+    s.ci = min(max(s.xa, 0) >> 1, max(C.m - 1, 0));
+    s.ci1 = min(s.ci + 1, max(C.m - 1, 0));
+}
+
+// Off-diagonal part of the nodal operator at the even (A) or odd (B) column
+// of the row in slot c (rows below / above in d / u), fem_off_t's rounding.
+template <class T>
+__device__ __forceinline__ T snoff_a(const FemOp &op, T wi, T wj, const T *za, const T *zb, int c, int d, int u) {
+    return fem_off_v<T>(op, wi, wj, lane_from_left(zb[d]), za[d], zb[d], lane_from_left(zb[c]), zb[c],
+                        lane_from_left(zb[u]), za[u], zb[u]);
+}
+template <class T>
+__device__ __forceinline__ T snoff_b(const FemOp &op, T wi, T wj, const T *za, const T *zb, int c, int d, int u) {
+    return fem_off_v<T>(op, wi, wj, za[d], zb[d], lane_from_right(za[d]), za[c], lane_from_right(za[c]), za[u], zb[u],
+                        lane_from_right(za[u]));
+}
+
+// Pass p of a step (row y - p, LAG 1).  Y: parity of the step's row.
+template <bool INT, class T, int W, int U, int Y, int p>
+__device__ __forceinline__ void snpass(const LevelT<T> &F, const FemOp &op, const SNLane<T> &s, int y, int ylo,
+                                       int yhi, SWin<T, W> &w) {
+#pragma clang fp contract(off)
+    constexpr int cj = ((p - 1) >> 1) & 1, ci = (p - 1) & 1;
+    if constexpr (((Y - p) & 1) == cj) {
+        constexpr int c = sslot<W>(U, p), d = sslot<W>(U, p + 1), u = sslot<W>(U, p - 1);
+        const int r = y - p;
+        const bool ok = r >= max(ylo + p, 1) && r <= min(yhi - p, F.n - 1);
+        const T wj = INT ? T(1) : (r == F.n - 1 ? T(0.5) : T(1));
+        if constexpr (ci == 0) {
+            const T v = (w.ba[c] - snoff_a<T>(op, s.wia, wj, w.za, w.zb, c, d, u)) *
+                        (INT ? s.ia0 : (r == F.n - 1 ? s.ia2 : s.ia0));
+            w.za[c] = (ok && s.ina) ? v : w.za[c];
+        } else {
+            const T v = (w.bb[c] - snoff_b<T>(op, s.wib, wj, w.za, w.zb, c, d, u)) *
+                        (INT ? s.ib0 : (r == F.n - 1 ? s.ib2 : s.ib0));
+            w.zb[c] = (ok && s.inb) ? v : w.zb[c];
+        }
+    }
+}
+
+template <bool INT, class T, int W, int U, int Y, int... Q>
+__device__ __forceinline__ void snpasses(const LevelT<T> &F, const FemOp &op, const SNLane<T> &s, int y, int ylo,
+                                         int yhi, SWin<T, W> &w, std::integer_sequence<int, Q...>) {
+    (snpass<INT, T, W, U, Y, Q + 1>(F, op, s, y, ylo, yhi, w), ...);
+}
+
+// Residual b - A z at both columns of row q in slot c (fem_resid_t's
+// rounding; row 0 has only the diagonal).
+template <bool INT, class T, int W>
+__device__ __forceinline__ void snresid(const LevelT<T> &F, const FemOp &op, const SNLane<T> &s, int q,
+                                        const SWin<T, W> &w, int c, int d, int u, T &ra, T &rb) {
+#pragma clang fp contract(off)
+    const bool edge = !INT && (q == 0 || q == F.n - 1);
+    const T wj = INT ? T(1) : (q == F.n - 1 ? T(0.5) : T(1));
+    const T acca = edge ? s.aa1 : s.aa0, accb = edge ? s.ab1 : s.ab0;
+    const T fa = w.ba[c] - (acca * w.za[c] + snoff_a<T>(op, s.wia, wj, w.za, w.zb, c, d, u));
+    const T fb = w.bb[c] - (accb * w.zb[c] + snoff_b<T>(op, s.wib, wj, w.za, w.zb, c, d, u));
+    ra = (!INT && q == 0) ? w.ba[c] - acca * w.za[c] : fa;
+    rb = (!INT && q == 0) ? w.bb[c] - accb * w.zb[c] : fb;
+}
+
+template <bool INT, class T, int P, int W, int U, int Y>
+__device__ __forceinline__ void sncompute(const LevelT<T> &F, const FemOp &op, const SNLane<T> &s, int y, int ylo,
+                                          int yhi, int q, int c, int d, int u, SWin<T, W> &w, T &ra, T &rb) {
+    snpasses<INT, T, W, U, Y>(F, op, s, y, ylo, yhi, w, std::make_integer_sequence<int, P>{});
+    snresid<INT, T, W>(F, op, s, q, w, c, d, u, ra, rb);
+}
+
+// Up launches: the two coarse values (columns I, I1) of coarse row J into
+// ring slot K.
+template <class T, int W, int K>
+__device__ __forceinline__ void snq_issue(const LevelT<T> &C, const SBufs &bf, const SNLane<T> &s, int J,
+                                          SWin<T, W> &w) {
+    const int r = C.m * min(max(J, 0), C.n - 1);
+    const bool in = s.ina || s.inb;
+    w.qc[K] = sload(T(), bf.coarse, in ? (int)sizeof(T) * (r + s.ci) : kSOut);
+    w.qa[K] = sload(T(), bf.coarse, in ? (int)sizeof(T) * (r + s.ci1) : kSOut);
+}
+
+// Up launches, when row y reaches the step front: z = zalt + the
+// prolongated correction (fem_prolong_all, none on row 0) from ring slots
+// KJ (coarse row J = y >> 1) and KJ1 (J + 1, or J itself on the last coarse
+// row), zero outside the level or the segment's rows.
+template <class T, int W, int K, int KJ, int KJ1>
+__device__ __forceinline__ void snrow_fix(const LevelT<T> &F, const LevelT<T> &C, const SNLane<T> &s, int y,
+                                          int ylast, SWin<T, W> &w) {
+    const bool row = y >= 0 && y < F.n && y <= ylast;
+    const bool same = min(max(y, 0), F.n - 1) / 2 + 1 > C.n - 1;
+    const T c1 = same ? w.qc[KJ] : w.qc[KJ1], a1 = same ? w.qa[KJ] : w.qa[KJ1];
+    const int jo = y & 1;
+    const T ea = fem_prolong_combine<T>(0, jo, w.qc[KJ], w.qa[KJ], c1, a1);
+    const T eb = fem_prolong_combine<T>(1, jo, w.qc[KJ], w.qa[KJ], c1, a1);
+    const T va = y > 0 ? w.za[K] + ea : w.za[K], vb = y > 0 ? w.zb[K] + eb : w.zb[K];
+    w.za[K] = (row && s.ina) ? va : T(0);
+    w.zb[K] = (row && s.inb) ? vb : T(0);
+    asm volatile("" : "+v"(w.za[K]), "+v"(w.zb[K]));
+}
+
+// One nodal step (row y, window position U, row parity Y): loads of row
+// y + kSAhead, finish row y (up), passes, residual of row q = y - P - 2;
+// down: zalt of row q for the segment's output rows [y0, y1), and the
+// full-weighting restriction (coarse row J collects rows 2J-1, 2J, 2J+1
+// in that order, weights 1/2, 1, 1/2, zero outside the level; coarse row 0
+// is zero) over the residual rows y0 - 1 .. y0 + S - 1; up: z of row q and
+// ||r||^2.
+template <class T, int P, bool UP, int W, int U, int Y>
+__device__ __forceinline__ void snstep(const LvArgs<T> &a, const SNLane<T> &s, const SBufs &bf, int y, int ylo,
+                                       int yhi, int y0, int y1, int ck, int tb, SWin<T, W> &w, T &acc, T &acc_next,
+                                       double &rr) {
+#pragma clang fp contract(off)
+    const LevelT<T> &F = a.F, &C = a.C;
+    __builtin_amdgcn_sched_barrier(0);
+    srow_issue<T, W, sslot<W>(U, -kSAhead)>(F, bf, s, y + kSAhead, yhi, w);
+    if constexpr (UP) {
+        constexpr int R = W / 2, A = U >> 1;
+        if constexpr (!(U & 1))
+            snq_issue<T, W, ((U + kSQAhead) / 2 + 1) % R>(C, bf, s, ck + (tb + U + kSQAhead) / 2 + 1, w);
+        snrow_fix<T, W, sslot<W>(U, 0), A % R, (A + 1) % R>(F, C, s, y, yhi, w);
+    }
+    constexpr int c = sslot<W>(U, P + 2), d = sslot<W>(U, P + 3), u = sslot<W>(U, P + 1);
+    const int q = y - P - 2;
+    T ra, rb;
+    if (q > 0 && y - 1 < F.n - 1) sncompute<true, T, P, W, U, Y>(F, a.fop, s, y, ylo, yhi, q, c, d, u, w, ra, rb);
+    else sncompute<false, T, P, W, U, Y>(F, a.fop, s, y, ylo, yhi, q, c, d, u, w, ra, rb);
+    const bool qrow = q >= y0 && q < y1;
+    const int base = F.m * q;
+    const bool va = qrow && s.outa, vb = qrow && s.outb;
+    sstore(bf.dst, va ? (int)sizeof(T) * (base + s.xa) : kSOut, w.za[c]);
+    sstore(bf.dst, vb ? (int)sizeof(T) * (base + s.xb) : kSOut, w.zb[c]);
+    if constexpr (UP) {
+        const double da = va ? (double)ra : 0.0, db = vb ? (double)rb : 0.0;
+        rr += da * da;
+        rr += db * db;
+    } else {
+        // restrict_at's row sum: columns 2I-1 (the left lane's odd one), 2I, 2I+1
+        T sx = T(0);
+        sx += s.wla * lane_from_left(rb);
+        sx += ra;
+        sx += s.wrb * rb;
+        const T wrow = q >= F.n ? T(0) : T(0.5);
+        if constexpr (!(Y & 1)) {
+            acc += (q >= F.n ? T(0) : T(1)) * sx;  // row 2J
+        } else {
+            acc += wrow * sx;  // row 2J + 1 closes coarse row J ...
+            const int J = q >> 1;
+            const bool st = J >= (y0 >> 1) && J < min((y0 + a.ty) >> 1, C.n) && s.outa;
+            sstore(bf.coarse, st ? (int)sizeof(T) * ((s.xa >> 1) + C.m * J) : kSOut, J == 0 ? T(0) : acc);
+            acc = T(0);  // ... and opens coarse row J + 1
+            acc += wrow * sx;
+        }
+    }
+    (void)acc_next;
+}
+
+// The rows loaded before the first step: z and b of rows ys..ys+kSAhead-1,
+// and (up) coarse rows ck .. ck + kSQAhead / 2.
+template <class T, int W, bool UP, int... K>
+__device__ __forceinline__ void snprologue(const LevelT<T> &F, const LevelT<T> &C, const SBufs &bf,
+                                           const SNLane<T> &s, int ys, int ylast, int ck, SWin<T, W> &w,
+                                           std::integer_sequence<int, K...>) {
+    (srow_issue<T, W, K>(F, bf, s, ys + K, ylast, w), ...);
+    if constexpr (UP) ((K <= kSQAhead / 2 ? snq_issue<T, W, K>(C, bf, s, ck + K, w) : void()), ...);
+}
+
+template <class T, int P, bool UP, int W, int YS, int... U>
+__device__ __forceinline__ void snstep_window(const LvArgs<T> &a, const SNLane<T> &s, const SBufs &bf, int ys, int tb,
+                                              int ylo, int yhi, int y0, int y1, int ck, SWin<T, W> &w, T &acc,
+                                              double &rr, std::integer_sequence<int, U...>) {
+    T dummy = T(0);
+    (snstep<T, P, UP, W, U, (YS + U) & 1>(a, s, bf, ys + tb + U, ylo, yhi, y0, y1, ck, tb, w, acc, dummy, rr), ...);
+}
+
+template <class T, int NU, bool UP>
+__device__ __forceinline__ void nstream_body(const LvArgs<T> &a, int item, double &rr) {
+    constexpr int P = 4 * NU;
+    constexpr int HH = 4 * NU + 2;  // columns and rows of halo (even)
+    constexpr int WOUT = kSW - 2 * HH;
+    constexpr int W = P + 4 + kSAhead;
+    static_assert(W % 2 == 0 && HH % 2 == 0, "even window, even first row");
+    static_assert(kSQAhead % 2 == 0 && W / 2 >= kSQAhead / 2 + 3, "the coarse ring holds the rows in use");
+    const LevelT<T> &F = a.F, &C = a.C;
+    const int S = a.ty;
+    const int sx = item % a.nbx, sy = item / a.nbx;
+    const int x0 = sx * WOUT, y0 = sy * S, y1 = min(y0 + S, F.n);
+    SNLane<T> s;
+    snlane_init<T>(F, C, x0, WOUT, HH, a.fop, s);
+    const long nf = (long)F.m * F.n * sizeof(T), nc = (long)C.m * C.n * sizeof(T);
+    SBufs bf;
+    bf.zsrc = srsrc(UP ? F.zalt : F.z, (!UP && a.flag) ? 0 : nf);
+    bf.b = srsrc(F.b, nf);
+    bf.dst = srsrc(UP ? F.z : F.zalt, nf);
+    bf.coarse = srsrc(UP ? C.z : C.b, nc);
+    SWin<T, W> w = {};
+    const int ys = y0 - HH, ylast = y0 + S + HH - 1;
+    const int ck = ys >> 1;  // ys is even: coarse row of the window's first row
+    snprologue<T, W, UP>(F, C, bf, s, ys, ylast, ck, w, std::make_integer_sequence<int, kSAhead>{});
+    // residual rows up to the segment's last output row (down: row 2J + 1
+    // of its last coarse row J, outside the level for the top one)
+    const int qlast = UP ? y1 - 1 : 2 * min((y0 + S) >> 1, C.n) - 1;
+    const int nsteps = qlast + P + 2 - ys + 1;
+    T acc = T(0);
+    for (int tb = 0; tb < nsteps; tb += W)
+        snstep_window<T, P, UP, W, 0>(a, s, bf, ys, tb, ys, ylast, y0, y1, ck, w, acc, rr,
+                                      std::make_integer_sequence<int, W>{});
+}
+
+template <class T, int NU>
+__global__ void __launch_bounds__(kSThreads) k_sndown(LvArgs<T> a) {
+    __shared__ double sh[10];
+    if (a.ctl.state->done && !a.fin_begin) return;
+    stamp_begin(a.ts);
+    const int item = stream_item(a.nb);
+    double pf[2 * kFinRegs];
+    if (a.fin) {
+        fin_load(a.ctl, a.fin, a.fin_begin, pf);
+        const int nd = fin_norm(a.ctl, a.fin, a.fin_begin, a.tol2, pf, sh);
+        if (nd == 3 && a.fin_begin && sizeof(T) == sizeof(double) && item < a.items)
+            stream_zero<T>(a, item, kSW - 2 * (4 * NU + 2));
+        if (nd) return;
+    }
+    double rr = 0;
+    if (item < a.items) nstream_body<T, NU, false>(a, item, rr);
+    stamp_end(a.ts);
+}
+
+template <class T, int NU>
+__global__ void __launch_bounds__(kSThreads) k_snup(LvArgs<T> a) {
+    __shared__ double sh[8];
+    if (a.ctl.state->done) return;
+    stamp_begin(a.ts);
+    const int bid = xcd_tile(blockIdx.x, a.nb);
+    const int item = stream_item(a.nb);
+    double rr = 0;
+    if (item < a.items) nstream_body<T, NU, true>(a, item, rr);
     if (a.flag) store_partial(a.ctl, rr, bid, sh);
     stamp_end(a.ts);
 }
