@@ -103,7 +103,10 @@ def operator_level(vdi, reps=3):
     """Operator-level solves (projection and Helmholtz systems alone): the
     64-mode Fourier inputs, every solve from zero to relative residual 1e-10,
     FP64 at 1024x1024, 2048x1024 and 4096x2048, mixed FP32/FP64 at 2048x1024
-    and 4096x2048.  GB/s = V-cycles x algorithmic bytes per V-cycle / time.
+    and 4096x2048.  GB/s = V-cycles x algorithmic bytes per V-cycle / time;
+    level0_down / level0_up: the level-0 launches alone, algorithmic bytes
+    per launch over their device-clock duration (first workgroup start to
+    last workgroup end).
     Then the advection passes alone on the same field at 1024x1024 and
     2048x1024 (GB/s = algorithmic bytes per launch / launch time)."""
     import numpy as np
@@ -115,6 +118,7 @@ def operator_level(vdi, reps=3):
     names = {0: "mac", 1: "fem", 2: "helmholtz"}
     for form, m, n in cases:
         for kind in (0, 1, 2):
+            log(f"operator level: {names[kind]} {m}x{n} {'fp64' if form == 0 else 'mixed'}")
             M, N, cx, cy, b = workloads.operator_case(kind, m, n)
             g = vdi.MGSystem(kind, M, N, cx, cy, 1)
             g.set_form(form)
@@ -131,8 +135,20 @@ def operator_level(vdi, reps=3):
                 times.append(time.perf_counter() - t0)
             t = min(times)
             gbs = cyc * g.cycle_bytes() / t / 1e9
-            out.append({"system": names[kind], "grid": [m, n], "form": "fp64" if form == 0 else "mixed",
-                        "vcycles": cyc, "ms_per_solve": 1e3 * t, "GB_per_s": gbs, "frac_of_8TBps": gbs / PEAK_HBM})
+            rec = {"system": names[kind], "grid": [m, n], "form": "fp64" if form == 0 else "mixed",
+                   "vcycles": cyc, "ms_per_solve": 1e3 * t, "GB_per_s": gbs, "frac_of_8TBps": gbs / PEAK_HBM}
+            # the level-0 launches alone: device-clock stamps over 5 V-cycles
+            for cls, key in ((vdi.KCLASS_MG_DOWN0, "level0_down"), (vdi.KCLASS_MG_UP0, "level0_up")):
+                g.time_kernels(cls, 64)
+                g.set_z(zero)
+                g.vcycles(5)
+                nl, ms, by = g.kernel_times()
+                g.time_kernels(-1, 0)
+                if nl:
+                    lg = by / (ms * 1e-3) / 1e9
+                    rec[key] = {"us_per_launch": 1e3 * ms / nl, "alg_MB_per_launch": by / nl / 1e6,
+                                "GB_per_s": lg, "frac_of_8TBps": lg / PEAK_HBM}
+            out.append(rec)
             g.close()
     for m, n in ((1024, 1024), (2048, 1024)):
         p = VdiParams(m=m, n=n, x_sym=1, implicit_visc=1, ax=0.0, bx=1.0, ay=0.0, by=n / m, rho=1.0, mu=1e-3,
@@ -206,6 +222,13 @@ def run_plain(vdi, args, p):
         sim.autotune(args.autotune)
         steps(3)
     log(f"plain: {args.warmup} warm-up steps, tiles tuned, schedules {schedules}")
+    # the FP64 level launches the tuner chose, per solver and level (tile or
+    # streaming launch)
+    tiles = {}
+    for k, name in ((vdi.MAC, "mac"), (vdi.FEM, "fem"), (vdi.VX, "vx"), (vdi.VY, "vy")):
+        idx = sim.tile_indices(k)
+        lc = (len(idx) - 1) // 4
+        tiles[name] = {"down": [vdi.tile_label(i) for i in idx[:lc]], "up": [vdi.tile_label(i) for i in idx[lc:2 * lc]]}
     # Which kernel function dominates the step: every launch of one function
     # (all levels) is timed by the kernels' own wall-clock stamps over a few
     # steps; the level classes are surveyed too.
@@ -259,6 +282,7 @@ def run_plain(vdi, args, p):
         "height_passes_last_step": height_passes,
         "budgets_mac_vx_vy_fem": {"before": b0, "after": budgets},
         "schedules": schedules,
+        "tiles_fp64": tiles,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM, "traffic": traffic,
                      "kernel": names[dominant], "launches": launches,
@@ -406,6 +430,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many doubles (counter calibration) and exit")
+    ap.add_argument("--operator-only", action="store_true",
+                    help="only run the operator-level solves and print their records (profiling) and exit")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -418,6 +444,9 @@ def main():
 
     if args.copy_probe > 0:
         print(json.dumps(copy_probe(vdi, args.copy_probe, 10)), flush=True)
+        return
+    if args.operator_only:
+        print(json.dumps({"operator_level": operator_level(vdi)}), flush=True)
         return
     p = drop_impact_params(args.m, args.n)
     plain = run_plain(vdi, args, p)
@@ -442,6 +471,7 @@ def main():
                    "captured_steps": plain["captured_steps"], "redone_steps": plain["redone_steps"],
                    "gas_height_passes_last_step": plain["height_passes_last_step"],
                    "vcycle_schedule_mac_fem_vx_vy": plain["schedules"],
+                   "level_launches_fp64": plain["tiles_fp64"],
                    "vcycle_budgets_mac_vx_vy_fem": plain["budgets_mac_vx_vy_fem"],
                    "concurrent_branches": not args.serial_visco,
                    "steps_per_s_with_kernel_stamps": plain["steps_per_s_stamped"]},

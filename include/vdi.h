@@ -96,6 +96,10 @@ typedef struct vdi_stats {
 #define VDI_KCLASS_FEM_DOWN 8   /* k_down<double, true>: nodal smooth + restrict */
 #define VDI_KCLASS_FEM_UP 9     /* k_up<double, true>: nodal prolong + smooth */
 #define VDI_KCLASS_XSOLVE 10    /* k_xsolve<double>: the three phases of a larger coarsest level's exact solve */
+#define VDI_KCLASS_SCC_DOWN 11  /* k_sdown<double, nu>: streaming cell-centred smooth + restrict */
+#define VDI_KCLASS_SCC_UP 12    /* k_sup<double, nu>: streaming cell-centred prolong + smooth */
+#define VDI_KCLASS_SFEM_DOWN 13 /* k_sndown<double, nu>: streaming nodal smooth + restrict */
+#define VDI_KCLASS_SFEM_UP 14   /* k_snup<double, nu>: streaming nodal prolong + smooth */
 
 typedef struct vdi_ctx vdi_ctx;
 typedef struct vdi_mg vdi_mg;
@@ -261,14 +265,26 @@ int vdi_mg_solve_noalloc(int kind, int m, int n, double cx, double cy, int x_sym
 
 /* ---- tile shapes of the multigrid level kernels ----
  * No counterpart in the reference (TGMG's sweeps are not tiled).  Shapes are
- * indexes into a fixed candidate list (vdi_tile_shape gives tx, ty).
+ * indexes into a fixed candidate list (vdi_tile_shape gives tx, ty; the last
+ * four are streaming launches, one wavefront per strip of tx = 128 columns
+ * less the halo and ty rows, for down and up launches).
  * prec 0 = FP64 hierarchy, 1 = FP32 hierarchy of the mixed form; which 0 =
  * down (smooth + restrict), 1 = up (prolong + smooth), 2 = mixed level-0 up. */
 int vdi_tile_shape(int shape, int *tx, int *ty);
+/* 1 when `shape` is a streaming launch, 0 when it is a tile, negative when
+ * out of range (no reference counterpart). */
+int vdi_tile_stream(int shape);
 int vdi_mg_set_tile(vdi_mg *g, int prec, int which, int level, int shape);
 /* Times every candidate for every level kernel on the system's own arrays and
  * keeps the fastest (z and b are restored). */
 int vdi_mg_autotune(vdi_mg *g, int reps);
+/* Measurement (no reference counterpart): vdi_time_kernels and
+ * vdi_kernel_times for a standalone system -- every launch of one kernel
+ * class (VDI_KCLASS_*, -1 to stop) is stamped with the device clock, up to
+ * max_launches launches; the totals cover the launches since the call that
+ * started the timing. */
+int vdi_mg_time_kernels(vdi_mg *g, int kclass, int max_launches);
+int vdi_mg_kernel_times(vdi_mg *g, long *launches, double *total_ms, double *alg_bytes);
 /* Same for the four solvers of a context. */
 int vdi_autotune(vdi_ctx *ctx, int reps);
 /* Writes the chosen shapes [prec][which][level] of solver `which_solver`

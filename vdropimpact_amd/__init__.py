@@ -36,12 +36,14 @@ PASS_GODUNOV, PASS_CORRECT_ADVECT = 0, 1
 # kernel classes for Fluid2D.time_kernels
 KCLASS_MG_DOWN0, KCLASS_MG_UP0, KCLASS_MG_MID, KCLASS_MG_COARSE, KCLASS_GAS, KCLASS_SIMPSON = range(6)
 KCLASS_CC_DOWN, KCLASS_CC_UP, KCLASS_FEM_DOWN, KCLASS_FEM_UP, KCLASS_XSOLVE = range(6, 11)
+KCLASS_SCC_DOWN, KCLASS_SCC_UP, KCLASS_SFEM_DOWN, KCLASS_SFEM_UP = range(11, 15)
 KCLASS_NAMES = {0: "k_cc_down/k_fem_down (level 0)", 1: "k_cc_up/k_fem_up (level 0)", 2: "level kernels 1..lc-1",
                 3: "k_coarse", 4: "k_gas", 5: "k_simpson"}
 # the same launches by kernel function, as rocprofv3 names them
 KFUNC_NAMES = {6: "vdi::k_down<double, false>", 7: "vdi::k_up<double, false>", 8: "vdi::k_down<double, true>",
                9: "vdi::k_up<double, true>", 3: "vdi::k_coarse<double>", 10: "vdi::k_xsolve<double>",
-               4: "vdi::k_gas", 5: "vdi::k_simpson"}
+               4: "vdi::k_gas", 5: "vdi::k_simpson", 11: "vdi::k_sdown<double, nu>", 12: "vdi::k_sup<double, nu>",
+               13: "vdi::k_sndown<double, nu>", 14: "vdi::k_snup<double, nu>"}
 # array ids
 (U, V, P, UL, VL, UD, VD, C0, C1, SRC, GP, GTAU, GP_OLD, GTAU_OLD, H, HX, HT, FBD, PG, PG_OLD,
  Z_MAC, Z_FEM, Z_VX, Z_VY) = range(24)
@@ -121,8 +123,11 @@ ABI = {
     "vdi_mg_synchronize": (_i, [_vp]),
     "vdi_mg_cycle_bytes": (_i, [_vp, _dp]),
     "vdi_tile_shape": (_i, [_i, _ip, _ip]),
+    "vdi_tile_stream": (_i, [_i]),
     "vdi_mg_set_tile": (_i, [_vp, _i, _i, _i, _i]),
     "vdi_mg_autotune": (_i, [_vp, _i]),
+    "vdi_mg_time_kernels": (_i, [_vp, _i, _i]),
+    "vdi_mg_kernel_times": (_i, [_vp, ctypes.POINTER(_l), _dp, _dp]),
     "vdi_autotune": (_i, [_vp, _i]),
     "vdi_get_tiles": (_i, [_vp, _i, _ip, _i]),
     "vdi_batch_create": (_i, [ctypes.POINTER(VdiParams), _i, _i, ctypes.POINTER(_vp)]),
@@ -364,13 +369,17 @@ class Fluid2D:
         """Picks the fastest tile shape of every multigrid level kernel."""
         _check(self.lib.vdi_autotune(self.h, reps), "autotune")
 
-    def tiles(self, which: int) -> list:
-        """Chosen tile shapes (tx, ty) of solver `which`: [prec][down/up][level] then the mixed level-0 up."""
+    def tile_indices(self, which: int) -> list:
+        """Chosen tile-shape indices of solver `which`: [prec][down/up][level] then the mixed level-0 up."""
         arr = (ctypes.c_int * 256)()
         n = self.lib.vdi_get_tiles(self.h, which, arr, 256)
         if n < 0:
             raise VdiError(-n, "get_tiles")
-        return [tile_shape(v) for v in list(arr)[:n]]
+        return list(arr)[:n]
+
+    def tiles(self, which: int) -> list:
+        """Chosen tile shapes (tx, ty) of solver `which`: [prec][down/up][level] then the mixed level-0 up."""
+        return [tile_shape(v) for v in self.tile_indices(which)]
 
 
     # ---- learned initial guess of the MAC solve ----
@@ -456,6 +465,12 @@ def tile_shape(index: int):
     tx, ty = ctypes.c_int(), ctypes.c_int()
     _check(lib.vdi_tile_shape(index, ctypes.byref(tx), ctypes.byref(ty)), "tile_shape")
     return tx.value, ty.value
+
+
+def tile_label(index: int) -> str:
+    """'64x32' for a tile, 'stream/16' for a streaming launch of 16-row segments."""
+    tx, ty = tile_shape(index)
+    return f"stream/{ty}" if load_library().vdi_tile_stream(index) == 1 else f"{tx}x{ty}"
 
 
 class Batch:
@@ -641,10 +656,22 @@ class MGSystem:
     def autotune(self, reps: int = 5):
         _check(self.lib.vdi_mg_autotune(self.h, reps), "vdi_mg_autotune")
 
+    def time_kernels(self, kclass: int, max_launches: int = 1024):
+        """Stamps every launch of one kernel class (KCLASS_*; -1 stops) with
+        the device clock."""
+        _check(self.lib.vdi_mg_time_kernels(self.h, kclass, max_launches), "vdi_mg_time_kernels")
+
+    def kernel_times(self):
+        """(launches, total ms, algorithmic bytes) of the stamped launches."""
+        n, t, b = ctypes.c_long(), ctypes.c_double(), ctypes.c_double()
+        _check(self.lib.vdi_mg_kernel_times(self.h, ctypes.byref(n), ctypes.byref(t), ctypes.byref(b)),
+               "vdi_mg_kernel_times")
+        return n.value, t.value, b.value
+
 
 NUM_TILE_SHAPES = 18
 
 __all__ = [
-    "Fluid2D", "MGSystem", "Batch", "UNet", "tile_shape", "NUM_TILE_SHAPES", "VdiParams", "VdiStats", "VdiError", "PhysicalConfig",
+    "Fluid2D", "MGSystem", "Batch", "UNet", "tile_shape", "tile_label", "NUM_TILE_SHAPES", "VdiParams", "VdiStats", "VdiError", "PhysicalConfig",
     "drop_impact_params", "h0_nd_for_hmin", "load_library", "LIB_PATH", "ABI",
 ]

@@ -1664,8 +1664,13 @@ template <class T> int up_b(const LvArgs<T> *A, const WorkItem *w, int nw, bool 
     VDI_LAUNCHED();
     return VDI_OK;
 }
+template <class T> int xsolve_grid(const XArgs<T> &a) {
+    return a.phase == 2 ? (a.P + kXColsV - 1) / kXColsV : a.nbr * a.nbc;
+}
+template int xsolve_grid<double>(const XArgs<double> &);
+template int xsolve_grid<float>(const XArgs<float> &);
 template <class T> int xsolve(const XArgs<T> &a, size_t lds, hipStream_t st) {
-    const int nb = a.phase == 2 ? (a.P + kXColsV - 1) / kXColsV : a.nbr * a.nbc;
+    const int nb = xsolve_grid(a);
     hipLaunchKernelGGL(k_xsolve<T>, dim3(nb), dim3(kXThreads), lds, st, a);
     VDI_LAUNCHED();
     return VDI_OK;
@@ -1739,9 +1744,12 @@ int KernelTimer::setup(int c, int capacity) {
     cls = c;
     if (c < 0) return VDI_OK;
     cap = capacity;
-    VDI_TRY(stamps.alloc(sizeof(Stamp) * (size_t)cap));
-    VDI_HIP(hipHostMalloc((void **)&host, sizeof(Stamp) * (size_t)cap, hipHostMallocDefault));
+    // workgroup stamps: up to 512 per launch on average, at most 4 Mi (64 MiB)
+    ncap = std::min<long>(4L << 20, std::max<long>(4096, 512L * cap));
+    VDI_TRY(stamps.alloc(sizeof(Stamp) * (size_t)ncap));
+    VDI_HIP(hipHostMalloc((void **)&host, sizeof(Stamp) * (size_t)ncap, hipHostMallocDefault));
     slot_bytes.assign(cap, 0.0);
+    off.assign(cap + 1, 0);
     int dev = 0, khz = 0;
     VDI_HIP(hipGetDevice(&dev));
     VDI_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
@@ -1750,20 +1758,32 @@ int KernelTimer::setup(int c, int capacity) {
 }
 
 int KernelTimer::reset(int a, int b, hipStream_t st) {
-    if (b <= a) return VDI_OK;
-    hipLaunchKernelGGL(k_stamp_reset, dim3(cdiv(b - a, 256)), dim3(256), 0, st, stamps.as<Stamp>() + a, b - a);
+    if (b <= a || a > next) return VDI_OK;
+    const long s0 = off[a], s1 = b >= cap ? ncap : off[std::min(b, next)];
+    if (s1 <= s0) return VDI_OK;
+    hipLaunchKernelGGL(k_stamp_reset, dim3((unsigned)((s1 - s0 + 255) / 256)), dim3(256), 0, st, stamps.as<Stamp>() + s0,
+                       (int)(s1 - s0));
     VDI_LAUNCHED();
     return VDI_OK;
 }
 
 int KernelTimer::harvest(int a, int b, hipStream_t st) {
+    b = std::min(b, next);
     if (b <= a) return VDI_OK;
-    VDI_HIP(hipMemcpyAsync(host + a, stamps.as<Stamp>() + a, sizeof(Stamp) * (size_t)(b - a), hipMemcpyDeviceToHost, st));
+    const long s0 = off[a], s1 = off[b];
+    VDI_HIP(hipMemcpyAsync(host + s0, stamps.as<Stamp>() + s0, sizeof(Stamp) * (size_t)(s1 - s0), hipMemcpyDeviceToHost,
+                           st));
     VDI_HIP(hipStreamSynchronize(st));
     for (int k = a; k < b; k++) {
-        const Stamp &s = host[k];
-        if (s.t0 == ~0ull || s.t1 < s.t0) continue;  // every workgroup returned at once
-        dur_ms.push_back((float)((double)(s.t1 - s.t0) / ticks_per_ms));
+        unsigned long long t0 = ~0ull, t1 = 0;
+        for (long i = off[k]; i < off[k + 1]; i++) {
+            const Stamp &s = host[i];
+            if (s.t0 == ~0ull || s.t1 < s.t0) continue;  // returned at once
+            t0 = std::min(t0, s.t0);
+            t1 = std::max(t1, s.t1);
+        }
+        if (t0 == ~0ull) continue;  // every workgroup returned at once
+        dur_ms.push_back((float)((double)(t1 - t0) / ticks_per_ms));
         bytes.push_back(slot_bytes[k]);
     }
     return VDI_OK;
@@ -1774,7 +1794,9 @@ void KernelTimer::release() {
     if (host) (void)hipHostFree(host);
     host = nullptr;
     cap = next = reserved = 0;
+    ncap = 0;
     slot_bytes.clear();
+    off.clear();
     dur_ms.clear();
     bytes.clear();
     cls = -1;
@@ -2275,6 +2297,12 @@ double MG::mx_bytes() const {
     return 4.0 * (3.0 * N0 + (double)lm[1] * ln[1]) + 8.0 * 3.0 * N0;
 }
 
+// Kernel-function class of an FP64 level launch (VDI_KCLASS_*).
+static int func_class(bool fem, bool up, bool stream) {
+    if (stream) return fem ? (up ? VDI_KCLASS_SFEM_UP : VDI_KCLASS_SFEM_DOWN) : (up ? VDI_KCLASS_SCC_UP : VDI_KCLASS_SCC_DOWN);
+    return fem ? (up ? VDI_KCLASS_FEM_UP : VDI_KCLASS_FEM_DOWN) : (up ? VDI_KCLASS_CC_UP : VDI_KCLASS_CC_DOWN);
+}
+
 // Enqueues the down kernel of level l of hierarchy H (FP64 or FP32).
 template <class T>
 int MG::launch_down(const std::vector<LevelT<T>> &H, int l, int zero_z, int fin, double tol2, int fin_begin) {
@@ -2285,7 +2313,7 @@ int MG::launch_down(const std::vector<LevelT<T>> &H, int l, int zero_z, int fin,
     a.tol2 = tol2;
     if (timer)
         a.ts = timer->slot(l == 0 ? VDI_KCLASS_MG_DOWN0 : VDI_KCLASS_MG_MID, down_bytes(prec, l, zero_z),
-                           prec ? -1 : (fem ? VDI_KCLASS_FEM_DOWN : VDI_KCLASS_CC_DOWN));
+                           prec ? -1 : func_class(fem, false, kTileShapes[tile[prec][0][l]].stream), a.nb);
     return mgk::down<T>(a, fem, down_lds(prec, l), st);
 }
 
@@ -2295,7 +2323,7 @@ int MG::launch_up(const std::vector<LevelT<T>> &H, int l, int norm, double tol2)
     LvArgs<T> a = up_args<T>(H, l, norm, tol2);
     if (timer)
         a.ts = timer->slot(l == 0 ? VDI_KCLASS_MG_UP0 : VDI_KCLASS_MG_MID, up_bytes(prec, l),
-                           prec ? -1 : (fem ? VDI_KCLASS_FEM_UP : VDI_KCLASS_CC_UP));
+                           prec ? -1 : func_class(fem, true, kTileShapes[tile[prec][1][l]].stream), a.nb);
     return mgk::up<T>(a, fem, up_lds(prec, l), st);
 }
 
@@ -2305,7 +2333,9 @@ int MG::launch_coarse(const std::vector<LevelT<T>> &H, double tol2) {
     if (big_) {
         for (int ph = 1; ph <= 3; ph++) {
             XArgs<T> x = x_args<T>(H, ph);
-            if (timer) x.ts = timer->slot(prec ? VDI_KCLASS_MG_COARSE : VDI_KCLASS_XSOLVE, coarse_bytes(prec) / 3.0);
+            if (timer)
+                x.ts = timer->slot(prec ? VDI_KCLASS_MG_COARSE : VDI_KCLASS_XSOLVE, coarse_bytes(prec) / 3.0, -1,
+                                   mgk::xsolve_grid(x));
             VDI_TRY(mgk::xsolve<T>(x, x_lds(ph), st));
         }
         return VDI_OK;
@@ -2357,14 +2387,14 @@ int MG::launch_cycle(double tol2, int fin_prev, int norm) {
         a.fin = fin;
         a.fin_begin = fin_begin;
         a.tol2 = tol2;
-        if (timer) a.ts = timer->slot(VDI_KCLASS_MG_DOWN0, down_bytes(1, 0, 1));
+        if (timer) a.ts = timer->slot(VDI_KCLASS_MG_DOWN0, down_bytes(1, 0, 1), -1, a.nb);
         VDI_TRY(mgk::down<float>(a, fem, down_lds(1, 0), st));
     }
     for (int l = 1; l < lc; l++) VDI_TRY(launch_down<float>(LF, l, 1));
     VDI_TRY(launch_coarse<float>(LF, tol2));
     for (int l = lc - 1; l >= 1; l--) VDI_TRY(launch_up<float>(LF, l, 0, tol2));
     MxArgs a = mx_args(par, tol2);
-    if (timer) a.ts = timer->slot(VDI_KCLASS_MG_UP0, mx_bytes());
+    if (timer) a.ts = timer->slot(VDI_KCLASS_MG_UP0, mx_bytes(), -1, a.nb);
     VDI_TRY(mgk::mx(a, fem, mx_lds(), st));
     if (norm || !can_fold_norm()) VDI_TRY(mgk::norm(norm_args(0, tol2), st));
     mx_enqueued_++;

@@ -405,6 +405,7 @@ template <class T> int down(const LvArgs<T> &a, bool fem, size_t lds, hipStream_
 template <class T> int up(const LvArgs<T> &a, bool fem, size_t lds, hipStream_t st);
 template <class T> int coarse(const CoarseArgs<T> &a, size_t lds, hipStream_t st);
 template <class T> int xsolve(const XArgs<T> &a, size_t lds, hipStream_t st);
+template <class T> int xsolve_grid(const XArgs<T> &a);
 int mx(const MxArgs &a, bool fem, size_t lds, hipStream_t st);
 int begin(const BeginArgs &a, hipStream_t st);
 int vec(const VecArgs &a, hipStream_t st);

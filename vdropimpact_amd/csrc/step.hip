@@ -1261,7 +1261,7 @@ static int enqueue_viscous_bc(vdi_ctx *c, const StepRec &a, hipStream_t st = nul
 static int enqueue_inviscid_bc(vdi_ctx *c, const StepRec &a, hipStream_t st = nullptr) {
     const int waves = c->g.n + c->g.ml;
     // reads p once; reads and writes two ghost cells of u and v per line
-    Stamp *ts = c->timer.slot(VDI_KCLASS_SIMPSON, 8.0 * (c->g.me + 8.0 * waves));
+    Stamp *ts = c->timer.slot(VDI_KCLASS_SIMPSON, 8.0 * (c->g.me + 8.0 * waves), -1, cdiv(waves, 4));
     hipLaunchKernelGGL(k_simpson, dim3(cdiv(waves, 4)), dim3(256), 0, st ? st : c->st, a, ts);
     VDI_LAUNCHED();
     return VDI_OK;
@@ -1770,7 +1770,7 @@ int vdi_synchronize(vdi_ctx *c) {
 }
 
 int vdi_time_kernels(vdi_ctx *c, int kclass, int max_launches) {
-    if (!c || max_launches < 0 || kclass < -1 || kclass > VDI_KCLASS_XSOLVE) return VDI_ERR_ARG;
+    if (!c || max_launches < 0 || kclass < -1 || kclass > VDI_KCLASS_SFEM_UP) return VDI_ERR_ARG;
     VDI_HIP(hipStreamSynchronize(c->st));
     c->drop_graphs();  // captured graphs own slots of the previous pool
     VDI_TRY(c->timer.setup(kclass, max_launches));
@@ -1779,12 +1779,11 @@ int vdi_time_kernels(vdi_ctx *c, int kclass, int max_launches) {
     return VDI_OK;
 }
 
-int vdi_kernel_times(vdi_ctx *c, long *launches, double *total_ms, double *alg_bytes) {
-    if (!c || !launches || !total_ms || !alg_bytes) return VDI_ERR_ARG;
-    VDI_HIP(hipStreamSynchronize(c->st));
-    const std::vector<float> &ms = c->timer.dur_ms;
-    // Launches that returned at once (issued after their solve had already
-    // converged) move no data; they are left out.
+// Totals of the harvested launches of a timer.  Launches that returned at
+// once (issued after their solve had already converged) move no data; they
+// are left out.
+static void timer_totals(const KernelTimer &tm, long *launches, double *total_ms, double *alg_bytes) {
+    const std::vector<float> &ms = tm.dur_ms;
     std::vector<float> sorted = ms;
     std::sort(sorted.begin(), sorted.end());
     const float med = sorted.empty() ? 0.f : sorted[sorted.size() / 2];
@@ -1794,11 +1793,17 @@ int vdi_kernel_times(vdi_ctx *c, long *launches, double *total_ms, double *alg_b
         if (ms[k] < 0.2f * med) continue;
         n++;
         t += ms[k];
-        b += c->timer.bytes[k];
+        b += tm.bytes[k];
     }
     *launches = n;
     *total_ms = t;
     *alg_bytes = b;
+}
+
+int vdi_kernel_times(vdi_ctx *c, long *launches, double *total_ms, double *alg_bytes) {
+    if (!c || !launches || !total_ms || !alg_bytes) return VDI_ERR_ARG;
+    VDI_HIP(hipStreamSynchronize(c->st));
+    timer_totals(c->timer, launches, total_ms, alg_bytes);
     return VDI_OK;
 }
 
@@ -1894,6 +1899,7 @@ int vdi_upload_state(vdi_ctx *c, const double *host, long len) {
 struct vdi_mg {
     MG g;
     hipStream_t st = nullptr;
+    KernelTimer timer;  // vdi_mg_time_kernels
     ~vdi_mg() {
         if (st) (void)hipStreamDestroy(st);
     }
@@ -2052,6 +2058,33 @@ int vdi_tile_shape(int shape, int *tx, int *ty) {
     *tx = kTileShapes[shape].tx;
     *ty = kTileShapes[shape].ty;
     return VDI_OK;
+}
+
+int vdi_mg_time_kernels(vdi_mg *h, int kclass, int max_launches) {
+    if (!h || max_launches < 0 || kclass < -1 || kclass > VDI_KCLASS_SFEM_UP) return VDI_ERR_ARG;
+    VDI_HIP(hipStreamSynchronize(h->st));
+    VDI_TRY(h->timer.setup(kclass, max_launches));
+    h->g.timer = kclass < 0 ? nullptr : &h->timer;
+    if (kclass >= 0) VDI_TRY(h->timer.reset(0, h->timer.cap, h->st));
+    h->timer.next = 0;
+    return VDI_OK;
+}
+
+int vdi_mg_kernel_times(vdi_mg *h, long *launches, double *total_ms, double *alg_bytes) {
+    if (!h || !launches || !total_ms || !alg_bytes) return VDI_ERR_ARG;
+    VDI_HIP(hipStreamSynchronize(h->st));
+    if (h->timer.cls >= 0) {
+        VDI_TRY(h->timer.harvest(0, h->timer.next, h->st));
+        VDI_TRY(h->timer.reset(0, h->timer.cap, h->st));
+        h->timer.next = 0;
+    }
+    timer_totals(h->timer, launches, total_ms, alg_bytes);
+    return VDI_OK;
+}
+
+int vdi_tile_stream(int shape) {
+    if (shape < 0 || shape >= kNumTileShapes) return VDI_ERR_ARG;
+    return kTileShapes[shape].stream;
 }
 
 int vdi_mg_set_tile(vdi_mg *h, int prec, int which, int level, int shape) {

@@ -116,22 +116,26 @@ struct LevelT {
 using Level = LevelT<double>;
 using LevelF = LevelT<float>;
 
-// Launch timing stamp: the wall-clock ticks at which the first workgroup of
-// a launch started and the last one finished (atomicMin / atomicMax).  A
-// launch whose workgroups all returned at once (a converged solve) leaves
-// t0 = ~0 and is not counted.
+// Workgroup timing stamp: the wall-clock ticks at which one workgroup of a
+// timed launch started and finished, stored by the workgroup itself (plain
+// stores into its own stamp: atomics of every workgroup on one stamp
+// serialised across the XCDs and lengthened the launch by a quarter).  A
+// launch's duration runs from its earliest start to its latest end; a
+// workgroup that returned at once (a converged solve) leaves t0 = ~0 and is
+// not counted.
 struct Stamp {
     unsigned long long t0, t1;
 };
 
 #ifdef __HIPCC__
+// ts: the launch's stamps, one per workgroup of its (1-D) grid.
 __device__ __forceinline__ void stamp_begin(Stamp *ts) {
-    if (ts && threadIdx.x == 0 && threadIdx.y == 0) atomicMin(&ts->t0, (unsigned long long)wall_clock64());
+    if (ts && threadIdx.x == 0 && threadIdx.y == 0) ts[blockIdx.x].t0 = (unsigned long long)wall_clock64();
 }
 __device__ __forceinline__ void stamp_end(Stamp *ts) {
     if (!ts) return;
     __syncthreads();
-    if (threadIdx.x == 0 && threadIdx.y == 0) atomicMax(&ts->t1, (unsigned long long)wall_clock64());
+    if (threadIdx.x == 0 && threadIdx.y == 0) ts[blockIdx.x].t1 = (unsigned long long)wall_clock64();
 }
 // Per-simulation clocks of a batched step, three counters per simulation:
 // busy[3 s] the workgroup busy ticks summed over its work items, busy[3 s + 1]
@@ -158,23 +162,27 @@ __device__ __forceinline__ void busy_end(unsigned long long *busy, int sim, unsi
 // replay.
 struct KernelTimer {
     int cls = -1;       // class being timed (VDI_KCLASS_*), -1 = off
-    DevBuf stamps;      // cap slots
+    DevBuf stamps;      // ncap workgroup stamps
     Stamp *host = nullptr;  // pinned mirror
-    int cap = 0;
+    int cap = 0;            // launch slots
+    long ncap = 0;          // workgroup stamps
     int next = 0, reserved = 0;  // slots [0, reserved) belong to captured graphs
     double ticks_per_ms = 1e5;
     std::vector<double> slot_bytes;
+    std::vector<long> off;  // launch slot k owns stamps [off[k], off[k + 1])
     std::vector<float> dur_ms;
     std::vector<double> bytes;
-    // A launch belongs to a level class c and (FP64 level kernels) to the
-    // class of its kernel function c2 (-1: none).
-    Stamp *slot(int c, double alg_bytes, int c2 = -1) {
-        if ((c != cls && c2 != cls) || next >= cap) return nullptr;
+    // A launch of nwg workgroups belongs to a level class c and (FP64 level
+    // kernels) to the class of its kernel function c2 (-1: none).
+    Stamp *slot(int c, double alg_bytes, int c2 = -1, int nwg = 1) {
+        if ((c != cls && c2 != cls) || next >= cap || off[next] + nwg > ncap) return nullptr;
         slot_bytes[next] = alg_bytes;
-        return stamps.as<Stamp>() + next++;
+        off[next + 1] = off[next] + nwg;
+        return stamps.as<Stamp>() + off[next++];
     }
     int setup(int c, int capacity);
-    // Enqueues the reset of slots [a, b) (a kernel, so it can be captured).
+    // Enqueues the reset of the stamps of slots [a, b) (b = cap: every stamp
+    // from slot a on; a kernel, so it can be captured).
     int reset(int a, int b, hipStream_t st);
     // Reads slots [a, b) after their launches have completed.
     int harvest(int a, int b, hipStream_t st);
