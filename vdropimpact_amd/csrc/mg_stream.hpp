@@ -38,6 +38,10 @@ constexpr int kSW = 128;       // columns of a strip (two per lane)
 constexpr int kSWaves = 4;     // independent wavefronts per workgroup
 constexpr int kSThreads = 64 * kSWaves;
 constexpr int kSAhead = 4;     // rows of z and b loaded ahead of the step's row
+// nodal up launches load two rows ahead: with four they need more than 256
+// registers, one wavefront per SIMD
+// (P > 0: the coarse ring of a window needs four rows)
+template <bool UP, int P> constexpr int kSNAhead = (UP && P > 0) ? 2 : kSAhead;
 constexpr int kSQAhead = 2;    // fine rows by which a coarse row (up launches) is loaded ahead
 
 // Lane k receives lane k-1's value (lane 0: zero) / lane k+1's (lane 63: zero).
@@ -639,7 +643,7 @@ __device__ __forceinline__ void snstep(const LvArgs<T> &a, const SNLane<T> &s, c
 #pragma clang fp contract(off)
     const LevelT<T> &F = a.F, &C = a.C;
     __builtin_amdgcn_sched_barrier(0);
-    srow_issue<T, W, sslot<W>(U, -kSAhead)>(F, bf, s, y + kSAhead, yhi, w);
+    srow_issue<T, W, sslot<W>(U, -kSNAhead<UP, P>)>(F, bf, s, y + kSNAhead<UP, P>, yhi, w);
     if constexpr (UP) {
         constexpr int R = W / 2, A = U >> 1;
         if constexpr (!(U & 1))
@@ -704,7 +708,7 @@ __device__ __forceinline__ void nstream_body(const LvArgs<T> &a, int item, doubl
     constexpr int P = 4 * NU;
     constexpr int HH = 4 * NU + 2;  // columns and rows of halo (even)
     constexpr int WOUT = kSW - 2 * HH;
-    constexpr int W = P + 4 + kSAhead;
+    constexpr int W = P + 4 + kSNAhead<UP, P>;
     static_assert(W % 2 == 0 && HH % 2 == 0, "even window, even first row");
     static_assert(kSQAhead % 2 == 0 && W / 2 >= kSQAhead / 2 + 3, "the coarse ring holds the rows in use");
     const LevelT<T> &F = a.F, &C = a.C;
@@ -722,7 +726,7 @@ __device__ __forceinline__ void nstream_body(const LvArgs<T> &a, int item, doubl
     SWin<T, W> w = {};
     const int ys = y0 - HH, ylast = y0 + S + HH - 1;
     const int ck = ys >> 1;  // ys is even: coarse row of the window's first row
-    snprologue<T, W, UP>(F, C, bf, s, ys, ylast, ck, w, std::make_integer_sequence<int, kSAhead>{});
+    snprologue<T, W, UP>(F, C, bf, s, ys, ylast, ck, w, std::make_integer_sequence<int, kSNAhead<UP, P>>{});
     // residual rows up to the segment's last output row (down: row 2J + 1
     // of its last coarse row J, outside the level for the top one)
     const int qlast = UP ? y1 - 1 : 2 * min((y0 + S) >> 1, C.n) - 1;
