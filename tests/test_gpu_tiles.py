@@ -53,6 +53,49 @@ def test_gpu_every_tile_shape_same_cycle(vdi, kind, m, n, cx, cy, form):
     ref.close()
 
 
+STREAM_SCHEDULES = [(0, 1), (1, 0), (1, 1), (3, 2), (4, 4)]
+
+
+@pytest.mark.parametrize("kind,m,n,cx,cy", [(0, 296, 136, 1e4, 6e3), (1, 297, 137, 0.01, 0.013),
+                                             (2, 296, 136, 0.13, 0.09)])
+@pytest.mark.parametrize("form", [0, 1])
+def test_gpu_streaming_launches_every_schedule(vdi, kind, m, n, cx, cy, form):
+    """The streaming launches (every sweep count 0..4 their kernels are
+    instantiated for, strips and segments that do not divide the level)
+    compute the tiled launches' V-cycle bit for bit; in the mixed form this
+    includes the level-0 up launch with its FP64 residual."""
+    stream = [s for s in range(vdi.NUM_TILE_SHAPES) if vdi.tile_stream(s)]
+    assert stream
+    rng = np.random.default_rng(11 + kind)
+    b = rng.standard_normal(m * n)
+    prec = 1 if form else 0
+    for i, (nu1, nu2) in enumerate(STREAM_SCHEDULES):
+        shape = stream[i % len(stream)]
+        say("schedule", (nu1, nu2), "shape", vdi.tile_label(shape), "kind", kind, "form", form)
+        ref = vdi.MGSystem(kind, m, n, cx, cy, 1, nu1, nu2)
+        ref.set_form(form)
+        ref.set_b(b)
+        ref.set_z(np.zeros(m * n))
+        ref.vcycles(3)
+        z_ref = ref.get_z()
+        g = vdi.MGSystem(kind, m, n, cx, cy, 1, nu1, nu2)
+        g.set_form(form)
+        for lv in range(g.levels):
+            for which in (0, 1):
+                try:
+                    g.set_tile(prec, which, lv, shape)
+                except vdi.VdiError:
+                    pass   # level handled by the coarsest-level kernels
+        if form:
+            g.set_tile(1, 2, 0, shape)
+        g.set_b(b)
+        g.set_z(np.zeros(m * n))
+        g.vcycles(3)
+        np.testing.assert_array_equal(g.get_z(), z_ref)
+        g.close()
+        ref.close()
+
+
 def test_gpu_autotuned_context_same_steps(vdi):
     from vdropimpact_amd.params import PhysicalConfig
     p = PhysicalConfig(m=256, n=128).params()

@@ -467,10 +467,15 @@ def tile_shape(index: int):
     return tx.value, ty.value
 
 
+def tile_stream(index: int) -> bool:
+    """True for a streaming-launch shape (one wavefront per strip and segment)."""
+    return load_library().vdi_tile_stream(index) == 1
+
+
 def tile_label(index: int) -> str:
     """'64x32' for a tile, 'stream/16' for a streaming launch of 16-row segments."""
     tx, ty = tile_shape(index)
-    return f"stream/{ty}" if load_library().vdi_tile_stream(index) == 1 else f"{tx}x{ty}"
+    return f"stream/{ty}" if tile_stream(index) else f"{tx}x{ty}"
 
 
 class Batch:

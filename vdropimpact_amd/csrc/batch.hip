@@ -641,7 +641,7 @@ int vdi_batch_create(const vdi_params *params, int count, int form, vdi_batch **
                         const int sh = g.tile[pr][w][l];
                         b->tiles[k][pr][w][l] = kTileShapes[sh].stream ? 0 : sh;
                     }
-                b->tiles[k][1][2][0] = g.tile_mx;
+                b->tiles[k][1][2][0] = kTileShapes[g.tile_mx].stream ? 0 : g.tile_mx;
                 break;
             }
     }

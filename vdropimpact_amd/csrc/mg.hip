@@ -654,6 +654,19 @@ __device__ __forceinline__ void up_body(const LvArgs<T> &a, int tile, T *lds, do
 // E.b), then z_out = z_in + e and the new FP64 residual r = b - A z_out,
 // which is reduced for the convergence test and stored in r_out as the next
 // cycle's right-hand side.  Inputs and outputs are distinct arrays.
+// A z of the mixed form's FP64 residual at (gx, gy), z on a tile of row pitch
+// RX (the rounding fixed in the source, shared with the streaming form).
+template <bool FEM>
+__device__ __forceinline__ double mx_az(const MxArgs &a, const double *zd, int l, int RX, int gx, int gy) {
+#pragma clang fp contract(off)
+    const Level &Z = a.Z;
+    if (FEM) {
+        const double acc = fem_acc<double>(a.fop, Z.m, Z.n, gx, gy);
+        return gy == 0 ? acc * zd[l] : acc * zd[l] + fem_off_t<double>(a.fop, Z.m, Z.n, zd, l, RX, gx, gy);
+    }
+    return fma(a.cop.s, cc_off_t<double>(Z.cx, Z.cy, zd, l, RX), cc_diag<double>(a.cop, Z.m, Z.n, Z.cx, Z.cy, gx, gy) * zd[l]);
+}
+
 template <bool FEM>
 __device__ __forceinline__ void mx_body(const MxArgs &a, int tile, double *lds_d, double *sh, int *last) {
     const Level &Z = a.Z;
@@ -726,15 +739,7 @@ __device__ __forceinline__ void mx_body(const MxArgs &a, int tile, double *lds_d
             if (gx >= Z.m || gy >= Z.n) continue;
             const int l = (y + t.H) * t.RX + x + t.H;
             const size_t g = gx + (size_t)Z.m * gy;
-            double az;
-            if (FEM) {
-                const double acc = fem_acc<double>(a.fop, Z.m, Z.n, gx, gy);
-                az = gy == 0 ? acc * zd[l] : acc * zd[l] + fem_off_t<double>(a.fop, Z.m, Z.n, zd, l, t.RX, gx, gy);
-            } else {
-                az = a.cop.s * cc_off_t<double>(Z.cx, Z.cy, zd, l, t.RX) +
-                     cc_diag<double>(a.cop, Z.m, Z.n, Z.cx, Z.cy, gx, gy) * zd[l];
-            }
-            const double r = bv[u] - az;
+            const double r = bv[u] - mx_az<FEM>(a, zd, l, t.RX, gx, gy);
             a.z_out[g] = zd[l];
             a.r_out[g] = (float)r;
             rr += r * r;
@@ -1622,8 +1627,20 @@ template <class T> int coarse(const CoarseArgs<T> &a, size_t lds, hipStream_t st
     VDI_LAUNCHED();
     return VDI_OK;
 }
+template <bool FEM>
+static void launch_smx(const MxArgs &a, hipStream_t st) {
+    switch (a.nu) {
+        case 0: hipLaunchKernelGGL((k_smx<FEM, 0>), dim3(a.nb), dim3(kSThreads), 0, st, a); break;
+        case 1: hipLaunchKernelGGL((k_smx<FEM, 1>), dim3(a.nb), dim3(kSThreads), 0, st, a); break;
+        case 2: hipLaunchKernelGGL((k_smx<FEM, 2>), dim3(a.nb), dim3(kSThreads), 0, st, a); break;
+        case 3: hipLaunchKernelGGL((k_smx<FEM, 3>), dim3(a.nb), dim3(kSThreads), 0, st, a); break;
+        default: hipLaunchKernelGGL((k_smx<FEM, 4>), dim3(a.nb), dim3(kSThreads), 0, st, a); break;
+    }
+}
 int mx(const MxArgs &a, bool fem, size_t lds, hipStream_t st) {
-    if (fem) hipLaunchKernelGGL(k_mx_up0<true>, dim3(a.nb), dim3(kBX, kBY), lds, st, a);
+    if (a.items && fem) launch_smx<true>(a, st);
+    else if (a.items) launch_smx<false>(a, st);
+    else if (fem) hipLaunchKernelGGL(k_mx_up0<true>, dim3(a.nb), dim3(kBX, kBY), lds, st, a);
     else hipLaunchKernelGGL(k_mx_up0<false>, dim3(a.nb), dim3(kBX, kBY), lds, st, a);
     VDI_LAUNCHED();
     return VDI_OK;
@@ -2056,6 +2073,7 @@ size_t MG::up_lds(int prec, int l) const {
 }
 size_t MG::mx_lds() const {
     const TileShape &t = kTileShapes[tile_mx];
+    if (t.stream) return 0;
     const int H = halo_up(fem, nu2, 1);
     return (size_t)(t.tx + 2 * H) * (t.ty + 2 * H) * (sizeof(double) + 2 * sizeof(float)) + kInvTab * sizeof(float);
 }
@@ -2186,8 +2204,14 @@ MxArgs MG::mx_args(int parity, double tol2) {
     const TileShape &t = kTileShapes[tile_mx];
     a.tx = t.tx;
     a.ty = t.ty;
-    a.nbx = cdiv(L[0].m, t.tx);
-    a.nb = a.nbx * cdiv(L[0].n, t.ty);
+    if (t.stream) {
+        a.nbx = cdiv(L[0].m, stream_wout(fem, nu2));
+        a.items = a.nbx * cdiv(L[0].n, t.ty);
+        a.nb = cdiv(a.items, kSWaves);
+    } else {
+        a.nbx = cdiv(L[0].m, t.tx);
+        a.nb = a.nbx * cdiv(L[0].n, t.ty);
+    }
     a.ctl = ctl();
     a.tol2 = tol2;
     a.ts = nullptr;
@@ -2589,11 +2613,11 @@ int MG::apply(const double *z, double *out) {
 
 int MG::set_tile(int prec, int which, int level, int shape) {
     if (prec < 0 || prec > 1 || which < 0 || which > 2 || shape < 0 || shape >= kNumTileShapes) return VDI_ERR_ARG;
-    // streaming launches: down and up launches, on levels whose arrays a
-    // buffer resource's 32-bit byte range covers
+    // streaming launches: on levels whose arrays a buffer resource's 32-bit
+    // byte range covers (the mixed form's level-0 up reads FP64 level 0)
     if (kTileShapes[shape].stream) {
-        if (which == 2) return VDI_ERR_ARG;
-        if (level >= 0 && level < lc && (long)lm[level] * ln[level] * (prec ? 4 : 8) >= (1L << 31) - (1L << 20))
+        const int elem = (which == 2 || !prec) ? 8 : 4;
+        if (level >= 0 && level < lc && (long)lm[level] * ln[level] * elem >= (1L << 31) - (1L << 20))
             return VDI_ERR_ARG;
     }
     if (which == 2) {

@@ -138,6 +138,7 @@ struct MxArgs {
     MGCtl ctl;
     double tol2;
     Stamp *ts;
+    int items;  // streaming launch (mg_stream.hpp): wavefront items; 0: tiled
 };
 
 // The coarsest level for the single-workgroup kernel.  LDS holds the level

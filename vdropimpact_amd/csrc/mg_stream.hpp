@@ -769,4 +769,192 @@ __global__ void __launch_bounds__(kSThreads) k_snup(LvArgs<T> a) {
     stamp_end(a.ts);
 }
 
+// ------------------------------------------------------------------------
+// The mixed form's level-0 up launch (mx_body) as a streaming launch
+// ------------------------------------------------------------------------
+// The FP32 correction e is prolongated and post-smoothed as in an up launch
+// (lag 1, with the FP32 level's own coefficients).  Behind the last pass an
+// FP64 stage forms z_out = z_in + e on a three-row ring (zero outside the
+// level), the FP64 residual r = b - A z_out with the level-0 operator
+// (mx_az's rounding), and stores z_out, r (FP32: the next cycle's
+// right-hand side) and ||r||^2.
+
+constexpr int kSMxAhead = 2;  // rows by which z_in and b (FP64) are loaded ahead
+// z_in's lookahead: its first row, y0 - 1, is loaded at step y0 + P - KZ,
+// which must not lie before the first step y0 - HV (P + HV = 1 for the
+// cell-centred levels without post-smoothing)
+template <int P, int HV> constexpr int kSMxZ = P + HV < kSMxAhead ? P + HV : kSMxAhead;
+
+template <int W>
+struct SMxWin {
+    double zia[W], zib[W], bza[W], bzb[W], zoa[W], zob[W];
+};
+
+struct SMxBufs {
+    SRsrc zin, b, zout, rout;
+};
+
+// FP64 residual of row q at both columns (mx_az), z_out rows q-1, q, q+1 in
+// slots d, c, u.
+template <bool FEM, bool INT, int W, class LN>
+__device__ __forceinline__ void smx_resid(const MxArgs &m, const LN &s, int q, const SMxWin<W> &x, int c, int d,
+                                          int u, double &ra, double &rb) {
+#pragma clang fp contract(off)
+    const Level &Z = m.Z;
+    if constexpr (FEM) {
+        const bool edge = !INT && (q == 0 || q == Z.n - 1);
+        const double wj = INT ? 1.0 : (q == Z.n - 1 ? 0.5 : 1.0);
+        const double acca = edge ? s.aa1 : s.aa0, accb = edge ? s.ab1 : s.ab0;
+        const double fa = acca * x.zoa[c] + snoff_a<double>(m.fop, s.wia, wj, x.zoa, x.zob, c, d, u);
+        const double fb = accb * x.zob[c] + snoff_b<double>(m.fop, s.wib, wj, x.zoa, x.zob, c, d, u);
+        ra = x.bza[c] - ((!INT && q == 0) ? acca * x.zoa[c] : fa);
+        rb = x.bzb[c] - ((!INT && q == 0) ? accb * x.zob[c] : fb);
+    } else {
+        ra = x.bza[c] - fma(m.cop.s, soff_a<double>(Z.cx, Z.cy, x.zoa, x.zob, c, d, u),
+                            by_row_class<INT, double>(s.da0, s.da1, s.da2, q, Z.n) * x.zoa[c]);
+        rb = x.bzb[c] - fma(m.cop.s, soff_b<double>(Z.cx, Z.cy, x.zoa, x.zob, c, d, u),
+                            by_row_class<INT, double>(s.db0, s.db1, s.db2, q, Z.n) * x.zob[c]);
+    }
+}
+
+// One step of the streaming mixed level-0 up launch (row y, window position
+// U, row parity Y); LE / LZ: the lane state for the FP32 level E and the FP64
+// level Z.
+template <bool FEM, int P, int KA, int KZ, int W, int U, int Y, class LE, class LZ>
+__device__ __forceinline__ void smx_step(const MxArgs &m, const LE &s, const LZ &sz, const SBufs &bf,
+                                         const SMxBufs &bd, int y, int ylo, int yhi, int y0, int y1, int ck, int tb,
+                                         SWin<float, W> &w, SMxWin<W> &x, double &rr) {
+#pragma clang fp contract(off)
+    const LevelF &E = m.E, &C = m.C;
+    const Level &Z = m.Z;
+    __builtin_amdgcn_sched_barrier(0);
+    // FP32 stage: the correction's loads, prolongation and passes (lag 1)
+    srow_issue<float, W, sslot<W>(U, -KA)>(E, bf, s, y + KA, yhi, w);
+    constexpr int R = W / 2, A = U >> 1;
+    const bool interior = y - P - 2 > 0 && y - 1 < E.n - 1;
+    if constexpr (FEM) {
+        if constexpr (!(U & 1))
+            snq_issue<float, W, ((U + kSQAhead) / 2 + 1) % R>(C, bf, s, ck + (tb + U + kSQAhead) / 2 + 1, w);
+        snrow_fix<float, W, sslot<W>(U, 0), A % R, (A + 1) % R>(E, C, s, y, yhi, w);
+        if (interior) snpasses<true, float, W, U, Y>(E, m.fop, s, y, ylo, yhi, w, std::make_integer_sequence<int, P>{});
+        else snpasses<false, float, W, U, Y>(E, m.fop, s, y, ylo, yhi, w, std::make_integer_sequence<int, P>{});
+    } else {
+        if constexpr (!(U & 1))
+            sq_issue<float, W, ((U + kSQAhead) / 2 + 1) % R>(C, bf, s, ck + (tb + U + kSQAhead) / 2 + 1, w);
+        if constexpr (!(U & 1)) srow_fix<float, W, sslot<W>(U, 0), A % R, (A + 1) % R>(E, C, m.cop, s, y, yhi, w);
+        else srow_fix<float, W, sslot<W>(U, 0), (A + 1) % R, A % R>(E, C, m.cop, s, y, yhi, w);
+        if (interior) spasses<true, float, W, U, Y, 1>(E, m.cop, s, y, ylo, yhi, w, std::make_integer_sequence<int, P>{});
+        else spasses<false, float, W, U, Y, 1>(E, m.cop, s, y, ylo, yhi, w, std::make_integer_sequence<int, P>{});
+    }
+    // FP64 stage: z_in and b loaded KZ and kSMxAhead rows ahead of their
+    // use, only on the rows the segment's residuals read (z_out rows y0 - 1
+    // .. y1, b rows y0 .. y1 - 1)
+    {
+        const int ri = y - P - 1 + KZ, rb = y - P - 2 + kSMxAhead;
+        constexpr int ki = sslot<W>(U, P + 1 - KZ), kb = sslot<W>(U, P + 2 - kSMxAhead);
+        const bool vi = ri >= max(y0 - 1, 0) && ri <= min(y1, Z.n - 1), vb = rb >= y0 && rb < y1;
+        x.zia[ki] = sload(0.0, bd.zin, (vi && s.ina) ? 8 * (Z.m * ri + s.xa) : kSOut);
+        x.zib[ki] = sload(0.0, bd.zin, (vi && s.inb) ? 8 * (Z.m * ri + s.xb) : kSOut);
+        x.bza[kb] = sload(0.0, bd.b, (vb && s.ina) ? 8 * (Z.m * rb + s.xa) : kSOut);
+        x.bzb[kb] = sload(0.0, bd.b, (vb && s.inb) ? 8 * (Z.m * rb + s.xb) : kSOut);
+    }
+    // z_out of row r1 = y - P - 1, whose correction is final (mx_body's zd)
+    {
+        const int r1 = y - P - 1;
+        constexpr int k1 = sslot<W>(U, P + 1);
+        const bool row = r1 >= 0 && r1 < Z.n;
+        x.zoa[k1] = (row && s.ina) ? x.zia[k1] + (double)w.za[k1] : 0.0;
+        x.zob[k1] = (row && s.inb) ? x.zib[k1] + (double)w.zb[k1] : 0.0;
+    }
+    // the residual of row q = y - P - 2 and the stores
+    constexpr int c = sslot<W>(U, P + 2), d = sslot<W>(U, P + 3), u = sslot<W>(U, P + 1);
+    const int q = y - P - 2;
+    double ra, rb;
+    if (q > 0 && q < Z.n - 1) smx_resid<FEM, true, W>(m, sz, q, x, c, d, u, ra, rb);
+    else smx_resid<FEM, false, W>(m, sz, q, x, c, d, u, ra, rb);
+    const bool qrow = q >= y0 && q < y1;
+    const int base = Z.m * q;
+    const bool va = qrow && s.outa, vb = qrow && s.outb;
+    sstore(bd.zout, va ? 8 * (base + s.xa) : kSOut, x.zoa[c]);
+    sstore(bd.zout, vb ? 8 * (base + s.xb) : kSOut, x.zob[c]);
+    sstore(bd.rout, va ? 4 * (base + s.xa) : kSOut, (float)ra);
+    sstore(bd.rout, vb ? 4 * (base + s.xb) : kSOut, (float)rb);
+    const double qa = va ? ra : 0.0, qb = vb ? rb : 0.0;
+    rr += qa * qa;
+    rr += qb * qb;
+}
+
+template <bool FEM, int P, int KA, int KZ, int W, int YS, class LE, class LZ, int... U>
+__device__ __forceinline__ void smx_window(const MxArgs &m, const LE &s, const LZ &sz, const SBufs &bf,
+                                           const SMxBufs &bd, int ys, int tb, int ylo, int yhi, int y0, int y1, int ck,
+                                           SWin<float, W> &w, SMxWin<W> &x, double &rr,
+                                           std::integer_sequence<int, U...>) {
+    (smx_step<FEM, P, KA, KZ, W, U, (YS + U) & 1>(m, s, sz, bf, bd, ys + tb + U, ylo, yhi, y0, y1, ck, tb, w, x, rr), ...);
+}
+
+template <bool FEM, int NU>
+__device__ __forceinline__ void mx_stream_body(const MxArgs &m, int item, double &rr) {
+    constexpr int P = FEM ? 4 * NU : 2 * NU;
+    constexpr int HH = FEM ? 4 * NU + 2 : (2 * NU + 2) & ~1;  // columns of halo (even)
+    constexpr int HV = FEM ? 4 * NU + 2 : 2 * NU + 1;          // rows of halo
+    constexpr int WOUT = kSW - 2 * HH;
+    constexpr int KA = FEM ? kSNAhead<true, P> : kSAhead;
+    constexpr int KZ = kSMxZ<P, HV>;
+    constexpr int W = P + 4 + KA;
+    static_assert(W % 2 == 0 && W / 2 >= kSQAhead / 2 + 3, "even window holding the coarse rows in use");
+    const LevelF &E = m.E, &C = m.C;
+    const int S = m.ty;
+    const int sx = item % m.nbx, sy = item / m.nbx;
+    const int x0 = sx * WOUT, y0 = sy * S, y1 = min(y0 + S, E.n);
+    const long nf = (long)E.m * E.n;
+    SBufs bf;
+    bf.zsrc = srsrc(E.zalt, nf * 4);
+    bf.b = srsrc(E.b, nf * 4);
+    bf.dst = srsrc(E.z, 0);
+    bf.coarse = srsrc(C.z, (long)C.m * C.n * 4);
+    SMxBufs bd;
+    bd.zin = srsrc(m.z_in, nf * 8);
+    bd.b = srsrc(m.Z.b, nf * 8);
+    bd.zout = srsrc(m.z_out, nf * 8);
+    bd.rout = srsrc(m.r_out, nf * 4);
+    SWin<float, W> w = {};
+    SMxWin<W> x = {};
+    const int ys = y0 - HV, ylast = y0 + S + HV - 1;
+    const int nsteps = (y1 - 1) + P + 2 - ys + 1;
+    if constexpr (FEM) {
+        SNLane<float> s;
+        snlane_init<float>(E, C, x0, WOUT, HH, m.fop, s);
+        SNLane<double> sz;
+        snlane_init<double>(m.Z, m.Z, x0, WOUT, HH, m.fop, sz);
+        const int ck = ys >> 1;
+        snprologue<float, W, true>(E, C, bf, s, ys, ylast, ck, w, std::make_integer_sequence<int, KA>{});
+        for (int tb = 0; tb < nsteps; tb += W)
+            smx_window<true, P, KA, KZ, W, 0>(m, s, sz, bf, bd, ys, tb, ys, ylast, y0, y1, ck, w, x, rr,
+                                          std::make_integer_sequence<int, W>{});
+    } else {
+        SLane<float> s;
+        slane_init<float>(E, C, m.cop, x0, WOUT, HH, s);
+        SLane<double> sz;
+        slane_init<double>(m.Z, m.Z, m.cop, x0, WOUT, HH, sz);
+        const int ck = (ys - 1) >> 1;
+        sprologue<float, W, true>(E, C, bf, s, ys, ylast, ck, w, std::make_integer_sequence<int, KA>{});
+        for (int tb = 0; tb < nsteps; tb += W)
+            smx_window<false, P, KA, KZ, W, HV & 1>(m, s, sz, bf, bd, ys, tb, ys, ylast, y0, y1, ck, w, x, rr,
+                                                std::make_integer_sequence<int, W>{});
+    }
+}
+
+template <bool FEM, int NU>
+__global__ void __launch_bounds__(kSThreads) k_smx(MxArgs a) {
+    __shared__ double sh[8];
+    if (a.ctl.state->done) return;
+    stamp_begin(a.ts);
+    const int bid = xcd_tile(blockIdx.x, a.nb);
+    const int item = stream_item(a.nb);
+    double rr = 0;
+    if (item < a.items) mx_stream_body<FEM, NU>(a, item, rr);
+    store_partial(a.ctl, rr, bid, sh);
+    stamp_end(a.ts);
+}
+
 }  // namespace vdi
