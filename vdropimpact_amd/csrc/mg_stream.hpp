@@ -376,12 +376,17 @@ __device__ __forceinline__ void sstep(const LvArgs<T> &a, const SLane<T> &s, con
 }
 
 // W consecutive steps (U = 0..W-1), so that every window slot is a
-// compile-time register; the first row ys has parity YS.
+// compile-time register; the first row ys has parity YS.  The steps stop at
+// step nsteps (a scalar branch per step; the && fold evaluates in order and
+// stops at the first false).
 template <class T, int P, bool UP, int LAG, int W, int YS, int... U>
 __device__ __forceinline__ void sstep_window(const LvArgs<T> &a, const SLane<T> &s, const SBufs &bf, int ys, int tb,
-                                             int ylo, int yhi, int y0, int y1, int ck, SWin<T, W> &w, T &acc,
-                                             double &rr, std::integer_sequence<int, U...>) {
-    (sstep<T, P, UP, LAG, W, U, (YS + U) & 1>(a, s, bf, ys + tb + U, ylo, yhi, y0, y1, ck, tb, w, acc, rr), ...);
+                                             int nsteps, int ylo, int yhi, int y0, int y1, int ck, SWin<T, W> &w,
+                                             T &acc, double &rr, std::integer_sequence<int, U...>) {
+    (void)((tb + U < nsteps &&
+            (sstep<T, P, UP, LAG, W, U, (YS + U) & 1>(a, s, bf, ys + tb + U, ylo, yhi, y0, y1, ck, tb, w, acc, rr),
+             true)) &&
+           ...);
 }
 
 // Item (strip, segment) of wavefront w of workgroup b.
@@ -430,12 +435,11 @@ __device__ __forceinline__ void stream_body(const LvArgs<T> &a, int item, double
     const int ys = y0 - HV, ylast = y0 + S + HV - 1;
     const int ck = (ys - 1) >> 1;  // ys is odd: coarse row of the window's first row pair
     sprologue<T, W, UP>(F, C, bf, s, ys, ylast, ck, w, std::make_integer_sequence<int, kSAhead>{});
-    // whole windows until the last output row has had its residual; the
-    // steps past it load nothing, update nothing and store nothing
+    // steps until the last output row has had its residual
     const int nsteps = (y1 - 1) + LAG * P + 2 - ys + 1;
     T acc = T(0);
     for (int tb = 0; tb < nsteps; tb += W)
-        sstep_window<T, P, UP, LAG, W, HV & 1>(a, s, bf, ys, tb, ys, ylast, y0, y1, ck, w, acc, rr,
+        sstep_window<T, P, UP, LAG, W, HV & 1>(a, s, bf, ys, tb, nsteps, ys, ylast, y0, y1, ck, w, acc, rr,
                                           std::make_integer_sequence<int, W>{});
 }
 
@@ -697,10 +701,13 @@ __device__ __forceinline__ void snprologue(const LevelT<T> &F, const LevelT<T> &
 
 template <class T, int P, bool UP, int W, int YS, int... U>
 __device__ __forceinline__ void snstep_window(const LvArgs<T> &a, const SNLane<T> &s, const SBufs &bf, int ys, int tb,
-                                              int ylo, int yhi, int y0, int y1, int ck, SWin<T, W> &w, T &acc,
-                                              double &rr, std::integer_sequence<int, U...>) {
+                                              int nsteps, int ylo, int yhi, int y0, int y1, int ck, SWin<T, W> &w,
+                                              T &acc, double &rr, std::integer_sequence<int, U...>) {
     T dummy = T(0);
-    (snstep<T, P, UP, W, U, (YS + U) & 1>(a, s, bf, ys + tb + U, ylo, yhi, y0, y1, ck, tb, w, acc, dummy, rr), ...);
+    (void)((tb + U < nsteps && (snstep<T, P, UP, W, U, (YS + U) & 1>(a, s, bf, ys + tb + U, ylo, yhi, y0, y1, ck, tb,
+                                                                     w, acc, dummy, rr),
+                                true)) &&
+           ...);
 }
 
 template <class T, int NU, bool UP>
@@ -733,7 +740,7 @@ __device__ __forceinline__ void nstream_body(const LvArgs<T> &a, int item, doubl
     const int nsteps = qlast + P + 2 - ys + 1;
     T acc = T(0);
     for (int tb = 0; tb < nsteps; tb += W)
-        snstep_window<T, P, UP, W, 0>(a, s, bf, ys, tb, ys, ylast, y0, y1, ck, w, acc, rr,
+        snstep_window<T, P, UP, W, 0>(a, s, bf, ys, tb, nsteps, ys, ylast, y0, y1, ck, w, acc, rr,
                                       std::make_integer_sequence<int, W>{});
 }
 
@@ -886,10 +893,13 @@ __device__ __forceinline__ void smx_step(const MxArgs &m, const LE &s, const LZ 
 
 template <bool FEM, int P, int KA, int KZ, int W, int YS, class LE, class LZ, int... U>
 __device__ __forceinline__ void smx_window(const MxArgs &m, const LE &s, const LZ &sz, const SBufs &bf,
-                                           const SMxBufs &bd, int ys, int tb, int ylo, int yhi, int y0, int y1, int ck,
-                                           SWin<float, W> &w, SMxWin<W> &x, double &rr,
+                                           const SMxBufs &bd, int ys, int tb, int nsteps, int ylo, int yhi, int y0,
+                                           int y1, int ck, SWin<float, W> &w, SMxWin<W> &x, double &rr,
                                            std::integer_sequence<int, U...>) {
-    (smx_step<FEM, P, KA, KZ, W, U, (YS + U) & 1>(m, s, sz, bf, bd, ys + tb + U, ylo, yhi, y0, y1, ck, tb, w, x, rr), ...);
+    (void)((tb + U < nsteps && (smx_step<FEM, P, KA, KZ, W, U, (YS + U) & 1>(m, s, sz, bf, bd, ys + tb + U, ylo, yhi,
+                                                                             y0, y1, ck, tb, w, x, rr),
+                                true)) &&
+           ...);
 }
 
 template <bool FEM, int NU>
@@ -929,7 +939,7 @@ __device__ __forceinline__ void mx_stream_body(const MxArgs &m, int item, double
         const int ck = ys >> 1;
         snprologue<float, W, true>(E, C, bf, s, ys, ylast, ck, w, std::make_integer_sequence<int, KA>{});
         for (int tb = 0; tb < nsteps; tb += W)
-            smx_window<true, P, KA, KZ, W, 0>(m, s, sz, bf, bd, ys, tb, ys, ylast, y0, y1, ck, w, x, rr,
+            smx_window<true, P, KA, KZ, W, 0>(m, s, sz, bf, bd, ys, tb, nsteps, ys, ylast, y0, y1, ck, w, x, rr,
                                           std::make_integer_sequence<int, W>{});
     } else {
         SLane<float> s;
@@ -939,7 +949,7 @@ __device__ __forceinline__ void mx_stream_body(const MxArgs &m, int item, double
         const int ck = (ys - 1) >> 1;
         sprologue<float, W, true>(E, C, bf, s, ys, ylast, ck, w, std::make_integer_sequence<int, KA>{});
         for (int tb = 0; tb < nsteps; tb += W)
-            smx_window<false, P, KA, KZ, W, HV & 1>(m, s, sz, bf, bd, ys, tb, ys, ylast, y0, y1, ck, w, x, rr,
+            smx_window<false, P, KA, KZ, W, HV & 1>(m, s, sz, bf, bd, ys, tb, nsteps, ys, ylast, y0, y1, ck, w, x, rr,
                                                 std::make_integer_sequence<int, W>{});
     }
 }
