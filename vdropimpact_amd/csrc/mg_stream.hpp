@@ -11,17 +11,19 @@
 // the even column c0 + 2k and the odd column c0 + 2k + 1 -- and a segment of
 // S rows, and walks up the rows of its strip.  The rows it is working on sit
 // in a rolling window of registers: z and b of both columns for the last
-// 2P + 4 rows and kSAhead rows loaded ahead.  A point's vertical neighbours
+// P + 4 rows and kSAhead rows loaded ahead.  A point's vertical neighbours
 // are in the lane's own registers; of its two horizontal neighbours one is
 // the lane's other column and the other comes from the next lane with one
 // DPP whole-wave shift (wave_shr:1 / wave_shl:1).  No LDS, no barriers.
 //
 // Schedule.  When row y has been loaded, pass p (p = 1..P, colour (p-1)&1)
-// runs on row y - 2p.  A point of row r in pass p then reads row r+1 after
-// pass p-1 (run one step earlier) and not yet pass p+1, and row r-1 after
-// pass p-1 and before pass p+1 -- exactly the values a global sweep gives --
-// and the P updates of one step are independent of each other.  The
-// residual, the stores and (down) the restriction run on row y - 2P - 2.
+// runs on row y - p, the passes of a step in order.  A point of row r in
+// pass p then reads row r+1 after pass p-1 (run earlier in the same step)
+// and not yet pass p+1, and row r-1 after pass p-1 and before pass p+1 --
+// exactly the values a global sweep gives.  The residual, the stores and
+// (down) the restriction run on row y - P - 2.  (The code also supports a
+// lag of 2, pass p on row y - 2p, whose P updates per step are independent;
+// its P more steps per wavefront cost more than the chained updates.)
 // Each pass shrinks the region whose values equal the global sweep's by one
 // row and column at each end, so a strip carries a halo of P + 2 columns
 // (even, so that the restriction's 2 x 2 blocks lie within one lane) and a
@@ -411,9 +413,8 @@ __device__ __forceinline__ void stream_body(const LvArgs<T> &a, int item, double
     constexpr int HH = (P + 2) & ~1;   // columns of halo (even)
     constexpr int HV = P + 1;          // rows of halo
     constexpr int WOUT = kSW - 2 * HH;
-    // pass lag: down launches 2 (independent updates per step); up launches
-    // 1, whose window also holds the coarse values in flight
-    constexpr int LAG = UP ? 1 : 2;
+    // pass lag 1 (see the schedule above)
+    constexpr int LAG = 1;
     constexpr int W = LAG * P + 4 + kSAhead;  // even: a window's rows keep their parities
     static_assert(W % 2 == 0 && (P + 1) % 2 == 1, "even window, odd first row");
     static_assert(kSQAhead % 2 == 0 && W / 2 >= kSQAhead / 2 + 3, "the coarse ring holds the rows in use");
