@@ -497,8 +497,11 @@ __global__ void __launch_bounds__(kSThreads) k_sup(LvArgs<T> a) {
 // and r+1 between the passes a global sweep would give it, as for the
 // cell-centred levels.  An update reads the other column's three rows, one
 // of them from the neighbouring lane.  Row 0 is Dirichlet and never updated.
-// The halo is 4 nu + 2 rows and columns: each pass shrinks the exact region
-// by one, the residual by one more, and the full-weighting restriction
+// The halo is 4 nu + 2 columns and 2 nu + 2 rows.  Along a row each pass
+// shrinks the exact region by one column (a pass's points read the columns
+// the pass before updated).  Across rows the passes come in pairs on rows of
+// one parity, so after pass p the region has lost ceil(p / 2) rows at each
+// end; the residual needs one row more, and the full-weighting restriction
 // reads residuals one node beyond the output.
 
 template <class T>
@@ -562,7 +565,7 @@ __device__ __forceinline__ void snpass(const LevelT<T> &F, const FemOp &op, cons
     if constexpr (((Y - p) & 1) == cj) {
         constexpr int c = sslot<W>(U, p), d = sslot<W>(U, p + 1), u = sslot<W>(U, p - 1);
         const int r = y - p;
-        const bool ok = r >= max(ylo + p, 1) && r <= min(yhi - p, F.n - 1);
+        const bool ok = r >= max(ylo + (p + 1) / 2, 1) && r <= min(yhi - (p + 1) / 2, F.n - 1);
         const T wj = INT ? T(1) : (r == F.n - 1 ? T(0.5) : T(1));
         if constexpr (ci == 0) {
             const T v = (w.ba[c] - snoff_a<T>(op, s.wia, wj, w.za, w.zb, c, d, u)) *
@@ -714,10 +717,11 @@ __device__ __forceinline__ void snstep_window(const LvArgs<T> &a, const SNLane<T
 template <class T, int NU, bool UP>
 __device__ __forceinline__ void nstream_body(const LvArgs<T> &a, int item, double &rr) {
     constexpr int P = 4 * NU;
-    constexpr int HH = 4 * NU + 2;  // columns and rows of halo (even)
+    constexpr int HH = 4 * NU + 2;  // columns of halo (even)
+    constexpr int HV = 2 * NU + 2;  // rows of halo (even)
     constexpr int WOUT = kSW - 2 * HH;
     constexpr int W = P + 4 + kSNAhead<UP, P>;
-    static_assert(W % 2 == 0 && HH % 2 == 0, "even window, even first row");
+    static_assert(W % 2 == 0 && HH % 2 == 0 && HV % 2 == 0, "even window, even first row");
     static_assert(kSQAhead % 2 == 0 && W / 2 >= kSQAhead / 2 + 3, "the coarse ring holds the rows in use");
     const LevelT<T> &F = a.F, &C = a.C;
     const int S = a.ty;
@@ -732,7 +736,7 @@ __device__ __forceinline__ void nstream_body(const LvArgs<T> &a, int item, doubl
     bf.dst = srsrc(UP ? F.z : F.zalt, nf);
     bf.coarse = srsrc(UP ? C.z : C.b, nc);
     SWin<T, W> w = {};
-    const int ys = y0 - HH, ylast = y0 + S + HH - 1;
+    const int ys = y0 - HV, ylast = y0 + S + HV - 1;
     const int ck = ys >> 1;  // ys is even: coarse row of the window's first row
     snprologue<T, W, UP>(F, C, bf, s, ys, ylast, ck, w, std::make_integer_sequence<int, kSNAhead<UP, P>>{});
     // residual rows up to the segment's last output row (down: row 2J + 1
@@ -907,7 +911,7 @@ template <bool FEM, int NU>
 __device__ __forceinline__ void mx_stream_body(const MxArgs &m, int item, double &rr) {
     constexpr int P = FEM ? 4 * NU : 2 * NU;
     constexpr int HH = FEM ? 4 * NU + 2 : (2 * NU + 2) & ~1;  // columns of halo (even)
-    constexpr int HV = FEM ? 4 * NU + 2 : 2 * NU + 1;          // rows of halo
+    constexpr int HV = FEM ? 2 * NU + 2 : 2 * NU + 1;          // rows of halo
     constexpr int WOUT = kSW - 2 * HH;
     constexpr int KA = FEM ? kSNAhead<true, P> : kSAhead;
     constexpr int KZ = kSMxZ<P, HV>;
