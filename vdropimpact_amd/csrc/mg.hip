@@ -184,22 +184,39 @@ __device__ __forceinline__ T fem_inv_acc(const LevelT<T> &L, int i, int j) {
     return T(((i == 0 || i == L.m - 1) ? 2.0 : 1.0) * ((j == 0 || j == L.n - 1) ? 2.0 : 1.0)) * L.inv_int;
 }
 
+// A tile of TX x TY points at (x0, y0) with a halo of H columns and HY rows
+// on each side (the nodal sweeps need fewer rows than columns).
 struct Tile {
-    int x0, y0, gx0, gy0, RX, RY, H, TX, TY;
+    int x0, y0, gx0, gy0, RX, RY, H, HY, TX, TY;
 };
 
-__device__ __forceinline__ Tile make_tile(int bx, int by, int H, int TX, int TY) {
+__device__ __forceinline__ Tile make_tile(int bx, int by, int H, int TX, int TY, int HY) {
     Tile t;
     t.H = H;
+    t.HY = HY;
     t.TX = TX;
     t.TY = TY;
     t.x0 = bx * TX;
     t.y0 = by * TY;
     t.gx0 = t.x0 - H;
-    t.gy0 = t.y0 - H;
+    t.gy0 = t.y0 - HY;
     t.RX = TX + 2 * H;
-    t.RY = TY + 2 * H;
+    t.RY = TY + 2 * HY;
     return t;
+}
+
+// Halo of a level launch with nu sweeps: columns, and rows (HY).  Cell-centred
+// red-black passes shrink the region whose values equal a global sweep's by
+// one point per pass both ways, plus one for the residual (norm or
+// restriction).  The nodal four-colour passes shrink it by one column per
+// pass, but by one row per pair of passes (a pair updates rows of one
+// parity): ceil(p / 2) rows after pass p, with the tile's first row even.
+// Nodal down launches need residuals one row and column beyond the tile (full
+// weighting), up launches with the norm the tile's own residuals; the row
+// halo is kept even so that the tile's first row stays even.
+__host__ __device__ __forceinline__ int halo_cols(bool fem, int nu, int extra) { return (fem ? 4 : 2) * nu + extra; }
+__host__ __device__ __forceinline__ int halo_rows(bool fem, int nu, int extra) {
+    return fem ? 2 * nu + 2 * extra : 2 * nu + extra;
 }
 
 // Bilinear weights of a cell-centred prolongation: the coarse value q00 of
@@ -357,7 +374,7 @@ __device__ __forceinline__ void tile_store(const LevelT<T> &F, T *dst, const T *
         for (int x = threadIdx.x; x < t.TX; x += kBX) {
             const int gx = t.x0 + x;
             if (gx >= F.m) break;
-            dst[gx + (size_t)F.m * gy] = sz[(y + t.H) * t.RX + x + t.H];
+            dst[gx + (size_t)F.m * gy] = sz[(y + t.HY) * t.RX + x + t.H];
         }
     }
 }
@@ -368,7 +385,7 @@ __device__ __forceinline__ void cc_tile_sweep(const LevelT<T> &L, const CCOp &op
                                               int s, int c, const T *itab) {
 #pragma clang fp contract(off)
     const int X0 = max(0, t.gx0 + s), X1 = min(L.m, t.x0 + t.TX + t.H - s);
-    const int Y0 = max(0, t.gy0 + s), Y1 = min(L.n, t.y0 + t.TY + t.H - s);
+    const int Y0 = max(0, t.gy0 + s), Y1 = min(L.n, t.y0 + t.TY + t.HY - s);
     const T sgn = T(op.s);
     for (int gy = Y0 + threadIdx.y; gy < Y1; gy += BY) {
         const int xs = X0 + (((X0 + gy) & 1) ^ c);
@@ -389,8 +406,9 @@ __device__ __forceinline__ void cc_tile_sweep(const LevelT<T> &L, const CCOp &op
 template <class T, int BX = kBX, int BY = kBY>
 __device__ __forceinline__ void fem_tile_sweep(const LevelT<T> &L, const FemOp &op, T *sz, const T *sb, const Tile &t,
                                                int s, int ci, int cj) {
+    const int sy = (s + 1) >> 1;  // rows lost after pass s (halo_rows)
     const int X0 = max(0, t.gx0 + s), X1 = min(L.m, t.x0 + t.TX + t.H - s);
-    const int Y0 = max(1, t.gy0 + s), Y1 = min(L.n, t.y0 + t.TY + t.H - s);
+    const int Y0 = max(1, t.gy0 + sy), Y1 = min(L.n, t.y0 + t.TY + t.HY - sy);
     const int ys = Y0 + ((Y0 ^ cj) & 1), xs = X0 + ((X0 ^ ci) & 1);
     const int nx = (X1 - xs + 1) >> 1, ny = (Y1 - ys + 1) >> 1;
     if (nx <= 0 || ny <= 0) return;
@@ -572,7 +590,8 @@ __device__ __forceinline__ int fin_norm(const MGCtl &ctl, int np, int pairs, dou
 template <class T, bool FEM>
 __device__ __forceinline__ void down_body(const LvArgs<T> &a, int tile, T *lds, double *sh) {
     const LevelT<T> &F = a.F, &C = a.C;
-    const Tile t = make_tile(tile % a.nbx, tile / a.nbx, FEM ? 4 * a.nu + 2 : 2 * a.nu + 1, a.tx, a.ty);
+    const Tile t = make_tile(tile % a.nbx, tile / a.nbx, halo_cols(FEM, a.nu, FEM ? 2 : 1), a.tx, a.ty,
+                             halo_rows(FEM, a.nu, 1));
     T *sz = lds, *sb = lds + t.RX * t.RY, *itab = sb + t.RX * t.RY;
     double pf[2 * kFinRegs];
     if (a.fin) fin_load(a.ctl, a.fin, a.fin_begin, pf);
@@ -613,7 +632,8 @@ __device__ __forceinline__ void down_body(const LvArgs<T> &a, int tile, T *lds, 
 template <class T, bool FEM>
 __device__ __forceinline__ void up_body(const LvArgs<T> &a, int tile, T *lds, double *sh, int *last) {
     const LevelT<T> &F = a.F;
-    const Tile t = make_tile(tile % a.nbx, tile / a.nbx, (FEM ? 4 : 2) * a.nu + a.flag, a.tx, a.ty);
+    const Tile t = make_tile(tile % a.nbx, tile / a.nbx, halo_cols(FEM, a.nu, a.flag), a.tx, a.ty,
+                             halo_rows(FEM, a.nu, a.flag));
     T *sz = lds, *sb = lds + t.RX * t.RY, *itab = sb + t.RX * t.RY;
     tile_load<T, FEM>(F, F.zalt, &a.C, a.cop, sz, sb, t);
     if (!FEM) cc_inv_table<T>(F, a.cop, itab, threadIdx.x + kBX * threadIdx.y);
@@ -639,7 +659,7 @@ __device__ __forceinline__ void up_body(const LvArgs<T> &a, int tile, T *lds, do
         for (int x = threadIdx.x; x < t.TX; x += kBX) {
             const int gx = t.x0 + x;
             if (gx >= F.m) break;
-            const int l = (y + t.H) * t.RX + x + t.H;
+            const int l = (y + t.HY) * t.RX + x + t.H;
             const double r = FEM ? fem_resid_t<T>(a.fop, F.m, F.n, sz, sb, t.RX, l, gx, gy)
                                  : cc_resid_t<T>(F, a.cop, sz, sb, t.RX, l, gx, gy);
             rr += r * r;
@@ -670,7 +690,7 @@ __device__ __forceinline__ double mx_az(const MxArgs &a, const double *zd, int l
 template <bool FEM>
 __device__ __forceinline__ void mx_body(const MxArgs &a, int tile, double *lds_d, double *sh, int *last) {
     const Level &Z = a.Z;
-    const Tile t = make_tile(tile % a.nbx, tile / a.nbx, (FEM ? 4 : 2) * a.nu + 1, a.tx, a.ty);
+    const Tile t = make_tile(tile % a.nbx, tile / a.nbx, halo_cols(FEM, a.nu, 1), a.tx, a.ty, halo_rows(FEM, a.nu, 1));
     const int R = t.RX * t.RY;
     double *zd = lds_d;
     float *se = reinterpret_cast<float *>(lds_d + R), *sr = se + R, *itab = sr + R;
@@ -709,7 +729,7 @@ __device__ __forceinline__ void mx_body(const MxArgs &a, int tile, double *lds_d
                     const int ry = k / RW, rx = k - ry * RW;
                     const int gx = t.x0 - 1 + rx, gy = t.y0 - 1 + ry;
                     const bool in = gy >= 0 && gy < Z.n && gx >= 0 && gx < Z.m;
-                    const int l = (ry + t.H - 1) * t.RX + rx + t.H - 1;
+                    const int l = (ry + t.HY - 1) * t.RX + rx + t.H - 1;
                     zd[l] = in ? zz[u] + (double)se[l] : 0.0;
                 }
             }
@@ -737,7 +757,7 @@ __device__ __forceinline__ void mx_body(const MxArgs &a, int tile, double *lds_d
             const int y = k / t.TX, x = k - y * t.TX;
             const int gx = t.x0 + x, gy = t.y0 + y;
             if (gx >= Z.m || gy >= Z.n) continue;
-            const int l = (y + t.H) * t.RX + x + t.H;
+            const int l = (y + t.HY) * t.RX + x + t.H;
             const size_t g = gx + (size_t)Z.m * gy;
             const double r = bv[u] - mx_az<FEM>(a, zd, l, t.RX, gx, gy);
             a.z_out[g] = zd[l];
@@ -857,6 +877,7 @@ namespace vdi {
 __device__ __forceinline__ Tile coarse_tile(int m, int n) {
     Tile t;
     t.H = 1;
+    t.HY = 1;
     t.TX = m;
     t.TY = n;
     t.x0 = t.y0 = 0;
@@ -1520,8 +1541,8 @@ __global__ void k_reset_state(MGState *st) {
 // Launchers
 // ------------------------------------------------------------------------
 
-static size_t tile_lds_bytes(int H, size_t elem, int tx, int ty) {
-    return 2 * elem * (size_t)(tx + 2 * H) * (ty + 2 * H) + kInvTab * elem;
+static size_t tile_lds_bytes(int H, int HY, size_t elem, int tx, int ty) {
+    return 2 * elem * (size_t)(tx + 2 * H) * (ty + 2 * HY) + kInvTab * elem;
 }
 
 namespace mgk {
@@ -1823,8 +1844,7 @@ void KernelTimer::release() {
 // Driver
 // ------------------------------------------------------------------------
 
-static int halo_down(bool fem, int nu) { return fem ? 4 * nu + 2 : 2 * nu + 1; }
-static int halo_up(bool fem, int nu, int norm) { return (fem ? 4 : 2) * nu + norm; }
+
 
 // Largest default shape that still gives every CU a tile.
 static int default_tile(int m, int n) {
@@ -2064,18 +2084,22 @@ MGCtl MG::ctl() const {
 size_t MG::down_lds(int prec, int l) const {
     const TileShape &t = kTileShapes[tile[prec][0][l]];
     if (t.stream) return 0;
-    return tile_lds_bytes(halo_down(fem, l == 0 ? nu1 : nu1s), prec ? sizeof(float) : sizeof(double), t.tx, t.ty);
+    const int nu = l == 0 ? nu1 : nu1s;
+    return tile_lds_bytes(halo_cols(fem, nu, fem ? 2 : 1), halo_rows(fem, nu, 1), prec ? sizeof(float) : sizeof(double),
+                          t.tx, t.ty);
 }
 size_t MG::up_lds(int prec, int l) const {
     const TileShape &t = kTileShapes[tile[prec][1][l]];
     if (t.stream) return 0;
-    return tile_lds_bytes(halo_up(fem, l == 0 ? nu2 : nu2s, l == 0), prec ? sizeof(float) : sizeof(double), t.tx, t.ty);
+    const int nu = l == 0 ? nu2 : nu2s, norm = l == 0;
+    return tile_lds_bytes(halo_cols(fem, nu, norm), halo_rows(fem, nu, norm), prec ? sizeof(float) : sizeof(double),
+                          t.tx, t.ty);
 }
 size_t MG::mx_lds() const {
     const TileShape &t = kTileShapes[tile_mx];
     if (t.stream) return 0;
-    const int H = halo_up(fem, nu2, 1);
-    return (size_t)(t.tx + 2 * H) * (t.ty + 2 * H) * (sizeof(double) + 2 * sizeof(float)) + kInvTab * sizeof(float);
+    const int H = halo_cols(fem, nu2, 1), HY = halo_rows(fem, nu2, 1);
+    return (size_t)(t.tx + 2 * H) * (t.ty + 2 * HY) * (sizeof(double) + 2 * sizeof(float)) + kInvTab * sizeof(float);
 }
 
 // Streaming launch of a level with nu sweeps (a.nu), segments of a.ty rows:
