@@ -89,6 +89,7 @@ struct SLane {
     // x of the even / odd one (clamped) and their ghost factors
     int ci, cia, cib;
     T fxa, fxb;
+    T fyb, fyt;  // ghost factors of a coarse row below / above the coarse level
 };
 
 template <class T>
@@ -128,6 +129,11 @@ __device__ __forceinline__ void slane_init(const LevelT<T> &F, const LevelT<T> &
     s.fxb = T(1);
     if (s.cia < 0) { s.fxa = T(bc_ghost_factor(op.bcl)); s.cia = s.ci; }
     if (s.cib >= C.m) { s.fxb = T(bc_ghost_factor(op.bcr)); s.cib = s.ci; }
+    // kept in vector registers: as scalars they are reloaded from the kernel
+    // arguments (and waited for) at every step
+    s.fyb = T(bc_ghost_factor(op.bcb));
+    s.fyt = T(bc_ghost_factor(op.bct));
+    asm volatile("" : "+v"(s.fyb), "+v"(s.fyt));
 }
 
 // Row class: 0 interior, 1 first, 2 last row.  INT: the row is known to be
@@ -182,14 +188,15 @@ struct SWin {
 };
 
 // Coarse rows of cc_prolong_at for fine row y (inside the level): J and the
-// neighbour across y, J2, with its ghost factor.
+// neighbour across y, J2, with its ghost factor (fyb / fyt below / above the
+// coarse level).
 template <class T>
-__device__ __forceinline__ void sprolong_rows(const LevelT<T> &C, const CCOp &op, int y, int &J, int &J2, T &fy) {
+__device__ __forceinline__ void sprolong_rows(const LevelT<T> &C, T fyb, T fyt, int y, int &J, int &J2, T &fy) {
     J = y >> 1;
     J2 = (y & 1) ? J + 1 : J - 1;
-    fy = T(1);
-    if (J2 < 0) { fy = T(bc_ghost_factor(op.bcb)); J2 = J; }
-    else if (J2 >= C.n) { fy = T(bc_ghost_factor(op.bct)); J2 = J; }
+    const bool lo = J2 < 0, hi = J2 >= C.n;
+    fy = lo ? fyb : (hi ? fyt : T(1));
+    J2 = (lo || hi) ? J : J2;
 }
 
 // Stage 1 of a row, kSAhead steps before it is needed: the loads of z and
@@ -232,7 +239,7 @@ __device__ __forceinline__ void srow_fix(const LevelT<T> &F, const LevelT<T> &C,
     const bool row = y >= 0 && y < F.n && y <= ylast;
     int J, J2;
     T fy;
-    sprolong_rows<T>(C, op, min(max(y, 0), F.n - 1), J, J2, fy);
+    sprolong_rows<T>(C, s.fyb, s.fyt, min(max(y, 0), F.n - 1), J, J2, fy);
     const bool same = J2 == J;  // J2 outside the coarse level: row J with the ghost factor
     const T c1 = same ? w.qc[KJ] : w.qc[KJ2], a1 = same ? w.qa[KJ] : w.qa[KJ2], b1 = same ? w.qb[KJ] : w.qb[KJ2];
     const T va = w.za[K] + cc_prolong_combine<T>(s.fxa, fy, w.qc[KJ], w.qa[KJ], c1, a1);
