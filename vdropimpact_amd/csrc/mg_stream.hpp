@@ -3,11 +3,12 @@
 // launch (k_smx).
 //
 // Included by mg.hip after the tiled level kernels, whose operator helpers
-// (cc_diag, cc_prolong_combine, fem_off_v, fem_prolong_combine, mx_az) and
-// norm bookkeeping (fin_load, fin_norm, store_partial) these kernels share,
-// so that they compute the tiled kernels' bits: the same Gauss-Seidel
-// sweeps, residual, restriction and prolongation as down_body / up_body /
-// mx_body, with every rounding in the same order.
+// (cc_diag, cc_prolong_combine, fem_off_v, fem_prolong_combine) and norm
+// bookkeeping (fin_load, fin_norm, store_partial) these kernels share, so
+// that they compute the tiled kernels' bits: the same Gauss-Seidel sweeps,
+// residual, restriction and prolongation as down_body / up_body / mx_body
+// (whose FP64 residual, mx_az, k_smx restates in the same order), with every
+// rounding in the same order.
 //
 // Layout.  One wavefront owns a strip of kSW = 128 columns -- lane k holds
 // the even column c0 + 2k and the odd column c0 + 2k + 1 -- and a segment of
