@@ -96,6 +96,44 @@ def test_gpu_streaming_launches_every_schedule(vdi, kind, m, n, cx, cy, form):
         ref.close()
 
 
+@pytest.mark.parametrize("kind,m,n,cx,cy", [(0, 2048, 1024, 1e4, 6e3), (1, 2049, 1025, 0.01, 0.013)])
+@pytest.mark.parametrize("form", [0, 1])
+def test_gpu_streaming_launches_full_size(vdi, kind, m, n, cx, cy, form):
+    """At an operator-level size (strips of 116 / 108 columns and segments of
+    16 and 64 rows do not divide the levels) the streaming launches on every
+    level, and in the mixed form the streaming level-0 up launch, give the
+    tiled launches' V-cycles bit for bit."""
+    rng = np.random.default_rng(21 + kind)
+    b = rng.standard_normal(m * n)
+    prec = 1 if form else 0
+    ref = vdi.MGSystem(kind, m, n, cx, cy, 1)
+    ref.set_form(form)
+    ref.set_b(b)
+    ref.set_z(np.zeros(m * n))
+    ref.vcycles(2)
+    z_ref = ref.get_z()
+    levels = ref.levels
+    ref.close()
+    for ty in (16, 64):
+        shape = next(s for s in range(vdi.NUM_TILE_SHAPES) if vdi.tile_stream(s) and vdi.tile_shape(s)[1] == ty)
+        say("full size", (m, n), "kind", kind, "form", form, "shape", vdi.tile_label(shape))
+        g = vdi.MGSystem(kind, m, n, cx, cy, 1)
+        g.set_form(form)
+        for lv in range(levels):
+            for which in (0, 1):
+                try:
+                    g.set_tile(prec, which, lv, shape)
+                except vdi.VdiError:
+                    pass   # level handled by the coarsest-level kernels
+        if form:
+            g.set_tile(1, 2, 0, shape)
+        g.set_b(b)
+        g.set_z(np.zeros(m * n))
+        g.vcycles(2)
+        np.testing.assert_array_equal(g.get_z(), z_ref)
+        g.close()
+
+
 def test_gpu_autotuned_context_same_steps(vdi):
     from vdropimpact_amd.params import PhysicalConfig
     p = PhysicalConfig(m=256, n=128).params()
