@@ -1209,7 +1209,8 @@ __global__ void __launch_bounds__(kCoarseThreads) k_coarse_b(const CoarseArgs<T>
 // diagonalised, MG::coarse_factors).  Phases 1 and 3: out = in M over rows
 // (M = U, then U^T), a workgroup owning kXRows x kXCols outputs with its
 // input rows and M's columns staged in LDS; phase 2: kXColsV columns of w
-// through V^T, the diagonal 1 / (lam + mu) and V.  Arithmetic in FP64.
+// through V^T, the diagonal 1 / (lam + mu) and V.  Arithmetic in FP64; each
+// output's dot product is shared by four lanes (x_dot_quad).
 // ------------------------------------------------------------------------
 
 // Right-hand side of unknown row t (level row t + yoff) at column p: the
@@ -1264,6 +1265,21 @@ __device__ __forceinline__ double x_dot(const double *x, int sx, const double *y
     return (s0 + s1) + (s2 + s3);
 }
 
+// x_dot on the four lanes k = 0..3 of a quad (consecutive lanes): lane k
+// accumulates x_dot's partial sum s_k (terms q = k, k + 4, ... below n - n % 4
+// in order; lane 0 then the remaining terms), and the quad combines them as
+// x_dot does, (s0 + s1) + (s2 + s3), so the result is x_dot's bit for bit.
+// Every lane of the wavefront must call it.
+__device__ __forceinline__ double x_dot_quad(const double *x, int sx, const double *y, int sy, int n, int k) {
+    const int n4 = n & ~3;
+    double s = 0;
+    for (int q = k; q < n4; q += 4) s = fma(x[q * sx], y[q * sy], s);
+    if (k == 0)
+        for (int q = n4; q < n; q++) s = fma(x[q * sx], y[q * sy], s);
+    const double s2 = s + __shfl_xor(s, 1);  // lanes 0, 1: s0 + s1; lanes 2, 3: s2 + s3
+    return s2 + __shfl_xor(s2, 2);
+}
+
 template <class T>
 __device__ void xsolve_body(const XArgs<T> &a, int blk, double *lds) {
     const int tid = threadIdx.x, P = a.P, Tn = a.Tn;
@@ -1284,9 +1300,10 @@ __device__ void xsolve_body(const XArgs<T> &a, int blk, double *lds) {
                 return a.phase == 1 ? x_rhs<T>(a, t, p) : double(a.w[(size_t)t * P + p]);
             });
         __syncthreads();
-        const int r = tid / kXCols, c = tid - r * kXCols, t = r0 + r, j = c0 + c;
-        const double acc = x_dot(in + r * P, 1, mx + c, kXCols, P);
-        if (t < Tn && j < P) {
+        const int o = tid >> 2, k = tid & 3;  // output o, quarter k of its dot product
+        const int r = o / kXCols, c = o - r * kXCols, t = r0 + r, j = c0 + c;
+        const double acc = x_dot_quad(in + r * P, 1, mx + c, kXCols, P, k);
+        if (k == 0 && t < Tn && j < P) {
             if (a.phase == 1) a.w[(size_t)t * P + j] = T(acc);
             else a.z[(size_t)(t + a.yoff) * a.m + j] = T(acc);
         }
@@ -1304,15 +1321,18 @@ __device__ void xsolve_body(const XArgs<T> &a, int blk, double *lds) {
             return double(a.w[(size_t)t * P + min(j0 + c, P - 1)]);
         });
     __syncthreads();
-    for (int i = tid; i < Tn * kXColsV; i += kXThreads) {
-        const int k = i / kXColsV, c = i - k * kXColsV;
-        hl[i] = x_dot(Vl + k, Tn, wl + c, kXColsV, Tn) * D[(size_t)k * P + min(j0 + c, P - 1)];
+    // outputs o < Tn kXColsV (at most kXThreads / 4: Tn <= 64), quarter k
+    const int no = Tn * kXColsV, o = min(tid >> 2, no - 1), q4 = tid & 3;
+    {
+        const int k = o / kXColsV, c = o - k * kXColsV;
+        const double v = x_dot_quad(Vl + k, Tn, wl + c, kXColsV, Tn, q4) * D[(size_t)k * P + min(j0 + c, P - 1)];
+        if (q4 == 0 && (tid >> 2) < no) hl[o] = v;
     }
     __syncthreads();
-    for (int i = tid; i < Tn * kXColsV; i += kXThreads) {
-        const int t = i / kXColsV, c = i - t * kXColsV;
-        const double acc = x_dot(Vl + t * Tn, 1, hl + c, kXColsV, Tn);
-        if (j0 + c < P) a.w[(size_t)t * P + j0 + c] = T(acc);
+    {
+        const int t = o / kXColsV, c = o - t * kXColsV;
+        const double acc = x_dot_quad(Vl + t * Tn, 1, hl + c, kXColsV, Tn, q4);
+        if (q4 == 0 && (tid >> 2) < no && j0 + c < P) a.w[(size_t)t * P + j0 + c] = T(acc);
     }
 }
 

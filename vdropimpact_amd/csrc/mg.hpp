@@ -60,10 +60,14 @@ constexpr int kSepMaxT = 64;
 // U, columns through V^T, 1 / (lam + mu) and V, rows times U^T.
 constexpr int kExactBig = 9000;
 constexpr int kExactBigSide = 256;
-constexpr int kXRows = 8, kXCols = 32;  // row-transform block (256 threads, one output each)
+// Every output of the transforms is a dot product that four lanes share
+// (x_dot_quad).  Row transforms: kXRows x kXCols outputs per workgroup;
+// column transform: kXColsV columns (of at most 64 unknowns) per workgroup.
+constexpr int kXRows = 2, kXCols = 32;
 static_assert(kXCols == 32, "the row transform's staging indexes columns by shift and mask");
-constexpr int kXColsV = 4;               // column-transform block: columns per workgroup
+constexpr int kXColsV = 1;
 constexpr int kXThreads = 256;
+static_assert(4 * kXRows * kXCols == kXThreads && 4 * 64 * kXColsV == kXThreads, "four lanes per output");
 // The exact solve's transforms take a block of kSepModes modes (rows) by
 // kSepCols columns per thread; its factors are staged kSepLoads values per
 // thread and round (one round for the 64 x 32 levels of the 1024 x 512
