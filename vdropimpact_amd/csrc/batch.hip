@@ -230,9 +230,10 @@ static BLaunch x_launch_t(vdi_batch *b, Blob &B, int k, int ph) {
         MG &g = *step_solver(b->sims[s], k);
         if (!g.big_exact()) continue;
         XArgs<T> a;
-        if constexpr (std::is_same<T, double>::value) a = g.x_args<double>(g.L, ph);
-        else a = g.x_args<float>(g.LF, ph);
-        const int nb = ph == 2 ? (a.P + kXColsV - 1) / kXColsV : a.nbr * a.nbc;
+        // the throughput layout: the batch keeps every CU busy
+        if constexpr (std::is_same<T, double>::value) a = g.x_args<double>(g.L, ph, 0);
+        else a = g.x_args<float>(g.LF, ph, 0);
+        const int nb = mgk::xsolve_grid(a);
         for (int t = 0; t < nb; t++) work.push_back(WorkItem{(int)recs.size(), t, s});
         recs.push_back(a);
         lds = std::max(lds, g.x_lds(ph));

@@ -1285,8 +1285,9 @@ __device__ void xsolve_body(const XArgs<T> &a, int blk, double *lds) {
     const int tid = threadIdx.x, P = a.P, Tn = a.Tn;
     const double *V = a.fac, *U = V + (size_t)Tn * Tn, *Ut = U + (size_t)P * P, *D = Ut + (size_t)P * P;
     if (a.phase != 2) {
-        const int rb = blk / a.nbc, cb = blk - rb * a.nbc, r0 = rb * kXRows, c0 = cb * kXCols;
-        double *in = lds, *mx = lds + kXRows * P;
+        const int R = a.quad ? kXRowsQ : kXRows;
+        const int rb = blk / a.nbc, cb = blk - rb * a.nbc, r0 = rb * R, c0 = cb * kXCols;
+        double *in = lds, *mx = lds + R * P;
         const double *M = a.phase == 1 ? U : Ut;
         x_stage2(
             mx, P * kXCols,
@@ -1294,15 +1295,16 @@ __device__ void xsolve_body(const XArgs<T> &a, int blk, double *lds) {
                 const int q = i >> 5, c = i & (kXCols - 1);
                 return M[(size_t)q * P + min(c0 + c, P - 1)];
             },
-            in, kXRows * P,
+            in, R * P,
             [&](int i) {
                 const int r = i / P, p = i - r * P, t = min(r0 + r, Tn - 1);
                 return a.phase == 1 ? x_rhs<T>(a, t, p) : double(a.w[(size_t)t * P + p]);
             });
         __syncthreads();
-        const int o = tid >> 2, k = tid & 3;  // output o, quarter k of its dot product
+        // output o (quad layout: quarter k of its dot product)
+        const int o = a.quad ? tid >> 2 : tid, k = a.quad ? tid & 3 : 0;
         const int r = o / kXCols, c = o - r * kXCols, t = r0 + r, j = c0 + c;
-        const double acc = x_dot_quad(in + r * P, 1, mx + c, kXCols, P, k);
+        const double acc = a.quad ? x_dot_quad(in + r * P, 1, mx + c, kXCols, P, k) : x_dot(in + r * P, 1, mx + c, kXCols, P);
         if (k == 0 && t < Tn && j < P) {
             if (a.phase == 1) a.w[(size_t)t * P + j] = T(acc);
             else a.z[(size_t)(t + a.yoff) * a.m + j] = T(acc);
@@ -1312,27 +1314,39 @@ __device__ void xsolve_body(const XArgs<T> &a, int blk, double *lds) {
             a.z[c0 + tid] = T(double(a.b[c0 + tid]) / fem_acc<double>(a.fop, a.m, a.n, c0 + tid, 0));
         return;
     }
-    const int j0 = blk * kXColsV;
-    double *Vl = lds, *wl = Vl + Tn * Tn, *hl = wl + Tn * kXColsV;
+    const int CV = a.quad ? kXColsVQ : kXColsV;
+    const int j0 = blk * CV;
+    double *Vl = lds, *wl = Vl + Tn * Tn, *hl = wl + Tn * CV;
     x_stage2(
-        Vl, Tn * Tn, [&](int i) { return V[i]; }, wl, Tn * kXColsV,
+        Vl, Tn * Tn, [&](int i) { return V[i]; }, wl, Tn * CV,
         [&](int i) {
-            const int t = i / kXColsV, c = i - t * kXColsV;
+            const int t = i / CV, c = i - t * CV;
             return double(a.w[(size_t)t * P + min(j0 + c, P - 1)]);
         });
     __syncthreads();
-    // outputs o < Tn kXColsV (at most kXThreads / 4: Tn <= 64), quarter k
-    const int no = Tn * kXColsV, o = min(tid >> 2, no - 1), q4 = tid & 3;
-    {
-        const int k = o / kXColsV, c = o - k * kXColsV;
-        const double v = x_dot_quad(Vl + k, Tn, wl + c, kXColsV, Tn, q4) * D[(size_t)k * P + min(j0 + c, P - 1)];
-        if (q4 == 0 && (tid >> 2) < no) hl[o] = v;
+    if (a.quad) {
+        // outputs o < Tn CV (at most kXThreads / 4: Tn <= 64), quarter q4
+        const int no = Tn * CV, o = min(tid >> 2, no - 1), q4 = tid & 3;
+        {
+            const int k = o / CV, c = o - k * CV;
+            const double v = x_dot_quad(Vl + k, Tn, wl + c, CV, Tn, q4) * D[(size_t)k * P + min(j0 + c, P - 1)];
+            if (q4 == 0 && (tid >> 2) < no) hl[o] = v;
+        }
+        __syncthreads();
+        const int t = o / CV, c = o - t * CV;
+        const double acc = x_dot_quad(Vl + t * Tn, 1, hl + c, CV, Tn, q4);
+        if (q4 == 0 && (tid >> 2) < no && j0 + c < P) a.w[(size_t)t * P + j0 + c] = T(acc);
+        return;
+    }
+    for (int i = tid; i < Tn * CV; i += kXThreads) {
+        const int k = i / CV, c = i - k * CV;
+        hl[i] = x_dot(Vl + k, Tn, wl + c, CV, Tn) * D[(size_t)k * P + min(j0 + c, P - 1)];
     }
     __syncthreads();
-    {
-        const int t = o / kXColsV, c = o - t * kXColsV;
-        const double acc = x_dot_quad(Vl + t * Tn, 1, hl + c, kXColsV, Tn, q4);
-        if (q4 == 0 && (tid >> 2) < no && j0 + c < P) a.w[(size_t)t * P + j0 + c] = T(acc);
+    for (int i = tid; i < Tn * CV; i += kXThreads) {
+        const int t = i / CV, c = i - t * CV;
+        const double acc = x_dot(Vl + t * Tn, 1, hl + c, CV, Tn);
+        if (j0 + c < P) a.w[(size_t)t * P + j0 + c] = T(acc);
     }
 }
 
@@ -1723,7 +1737,8 @@ template <class T> int up_b(const LvArgs<T> *A, const WorkItem *w, int nw, bool 
     return VDI_OK;
 }
 template <class T> int xsolve_grid(const XArgs<T> &a) {
-    return a.phase == 2 ? (a.P + kXColsV - 1) / kXColsV : a.nbr * a.nbc;
+    const int cv = a.quad ? kXColsVQ : kXColsV;
+    return a.phase == 2 ? (a.P + cv - 1) / cv : a.nbr * a.nbc;
 }
 template int xsolve_grid<double>(const XArgs<double> &);
 template int xsolve_grid<float>(const XArgs<float> &);
@@ -2334,8 +2349,9 @@ double MG::coarse_bytes(int prec) const {
 }
 
 template <class T>
-XArgs<T> MG::x_args(const std::vector<LevelT<T>> &H, int phase) {
+XArgs<T> MG::x_args(const std::vector<LevelT<T>> &H, int phase, int quad) {
     XArgs<T> a{};
+    a.quad = quad;
     const LevelT<T> &v = H[lc];
     a.phase = phase;
     a.m = v.m;
@@ -2343,7 +2359,8 @@ XArgs<T> MG::x_args(const std::vector<LevelT<T>> &H, int phase) {
     a.yoff = sep_t0_;
     a.Tn = sep_T_;
     a.P = sep_P_;
-    a.nbr = (sep_T_ + kXRows - 1) / kXRows;
+    const int R = quad ? kXRowsQ : kXRows;
+    a.nbr = (sep_T_ + R - 1) / R;
     a.nbc = (sep_P_ + kXCols - 1) / kXCols;
     a.b = v.b;
     a.w = v.zalt;
@@ -2805,8 +2822,8 @@ template LvArgs<float> MG::down_args<float>(const std::vector<LevelF> &, int, in
 template LvArgs<double> MG::up_args<double>(const std::vector<Level> &, int, int, double);
 template LvArgs<float> MG::up_args<float>(const std::vector<LevelF> &, int, int, double);
 template CoarseArgs<double> MG::coarse_args<double>(const std::vector<Level> &, double);
-template XArgs<double> MG::x_args<double>(const std::vector<Level> &, int);
-template XArgs<float> MG::x_args<float>(const std::vector<LevelF> &, int);
+template XArgs<double> MG::x_args<double>(const std::vector<Level> &, int, int);
+template XArgs<float> MG::x_args<float>(const std::vector<LevelF> &, int, int);
 template CoarseArgs<float> MG::coarse_args<float>(const std::vector<LevelF> &, double);
 
 }  // namespace vdi

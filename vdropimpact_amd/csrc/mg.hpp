@@ -60,14 +60,19 @@ constexpr int kSepMaxT = 64;
 // U, columns through V^T, 1 / (lam + mu) and V, rows times U^T.
 constexpr int kExactBig = 9000;
 constexpr int kExactBigSide = 256;
-// Every output of the transforms is a dot product that four lanes share
-// (x_dot_quad).  Row transforms: kXRows x kXCols outputs per workgroup;
-// column transform: kXColsV columns (of at most 64 unknowns) per workgroup.
-constexpr int kXRows = 2, kXCols = 32;
+// Two block layouts of the transforms, with the same bits.  Throughput
+// (batched launches, where every CU is busy): one output per thread, row
+// transforms kXRows x kXCols outputs per workgroup, the column transform
+// kXColsV columns (of at most 64 unknowns).  Latency (direct launches, a
+// single system leaving most CUs idle): every output's dot product shared by
+// four lanes (x_dot_quad) on four times the workgroups, kXRowsQ x kXCols and
+// kXColsVQ.
+constexpr int kXRows = 8, kXCols = 32, kXColsV = 4;
+constexpr int kXRowsQ = 2, kXColsVQ = 1;
 static_assert(kXCols == 32, "the row transform's staging indexes columns by shift and mask");
-constexpr int kXColsV = 1;
 constexpr int kXThreads = 256;
-static_assert(4 * kXRows * kXCols == kXThreads && 4 * 64 * kXColsV == kXThreads, "four lanes per output");
+static_assert(kXRows * kXCols == kXThreads && 4 * kXRowsQ * kXCols == kXThreads && 4 * 64 * kXColsVQ == kXThreads,
+              "one output per thread / four lanes per output");
 // The exact solve's transforms take a block of kSepModes modes (rows) by
 // kSepCols columns per thread; its factors are staged kSepLoads values per
 // thread and round (one round for the 64 x 32 levels of the 1024 x 512
@@ -168,6 +173,7 @@ struct CoarseDesc {
 template <class T>
 struct XArgs {
     int phase, m, n, yoff, Tn, P, nbr, nbc;
+    int quad;  // 1: latency layout (kXRowsQ, kXColsVQ, four lanes per output)
     const T *b;
     T *w, *z;
     const double *fac;  // V (Tn x Tn), U (P x P), U^T, 1 / (lam + mu) (Tn x P)
@@ -378,7 +384,7 @@ class MG {
     template <class T> int launch_up(const std::vector<LevelT<T>> &H, int l, int norm, double tol2);
     template <class T> int launch_coarse(const std::vector<LevelT<T>> &H, double tol2);
   public:
-    template <class T> XArgs<T> x_args(const std::vector<LevelT<T>> &H, int phase);
+    template <class T> XArgs<T> x_args(const std::vector<LevelT<T>> &H, int phase, int quad = 1);
     size_t x_lds(int phase) const;
     bool big_exact() const { return big_; }
   private:
