@@ -677,6 +677,6 @@ class MGSystem:
 NUM_TILE_SHAPES = 18
 
 __all__ = [
-    "Fluid2D", "MGSystem", "Batch", "UNet", "tile_shape", "tile_label", "NUM_TILE_SHAPES", "VdiParams", "VdiStats", "VdiError", "PhysicalConfig",
+    "Fluid2D", "MGSystem", "Batch", "UNet", "tile_shape", "tile_label", "tile_stream", "NUM_TILE_SHAPES", "VdiParams", "VdiStats", "VdiError", "PhysicalConfig",
     "drop_impact_params", "h0_nd_for_hmin", "load_library", "LIB_PATH", "ABI",
 ]
