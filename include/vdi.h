@@ -266,7 +266,7 @@ int vdi_mg_solve_noalloc(int kind, int m, int n, double cx, double cy, int x_sym
 /* ---- tile shapes of the multigrid level kernels ----
  * No counterpart in the reference (TGMG's sweeps are not tiled).  Shapes are
  * indexes into a fixed candidate list (vdi_tile_shape gives tx, ty; the last
- * seven are streaming launches, one wavefront per strip of tx = 128 columns
+ * eight are streaming launches, one wavefront per strip of tx = 128 columns
  * less the halo and ty rows).
  * prec 0 = FP64 hierarchy, 1 = FP32 hierarchy of the mixed form; which 0 =
  * down (smooth + restrict), 1 = up (prolong + smooth), 2 = mixed level-0 up. */

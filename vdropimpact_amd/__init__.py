@@ -674,7 +674,7 @@ class MGSystem:
         return n.value, t.value, b.value
 
 
-NUM_TILE_SHAPES = 21
+NUM_TILE_SHAPES = 22
 
 __all__ = [
     "Fluid2D", "MGSystem", "Batch", "UNet", "tile_shape", "tile_label", "tile_stream", "NUM_TILE_SHAPES", "VdiParams", "VdiStats", "VdiError", "PhysicalConfig",

@@ -44,15 +44,15 @@ namespace vdi {
 // node adds a column and a row of nearly empty tiles to the power-of-two
 // shapes: at 1025 x 513, 56 x 40 gives 247 tiles (one per CU) where 64 x 32
 // gives 289 and 64 x 64 gives 153; at 513 x 257, 28 x 20 gives 247.
-// The last seven are streaming launches (segments of 16 .. 128 rows, then 20,
-// 40 and 12).  A launch is fastest when its wavefronts come to just under a
+// The last eight are streaming launches (segments of 16 .. 128 rows, then 20,
+// 40, 12 and 10).  A launch is fastest when its wavefronts come to just under a
 // whole number per SIMD: at 2048 x 1024, 16 rows give a few more wavefronts
 // than the SIMDs and 20 rows fewer; at 4096 x 2048, 32 rows give 2.25 per
 // SIMD and 40 rows 1.8.
 const TileShape kTileShapes[kNumTileShapes] = {{64, 32, 0},  {32, 8, 0},   {32, 16, 0},  {64, 8, 0},  {64, 16, 0},
                                                {32, 32, 0},  {128, 16, 0}, {128, 32, 0}, {64, 64, 0}, {56, 40, 0},
                                                {72, 36, 0},  {48, 24, 0},  {28, 20, 0},  {36, 16, 0}, {128, 16, 1},
-                                               {128, 32, 1}, {128, 64, 1}, {128, 128, 1}, {128, 20, 1}, {128, 40, 1}, {128, 12, 1}};
+                                               {128, 32, 1}, {128, 64, 1}, {128, 128, 1}, {128, 20, 1}, {128, 40, 1}, {128, 12, 1}, {128, 10, 1}};
 
 // Position b of n work items -> item index such that the items of one XCD
 // (b mod 8) are contiguous.

@@ -105,7 +105,7 @@ struct TileShape {
 };
 
 // Tile shapes the autotuner chooses from.
-constexpr int kNumTileShapes = 21;
+constexpr int kNumTileShapes = 22;
 extern const TileShape kTileShapes[kNumTileShapes];
 
 // Down / up kernel of one level.
