@@ -69,31 +69,34 @@ def test_gpu_streaming_launches_every_schedule(vdi, kind, m, n, cx, cy, form):
     rng = np.random.default_rng(11 + kind)
     b = rng.standard_normal(m * n)
     prec = 1 if form else 0
+    ns = len(STREAM_SCHEDULES)
     for i, (nu1, nu2) in enumerate(STREAM_SCHEDULES):
-        shape = stream[i % len(stream)]
-        say("schedule", (nu1, nu2), "shape", vdi.tile_label(shape), "kind", kind, "form", form)
         ref = vdi.MGSystem(kind, m, n, cx, cy, 1, nu1, nu2)
         ref.set_form(form)
         ref.set_b(b)
         ref.set_z(np.zeros(m * n))
         ref.vcycles(3)
         z_ref = ref.get_z()
-        g = vdi.MGSystem(kind, m, n, cx, cy, 1, nu1, nu2)
-        g.set_form(form)
-        for lv in range(g.levels):
-            for which in (0, 1):
-                try:
-                    g.set_tile(prec, which, lv, shape)
-                except vdi.VdiError:
-                    pass   # level handled by the coarsest-level kernels
-        if form:
-            g.set_tile(1, 2, 0, shape)
-        g.set_b(b)
-        g.set_z(np.zeros(m * n))
-        g.vcycles(3)
-        np.testing.assert_array_equal(g.get_z(), z_ref)
-        g.close()
         ref.close()
+        # two shapes per schedule, so that every streaming shape meets a
+        # schedule other than the default
+        for shape in sorted({stream[i % len(stream)], stream[(i + ns) % len(stream)]}):
+            say("schedule", (nu1, nu2), "shape", vdi.tile_label(shape), "kind", kind, "form", form)
+            g = vdi.MGSystem(kind, m, n, cx, cy, 1, nu1, nu2)
+            g.set_form(form)
+            for lv in range(g.levels):
+                for which in (0, 1):
+                    try:
+                        g.set_tile(prec, which, lv, shape)
+                    except vdi.VdiError:
+                        pass   # level handled by the coarsest-level kernels
+            if form:
+                g.set_tile(1, 2, 0, shape)
+            g.set_b(b)
+            g.set_z(np.zeros(m * n))
+            g.vcycles(3)
+            np.testing.assert_array_equal(g.get_z(), z_ref)
+            g.close()
 
 
 @pytest.mark.parametrize("kind,m,n,cx,cy", [(0, 2048, 1024, 1e4, 6e3), (1, 2049, 1025, 0.01, 0.013)])
