@@ -58,3 +58,22 @@ def test_missing_library_fails_loudly(tmp_path, monkeypatch):
     monkeypatch.setattr(vdi, "_lib", None)
     with pytest.raises(RuntimeError, match="not built"):
         vdi.load_library(str(tmp_path / "nope.so"))
+
+
+def test_tile_shape_table_matches_library():
+    """The package's NUM_TILE_SHAPES is the length of the library's candidate
+    list (vdi_tile_shape / vdi_tile_stream are host-only: no GPU is needed),
+    and the streaming launches are 128 columns wide with even segments."""
+    lib = vdi.load_library()
+    tx, ty = ctypes.c_int(), ctypes.c_int()
+    n_stream = 0
+    for s in range(vdi.NUM_TILE_SHAPES):
+        assert lib.vdi_tile_shape(s, ctypes.byref(tx), ctypes.byref(ty)) == vdi.OK
+        st = lib.vdi_tile_stream(s)
+        assert st in (0, 1)
+        if st:
+            n_stream += 1
+            assert tx.value == 128 and ty.value % 2 == 0
+    assert n_stream >= 1
+    assert lib.vdi_tile_shape(vdi.NUM_TILE_SHAPES, ctypes.byref(tx), ctypes.byref(ty)) != vdi.OK
+    assert lib.vdi_tile_stream(vdi.NUM_TILE_SHAPES) < 0

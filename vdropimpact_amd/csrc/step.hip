@@ -2083,7 +2083,7 @@ int vdi_mg_kernel_times(vdi_mg *h, long *launches, double *total_ms, double *alg
 }
 
 int vdi_tile_stream(int shape) {
-    if (shape < 0 || shape >= kNumTileShapes) return VDI_ERR_ARG;
+    if (shape < 0 || shape >= kNumTileShapes) return -1;  // documented: negative when out of range
     return kTileShapes[shape].stream;
 }
 
